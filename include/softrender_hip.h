@@ -1,0 +1,236 @@
+/*
+ * softrender_hip.h — C ABI of the MI355X rendering path.
+ *
+ * This is the drop-in boundary for softrender's draw path.  The reference is a
+ * generic Rust builder chain whose "plugins" are monomorphised closures:
+ *
+ *   Pipeline::render_mesh            src/pipeline/mod.rs:146-155
+ *   VertexShader::run / run_to_fragment
+ *                                    src/pipeline/stages/vertex.rs:87-160
+ *   GeometryShader::run / clip_primitives / finish
+ *                                    src/pipeline/stages/geometry.rs:60-336
+ *   FragmentShader::{with_faces_culled, with_antialiased_lines,
+ *                    with_tile_size, with_blend, run}
+ *                                    src/pipeline/stages/fragment.rs:82-319
+ *   Framebuffer::clear / pixel + depth + stencil accessors
+ *                                    src/framebuffer/renderbuffer/mod.rs:58-133
+ *
+ * Closures cannot run on the device, so each shader stage is a program id
+ * plus a uniform block; the builder state (cull, AA lines, tile size, blend,
+ * stencil config, stencil reference value) travels in sr_draw_desc.  One
+ * sr_draw() is one `render_mesh(..).run(..)[.clip_primitives()].finish(..)
+ * .run(..)` chain: the framebuffer afterwards holds exactly what the
+ * reference's single-threaded pipeline would leave in it.
+ *
+ * Conventions: every function returns an int status (SR_OK or a negative
+ * SR_ERR_*); no C++ exception crosses the ABI; handles are owned by the
+ * library until their _free or _destroy call; there are no callbacks into the
+ * host.  A context is externally synchronised (the reference's stages hold
+ * `&mut Pipeline`, src/pipeline/stages/vertex.rs:30-35); separate contexts
+ * may be used from separate threads, each with its own HIP stream.
+ */
+#ifndef SOFTRENDER_HIP_H
+#define SOFTRENDER_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------ */
+#define SR_OK 0
+#define SR_ERR_INVALID_ARGUMENT (-1)   /* reference: assert!/assert_eq! panics, pipeline/mod.rs:129-130,148 */
+#define SR_ERR_OUT_OF_MEMORY (-2)
+#define SR_ERR_DEVICE (-3)             /* a HIP runtime call failed */
+#define SR_ERR_INVALID_COORDINATE (-4) /* reference: cast(..).unwrap() panics on NaN, triangle.rs:70, point.rs:47 */
+#define SR_ERR_UNSUPPORTED (-5)
+#define SR_ERR_BUDGET (-6)             /* serving: request does not fit the HBM budget */
+#define SR_ERR_NOT_READY (-7)          /* serving: ticket still in flight */
+#define SR_ERR_NO_DEVICE (-8)
+
+/* ---- enums (values are part of the ABI) -------------------------------- */
+/* primitive.rs:7-167: zero-sized type tags selecting index assembly */
+#define SR_PRIM_POINTS 1
+#define SR_PRIM_LINES 2
+#define SR_PRIM_TRIANGLES 3
+
+/* Vertex programs (stand-ins for the `Fn(&V,&U) -> ClipVertex` closure,
+ * vertex.rs:87-89).  Both write clip = MVP * (p,1) and a normalised
+ * world-space normal n' = normalize(N * n). */
+#define SR_VP_MVP_NORMAL_UV 0    /* varyings: nx ny nz u v        (K=5) */
+#define SR_VP_MVP_NORMAL_WORLD 1 /* varyings: nx ny nz wx wy wz   (K=6) */
+
+/* Geometry programs (GeometryShader::run, geometry.rs:132-258). */
+#define SR_GS_NONE 0           /* run_to_fragment / finish without a program */
+#define SR_GS_CLIP_REFERENCE 1 /* clip_primitives, geometry.rs:261-336, bit-exact */
+#define SR_GS_CLIP_NEAR 2      /* Sutherland-Hodgman against the Near plane only */
+
+/* Fragment programs (the `Fn(&ScreenVertex,&U) -> Fragment` closure,
+ * fragment.rs:168-169). */
+#define SR_FP_FLAT 0        /* constant colour fs_uniforms[0..3] */
+#define SR_FP_BLINN_PHONG 1 /* needs SR_VP_MVP_NORMAL_WORLD */
+#define SR_FP_TEX_LAMBERT 2 /* needs SR_VP_MVP_NORMAL_UV and a texture */
+#define SR_FP_UV_COLOR 3    /* r=u g=v b=fs[0] a=fs[1]; needs SR_VP_MVP_NORMAL_UV */
+
+/* Blend (color/blend.rs:9-74): `()` replaces, OVER is src-alpha over. */
+#define SR_BLEND_REPLACE 0
+#define SR_BLEND_OVER 1
+
+/* Face culling (geometry/winding.rs:11-43; triangle.rs:54-61) */
+#define SR_CULL_NONE 0
+#define SR_CULL_CW 1
+#define SR_CULL_CCW 2
+
+/* StencilTest, stencil.rs:90-107 */
+#define SR_STENCIL_ALWAYS 0
+#define SR_STENCIL_NEVER 1
+#define SR_STENCIL_LESS 2      /* ref <  fb */
+#define SR_STENCIL_GREATER 3   /* ref >  fb */
+#define SR_STENCIL_LEQUAL 4    /* ref <= fb */
+#define SR_STENCIL_GEQUAL 5    /* ref >= fb */
+#define SR_STENCIL_EQUAL 6
+#define SR_STENCIL_NOTEQUAL 7
+/* StencilOp, stencil.rs:128-142 */
+#define SR_STENCIL_KEEP 0
+#define SR_STENCIL_INVERT 1
+#define SR_STENCIL_ZERO 2
+#define SR_STENCIL_REPLACE 3
+#define SR_STENCIL_INCR 4
+#define SR_STENCIL_DECR 5
+
+/* Alpha type of the colour attachment (color/mod.rs:22-32, helper.rs:31-73):
+ * U8 is the library's Vector4<u8> (Alpha = u8: Wu coverage collapses to 0/1);
+ * F32 is a #[repr(C)] RGBA8 colour whose Alpha is f32. */
+#define SR_ALPHA_U8 0
+#define SR_ALPHA_F32 1
+
+/* texture.rs:20-40 */
+#define SR_FILTER_NEAREST 0
+#define SR_FILTER_BILINEAR 1
+#define SR_EDGE_CLAMP 0
+#define SR_EDGE_WRAP 1
+#define SR_EDGE_BORDER 2
+
+/* sr_draw_desc.exec_flags */
+#define SR_EXEC_EXHAUSTIVE 1u /* every triangle is tested in every screen tile (reference order, no pruning) */
+#define SR_EXEC_TIMING 2u     /* record per-stage HIP events (read with sr_ctx_stage_times) */
+
+/* ---- plain data -------------------------------------------------------- */
+typedef struct sr_viewport {
+    /* clipvertex.rs:40-64 Viewport{x,y,width,height,near,far} */
+    float x, y, width, height, near_z, far_z;
+} sr_viewport;
+
+/* Vertex streams, structure-of-arrays.  mesh.rs:12-64 keeps AoS
+ * `Vec<V>` with a Point3 position plus user data; the device wants one
+ * coalesced stream per component.  normal/uv streams may be NULL (zeros). */
+typedef struct sr_vertex_soa {
+    const float *px, *py, *pz;
+    const float *nx, *ny, *nz;
+    const float *u, *v;
+} sr_vertex_soa;
+
+typedef struct sr_draw_desc {
+    int32_t vertex_program;   /* SR_VP_* */
+    int32_t geometry_program; /* SR_GS_* */
+    int32_t fragment_program; /* SR_FP_* */
+    int32_t blend;            /* SR_BLEND_* */
+    int32_t cull;             /* SR_CULL_*, fragment.rs:82-91 */
+    int32_t aa_lines;         /* Xiaolin Wu if non-zero, fragment.rs:96-105 */
+    int32_t tile_w, tile_h;   /* screen tile, fragment.rs:29,107-116 (default 128x128) */
+    int32_t stencil_test;     /* SR_STENCIL_* test, StencilConfig::get_test */
+    int32_t stencil_op;       /* SR_STENCIL_* op,   StencilConfig::get_op */
+    int32_t stencil_wrap;     /* Increment/Decrement { wrap } */
+    uint32_t stencil_ref;     /* render_mesh(.., Some(v)), pipeline/mod.rs:154 (u8) */
+    int32_t alpha_mode;       /* SR_ALPHA_* */
+    int32_t tex_filter;       /* SR_FILTER_* */
+    int32_t tex_edge;         /* SR_EDGE_* */
+    uint32_t tex_border;      /* RGBA8 for SR_EDGE_BORDER */
+    sr_viewport viewport;
+    /* vs_uniforms: [0..16) MVP row-major, [16..25) normal matrix (3x3,
+     * row-major), [25..37) model matrix rows 0..2 (3x4, row-major). */
+    float vs_uniforms[48];
+    /* fs_uniforms: program specific, documented with each program. */
+    float fs_uniforms[32];
+    int32_t exec_tile;   /* execution tile edge in pixels: 0 = auto, else 8/16/32/64 */
+    uint32_t exec_flags; /* SR_EXEC_* */
+    int32_t reserved[6];
+} sr_draw_desc;
+
+typedef struct sr_draw_stats {
+    uint64_t triangles_in, triangles_setup, lines_in, points_in;
+    uint64_t bin_pairs;  /* (item, execution tile) pairs */
+    uint64_t generated_tris, generated_lines, generated_points;
+} sr_draw_stats;
+
+/* ---- opaque handles ----------------------------------------------------- */
+typedef struct sr_context_t* sr_context;
+typedef struct sr_mesh_t* sr_mesh;
+typedef struct sr_texture_t* sr_texture;
+typedef struct sr_framebuffer_t* sr_framebuffer;
+typedef struct sr_server_t* sr_server;
+
+/* ---- context ------------------------------------------------------------ */
+/* Pipeline::new (pipeline/mod.rs:110-117) creates the CPU pool; here the
+ * context owns a HIP stream and a grow-only device workspace. */
+int sr_ctx_create(int device, sr_context* out);
+int sr_ctx_destroy(sr_context ctx);
+const char* sr_last_error(sr_context ctx);
+int sr_finish(sr_context ctx); /* wait for all queued work */
+void* sr_ctx_stream(sr_context ctx); /* the context's hipStream_t */
+/* Per-stage device times (ms) of the last draw run with SR_EXEC_TIMING.
+ * names: "vertex","geometry","setup","scan","emit","sort","ranges","strips","raster" */
+int sr_ctx_stage_times(sr_context ctx, float* ms_out, int n_max, int* n_out);
+int sr_ctx_last_stats(sr_context ctx, sr_draw_stats* out);
+
+/* ---- mesh (mesh.rs:12-64) ---------------------------------------------- */
+/* indices == NULL means non-indexed (index i = i); n_indices must be a
+ * multiple of the primitive's vertex count (assert_eq!, pipeline/mod.rs:148). */
+int sr_mesh_upload(sr_context ctx, const sr_vertex_soa* vertices, uint32_t n_vertices,
+                   const uint32_t* indices, uint32_t n_indices, int32_t primitive,
+                   sr_mesh* out);
+int sr_mesh_free(sr_mesh mesh);
+
+/* ---- texture (texture.rs:14-62; sampling is build-defined) -------------- */
+int sr_texture_upload(sr_context ctx, uint32_t width, uint32_t height,
+                      const uint32_t* rgba8, sr_texture* out);
+int sr_texture_free(sr_texture tex);
+
+/* ---- framebuffer (renderbuffer/mod.rs:14-133) --------------------------- */
+/* Width and height must be non-zero (pipeline/mod.rs:129-130).  Planes are
+ * row-major, pixel (x,y) at x + y*W (geometry/coordinate.rs:47-52). */
+int sr_fb_create(sr_context ctx, uint32_t width, uint32_t height, sr_framebuffer* out);
+int sr_fb_free(sr_framebuffer fb);
+/* Framebuffer::clear sets colour and resets depth to far (f32::MIN) and
+ * stencil to 0 (renderbuffer/mod.rs:125-133); depth/stencil are explicit here. */
+int sr_fb_clear(sr_framebuffer fb, uint32_t rgba, float depth, uint8_t stencil);
+/* Any of the output pointers may be NULL. */
+int sr_fb_read(sr_framebuffer fb, uint32_t* rgba, float* depth, uint8_t* stencil);
+int sr_fb_write(sr_framebuffer fb, const uint32_t* rgba, const float* depth,
+                const uint8_t* stencil);
+int sr_fb_size(sr_framebuffer fb, uint32_t* width, uint32_t* height);
+
+/* ---- draw --------------------------------------------------------------- */
+/* One reference draw chain (see top of file).  `tex` may be NULL unless the
+ * fragment program samples.  Asynchronous on the context stream except
+ * where the workspace has to grow. */
+int sr_draw(sr_context ctx, sr_framebuffer fb, sr_mesh mesh, const sr_draw_desc* desc,
+            sr_texture tex);
+
+/* ---- batching: many independent draws in one persistent launch ---------- */
+typedef struct sr_batch_item {
+    sr_framebuffer fb;
+    sr_mesh mesh;
+    const sr_draw_desc* desc;
+    sr_texture tex;
+} sr_batch_item;
+/* Renders n independent (framebuffer, mesh) draws; each framebuffer must
+ * appear at most once.  Results equal n separate sr_draw calls. */
+int sr_batch_draw(sr_context ctx, const sr_batch_item* items, uint32_t n);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SOFTRENDER_HIP_H */

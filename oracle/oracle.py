@@ -1,0 +1,153 @@
+"""Python loader for the CPU restatement (liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and
+bench.py's cpu_baseline leg, as the checker (and the timed CPU baseline).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+sys.path.insert(0, os.path.join(ROOT, "rust-softrender_amd"))
+from softrender_hip import DrawDesc, VertexSoA, vertex_soa, make_desc  # noqa: E402  (struct layouts only)
+
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+F32_MIN = float(np.finfo(np.float32).min)
+
+
+class OrFb(C.Structure):
+    _fields_ = [("w", C.c_uint32), ("h", C.c_uint32), ("color", C.POINTER(C.c_uint32)),
+                ("depth", C.POINTER(C.c_float)), ("stencil", C.POINTER(C.c_uint8))]
+
+
+class OrMesh(C.Structure):
+    _fields_ = [("v", VertexSoA), ("n_vertices", C.c_uint32), ("indices", C.POINTER(C.c_uint32)),
+                ("n_indices", C.c_uint32), ("primitive", C.c_int32)]
+
+
+class OrTexture(C.Structure):
+    _fields_ = [("w", C.c_uint32), ("h", C.c_uint32), ("rgba", C.POINTER(C.c_uint32))]
+
+
+STAT_FIELDS = ["tiles", "tri_calls", "tri_culled", "bbox_tests", "strip_tests", "covered",
+               "shaded", "strip_shaded", "line_calls", "line_accepted", "line_plots",
+               "line_shaded", "point_calls", "point_hits", "point_shaded", "gen_points",
+               "gen_lines", "gen_tris", "oob_plots", "invalid"]
+
+
+class OrStats(C.Structure):
+    _fields_ = [(n, C.c_uint64) for n in STAT_FIELDS]
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} not built (make -C oracle)")
+        L = C.CDLL(LIB_PATH)
+        L.or_draw.argtypes = [C.POINTER(OrFb), C.POINTER(OrMesh), C.POINTER(DrawDesc),
+                              C.POINTER(OrTexture), C.c_int, C.POINTER(OrStats)]
+        L.or_draw.restype = C.c_int
+        L.or_fb_clear.argtypes = [C.POINTER(OrFb), C.c_uint32, C.c_float, C.c_uint8]
+        fp = C.POINTER(C.c_float)
+        L.or_normalize.argtypes = [fp, C.POINTER(DrawDesc._fields_[16][1]), fp]
+        L.or_clip_triangle_reference.argtypes = [fp, fp, fp, C.c_int, fp, C.c_int]
+        L.or_clip_triangle_reference.restype = C.c_int
+        L.or_clip_triangle_near.argtypes = [fp, fp, fp, C.c_int, fp, C.c_int]
+        L.or_clip_triangle_near.restype = C.c_int
+        L.or_clip_line_reference.argtypes = [fp, fp, C.c_int, fp]
+        L.or_clip_line_reference.restype = C.c_int
+        L.or_tiles.argtypes = [C.c_uint32] * 4 + [C.POINTER(C.c_uint32), C.c_int]
+        L.or_tiles.restype = C.c_int
+        L.or_liang_barsky.argtypes = [C.c_float] * 8 + [fp]
+        L.or_liang_barsky.restype = C.c_int
+        i64p, f64p = C.POINTER(C.c_int64), C.POINTER(C.c_double)
+        L.or_bresenham.argtypes = [C.c_int64] * 4 + [i64p, f64p, C.c_int]
+        L.or_bresenham.restype = C.c_int
+        L.or_wu.argtypes = [C.c_double] * 4 + [i64p, f64p, C.c_int]
+        L.or_wu.restype = C.c_int
+        L.or_stencil_test.argtypes = [C.c_int, C.c_uint32, C.c_uint32]
+        L.or_stencil_test.restype = C.c_int
+        L.or_stencil_op.argtypes = [C.c_int, C.c_int, C.c_uint32, C.c_uint32]
+        L.or_stencil_op.restype = C.c_uint32
+        L.or_sample.argtypes = [C.POINTER(OrTexture), C.c_float, C.c_float, C.c_int, C.c_int,
+                                C.c_uint32, fp]
+        L.or_blend.argtypes = [C.c_int, C.c_uint32, C.c_uint32]
+        L.or_blend.restype = C.c_uint32
+        L.or_mul_alpha.argtypes = [C.c_int, C.c_uint32, C.c_double]
+        L.or_mul_alpha.restype = C.c_uint32
+        L.or_hypotf.argtypes = [C.c_float, C.c_float]
+        L.or_hypotf.restype = C.c_float
+        L.or_div_f32.argtypes = [C.c_float, C.c_float]
+        L.or_div_f32.restype = C.c_float
+        _lib = L
+    return _lib
+
+
+class Framebuffer:
+    """Host framebuffer in the reference's clear state (renderbuffer/mod.rs:22-30)."""
+
+    def __init__(self, w, h, clear=0, depth=F32_MIN, stencil=0):
+        self.w, self.h = w, h
+        n = w * h
+        self.color = np.full(n, clear, np.uint32)
+        self.depth = np.full(n, depth, np.float32)
+        self.stencil = np.full(n, stencil, np.uint8)
+
+    def struct(self):
+        return OrFb(self.w, self.h, self.color.ctypes.data_as(C.POINTER(C.c_uint32)),
+                    self.depth.ctypes.data_as(C.POINTER(C.c_float)),
+                    self.stencil.ctypes.data_as(C.POINTER(C.c_uint8)))
+
+    def copy(self):
+        f = Framebuffer(self.w, self.h)
+        f.color, f.depth, f.stencil = self.color.copy(), self.depth.copy(), self.stencil.copy()
+        return f
+
+
+def _texture(tex):
+    if tex is None:
+        return None, None
+    w, h, rgba = tex
+    rgba = np.ascontiguousarray(rgba, np.uint32)
+    return OrTexture(w, h, rgba.ctypes.data_as(C.POINTER(C.c_uint32))), rgba
+
+
+def draw(fb: Framebuffer, mesh: dict, desc: DrawDesc, tex=None, nthreads: int = 1):
+    """Runs one draw chain; returns (status, stats dict)."""
+    soa, keep = vertex_soa(mesh)
+    idx = mesh.get("indices")
+    n = len(keep["px"])
+    if idx is not None:
+        idx = np.ascontiguousarray(idx, np.uint32)
+        om = OrMesh(soa, n, idx.ctypes.data_as(C.POINTER(C.c_uint32)), len(idx), mesh["primitive"])
+    else:
+        om = OrMesh(soa, n, None, 0, mesh["primitive"])
+    ot, keep_t = _texture(tex)
+    st = OrStats()
+    fbs = fb.struct()
+    rc = lib().or_draw(C.byref(fbs), C.byref(om), C.byref(desc),
+                       C.byref(ot) if ot is not None else None, nthreads, C.byref(st))
+    return rc, {k: getattr(st, k) for k in STAT_FIELDS}
+
+
+def render_scene(scene: dict, nthreads: int = 1):
+    """Clear + all draws of a scene dict from scenes.py; returns (fb, stats list)."""
+    fb = Framebuffer(scene["width"], scene["height"], scene.get("clear", 0))
+    draws = scene.get("draws") or [scene]
+    stats = []
+    for d in draws:
+        desc = make_desc(**d["desc"])
+        rc, st = draw(fb, d, desc, scene.get("texture"), nthreads)
+        if rc != 0:
+            raise RuntimeError(f"oracle draw failed: {rc}")
+        stats.append(st)
+    return fb, stats

@@ -1,0 +1,224 @@
+// sr_device.hpp — device-side building blocks: exact f32 helpers, the
+// fragment programs, the texture sampler, blending and the stencil test/op.
+//
+// Every arithmetic expression is written in the reference's operation
+// order and compiled with -ffp-contract=off, so the device result is the
+// same IEEE value the reference computes on the CPU (no FMA contraction,
+// correctly rounded division and sqrt).
+#pragma once
+
+#include "sr_internal.hpp"
+
+namespace sr {
+
+// f32 quotient num/den formed as RN_f32(RN_f64(num * RN_f64(1/den))).
+// This equals the correctly rounded f32 division for every pair whose
+// quotient is a normal f32: num and den have 24-bit significands, so num/den
+// is never exactly a 25-bit midpoint and lies at least 2^-49 (relative)
+// from every one, while the f64 product is within 2^-52 of num/den.
+// Zero, infinite and NaN operands follow IEEE (x*inf, 0*inf = NaN).
+__device__ __forceinline__ float quot(float num, double R) { return (float)((double)num * R); }
+
+__device__ __forceinline__ uint32_t quant8(float x) {
+    float c = fminf(fmaxf(x, 0.0f), 1.0f);
+    return (uint32_t)(c * 255.0f + 0.5f);
+}
+
+__device__ __forceinline__ uint32_t pack4(uint32_t r, uint32_t g, uint32_t b, uint32_t a) {
+    return r | (g << 8) | (b << 16) | (a << 24);
+}
+
+__device__ __forceinline__ float dot3(float ax, float ay, float az, float bx, float by, float bz) {
+    return ((ax * bx) + (ay * by)) + (az * bz);
+}
+
+__device__ __forceinline__ void normalize3(float& x, float& y, float& z) {
+    float len = sqrtf(dot3(x, y, z, x, y, z));
+    x = x / len;
+    y = y / len;
+    z = z / len;
+}
+
+__device__ __forceinline__ float clampf(float x, float lo, float hi) {
+    return fminf(fmaxf(x, lo), hi);
+}
+
+// StencilTest::test(value = framebuffer, mask = ref), stencil.rs:109-124
+__device__ __forceinline__ bool stencil_test(int test, uint32_t value, uint32_t mask) {
+    switch (test) {
+        case SR_STENCIL_ALWAYS: return true;
+        case SR_STENCIL_NEVER: return false;
+        case SR_STENCIL_LESS: return mask < value;
+        case SR_STENCIL_LEQUAL: return mask <= value;
+        case SR_STENCIL_GREATER: return mask > value;
+        case SR_STENCIL_GEQUAL: return mask >= value;
+        case SR_STENCIL_EQUAL: return mask == value;
+        default: return mask != value;
+    }
+}
+
+// StencilOp::op, stencil.rs:144-159 (u8)
+__device__ __forceinline__ uint32_t stencil_op(int op, int wrap, uint32_t v, uint32_t mask) {
+    switch (op) {
+        case SR_STENCIL_KEEP: return v;
+        case SR_STENCIL_INVERT: return (~v) & 0xFFu;
+        case SR_STENCIL_ZERO: return 0u;
+        case SR_STENCIL_REPLACE: return mask & 0xFFu;
+        case SR_STENCIL_INCR: return wrap ? ((v + 1u) & 0xFFu) : (v == 255u ? 255u : v + 1u);
+        default: return wrap ? ((v - 1u) & 0xFFu) : (v == 0u ? 0u : v - 1u);
+    }
+}
+
+// The op applied n times (test Always): the value after n visits.
+__device__ __forceinline__ uint32_t stencil_op_n(int op, int wrap, uint32_t v, uint32_t mask,
+                                                 uint32_t n) {
+    if (n == 0) return v;
+    switch (op) {
+        case SR_STENCIL_KEEP: return v;
+        case SR_STENCIL_INVERT: return (n & 1u) ? ((~v) & 0xFFu) : v;
+        case SR_STENCIL_ZERO: return 0u;
+        case SR_STENCIL_REPLACE: return mask & 0xFFu;
+        case SR_STENCIL_INCR: return wrap ? ((v + n) & 0xFFu) : (n >= 255u - v ? 255u : v + n);
+        default: return wrap ? ((v - n) & 0xFFu) : (n >= v ? 0u : v - n);
+    }
+}
+
+// unorm8 -> f32 through a 256-entry table filled with (float)i / 255.0f.
+struct Lut {
+    const float* t;
+    __device__ __forceinline__ float operator()(uint32_t c) const { return t[c & 0xFFu]; }
+};
+
+__device__ __forceinline__ void texel(const RasterParams& P, int ix, int iy, Lut lut, float o[4]) {
+    const int W = (int)P.texW, H = (int)P.texH;
+    uint32_t c;
+    if (P.tex_edge == SR_EDGE_WRAP) {
+        ix = ((ix % W) + W) % W;
+        iy = ((iy % H) + H) % H;
+        c = P.tex[(size_t)iy * W + ix];
+    } else if (P.tex_edge == SR_EDGE_BORDER) {
+        c = (ix < 0 || iy < 0 || ix >= W || iy >= H) ? P.tex_border : P.tex[(size_t)iy * W + ix];
+    } else {
+        ix = ix < 0 ? 0 : (ix > W - 1 ? W - 1 : ix);
+        iy = iy < 0 ? 0 : (iy > H - 1 ? H - 1 : iy);
+        c = P.tex[(size_t)iy * W + ix];
+    }
+    o[0] = lut(c);
+    o[1] = lut(c >> 8);
+    o[2] = lut(c >> 16);
+    o[3] = lut(c >> 24);
+}
+
+// Texture sampling (texture.rs:14-62 declares Filter/Edge; the sampling
+// itself is build-defined): texel centres at (i + 0.5) / W.
+__device__ __forceinline__ void sample(const RasterParams& P, float u, float v, Lut lut,
+                                       float out[4]) {
+    if (P.tex_filter == SR_FILTER_NEAREST) {
+        float fx = clampf(u * (float)P.texW, -1.0e8f, 1.0e8f);
+        float fy = clampf(v * (float)P.texH, -1.0e8f, 1.0e8f);
+        texel(P, (int)floorf(fx), (int)floorf(fy), lut, out);
+        return;
+    }
+    float fx = clampf(u * (float)P.texW - 0.5f, -1.0e8f, 1.0e8f);
+    float fy = clampf(v * (float)P.texH - 0.5f, -1.0e8f, 1.0e8f);
+    float x0 = floorf(fx), y0 = floorf(fy);
+    float ax = fx - x0, ay = fy - y0;
+    int ix = (int)x0, iy = (int)y0;
+    float t00[4], t10[4], t01[4], t11[4];
+    texel(P, ix, iy, lut, t00);
+    texel(P, ix + 1, iy, lut, t10);
+    texel(P, ix, iy + 1, lut, t01);
+    texel(P, ix + 1, iy + 1, lut, t11);
+    float bx = 1.0f - ax, by = 1.0f - ay;
+#pragma unroll
+    for (int c = 0; c < 4; c++) {
+        float top = (t00[c] * bx) + (t10[c] * ax);
+        float bot = (t01[c] * bx) + (t11[c] * ax);
+        out[c] = (top * by) + (bot * ay);
+    }
+}
+
+// Fragment programs (the closure passed to FragmentShader::run,
+// fragment.rs:168-169).  k = interpolated varyings.
+template <int FP>
+__device__ __forceinline__ uint32_t fragment(const RasterParams& P, const float* k, Lut lut) {
+    const float* U = P.fsu;
+    if constexpr (FP == SR_FP_BLINN_PHONG) {
+        float nx = k[0], ny = k[1], nz = k[2];
+        normalize3(nx, ny, nz);
+        float vx = U[3] - k[3], vy = U[4] - k[4], vz = U[5] - k[5];
+        normalize3(vx, vy, vz);
+        float ndl = dot3(nx, ny, nz, U[0], U[1], U[2]);
+        float diff = ndl > 0.0f ? ndl : 0.0f;
+        float hx = U[0] + vx, hy = U[1] + vy, hz = U[2] + vz;
+        normalize3(hx, hy, hz);
+        float ndh = dot3(nx, ny, nz, hx, hy, hz);
+        float s = ndh > 0.0f ? ndh : 0.0f;
+        s = s * s; s = s * s; s = s * s; s = s * s; s = s * s;
+        float lit = U[9] + U[10] * diff;
+        float spec = U[11] * s;
+        return pack4(quant8(U[6] * lit + spec), quant8(U[7] * lit + spec),
+                     quant8(U[8] * lit + spec), quant8(U[12]));
+    } else if constexpr (FP == SR_FP_TEX_LAMBERT) {
+        float nx = k[0], ny = k[1], nz = k[2];
+        normalize3(nx, ny, nz);
+        float ndl = dot3(nx, ny, nz, U[0], U[1], U[2]);
+        float diff = ndl > 0.0f ? ndl : 0.0f;
+        float lit = U[3] + U[4] * diff;
+        float t[4];
+        sample(P, k[3], k[4], lut, t);
+        return pack4(quant8(t[0] * lit), quant8(t[1] * lit), quant8(t[2] * lit), quant8(t[3] * U[5]));
+    } else if constexpr (FP == SR_FP_UV_COLOR) {
+        return pack4(quant8(k[3]), quant8(k[4]), quant8(U[0]), quant8(U[1]));
+    } else {
+        return pack4(quant8(U[0]), quant8(U[1]), quant8(U[2]), quant8(U[3]));
+    }
+}
+
+template <int FP>
+constexpr int fp_varyings() {
+    return FP == SR_FP_BLINN_PHONG ? 6 : (FP == SR_FP_TEX_LAMBERT || FP == SR_FP_UV_COLOR) ? 5 : 0;
+}
+
+// Blend::blend(src, dst), color/blend.rs:9-31
+template <int BLEND>
+__device__ __forceinline__ uint32_t blend(uint32_t s, uint32_t d, Lut lut) {
+    if constexpr (BLEND == SR_BLEND_OVER) {
+        float a = lut(s >> 24), ia = 1.0f - a;
+        uint32_t r = quant8((lut(s) * a) + (lut(d) * ia));
+        uint32_t g = quant8((lut(s >> 8) * a) + (lut(d >> 8) * ia));
+        uint32_t b = quant8((lut(s >> 16) * a) + (lut(d >> 16) * ia));
+        uint32_t oa = quant8(a + (lut(d >> 24) * ia));
+        return pack4(r, g, b, oa);
+    } else {
+        return s;
+    }
+}
+
+// c.mul_alpha(ColorAlpha::from_scalar(alpha)), line.rs:100
+__device__ __forceinline__ uint32_t sat_u8(float f) {
+    if (!(f > 0.0f)) return 0u;
+    if (f >= 255.0f) return 255u;
+    return (uint32_t)f;
+}
+__device__ __forceinline__ uint32_t mul_alpha(int mode, uint32_t c, double alpha) {
+    uint32_t a = c >> 24, na;
+    if (mode == SR_ALPHA_F32) {
+        na = sat_u8((float)a * (float)alpha);
+    } else {
+        uint32_t au8 = (alpha >= 0.0 && alpha <= 255.0) ? (uint32_t)alpha : 0u;
+        na = sat_u8((float)a * ((float)au8 / 255.0f));
+    }
+    return (c & 0x00FFFFFFu) | (na << 24);
+}
+
+// Work-group index -> tile index so that each XCD (work-groups are dealt to
+// the 8 XCDs round-robin) gets one contiguous run of tiles and keeps its
+// neighbours' vertex data in its own L2.
+__device__ __forceinline__ uint32_t xcd_remap(uint32_t b, uint32_t n) {
+    const uint32_t x = b & 7u, k = b >> 3;
+    const uint32_t q = n >> 3, r = n & 7u;
+    return x * q + (x < r ? x : r) + k;
+}
+
+}  // namespace sr

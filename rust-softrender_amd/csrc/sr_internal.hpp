@@ -1,0 +1,96 @@
+// sr_internal.hpp — types shared by the kernels (sr_kernels.hip) and the
+// host runtime (sr_runtime.hip).  Nothing here crosses the C ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/softrender_hip.h"
+
+namespace sr {
+
+constexpr int MAXK = 8;          // varyings per vertex
+constexpr int SCAN_TILE = 4096;  // elements per scan / radix block (256 threads x 16)
+
+// Screen- (or clip-) space vertices, structure of arrays:
+//   p[0..4)[i]  = x, y, z, w          p[4+k][i] = varying k
+// One allocation; stride = capacity in vertices.
+struct VertexBuf {
+    float* base;
+    uint32_t stride;
+    __host__ __device__ float* comp(int c) const { return base + (size_t)c * stride; }
+};
+
+// Per-triangle setup record, written once by k_tri_setup and read by every
+// raster workgroup whose tile the triangle touches.  80 bytes.
+struct alignas(16) TriRec {
+    double R;             // RN64(1 / (double)det): f32 quotients are formed as (float)(num * R)
+    float x3, y3;         // screen position of the third vertex
+    float e23y, e23x;     // (y2 - y3), (x3 - x2)   — triangle.rs:108
+    float e31y, e31x;     // (y3 - y1), (x1 - x3)   — triangle.rs:109
+    float z1, z2, z3;
+    int32_t X0, X1, Y0, Y1;  // floor of the f32 bbox, clamped to [-1, W] x [-1, H]
+    uint32_t i0, i1, i2;     // vertex ids in the screen vertex buffer
+};
+static_assert(sizeof(TriRec) == 80, "TriRec layout");
+
+// Per (line, screen tile) work item, written by k_line_walk.
+struct alignas(16) LineItem {
+    uint32_t line;        // primitive index
+    uint32_t tile;        // semantic tile index (row-major)
+    uint32_t plot_off;    // first plot record
+    uint32_t plot_cnt;    // number of plot records
+    float cx1, cy1;       // clipped start (liang_barsky output)
+    float dlen;           // hypot of the clipped segment
+    uint32_t pad;
+};
+
+// One plot of a line item: pixel index (already wrapped like x + y*W) and
+// the coverage alpha, f64 as the reference computes it.
+struct alignas(8) Plot {
+    double alpha;
+    uint32_t index;
+    int32_t x, y;  // plot coordinates before the x + y*W wrap (used for t)
+    uint32_t pad;
+};
+
+// Everything a raster workgroup needs, passed by value.
+struct RasterParams {
+    // framebuffer
+    uint32_t W, H;
+    uint32_t* color;
+    float* depth;
+    uint8_t* stencil;
+    const uint32_t* strip_counts;  // per-pixel stencil visits outside bboxes (or null)
+    // screen tiles of the reference (inclusive rectangles, fragment.rs:188-216)
+    uint32_t tw, th;
+    uint32_t scols, srows;
+    // execution tiles
+    uint32_t ets, ntx, nty;
+    const uint32_t* tile_start;
+    const uint32_t* tile_end;
+    const uint32_t* items;  // sorted item ids per execution tile
+    uint32_t exhaustive_n;  // >0: every tile walks items 0..n-1 (no bins)
+    // primitives
+    const TriRec* tris;
+    const LineItem* line_items;
+    const Plot* plots;
+    VertexBuf verts;        // screen-space vertices
+    const uint32_t* line_v; // 2 vertex ids per line
+    const uint32_t* point_v;// vertex id per point
+    uint32_t n_prims;
+    int K;
+    // programs
+    int fp, blend, alpha_mode;
+    int s_test, s_op, s_wrap;
+    uint32_t s_ref;
+    int smode;              // 0: stencil untouched, 1: count visits, 2: track in order
+    const uint32_t* tex;
+    uint32_t texW, texH;
+    int tex_filter, tex_edge;
+    uint32_t tex_border;
+    float fsu[32];
+    uint32_t* error_flags;
+};
+
+}  // namespace sr

@@ -1,0 +1,792 @@
+// sr_kernels.hip — gfx950 kernels of the draw path.
+//
+//   k_vertex        vertex stage + viewport transform  (vertex.rs:87-160,
+//                   clipvertex.rs:89-126)
+//   k_gs_*          geometry-stage clipping             (geometry.rs:261-336)
+//   k_tri_setup     cull, det, bbox, bin counts         (triangle.rs:54-78)
+//   k_scan_*        exclusive scans (wave prefix sums)
+//   k_*_emit        (execution tile, item) pairs in item order
+//   k_radix_*       stable LSD radix sort of pairs by tile (ballot ranking)
+//   k_ranges        per-tile [start, end) in the sorted pairs
+//   k_strips_*      stencil visits on clamped bboxes outside the bbox
+//   k_raster_*      one work-group per execution tile: colour/depth/stencil
+//                   of its pixels stay on-chip, every item is applied in the
+//                   reference's per-pixel order, the tile is written once.
+#include "sr_device.hpp"
+#include "sr_kernels.hpp"
+
+namespace sr {
+
+// ===========================================================================
+// Vertex stage
+// ===========================================================================
+// Screen transform of one position: ClipVertex::normalize,
+// geometry/clipvertex.rs:89-126.  The viewport matrix product keeps its zero
+// terms (0 * inf is NaN in the reference as well).
+__device__ __forceinline__ void normalize_pos(float x, float y, float z, float w,
+                                              const sr_viewport& vp, float out[4]) {
+    const float left = vp.x, bottom = vp.y;
+    const float right = left + vp.width, top = bottom + vp.height;
+    const float nr = vp.near_z, fr = vp.far_z;
+    const float m00 = (right - left) / 2.0f, m03 = (right + left) / 2.0f;
+    const float m11 = (top - bottom) / -2.0f, m13 = (top + bottom) / 2.0f;
+    const float m22 = (fr - nr) / -2.0f, m23 = (fr + nr) / -2.0f;
+    const float xn = x / w, yn = y / w, zn = z / w;
+    float a;
+    a = 0.0f; a = a + m00 * xn; a = a + 0.0f * yn; a = a + 0.0f * zn; a = a + m03 * 1.0f; out[0] = a;
+    a = 0.0f; a = a + 0.0f * xn; a = a + m11 * yn; a = a + 0.0f * zn; a = a + m13 * 1.0f; out[1] = a;
+    a = 0.0f; a = a + 0.0f * xn; a = a + 0.0f * yn; a = a + m22 * zn; a = a + m23 * 1.0f; out[2] = a;
+    out[3] = 1.0f / w;
+}
+
+// One vertex program invocation per thread.  Inputs are coalesced SoA
+// streams; outputs are SoA (x, y, z, w, varyings).
+__global__ __launch_bounds__(256) void k_vertex(VSParams P) {
+    const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+    if (i >= P.n) return;
+    const float* U = P.U;
+    const float pin[4] = {P.px[i], P.py[i], P.pz[i], 1.0f};
+    const float nin[3] = {P.nx ? P.nx[i] : 0.0f, P.ny ? P.ny[i] : 0.0f, P.nz ? P.nz[i] : 0.0f};
+    float clip[4];
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+        float acc = 0.0f;
+#pragma unroll
+        for (int k = 0; k < 4; k++) acc = acc + U[r * 4 + k] * pin[k];
+        clip[r] = acc;
+    }
+    float n[3];
+#pragma unroll
+    for (int r = 0; r < 3; r++) {
+        float acc = 0.0f;
+#pragma unroll
+        for (int k = 0; k < 3; k++) acc = acc + U[16 + r * 3 + k] * nin[k];
+        n[r] = acc;
+    }
+    normalize3(n[0], n[1], n[2]);
+    float var[6];
+    var[0] = n[0]; var[1] = n[1]; var[2] = n[2];
+    if (P.program == SR_VP_MVP_NORMAL_WORLD) {
+#pragma unroll
+        for (int r = 0; r < 3; r++) {
+            float acc = 0.0f;
+#pragma unroll
+            for (int k = 0; k < 4; k++) acc = acc + U[25 + r * 4 + k] * pin[k];
+            var[3 + r] = acc;
+        }
+    } else {
+        var[3] = P.u ? P.u[i] : 0.0f;
+        var[4] = P.v ? P.v[i] : 0.0f;
+        var[5] = 0.0f;
+    }
+    float out[4];
+    if (P.normalize) {
+        normalize_pos(clip[0], clip[1], clip[2], clip[3], P.viewport, out);
+    } else {
+        out[0] = clip[0]; out[1] = clip[1]; out[2] = clip[2]; out[3] = clip[3];
+    }
+    const VertexBuf& o = P.out;
+    o.comp(0)[i] = out[0];
+    o.comp(1)[i] = out[1];
+    o.comp(2)[i] = out[2];
+    o.comp(3)[i] = out[3];
+    for (int k = 0; k < P.K; k++) o.comp(4 + k)[i] = var[k];
+}
+
+// ===========================================================================
+// Geometry stage: clipping programs (clip space), fused with finish()
+// ===========================================================================
+struct CV {
+    float p[4];
+    float k[6];
+};
+
+__device__ __forceinline__ void load_cv(const VertexBuf& v, uint32_t i, int K, CV& o) {
+#pragma unroll
+    for (int c = 0; c < 4; c++) o.p[c] = v.comp(c)[i];
+#pragma unroll
+    for (int c = 0; c < 6; c++) o.k[c] = c < K ? v.comp(4 + c)[i] : 0.0f;
+}
+
+// ClippingPlane::has_inside, geometry/clip.rs:33-44 (Left..Far = 0..5)
+__device__ __forceinline__ bool inside(int pl, const CV& v) {
+    const float x = v.p[0], y = v.p[1], z = v.p[2], w = v.p[3];
+    switch (pl) {
+        case 0: return x >= -w;
+        case 1: return x <= w;
+        case 2: return y >= -w;
+        case 3: return y <= w;
+        case 4: return z >= 0.0f;
+        default: return z <= w;
+    }
+}
+
+// ClippingPlane::intersect, geometry/clip.rs:47-62; lerp per
+// numeric/interpolate.rs:55-58: ((1 - t) * a) + (t * b).
+__device__ __forceinline__ void intersect(int pl, const CV& v1, const CV& v2, CV& o) {
+    const float x1 = v1.p[0], y1 = v1.p[1], z1 = v1.p[2], w1 = v1.p[3];
+    const float x2 = v2.p[0], y2 = v2.p[1], z2 = v2.p[2], w2 = v2.p[3];
+    float a, b;
+    switch (pl) {
+        case 0: a = w1 + x1; b = w2 + x2; break;
+        case 1: a = w1 - x1; b = w2 - x2; break;
+        case 2: a = w1 + y1; b = w2 + y2; break;
+        case 3: a = w1 - y1; b = w2 - y2; break;
+        case 4: a = z1; b = z2; break;
+        default: a = w1 - z1; b = w2 - z2; break;
+    }
+    const float t = a / (a - b);
+    const float omt = 1.0f - t;
+#pragma unroll
+    for (int c = 0; c < 4; c++) o.p[c] = (omt * v1.p[c]) + (t * v2.p[c]);
+#pragma unroll
+    for (int c = 0; c < 6; c++) o.k[c] = (omt * v1.k[c]) + (t * v2.k[c]);
+}
+
+__device__ __forceinline__ uint32_t prim_vertex(const uint32_t* idx, uint32_t i) {
+    return idx ? idx[i] : i;
+}
+
+// Number of output primitives of one input primitive.
+template <int MODE, int PRIM>
+__device__ uint32_t gs_count_one(const GSParams& P, uint32_t t) {
+    if constexpr (PRIM == SR_PRIM_TRIANGLES) {
+        CV v[3];
+        for (int j = 0; j < 3; j++) load_cv(P.in, prim_vertex(P.idx, 3 * t + j), 0, v[j]);
+        int n = 0;
+        for (int e = 0; e < 3; e++) {
+            const CV& s = v[e];
+            const CV& p = v[(e + 1) % 3];
+            if constexpr (MODE == SR_GS_CLIP_NEAR) {
+                const bool si = inside(4, s), pi = inside(4, p);
+                n += (si != pi) + pi;
+            } else {
+                for (int pl = 0; pl < 6; pl++) {
+                    const bool si = inside(pl, s), pi = inside(pl, p);
+                    n += (si != pi) + pi;
+                }
+            }
+        }
+        return n == 3 ? 1u : (n > 3 ? (uint32_t)(n - 2) : 0u);
+    } else if constexpr (PRIM == SR_PRIM_LINES) {
+        CV s, e;
+        load_cv(P.in, prim_vertex(P.idx, 2 * t), 0, s);
+        load_cv(P.in, prim_vertex(P.idx, 2 * t + 1), 0, e);
+        if constexpr (MODE == SR_GS_CLIP_NEAR) {
+            return (inside(4, s) || inside(4, e)) ? 1u : 0u;
+        } else {
+            // geometry.rs:300-328; the intersection only moves endpoints
+            // along the segment, so run it exactly to decide acceptance.
+            int ints = 0;
+            for (int pl = 0; pl < 6; pl++) {
+                const bool si = inside(pl, s), pi = inside(pl, e);
+                if (si != pi) {
+                    CV I;
+                    intersect(pl, s, e, I);
+                    if (si) e = I; else s = I;
+                    ints += 1;
+                } else if (!si) {
+                    return 0u;
+                }
+                if (ints > 2) break;
+            }
+            return 1u;
+        }
+    } else {
+        CV p;
+        load_cv(P.in, prim_vertex(P.idx, t), 0, p);
+        if constexpr (MODE == SR_GS_CLIP_NEAR) return inside(4, p) ? 1u : 0u;
+        for (int pl = 0; pl < 6; pl++)
+            if (!inside(pl, p)) return 0u;
+        return 1u;
+    }
+}
+
+template <int MODE, int PRIM>
+__global__ __launch_bounds__(256) void k_gs_count(GSParams P) {
+    const uint32_t t = blockIdx.x * 256u + threadIdx.x;
+    if (t >= P.n_in) return;
+    P.counts[t] = gs_count_one<MODE, PRIM>(P, t);
+}
+
+__device__ __forceinline__ void store_screen(const GSParams& P, uint32_t slot, const CV& v) {
+    float s[4];
+    normalize_pos(v.p[0], v.p[1], v.p[2], v.p[3], P.viewport, s);
+    for (int c = 0; c < 4; c++) P.out.comp(c)[slot] = s[c];
+    for (int c = 0; c < P.K; c++) P.out.comp(4 + c)[slot] = v.k[c];
+}
+
+template <int MODE, int PRIM>
+__global__ __launch_bounds__(256) void k_gs_emit(GSParams P) {
+    const uint32_t t = blockIdx.x * 256u + threadIdx.x;
+    if (t >= P.n_in) return;
+    uint32_t o = P.offsets[t];
+    if constexpr (PRIM == SR_PRIM_TRIANGLES) {
+        if (P.counts[t] == 0) return;
+        CV v[3];
+        for (int j = 0; j < 3; j++) load_cv(P.in, prim_vertex(P.idx, 3 * t + j), P.K, v[j]);
+        CV poly[36];
+        int n = 0;
+        for (int e = 0; e < 3; e++) {
+            const CV& s = v[e];
+            const CV& p = v[(e + 1) % 3];
+            if constexpr (MODE == SR_GS_CLIP_NEAR) {
+                const bool si = inside(4, s), pi = inside(4, p);
+                if (si != pi) intersect(4, s, p, poly[n++]);
+                if (pi) poly[n++] = p;
+            } else {
+                for (int pl = 0; pl < 6; pl++) {
+                    const bool si = inside(pl, s), pi = inside(pl, p);
+                    if (si != pi) intersect(pl, s, p, poly[n++]);
+                    if (pi) poly[n++] = p;
+                }
+            }
+        }
+        if (n == 3) {
+            for (int j = 0; j < 3; j++) store_screen(P, 3 * o + j, poly[j]);
+        } else if (n > 3) {
+            for (int i = 0; i < n - 2; i++) {  // emit_triangle(last, poly[i], poly[i+1])
+                store_screen(P, 3 * (o + i) + 0, poly[n - 1]);
+                store_screen(P, 3 * (o + i) + 1, poly[i]);
+                store_screen(P, 3 * (o + i) + 2, poly[i + 1]);
+            }
+        }
+    } else if constexpr (PRIM == SR_PRIM_LINES) {
+        if (P.counts[t] == 0) return;
+        CV s, e;
+        load_cv(P.in, prim_vertex(P.idx, 2 * t), P.K, s);
+        load_cv(P.in, prim_vertex(P.idx, 2 * t + 1), P.K, e);
+        if constexpr (MODE == SR_GS_CLIP_NEAR) {
+            const bool si = inside(4, s), pi = inside(4, e);
+            if (si != pi) {
+                CV I;
+                intersect(4, s, e, I);
+                if (si) e = I; else s = I;
+            }
+        } else {
+            int ints = 0;
+            for (int pl = 0; pl < 6; pl++) {
+                const bool si = inside(pl, s), pi = inside(pl, e);
+                if (si != pi) {
+                    CV I;
+                    intersect(pl, s, e, I);
+                    if (si) e = I; else s = I;
+                    ints += 1;
+                }
+                if (ints > 2) break;
+            }
+        }
+        store_screen(P, 2 * o, s);
+        store_screen(P, 2 * o + 1, e);
+    } else {
+        if (P.counts[t] == 0) return;
+        CV p;
+        load_cv(P.in, prim_vertex(P.idx, t), P.K, p);
+        store_screen(P, o, p);
+    }
+}
+
+// ===========================================================================
+// Triangle setup: cull, det, bounding box, execution-tile counts
+// (rasterize_triangle's per-triangle part, triangle.rs:48-78)
+// ===========================================================================
+__global__ __launch_bounds__(256) void k_tri_setup(SetupParams P) {
+    const uint32_t t = blockIdx.x * 256u + threadIdx.x;
+    if (t >= P.n) return;
+    const uint32_t i0 = prim_vertex(P.idx, 3 * t), i1 = prim_vertex(P.idx, 3 * t + 1),
+                   i2 = prim_vertex(P.idx, 3 * t + 2);
+    const float* sx = P.v.comp(0);
+    const float* sy = P.v.comp(1);
+    const float* sz = P.v.comp(2);
+    const float x1 = sx[i0], y1 = sy[i0], x2 = sx[i1], y2 = sy[i1], x3 = sx[i2], y3 = sy[i2];
+    uint32_t count = 0;
+    if (P.cull != SR_CULL_NONE) {
+        // shoelace, triangle.rs:56 (left-to-right evaluation)
+        const float A = x1 * y2 + x2 * y3 + x3 * y1 - x2 * y1 - x3 * y2 - x1 * y3;
+        const int wind = signbit(A) ? SR_CULL_CW : SR_CULL_CCW;
+        if (wind == P.cull) {
+            P.counts[t] = 0;
+            P.recs[t].X0 = 1;  // marks "culled" for the strip pass: X0 > X1
+            P.recs[t].X1 = 0;
+            return;
+        }
+    }
+    const float det = (y2 - y3) * (x1 - x3) + (x3 - x2) * (y1 - y3);  // triangle.rs:64
+    const float mnx = fminf(fminf(x1, x2), x3), mxx = fmaxf(fmaxf(x1, x2), x3);
+    const float mny = fminf(fminf(y1, y2), y3), mxy = fmaxf(fmaxf(y1, y2), y3);
+    TriRec r;
+    r.R = 1.0 / (double)det;
+    r.x3 = x3; r.y3 = y3;
+    r.e23y = y2 - y3; r.e23x = x3 - x2;
+    r.e31y = y3 - y1; r.e31x = x1 - x3;
+    r.z1 = sz[i0]; r.z2 = sz[i1]; r.z3 = sz[i2];
+    r.i0 = i0; r.i1 = i1; r.i2 = i2;
+    if (mnx != mnx || mxx != mxx || mny != mny || mxy != mxy) {
+        // cast(NaN).unwrap() panics in clamp_as_int!, triangle.rs:70
+        atomicOr(&P.err[0], 1u);
+        r.X0 = 1; r.X1 = 0; r.Y0 = 1; r.Y1 = 0;
+        P.recs[t] = r;
+        P.counts[t] = 0;
+        return;
+    }
+    const float W = (float)P.W, H = (float)P.H;
+    r.X0 = (int32_t)floorf(clampf(mnx, -1.0f, W));
+    r.X1 = (int32_t)floorf(clampf(mxx, -1.0f, W));
+    r.Y0 = (int32_t)floorf(clampf(mny, -1.0f, H));
+    r.Y1 = (int32_t)floorf(clampf(mxy, -1.0f, H));
+    P.recs[t] = r;
+    const int32_t px0 = r.X0 > 0 ? r.X0 : 0, px1 = r.X1 < (int32_t)P.W - 1 ? r.X1 : (int32_t)P.W - 1;
+    const int32_t py0 = r.Y0 > 0 ? r.Y0 : 0, py1 = r.Y1 < (int32_t)P.H - 1 ? r.Y1 : (int32_t)P.H - 1;
+    if (px0 <= px1 && py0 <= py1) {
+        const uint32_t ex0 = px0 / P.ets, ex1 = px1 / P.ets, ey0 = py0 / P.ets, ey1 = py1 / P.ets;
+        count = (ex1 - ex0 + 1) * (ey1 - ey0 + 1);
+    }
+    P.counts[t] = count;
+}
+
+__global__ __launch_bounds__(256) void k_tri_emit(EmitParams P) {
+    const uint32_t t = blockIdx.x * 256u + threadIdx.x;
+    if (t >= P.n) return;
+    if (P.counts[t] == 0) return;
+    const TriRec& r = P.recs[t];
+    const int32_t px0 = r.X0 > 0 ? r.X0 : 0, px1 = r.X1 < (int32_t)P.W - 1 ? r.X1 : (int32_t)P.W - 1;
+    const int32_t py0 = r.Y0 > 0 ? r.Y0 : 0, py1 = r.Y1 < (int32_t)P.H - 1 ? r.Y1 : (int32_t)P.H - 1;
+    const uint32_t ex0 = px0 / P.ets, ex1 = px1 / P.ets, ey0 = py0 / P.ets, ey1 = py1 / P.ets;
+    uint32_t o = P.offsets[t];
+    for (uint32_t ey = ey0; ey <= ey1; ey++)
+        for (uint32_t ex = ex0; ex <= ex1; ex++) {
+            P.keys[o] = P.key_base + ey * P.ntx + ex;
+            P.vals[o] = P.val_base + t;
+            o++;
+        }
+}
+
+// ===========================================================================
+// Scans (exclusive, u32).  256 threads, 16 rounds of 256 elements per block.
+// ===========================================================================
+__device__ __forceinline__ uint32_t block_scan_256(uint32_t v, uint32_t& total, uint32_t* lds4) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    uint32_t x = v;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t y = __shfl_up(x, o, 64);
+        if (lane >= o) x += y;
+    }
+    if (lane == 63) lds4[wave] = x;
+    __syncthreads();
+    uint32_t off = 0;
+    total = 0;
+#pragma unroll
+    for (int w = 0; w < 4; w++) {
+        const uint32_t s = lds4[w];
+        if (w < wave) off += s;
+        total += s;
+    }
+    __syncthreads();
+    return x - v + off;
+}
+
+__global__ __launch_bounds__(256) void k_scan_reduce(const uint32_t* in, uint32_t n, uint32_t* sums) {
+    __shared__ uint32_t lds4[4];
+    const uint32_t base = blockIdx.x * SCAN_TILE;
+    uint32_t acc = 0;
+    for (int r = 0; r < 16; r++) {
+        const uint32_t e = base + r * 256u + threadIdx.x;
+        acc += e < n ? in[e] : 0u;
+    }
+    uint32_t tot;
+    block_scan_256(acc, tot, lds4);
+    if (threadIdx.x == 0) sums[blockIdx.x] = tot;
+}
+
+// Single work-group scan of the block sums; also writes the grand total.
+__global__ __launch_bounds__(256) void k_scan_top(uint32_t* sums, uint32_t nb, uint32_t* total_out) {
+    __shared__ uint32_t lds4[4];
+    uint32_t run = 0;
+    for (uint32_t b = 0; b < nb; b += 256) {
+        const uint32_t e = b + threadIdx.x;
+        const uint32_t v = e < nb ? sums[e] : 0u;
+        uint32_t tot;
+        const uint32_t ex = block_scan_256(v, tot, lds4);
+        if (e < nb) sums[e] = run + ex;
+        run += tot;
+    }
+    if (threadIdx.x == 0 && total_out) *total_out = run;
+}
+
+__global__ __launch_bounds__(256) void k_scan_down(const uint32_t* in, uint32_t* out, uint32_t n,
+                                                  const uint32_t* block_off) {
+    __shared__ uint32_t lds4[4];
+    const uint32_t base = blockIdx.x * SCAN_TILE;
+    uint32_t run = block_off[blockIdx.x];
+    for (int r = 0; r < 16; r++) {
+        const uint32_t e = base + r * 256u + threadIdx.x;
+        const uint32_t v = e < n ? in[e] : 0u;
+        uint32_t tot;
+        const uint32_t ex = block_scan_256(v, tot, lds4);
+        if (e < n) out[e] = run + ex;
+        run += tot;
+    }
+}
+
+// ===========================================================================
+// Stable LSD radix sort of (key, value) by key, 8-bit digits.
+// ===========================================================================
+__global__ __launch_bounds__(256) void k_radix_hist(const uint32_t* keys, uint32_t n, int shift,
+                                                   uint32_t* hist, uint32_t nblocks) {
+    __shared__ uint32_t h[256];
+    h[threadIdx.x] = 0;
+    __syncthreads();
+    const uint32_t base = blockIdx.x * SCAN_TILE;
+    for (int r = 0; r < 16; r++) {
+        const uint32_t e = base + r * 256u + threadIdx.x;
+        if (e < n) atomicAdd(&h[(keys[e] >> shift) & 255u], 1u);
+    }
+    __syncthreads();
+    hist[threadIdx.x * nblocks + blockIdx.x] = h[threadIdx.x];
+}
+
+// Elements are taken in 16 rounds of 256 in index order.  Within a round a
+// lane's rank among equal digits comes from 8 ballots (a wave-wide match on
+// the digit), the waves before it from per-wave digit counts in LDS.  The
+// output is therefore stable: equal keys keep their input order.
+__global__ __launch_bounds__(256) void k_radix_scatter(const uint32_t* keys_in,
+                                                      const uint32_t* vals_in, uint32_t n,
+                                                      int shift, const uint32_t* offsets,
+                                                      uint32_t nblocks, uint32_t* keys_out,
+                                                      uint32_t* vals_out) {
+    __shared__ uint32_t run[256];
+    __shared__ uint32_t wcnt[4][256];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    run[tid] = offsets[tid * nblocks + blockIdx.x];
+    const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    const uint32_t base = blockIdx.x * SCAN_TILE;
+    for (int r = 0; r < 16; r++) {
+        const uint32_t e = base + r * 256u + tid;
+        const bool valid = e < n;
+        const uint32_t k = valid ? keys_in[e] : 0u;
+        const uint32_t v = valid ? vals_in[e] : 0u;
+        const uint32_t d = (k >> shift) & 255u;
+        uint64_t mask = __ballot(valid);
+#pragma unroll
+        for (int b = 0; b < 8; b++) {
+            const bool bit = (d >> b) & 1u;
+            const uint64_t bal = __ballot(bit);
+            mask &= bit ? bal : ~bal;
+        }
+        const uint32_t rank = __popcll(mask & lt);
+        const uint32_t cnt = __popcll(mask);
+#pragma unroll
+        for (int w = 0; w < 4; w++) wcnt[w][tid] = 0;
+        __syncthreads();
+        if (valid && rank == 0) wcnt[wave][d] = cnt;
+        __syncthreads();
+        if (valid) {
+            uint32_t pre = 0;
+            for (int w = 0; w < wave; w++) pre += wcnt[w][d];
+            const uint32_t pos = run[d] + pre + rank;
+            keys_out[pos] = k;
+            vals_out[pos] = v;
+        }
+        __syncthreads();
+        run[tid] += wcnt[0][tid] + wcnt[1][tid] + wcnt[2][tid] + wcnt[3][tid];
+        __syncthreads();
+    }
+}
+
+__global__ __launch_bounds__(256) void k_fill_u32(uint32_t* p, uint32_t v, uint32_t n) {
+    const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+__global__ __launch_bounds__(256) void k_ranges(const uint32_t* keys, uint32_t n, uint32_t key_base,
+                                               uint32_t* start, uint32_t* end) {
+    const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t k = keys[i];
+    if (i == 0 || keys[i - 1] != k) start[k - key_base] = i;
+    if (i == n - 1 || keys[i + 1] != k) end[k - key_base] = i + 1;
+}
+
+// ===========================================================================
+// Framebuffer fill (Framebuffer::clear, renderbuffer/mod.rs:125-133)
+// ===========================================================================
+__global__ __launch_bounds__(256) void k_fb_clear(uint32_t* color, float* depth, uint8_t* stencil,
+                                                 uint32_t n, uint32_t rgba, float d, uint32_t s) {
+    const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+    if (i >= n) return;
+    color[i] = rgba;
+    depth[i] = d;
+    stencil[i] = (uint8_t)s;
+}
+
+// ===========================================================================
+// Stencil visits outside the bounding box (test Always, op != Keep).
+//
+// rasterize_triangle runs the stencil op on every pixel of the bbox clamped
+// into each screen tile, before the coverage test (triangle.rs:66-99).  A
+// tile the bbox misses still gets a one-pixel-wide strip (or a corner) on its
+// edge.  With test Always the stencil never gates shading and the op
+// commutes with itself, so the final value is op^n(s0) with n the number of
+// visits.  The raster counts visits inside bboxes; this pass counts the rest
+// for every non-culled triangle: per tile column the bbox is Left / Over /
+// Right of it, per tile row Above / Over / Below, and the strip cases are
+// separable, so they reduce to difference arrays over tile columns x pixel
+// rows (L, R), tile rows x pixel columns (A, B) and the tile grid (corners).
+// ===========================================================================
+__device__ __forceinline__ void strip_status(int lo, int hi, uint32_t t, uint32_t n, uint32_t len,
+                                             uint32_t& nbefore, uint32_t& first_after) {
+    // tiles entirely before the box (their max < lo) and the first tile
+    // entirely after it (its min > hi)
+    if (lo >= (int)len) nbefore = n;
+    else if (lo <= 0) nbefore = 0;
+    else nbefore = min((uint32_t)(lo - 1) / t, n - 1);
+    first_after = hi < 0 ? 0u : min((uint32_t)hi / t + 1u, n);
+}
+
+__global__ __launch_bounds__(256) void k_strips_accum(StripParams P) {
+    const uint32_t t = blockIdx.x * 256u + threadIdx.x;
+    if (t >= P.n) return;
+    const TriRec& r = P.recs[t];
+    const int X0 = r.X0, X1 = r.X1, Y0 = r.Y0, Y1 = r.Y1;
+    if (X0 > X1) return;  // culled
+    uint32_t nR, jL, nB, iA;  // columns right-of-box-status prefix etc.
+    strip_status(X0, X1, P.tw, P.scols, P.W, nR, jL);
+    strip_status(Y0, Y1, P.th, P.srows, P.H, nB, iA);
+    if (jL < nR) jL = nR;
+    if (iA < nB) iA = nB;
+    const uint32_t HS = P.H + 1, WS = P.W + 1, CS = P.scols + 1;
+    const uint32_t xmax = P.W - 1, ymax = P.H - 1;
+    // rows over the box: (L,O) and (R,O) strips on tile columns
+    for (uint32_t i = nB; i < iA; i++) {
+        const int ty0 = (int)(i * P.th), ty1 = (int)min(i * P.th + P.th, ymax);
+        const int ylo = max(Y0, ty0), yhi = min(Y1, ty1);
+        if (ylo > yhi) continue;
+        if (jL < P.scols) {  // columns j >= jL see the box on their left
+            atomicAdd(&P.colL[jL * HS + ylo], 1u);
+            atomicAdd(&P.colL[jL * HS + yhi + 1], 0xFFFFFFFFu);
+        }
+        if (nR > 0) {  // columns j < nR see it on their right
+            atomicAdd(&P.colR[0 * HS + ylo], 1u);
+            atomicAdd(&P.colR[0 * HS + yhi + 1], 0xFFFFFFFFu);
+            atomicAdd(&P.colR[nR * HS + ylo], 0xFFFFFFFFu);
+            atomicAdd(&P.colR[nR * HS + yhi + 1], 1u);
+        }
+    }
+    // columns over the box: (O,A) and (O,B) strips on tile rows
+    for (uint32_t j = nR; j < jL; j++) {
+        const int tx0 = (int)(j * P.tw), tx1 = (int)min(j * P.tw + P.tw, xmax);
+        const int xlo = max(X0, tx0), xhi = min(X1, tx1);
+        if (xlo > xhi) continue;
+        if (iA < P.srows) {
+            atomicAdd(&P.rowT[iA * WS + xlo], 1u);
+            atomicAdd(&P.rowT[iA * WS + xhi + 1], 0xFFFFFFFFu);
+        }
+        if (nB > 0) {
+            atomicAdd(&P.rowB[0 * WS + xlo], 1u);
+            atomicAdd(&P.rowB[0 * WS + xhi + 1], 0xFFFFFFFFu);
+            atomicAdd(&P.rowB[nB * WS + xlo], 0xFFFFFFFFu);
+            atomicAdd(&P.rowB[nB * WS + xhi + 1], 1u);
+        }
+    }
+    // corners: 2D difference arrays over the tile grid [row][col]
+    if (jL < P.scols && iA < P.srows) atomicAdd(&P.cLL[iA * CS + jL], 1u);      // left & above
+    if (jL < P.scols && nB > 0) {                                               // left & below
+        atomicAdd(&P.cLR[0 * CS + jL], 1u);
+        atomicAdd(&P.cLR[nB * CS + jL], 0xFFFFFFFFu);
+    }
+    if (nR > 0 && iA < P.srows) {                                               // right & above
+        atomicAdd(&P.cRL[iA * CS + 0], 1u);
+        atomicAdd(&P.cRL[iA * CS + nR], 0xFFFFFFFFu);
+    }
+    if (nR > 0 && nB > 0) {                                                     // right & below
+        atomicAdd(&P.cRR[0], 1u);
+        atomicAdd(&P.cRR[nB * CS], 0xFFFFFFFFu);
+        atomicAdd(&P.cRR[nR], 0xFFFFFFFFu);
+        atomicAdd(&P.cRR[nB * CS + nR], 1u);
+    }
+}
+
+// In-place inclusive prefix over `len` elements of each of `rows` rows
+// (stride `stride`, element step `step`), one work-group per row.
+__global__ __launch_bounds__(256) void k_prefix_rows(uint32_t* a, uint32_t rows, uint32_t len,
+                                                    uint32_t stride, uint32_t step) {
+    __shared__ uint32_t lds4[4];
+    const uint32_t row = blockIdx.x;
+    if (row >= rows) return;
+    uint32_t* base = a + (size_t)row * stride;
+    uint32_t run = 0;
+    for (uint32_t c = 0; c < len; c += 256) {
+        const uint32_t e = c + threadIdx.x;
+        const uint32_t v = e < len ? base[(size_t)e * step] : 0u;
+        uint32_t tot;
+        const uint32_t ex = block_scan_256(v, tot, lds4);
+        if (e < len) base[(size_t)e * step] = run + ex + v;
+        run += tot;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_strips_final(StripParams P) {
+    const uint32_t p = blockIdx.x * 256u + threadIdx.x;
+    if (p >= P.W * P.H) return;
+    const uint32_t x = p % P.W, y = p / P.W;
+    const uint32_t HS = P.H + 1, WS = P.W + 1, CS = P.scols + 1;
+    const uint32_t xmax = P.W - 1, ymax = P.H - 1;
+    // tile columns whose left (L) / right (R) edge is x; rows likewise
+    int jl = -1, jr0 = -1, jr1 = -1, it = -1, ib0 = -1, ib1 = -1;
+    if (x % P.tw == 0 && x / P.tw < P.scols) jl = (int)(x / P.tw);
+    if (x % P.tw == 0 && x > 0) {
+        const uint32_t j = x / P.tw - 1;
+        if (j < P.scols && min((j + 1) * P.tw, xmax) == x) jr0 = (int)j;
+    }
+    if (min(P.scols * P.tw, xmax) == x && (int)P.scols - 1 != jr0) jr1 = (int)P.scols - 1;
+    if (y % P.th == 0 && y / P.th < P.srows) it = (int)(y / P.th);
+    if (y % P.th == 0 && y > 0) {
+        const uint32_t i = y / P.th - 1;
+        if (i < P.srows && min((i + 1) * P.th, ymax) == y) ib0 = (int)i;
+    }
+    if (min(P.srows * P.th, ymax) == y && (int)P.srows - 1 != ib0) ib1 = (int)P.srows - 1;
+    uint32_t n = 0;
+    if (jl >= 0) n += P.colL[jl * HS + y];
+    if (jr0 >= 0) n += P.colR[jr0 * HS + y];
+    if (jr1 >= 0) n += P.colR[jr1 * HS + y];
+    if (it >= 0) n += P.rowT[it * WS + x];
+    if (ib0 >= 0) n += P.rowB[ib0 * WS + x];
+    if (ib1 >= 0) n += P.rowB[ib1 * WS + x];
+    const int jrs[2] = {jr0, jr1}, ibs[2] = {ib0, ib1};
+    if (jl >= 0 && it >= 0) n += P.cLL[it * CS + jl];
+    for (int b = 0; b < 2; b++)
+        if (jl >= 0 && ibs[b] >= 0) n += P.cLR[ibs[b] * CS + jl];
+    for (int a = 0; a < 2; a++) {
+        if (jrs[a] >= 0 && it >= 0) n += P.cRL[it * CS + jrs[a]];
+        for (int b = 0; b < 2; b++)
+            if (jrs[a] >= 0 && ibs[b] >= 0) n += P.cRR[ibs[b] * CS + jrs[a]];
+    }
+    P.out[p] = n;
+}
+
+// ===========================================================================
+// Host-side launchers
+// ===========================================================================
+static inline uint32_t blocks_for(uint64_t n, uint32_t per) { return (uint32_t)((n + per - 1) / per); }
+
+void launch_vertex(const VSParams& P, hipStream_t s) {
+    if (P.n) hipLaunchKernelGGL(k_vertex, dim3(blocks_for(P.n, 256)), dim3(256), 0, s, P);
+}
+
+void launch_gs_count(int mode, int prim, const GSParams& P, hipStream_t s) {
+    if (!P.n_in) return;
+    const dim3 g(blocks_for(P.n_in, 256)), b(256);
+#define GSC(M, PR) if (mode == M && prim == PR) { hipLaunchKernelGGL((k_gs_count<M, PR>), g, b, 0, s, P); return; }
+    GSC(SR_GS_CLIP_NEAR, SR_PRIM_TRIANGLES) GSC(SR_GS_CLIP_NEAR, SR_PRIM_LINES) GSC(SR_GS_CLIP_NEAR, SR_PRIM_POINTS)
+    GSC(SR_GS_CLIP_REFERENCE, SR_PRIM_TRIANGLES) GSC(SR_GS_CLIP_REFERENCE, SR_PRIM_LINES)
+    GSC(SR_GS_CLIP_REFERENCE, SR_PRIM_POINTS)
+#undef GSC
+}
+
+void launch_gs_emit(int mode, int prim, const GSParams& P, hipStream_t s) {
+    if (!P.n_in) return;
+    const dim3 g(blocks_for(P.n_in, 256)), b(256);
+#define GSE(M, PR) if (mode == M && prim == PR) { hipLaunchKernelGGL((k_gs_emit<M, PR>), g, b, 0, s, P); return; }
+    GSE(SR_GS_CLIP_NEAR, SR_PRIM_TRIANGLES) GSE(SR_GS_CLIP_NEAR, SR_PRIM_LINES) GSE(SR_GS_CLIP_NEAR, SR_PRIM_POINTS)
+    GSE(SR_GS_CLIP_REFERENCE, SR_PRIM_TRIANGLES) GSE(SR_GS_CLIP_REFERENCE, SR_PRIM_LINES)
+    GSE(SR_GS_CLIP_REFERENCE, SR_PRIM_POINTS)
+#undef GSE
+}
+
+void launch_tri_setup(const SetupParams& P, hipStream_t s) {
+    if (P.n) hipLaunchKernelGGL(k_tri_setup, dim3(blocks_for(P.n, 256)), dim3(256), 0, s, P);
+}
+void launch_tri_emit(const EmitParams& P, hipStream_t s) {
+    if (P.n) hipLaunchKernelGGL(k_tri_emit, dim3(blocks_for(P.n, 256)), dim3(256), 0, s, P);
+}
+
+uint32_t scan_tmp_words(uint32_t n) { return blocks_for(n, SCAN_TILE) + 1; }
+
+void launch_scan(const uint32_t* in, uint32_t* out, uint32_t n, uint32_t* tmp, uint32_t* total,
+                 hipStream_t s) {
+    const uint32_t nb = blocks_for(n, SCAN_TILE);
+    if (n == 0) {
+        hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(256), 0, s, total, 0u, 1u);
+        return;
+    }
+    hipLaunchKernelGGL(k_scan_reduce, dim3(nb), dim3(256), 0, s, in, n, tmp);
+    hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(256), 0, s, tmp, nb, total);
+    hipLaunchKernelGGL(k_scan_down, dim3(nb), dim3(256), 0, s, in, out, n, (const uint32_t*)tmp);
+}
+
+uint32_t radix_hist_words(uint32_t n) { return 256u * blocks_for(n, SCAN_TILE); }
+
+int radix_sort(uint32_t* keys, uint32_t* vals, uint32_t* keys_tmp, uint32_t* vals_tmp, uint32_t n,
+               int key_bits, uint32_t* hist, uint32_t* scan_tmp, uint32_t* total, hipStream_t s) {
+    if (n <= 1) return 0;
+    const uint32_t nb = blocks_for(n, SCAN_TILE);
+    const int passes = (key_bits + 7) / 8;
+    uint32_t *ki = keys, *vi = vals, *ko = keys_tmp, *vo = vals_tmp;
+    for (int p = 0; p < passes; p++) {
+        const int shift = 8 * p;
+        hipLaunchKernelGGL(k_radix_hist, dim3(nb), dim3(256), 0, s, ki, n, shift, hist, nb);
+        launch_scan(hist, hist, 256u * nb, scan_tmp, total, s);
+        hipLaunchKernelGGL(k_radix_scatter, dim3(nb), dim3(256), 0, s, ki, vi, n, shift,
+                           (const uint32_t*)hist, nb, ko, vo);
+        uint32_t* t;
+        t = ki; ki = ko; ko = t;
+        t = vi; vi = vo; vo = t;
+    }
+    return passes & 1;  // 1: result is in the tmp buffers
+}
+
+void launch_fill(uint32_t* p, uint32_t v, uint32_t n, hipStream_t s) {
+    if (n) hipLaunchKernelGGL(k_fill_u32, dim3(blocks_for(n, 256)), dim3(256), 0, s, p, v, n);
+}
+
+void launch_ranges(const uint32_t* keys, uint32_t n, uint32_t key_base, uint32_t* start,
+                   uint32_t* end, hipStream_t s) {
+    if (n) hipLaunchKernelGGL(k_ranges, dim3(blocks_for(n, 256)), dim3(256), 0, s, keys, n, key_base,
+                              start, end);
+}
+
+void launch_fb_clear(uint32_t* color, float* depth, uint8_t* stencil, uint32_t n, uint32_t rgba,
+                     float d, uint32_t st, hipStream_t s) {
+    if (n) hipLaunchKernelGGL(k_fb_clear, dim3(blocks_for(n, 256)), dim3(256), 0, s, color, depth,
+                              stencil, n, rgba, d, st);
+}
+
+void launch_raster_tris(const RasterParams& P, uint32_t ntiles, hipStream_t s) {
+    switch (P.ets) {
+        case 8: launch_raster_tris_8(P, ntiles, s); break;
+        case 32: launch_raster_tris_32(P, ntiles, s); break;
+        case 64: launch_raster_tris_64(P, ntiles, s); break;
+        default: launch_raster_tris_16(P, ntiles, s); break;
+    }
+}
+
+void launch_strips(const StripParams& P, hipStream_t s) {
+    const uint32_t HS = P.H + 1, WS = P.W + 1, CS = P.scols + 1, RS = P.srows + 1;
+    if (P.n) hipLaunchKernelGGL(k_strips_accum, dim3(blocks_for(P.n, 256)), dim3(256), 0, s, P);
+    // colL/colR: prefix along pixel rows (per tile column), then across tile columns
+    hipLaunchKernelGGL(k_prefix_rows, dim3(CS), dim3(256), 0, s, P.colL, CS, HS, HS, 1u);
+    hipLaunchKernelGGL(k_prefix_rows, dim3(HS), dim3(256), 0, s, P.colL, HS, CS, 1u, HS);
+    hipLaunchKernelGGL(k_prefix_rows, dim3(CS), dim3(256), 0, s, P.colR, CS, HS, HS, 1u);
+    hipLaunchKernelGGL(k_prefix_rows, dim3(HS), dim3(256), 0, s, P.colR, HS, CS, 1u, HS);
+    hipLaunchKernelGGL(k_prefix_rows, dim3(RS), dim3(256), 0, s, P.rowT, RS, WS, WS, 1u);
+    hipLaunchKernelGGL(k_prefix_rows, dim3(WS), dim3(256), 0, s, P.rowT, WS, RS, 1u, WS);
+    hipLaunchKernelGGL(k_prefix_rows, dim3(RS), dim3(256), 0, s, P.rowB, RS, WS, WS, 1u);
+    hipLaunchKernelGGL(k_prefix_rows, dim3(WS), dim3(256), 0, s, P.rowB, WS, RS, 1u, WS);
+    uint32_t* corners[4] = {P.cLL, P.cLR, P.cRL, P.cRR};
+    for (uint32_t* c : corners) {
+        hipLaunchKernelGGL(k_prefix_rows, dim3(RS), dim3(256), 0, s, c, RS, CS, CS, 1u);
+        hipLaunchKernelGGL(k_prefix_rows, dim3(CS), dim3(256), 0, s, c, CS, RS, 1u, CS);
+    }
+    hipLaunchKernelGGL(k_strips_final, dim3(blocks_for((uint64_t)P.W * P.H, 256)), dim3(256), 0, s, P);
+}
+
+// Lines and points: the raster kernels live further down.
+void launch_line_tiles(const LineSetupParams&, int, hipStream_t) {}
+void launch_line_plots(const PlotParams&, int, hipStream_t) {}
+void launch_line_emit(const PlotParams&, const uint32_t*, uint32_t*, uint32_t*, hipStream_t) {}
+void launch_point_setup(const PointSetupParams&, hipStream_t) {}
+void launch_raster_lines(const RasterParams&, uint32_t, hipStream_t) {}
+void launch_raster_points(const RasterParams&, uint32_t, hipStream_t) {}
+
+}  // namespace sr

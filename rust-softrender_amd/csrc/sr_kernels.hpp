@@ -1,0 +1,137 @@
+// sr_kernels.hpp — kernel parameter blocks and host-side launchers.
+#pragma once
+
+#include "sr_internal.hpp"
+
+namespace sr {
+
+struct VSParams {
+    const float *px, *py, *pz, *nx, *ny, *nz, *u, *v;
+    uint32_t n;
+    int program;
+    int K;
+    int normalize;  // 1: run_to_fragment (viewport fused), 0: keep clip space
+    sr_viewport viewport;
+    float U[48];
+    VertexBuf out;
+};
+
+struct GSParams {
+    VertexBuf in;        // clip-space vertices
+    const uint32_t* idx; // mesh indices (null: implicit)
+    uint32_t n_in;       // input primitives
+    int K;
+    uint32_t* counts;    // outputs per input primitive
+    const uint32_t* offsets;
+    VertexBuf out;       // generated vertices, screen space (finish fused)
+    sr_viewport viewport;
+};
+
+struct SetupParams {
+    VertexBuf v;
+    const uint32_t* idx;  // null: vertex 3t+k
+    uint32_t n;
+    int cull;
+    uint32_t W, H, ets;
+    TriRec* recs;
+    uint32_t* counts;
+    uint32_t* err;
+};
+
+struct EmitParams {
+    const TriRec* recs;
+    const uint32_t* counts;
+    const uint32_t* offsets;
+    uint32_t n, W, H, ets, ntx;
+    uint32_t key_base, val_base;
+    uint32_t* keys;
+    uint32_t* vals;
+};
+
+struct LineSetupParams {
+    VertexBuf v;
+    const uint32_t* idx;
+    uint32_t n_lines;
+    uint32_t W, H, tw, th, scols, srows;
+    int aa;
+    uint32_t* tile_counts;  // per line: number of accepted screen tiles (pass 1)
+    const uint32_t* tile_offsets;
+    LineItem* items;        // pass 2
+    uint32_t* plot_counts;  // per item
+    uint32_t* err;
+};
+
+struct PlotParams {
+    VertexBuf v;
+    LineItem* items;
+    uint32_t n_items;
+    const uint32_t* plot_offsets;
+    Plot* plots;
+    uint32_t W, H, tw, th, scols;
+    int aa;
+    uint32_t ets, ntx;
+    uint32_t* bin_counts;  // per item: execution tiles touched
+};
+
+struct PointSetupParams {
+    VertexBuf v;
+    const uint32_t* idx;
+    uint32_t n;
+    uint32_t W, H, tw, th, scols, srows, ets, ntx;
+    uint32_t* counts;  // 0/1
+    uint32_t* keys;    // written at offsets
+    const uint32_t* offsets;
+    uint32_t* vals;
+    int pass;
+    uint32_t* err;
+};
+
+struct StripParams {
+    const TriRec* recs;
+    uint32_t n;
+    uint32_t W, H, tw, th, scols, srows;
+    uint32_t* colL;  // [scols+1][H+1]  rows of column tx0_j, difference along y
+    uint32_t* colR;  // [scols+1][H+1]
+    uint32_t* rowT;  // [srows+1][W+1]
+    uint32_t* rowB;  // [srows+1][W+1]
+    uint32_t* cLL;   // corners [srows+1][scols+1]
+    uint32_t* cLR;
+    uint32_t* cRL;
+    uint32_t* cRR;
+    uint32_t* out;   // per pixel
+};
+
+void launch_vertex(const VSParams& P, hipStream_t s);
+void launch_gs_count(int mode, int prim, const GSParams& P, hipStream_t s);
+void launch_gs_emit(int mode, int prim, const GSParams& P, hipStream_t s);
+void launch_tri_setup(const SetupParams& P, hipStream_t s);
+void launch_tri_emit(const EmitParams& P, hipStream_t s);
+uint32_t scan_tmp_words(uint32_t n);
+void launch_scan(const uint32_t* in, uint32_t* out, uint32_t n, uint32_t* tmp, uint32_t* total,
+                 hipStream_t s);
+uint32_t radix_hist_words(uint32_t n);
+// returns 1 if the sorted data ended up in the *_tmp buffers
+int radix_sort(uint32_t* keys, uint32_t* vals, uint32_t* keys_tmp, uint32_t* vals_tmp, uint32_t n,
+               int key_bits, uint32_t* hist, uint32_t* scan_tmp, uint32_t* total, hipStream_t s);
+void launch_fill(uint32_t* p, uint32_t v, uint32_t n, hipStream_t s);
+void launch_ranges(const uint32_t* keys, uint32_t n, uint32_t key_base, uint32_t* start,
+                   uint32_t* end, hipStream_t s);
+void launch_fb_clear(uint32_t* color, float* depth, uint8_t* stencil, uint32_t n, uint32_t rgba,
+                     float d, uint32_t st, hipStream_t s);
+void launch_raster_tris(const RasterParams& P, uint32_t ntiles, hipStream_t s);
+void launch_raster_tris_8(const RasterParams& P, uint32_t ntiles, hipStream_t s);
+void launch_raster_tris_16(const RasterParams& P, uint32_t ntiles, hipStream_t s);
+void launch_raster_tris_32(const RasterParams& P, uint32_t ntiles, hipStream_t s);
+void launch_raster_tris_64(const RasterParams& P, uint32_t ntiles, hipStream_t s);
+
+// lines / points / stencil strips
+void launch_line_tiles(const LineSetupParams& P, int pass, hipStream_t s);
+void launch_line_plots(const PlotParams& P, int pass, hipStream_t s);
+void launch_line_emit(const PlotParams& P, const uint32_t* offsets, uint32_t* keys, uint32_t* vals,
+                      hipStream_t s);
+void launch_point_setup(const PointSetupParams& P, hipStream_t s);
+void launch_raster_lines(const RasterParams& P, uint32_t ntiles, hipStream_t s);
+void launch_raster_points(const RasterParams& P, uint32_t ntiles, hipStream_t s);
+void launch_strips(const StripParams& P, hipStream_t s);
+
+}  // namespace sr

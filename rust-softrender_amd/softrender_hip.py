@@ -1,0 +1,278 @@
+"""ctypes bindings for libsoftrender_hip.so (the C ABI in include/softrender_hip.h).
+
+Plumbing for tests, smoke and bench: every call goes through the C ABI into
+the HIP kernels.  There is no CPU fallback — if the shared library is
+missing this module raises at import.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libsoftrender_hip.so")
+
+# status codes (include/softrender_hip.h)
+SR_OK = 0
+ERRORS = {
+    -1: "SR_ERR_INVALID_ARGUMENT", -2: "SR_ERR_OUT_OF_MEMORY", -3: "SR_ERR_DEVICE",
+    -4: "SR_ERR_INVALID_COORDINATE", -5: "SR_ERR_UNSUPPORTED", -6: "SR_ERR_BUDGET",
+    -7: "SR_ERR_NOT_READY", -8: "SR_ERR_NO_DEVICE",
+}
+
+EXPORTED = [
+    "sr_ctx_create", "sr_ctx_destroy", "sr_last_error", "sr_finish", "sr_ctx_stream",
+    "sr_ctx_stage_times", "sr_ctx_last_stats", "sr_mesh_upload", "sr_mesh_free",
+    "sr_texture_upload", "sr_texture_free", "sr_fb_create", "sr_fb_free", "sr_fb_clear",
+    "sr_fb_read", "sr_fb_write", "sr_fb_size", "sr_draw", "sr_batch_draw",
+]
+
+
+class Viewport(C.Structure):
+    _fields_ = [("x", C.c_float), ("y", C.c_float), ("width", C.c_float),
+                ("height", C.c_float), ("near_z", C.c_float), ("far_z", C.c_float)]
+
+
+class VertexSoA(C.Structure):
+    _fields_ = [(n, C.POINTER(C.c_float)) for n in ("px", "py", "pz", "nx", "ny", "nz", "u", "v")]
+
+
+class DrawDesc(C.Structure):
+    _fields_ = [
+        ("vertex_program", C.c_int32), ("geometry_program", C.c_int32),
+        ("fragment_program", C.c_int32), ("blend", C.c_int32), ("cull", C.c_int32),
+        ("aa_lines", C.c_int32), ("tile_w", C.c_int32), ("tile_h", C.c_int32),
+        ("stencil_test", C.c_int32), ("stencil_op", C.c_int32), ("stencil_wrap", C.c_int32),
+        ("stencil_ref", C.c_uint32), ("alpha_mode", C.c_int32), ("tex_filter", C.c_int32),
+        ("tex_edge", C.c_int32), ("tex_border", C.c_uint32), ("viewport", Viewport),
+        ("vs_uniforms", C.c_float * 48), ("fs_uniforms", C.c_float * 32),
+        ("exec_tile", C.c_int32), ("exec_flags", C.c_uint32), ("reserved", C.c_int32 * 6),
+    ]
+
+
+class DrawStats(C.Structure):
+    _fields_ = [(n, C.c_uint64) for n in (
+        "triangles_in", "triangles_setup", "lines_in", "points_in", "bin_pairs",
+        "generated_tris", "generated_lines", "generated_points")]
+
+
+class BatchItem(C.Structure):
+    _fields_ = [("fb", C.c_void_p), ("mesh", C.c_void_p), ("desc", C.POINTER(DrawDesc)),
+                ("tex", C.c_void_p)]
+
+
+def make_desc(**kw) -> DrawDesc:
+    """DrawDesc with the reference defaults: tile 128x128 (fragment.rs:29),
+    no cull, `()` blend, stencil Always/Keep (stencil.rs:169-175)."""
+    d = DrawDesc()
+    d.tile_w = kw.pop("tile_w", 128)
+    d.tile_h = kw.pop("tile_h", 128)
+    vp = kw.pop("viewport", (0.0, 0.0, 1.0, 1.0, 0.0, 1.0))
+    d.viewport = Viewport(*[float(v) for v in vp])
+    vsu = kw.pop("vs_uniforms", None)
+    if vsu is not None:
+        d.vs_uniforms[:] = [float(v) for v in np.asarray(vsu, np.float32).reshape(48)]
+    fsu = kw.pop("fs_uniforms", None)
+    if fsu is not None:
+        d.fs_uniforms[:] = [float(v) for v in np.asarray(fsu, np.float32).reshape(32)]
+    for k, v in kw.items():
+        if not hasattr(d, k):
+            raise KeyError(k)
+        setattr(d, k, v)
+    return d
+
+
+def f32p(a):
+    if a is None:
+        return None
+    return a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+def vertex_soa(s: dict):
+    keep = {k: (None if s.get(k) is None else np.ascontiguousarray(s[k], np.float32))
+            for k in ("px", "py", "pz", "nx", "ny", "nz", "u", "v")}
+    soa = VertexSoA(*[f32p(keep[k]) for k in ("px", "py", "pz", "nx", "ny", "nz", "u", "v")])
+    return soa, keep
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is not built (run __graft_entry__.build())")
+        L = C.CDLL(LIB_PATH)
+        vp = C.c_void_p
+        L.sr_ctx_create.argtypes = [C.c_int, C.POINTER(vp)]
+        L.sr_ctx_destroy.argtypes = [vp]
+        L.sr_last_error.argtypes = [vp]
+        L.sr_last_error.restype = C.c_char_p
+        L.sr_finish.argtypes = [vp]
+        L.sr_ctx_stream.argtypes = [vp]
+        L.sr_ctx_stream.restype = vp
+        L.sr_ctx_stage_times.argtypes = [vp, C.POINTER(C.c_float), C.c_int, C.POINTER(C.c_int)]
+        L.sr_ctx_last_stats.argtypes = [vp, C.POINTER(DrawStats)]
+        L.sr_mesh_upload.argtypes = [vp, C.POINTER(VertexSoA), C.c_uint32, C.POINTER(C.c_uint32),
+                                     C.c_uint32, C.c_int32, C.POINTER(vp)]
+        L.sr_mesh_free.argtypes = [vp]
+        L.sr_texture_upload.argtypes = [vp, C.c_uint32, C.c_uint32, C.POINTER(C.c_uint32),
+                                        C.POINTER(vp)]
+        L.sr_texture_free.argtypes = [vp]
+        L.sr_fb_create.argtypes = [vp, C.c_uint32, C.c_uint32, C.POINTER(vp)]
+        L.sr_fb_free.argtypes = [vp]
+        L.sr_fb_clear.argtypes = [vp, C.c_uint32, C.c_float, C.c_uint8]
+        L.sr_fb_read.argtypes = [vp, C.POINTER(C.c_uint32), C.POINTER(C.c_float),
+                                 C.POINTER(C.c_uint8)]
+        L.sr_fb_write.argtypes = [vp, C.POINTER(C.c_uint32), C.POINTER(C.c_float),
+                                  C.POINTER(C.c_uint8)]
+        L.sr_fb_size.argtypes = [vp, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]
+        L.sr_draw.argtypes = [vp, vp, vp, C.POINTER(DrawDesc), vp]
+        L.sr_batch_draw.argtypes = [vp, C.POINTER(BatchItem), C.c_uint32]
+        for name in EXPORTED:
+            f = getattr(L, name)
+            if f.restype is None and name not in ("sr_last_error", "sr_ctx_stream"):
+                f.restype = C.c_int
+        _lib = L
+    return _lib
+
+
+class SoftrenderError(RuntimeError):
+    def __init__(self, code, msg=""):
+        super().__init__(f"{ERRORS.get(code, code)}: {msg}")
+        self.code = code
+
+
+class Context:
+    """Owns a HIP stream and device workspace (the Pipeline's role)."""
+
+    def __init__(self, device: int = 0):
+        self.h = C.c_void_p()
+        self._check(lib().sr_ctx_create(device, C.byref(self.h)))
+
+    def _check(self, rc):
+        if rc != SR_OK:
+            msg = lib().sr_last_error(self.h) if self.h else b""
+            raise SoftrenderError(rc, (msg or b"").decode())
+        return rc
+
+    def close(self):
+        if self.h:
+            lib().sr_ctx_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def finish(self):
+        self._check(lib().sr_finish(self.h))
+
+    def stream(self) -> int:
+        return lib().sr_ctx_stream(self.h) or 0
+
+    def stage_times(self):
+        buf = (C.c_float * 16)()
+        n = C.c_int()
+        self._check(lib().sr_ctx_stage_times(self.h, buf, 16, C.byref(n)))
+        names = ["vertex", "geometry", "setup", "scan", "emit", "sort", "ranges", "strips",
+                 "raster"]
+        return {names[i]: buf[i] for i in range(min(n.value, len(names)))}
+
+    def last_stats(self) -> dict:
+        st = DrawStats()
+        self._check(lib().sr_ctx_last_stats(self.h, C.byref(st)))
+        return {k: getattr(st, k) for k, _ in DrawStats._fields_}
+
+    def mesh(self, s: dict) -> "Mesh":
+        return Mesh(self, s)
+
+    def texture(self, w, h, rgba) -> "Texture":
+        return Texture(self, w, h, rgba)
+
+    def framebuffer(self, w, h) -> "Framebuffer":
+        return Framebuffer(self, w, h)
+
+    def draw(self, fb: "Framebuffer", mesh: "Mesh", desc: DrawDesc, tex: "Texture" = None):
+        self._check(lib().sr_draw(self.h, fb.h, mesh.h, C.byref(desc), tex.h if tex else None))
+
+    def batch_draw(self, items):
+        """items: list of (fb, mesh, desc, tex)."""
+        arr = (BatchItem * len(items))()
+        keep = []
+        for i, (fb, mesh, desc, tex) in enumerate(items):
+            keep.append(desc)
+            arr[i] = BatchItem(fb.h, mesh.h, C.pointer(desc), tex.h if tex else None)
+        self._check(lib().sr_batch_draw(self.h, arr, len(items)))
+
+
+class Mesh:
+    def __init__(self, ctx: Context, s: dict):
+        self.ctx = ctx
+        self.h = C.c_void_p()
+        soa, keep = vertex_soa(s)
+        n = len(keep["px"])
+        idx = s.get("indices")
+        if idx is not None:
+            idx = np.ascontiguousarray(idx, np.uint32)
+            ip, ni = idx.ctypes.data_as(C.POINTER(C.c_uint32)), len(idx)
+        else:
+            ip, ni = None, 0
+        ctx._check(lib().sr_mesh_upload(ctx.h, C.byref(soa), n, ip, ni, s["primitive"],
+                                        C.byref(self.h)))
+
+    def free(self):
+        if self.h:
+            lib().sr_mesh_free(self.h)
+            self.h = C.c_void_p()
+
+
+class Texture:
+    def __init__(self, ctx: Context, w: int, h: int, rgba):
+        self.h = C.c_void_p()
+        a = np.ascontiguousarray(rgba, np.uint32)
+        ctx._check(lib().sr_texture_upload(ctx.h, w, h, a.ctypes.data_as(C.POINTER(C.c_uint32)),
+                                           C.byref(self.h)))
+
+    def free(self):
+        if self.h:
+            lib().sr_texture_free(self.h)
+            self.h = C.c_void_p()
+
+
+class Framebuffer:
+    def __init__(self, ctx: Context, w: int, h: int):
+        self.ctx, self.w, self.h_ = ctx, w, h
+        self.h = C.c_void_p()
+        ctx._check(lib().sr_fb_create(ctx.h, w, h, C.byref(self.h)))
+
+    def clear(self, rgba: int, depth: float = float(np.finfo(np.float32).min), stencil: int = 0):
+        self.ctx._check(lib().sr_fb_clear(self.h, rgba, depth, stencil))
+
+    def read(self):
+        n = self.w * self.h_
+        col = np.empty(n, np.uint32)
+        dep = np.empty(n, np.float32)
+        st = np.empty(n, np.uint8)
+        self.ctx._check(lib().sr_fb_read(self.h, col.ctypes.data_as(C.POINTER(C.c_uint32)),
+                                         dep.ctypes.data_as(C.POINTER(C.c_float)),
+                                         st.ctypes.data_as(C.POINTER(C.c_uint8))))
+        return col, dep, st
+
+    def write(self, col, dep, st):
+        col = np.ascontiguousarray(col, np.uint32)
+        dep = np.ascontiguousarray(dep, np.float32)
+        st = np.ascontiguousarray(st, np.uint8)
+        self.ctx._check(lib().sr_fb_write(self.h, col.ctypes.data_as(C.POINTER(C.c_uint32)),
+                                          dep.ctypes.data_as(C.POINTER(C.c_float)),
+                                          st.ctypes.data_as(C.POINTER(C.c_uint8))))
+
+    def free(self):
+        if self.h:
+            lib().sr_fb_free(self.h)
+            self.h = C.c_void_p()
