@@ -34,25 +34,32 @@ struct alignas(16) TriRec {
 };
 static_assert(sizeof(TriRec) == 80, "TriRec layout");
 
-// Per (line, screen tile) work item, written by k_line_walk.
+// Per (line, screen tile) work item: the segment clipped to one screen
+// tile by liang_barsky_iterative (geometry/line.rs:6-54), as rasterize_line
+// receives it (line.rs:51-52).  64 bytes.
 struct alignas(16) LineItem {
     uint32_t line;        // primitive index
-    uint32_t tile;        // semantic tile index (row-major)
+    uint32_t tile;        // screen tile index (row-major, the reference's tile order)
     uint32_t plot_off;    // first plot record
     uint32_t plot_cnt;    // number of plot records
-    float cx1, cy1;       // clipped start (liang_barsky output)
-    float dlen;           // hypot of the clipped segment
+    float cx1, cy1;       // clipped start
+    float cx2, cy2;       // clipped end
+    float dlen;           // hypot of the clipped segment, line.rs:52
+    int32_t bx0, by0, bx1, by1;  // execution tiles touched by its plots (bx0 > bx1: none)
     uint32_t pad;
 };
+static_assert(sizeof(LineItem) == 64, "LineItem layout");
 
-// One plot of a line item: pixel index (already wrapped like x + y*W) and
-// the coverage alpha, f64 as the reference computes it.
+// One plot of a line item: framebuffer index (x + y*W, line.rs:56-58), the
+// plot coordinates (for the pixel centre in line.rs:72-73) and the
+// coverage alpha in f64 as draw_line_xiaolin_wu computes it.
 struct alignas(8) Plot {
     double alpha;
     uint32_t index;
-    int32_t x, y;  // plot coordinates before the x + y*W wrap (used for t)
+    int32_t x, y;
     uint32_t pad;
 };
+static_assert(sizeof(Plot) == 24, "Plot layout");
 
 // Everything a raster workgroup needs, passed by value.
 struct RasterParams {

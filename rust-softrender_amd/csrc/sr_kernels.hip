@@ -666,6 +666,275 @@ __global__ __launch_bounds__(256) void k_strips_final(StripParams P) {
 }
 
 // ===========================================================================
+// Lines (pipeline/stages/rasterization/line.rs:22-238)
+// ===========================================================================
+// liang_barsky_iterative, geometry/line.rs:6-54
+__device__ __forceinline__ bool liang_barsky(float x1, float y1, float x2, float y2, float xmin,
+                                             float ymin, float xmax, float ymax, float out[4]) {
+    float t0 = 0.0f, t1 = 1.0f;
+    const float dx = x2 - x1, dy = y2 - y1;
+#pragma unroll
+    for (int edge = 0; edge < 4; edge++) {
+        float p, q;
+        switch (edge) {
+            case 0: p = -dx; q = x1 - xmin; break;
+            case 1: p = dx; q = xmax - x1; break;
+            case 2: p = -dy; q = y1 - ymin; break;
+            default: p = dy; q = ymax - y1; break;
+        }
+        if (p == 0.0f && q < 0.0f) return false;
+        const float r = q / p;
+        if (p < 0.0f) {
+            if (r > t1) return false;
+            else if (r > t0) t0 = r;
+        } else if (p > 0.0f) {
+            if (r < t0) return false;
+            else if (r < t1) t1 = r;
+        }
+    }
+    out[0] = x1 + t0 * dx;
+    out[1] = y1 + t0 * dy;
+    out[2] = x1 + t1 * dx;
+    out[3] = y1 + t1 * dy;
+    return true;
+}
+
+// Rust f32::hypot is the platform libm hypotf; glibc evaluates it as
+// sqrt(x*x + y*y) in double, rounded once to float (infinities first).
+__device__ __forceinline__ float hypot_libm(float a, float b) {
+    if (__builtin_isinf(a) || __builtin_isinf(b)) return __builtin_inff();
+    const double x = (double)a, y = (double)b;
+    return (float)sqrt(x * x + y * y);
+}
+
+// Per line: the screen tiles whose closed rectangle the segment reaches
+// (liang_barsky accepts), in the reference's row-major tile order.
+__global__ __launch_bounds__(256) void k_line_tiles(LineSetupParams P, int pass) {
+    const uint32_t l = blockIdx.x * 256u + threadIdx.x;
+    if (l >= P.n_lines) return;
+    const uint32_t i0 = prim_vertex(P.idx, 2 * l), i1 = prim_vertex(P.idx, 2 * l + 1);
+    const float x1 = P.v.comp(0)[i0], y1 = P.v.comp(1)[i0];
+    const float x2 = P.v.comp(0)[i1], y2 = P.v.comp(1)[i1];
+    if (!(__builtin_isfinite(x1) && __builtin_isfinite(y1) && __builtin_isfinite(x2) &&
+          __builtin_isfinite(y2))) {
+        // cast(x).unwrap() on a non-finite coordinate panics, line.rs:112-116
+        if (pass == 0) {
+            P.counts[l] = 0;
+            atomicOr(&P.err[0], 2u);
+        }
+        return;
+    }
+    // candidate tiles: the segment's box grown by a pixel (a superset of what
+    // liang_barsky can accept; each candidate is then tested exactly)
+    const float lo_x = fminf(x1, x2), hi_x = fmaxf(x1, x2);
+    const float lo_y = fminf(y1, y2), hi_y = fmaxf(y1, y2);
+    const float gx = 1.0f + 1e-6f * fmaxf(fabsf(lo_x), fabsf(hi_x));
+    const float gy = 1.0f + 1e-6f * fmaxf(fabsf(lo_y), fabsf(hi_y));
+    const float W = (float)P.W, H = (float)P.H;
+    const int px0 = (int)floorf(clampf(lo_x - gx, 0.0f, W - 1.0f));
+    const int px1 = (int)floorf(clampf(hi_x + gx, -1.0f, W - 1.0f));
+    const int py0 = (int)floorf(clampf(lo_y - gy, 0.0f, H - 1.0f));
+    const int py1 = (int)floorf(clampf(hi_y + gy, -1.0f, H - 1.0f));
+    uint32_t n = 0;
+    uint32_t o = pass ? P.offsets[l] : 0u;
+    if (px0 <= px1 && py0 <= py1) {
+        const uint32_t jlo0 = (uint32_t)px0 / P.tw, jhi = min((uint32_t)px1 / P.tw, P.scols - 1);
+        const uint32_t ilo0 = (uint32_t)py0 / P.th, ihi = min((uint32_t)py1 / P.th, P.srows - 1);
+        const uint32_t jlo = jlo0 > 0 ? jlo0 - 1 : 0, ilo = ilo0 > 0 ? ilo0 - 1 : 0;
+        const uint32_t xmax = P.W - 1, ymax = P.H - 1;
+        for (uint32_t i = ilo; i <= ihi; i++) {
+            const float ty0 = (float)(i * P.th), ty1 = (float)min(i * P.th + P.th, ymax);
+            for (uint32_t j = jlo; j <= jhi; j++) {
+                const float tx0 = (float)(j * P.tw), tx1 = (float)min(j * P.tw + P.tw, xmax);
+                float c[4];
+                if (!liang_barsky(x1, y1, x2, y2, tx0, ty0, tx1, ty1, c)) continue;
+                if (pass) {
+                    LineItem it;
+                    it.line = l;
+                    it.tile = i * P.scols + j;
+                    it.plot_off = 0;
+                    it.plot_cnt = 0;
+                    it.cx1 = c[0]; it.cy1 = c[1]; it.cx2 = c[2]; it.cy2 = c[3];
+                    it.dlen = hypot_libm(c[0] - c[2], c[1] - c[3]);  // line.rs:52
+                    it.bx0 = 1; it.bx1 = 0; it.by0 = 1; it.by1 = 0;
+                    it.pad = 0;
+                    P.items[o + n] = it;
+                }
+                n++;
+            }
+        }
+    }
+    if (pass == 0) P.counts[l] = n;
+}
+
+// f64 -> i64 `as` cast (saturating, NaN -> 0)
+__device__ __forceinline__ int64_t f2i64(double v) {
+    if (v != v) return 0;
+    if (v >= 9.2233720368547758e18) return INT64_MAX;
+    if (v <= -9.2233720368547758e18) return INT64_MIN;
+    return (int64_t)v;
+}
+__device__ __forceinline__ double fract64(double v) { return v - trunc(v); }  // f64::fract
+
+// draw_line_bresenham, line.rs:125-151
+template <class F>
+__device__ void bresenham_walk(int64_t x0, int64_t y0, int64_t x1, int64_t y1, F&& plot) {
+    const int64_t dx = x1 - x0 < 0 ? -(x1 - x0) : x1 - x0;
+    const int64_t dy = -(y1 - y0 < 0 ? -(y1 - y0) : y1 - y0);
+    const int64_t sx = x0 < x1 ? 1 : -1, sy = y0 < y1 ? 1 : -1;
+    int64_t err = dx + dy;
+    for (;;) {
+        plot(x0, y0, 1.0);
+        if (x0 == x1 && y0 == y1) break;
+        const int64_t e2 = 2 * err;
+        if (e2 >= dy) { err += dy; x0 += sx; }
+        if (e2 <= dx) { err += dx; y0 += sy; }
+    }
+}
+
+// draw_line_xiaolin_wu, line.rs:159-238 (f64, exactly as written)
+template <class F>
+__device__ void wu_walk(double x0, double y0, double x1, double y1, F&& plot) {
+    const bool steep = fabs(y1 - y0) > fabs(x1 - x0);
+    double t;
+    if (steep) { t = x0; x0 = y0; y0 = t; t = x1; x1 = y1; y1 = t; }
+    if (x0 > x1) { t = x0; x0 = x1; x1 = t; t = y0; y0 = y1; y1 = t; }
+    const double dx = x1 - x0, dy = y1 - y0;
+    const double gradient = dx < 0.0001 ? 1.0 : dy / dx;
+    auto pf = [&](double X, double Y, double o) { plot(f2i64(X), f2i64(Y), o); };
+    double xend = round(x0);
+    double yend = y0 + gradient * (xend - x0);
+    double xgap = 1.0 - fract64(x0 + 0.5);
+    const double xpxl1 = xend, ypxl1 = trunc(yend);
+    if (steep) {
+        pf(ypxl1, xpxl1, (1.0 - fract64(yend)) * xgap);
+        pf(ypxl1 + 1.0, xpxl1, fract64(yend) * xgap);
+    } else {
+        pf(xpxl1, ypxl1, (1.0 - fract64(yend)) * xgap);
+        pf(xpxl1, ypxl1 + 1.0, fract64(yend) * xgap);
+    }
+    double intery = yend + gradient;
+    xend = round(x1);
+    yend = y1 + gradient * (xend - x1);
+    xgap = fract64(x1 + 0.5);
+    const double xpxl2 = xend, ypxl2 = trunc(yend);
+    if (steep) {
+        pf(ypxl2, xpxl2, (1.0 - fract64(yend)) * xgap);
+        pf(ypxl2 + 1.0, xpxl2, fract64(yend) * xgap);
+    } else {
+        pf(xpxl2, ypxl2, (1.0 - fract64(yend)) * xgap);
+        pf(xpxl2, ypxl2 + 1.0, fract64(yend) * xgap);
+    }
+    double x = xpxl1 + 1.0;
+    if (steep) {
+        while (x <= (xpxl2 - 1.0)) {
+            const double y = trunc(intery);
+            pf(y, x, 1.0 - fract64(intery));
+            pf(y + 1.0, x, fract64(intery));
+            intery += gradient;
+            x += 1.0;
+        }
+    } else {
+        while (x <= (xpxl2 - 1.0)) {
+            const double y = trunc(intery);
+            pf(x, y, 1.0 - fract64(intery));
+            pf(x, y + 1.0, fract64(intery));
+            intery += gradient;
+            x += 1.0;
+        }
+    }
+}
+
+// Walks one line item's plots: pass 0 counts them and the execution tiles
+// they touch, pass 1 writes them.  A plot is kept when x >= 0 and y >= 0
+// (line.rs:55) and x + y*W lies inside the framebuffer (past its end the
+// reference writes out of bounds; such plots are dropped).
+__global__ __launch_bounds__(256) void k_line_walk(WalkParams P, int pass) {
+    const uint32_t it_id = blockIdx.x * 256u + threadIdx.x;
+    if (it_id >= P.n_items) return;
+    LineItem& it = P.items[it_id];
+    const uint64_t area = (uint64_t)P.W * P.H;
+    uint32_t n = 0;
+    uint32_t o = pass ? P.plot_offsets[it_id] : 0u;
+    int bx0 = INT32_MAX, by0 = INT32_MAX, bx1 = -1, by1 = -1;
+    auto plot = [&](int64_t x, int64_t y, double alpha) {
+        if (!(x >= 0 && y >= 0)) return;
+        const uint64_t index = (uint64_t)(uint32_t)x + (uint64_t)(uint32_t)y * P.W;
+        if (x > 0xFFFFFFFFll || y > 0xFFFFFFFFll || index >= area) return;
+        if (pass) {
+            Plot pl;
+            pl.alpha = alpha;
+            pl.index = (uint32_t)index;
+            pl.x = (int32_t)x;
+            pl.y = (int32_t)y;
+            pl.pad = 0;
+            P.plots[o + n] = pl;
+        } else {
+            const int ex = (int)((uint32_t)(index % P.W) / P.ets), ey = (int)((uint32_t)(index / P.W) / P.ets);
+            bx0 = min(bx0, ex); bx1 = max(bx1, ex);
+            by0 = min(by0, ey); by1 = max(by1, ey);
+        }
+        n++;
+    };
+    if (P.aa) {
+        wu_walk((double)it.cx1, (double)it.cy1, (double)it.cx2, (double)it.cy2, plot);
+    } else {
+        // cast::<f32, i64>(..).unwrap(): truncation (the clipped points are finite)
+        bresenham_walk((int64_t)it.cx1, (int64_t)it.cy1, (int64_t)it.cx2, (int64_t)it.cy2, plot);
+    }
+    if (pass == 0) {
+        P.plot_counts[it_id] = n;
+        it.plot_cnt = n;
+        if (n) {
+            it.bx0 = bx0; it.bx1 = bx1; it.by0 = by0; it.by1 = by1;
+            P.bin_counts[it_id] = (uint32_t)((bx1 - bx0 + 1) * (by1 - by0 + 1));
+        } else {
+            it.bx0 = 1; it.bx1 = 0; it.by0 = 1; it.by1 = 0;
+            P.bin_counts[it_id] = 0;
+        }
+    } else {
+        it.plot_off = o;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_item_emit(ItemEmitParams P) {
+    const uint32_t t = blockIdx.x * 256u + threadIdx.x;
+    if (t >= P.n || P.counts[t] == 0) return;
+    const LineItem& it = P.items[t];
+    uint32_t o = P.offsets[t];
+    for (int ey = it.by0; ey <= it.by1; ey++)
+        for (int ex = it.bx0; ex <= it.bx1; ex++) {
+            P.keys[o] = (uint32_t)ey * P.ntx + (uint32_t)ex;
+            P.vals[o] = t;
+            o++;
+        }
+}
+
+// ===========================================================================
+// Points (pipeline/stages/rasterization/point.rs:21-86)
+// ===========================================================================
+// A point is rasterised in the screen tile whose half-open box
+// [tx0, tx1) x [ty0, ty1) holds it (point.rs:46).  The boxes of consecutive
+// tiles share only their closed ends, so a point lands in at most one tile;
+// x >= W-1 or y >= H-1 never lands (the last tile ends at W-1 / H-1).
+__global__ __launch_bounds__(256) void k_point_setup(PointSetupParams P) {
+    const uint32_t t = blockIdx.x * 256u + threadIdx.x;
+    if (t >= P.n) return;
+    const uint32_t i = prim_vertex(P.idx, t);
+    const float x = P.v.comp(0)[i], y = P.v.comp(1)[i];
+    const bool in = x >= 0.0f && x < (float)(P.W - 1) && y >= 0.0f && y < (float)(P.H - 1);
+    if (P.pass == 0) {
+        P.counts[t] = in ? 1u : 0u;
+        return;
+    }
+    if (!in) return;
+    const uint32_t px = (uint32_t)x, py = (uint32_t)y;  // point.rs:47
+    const uint32_t o = P.offsets[t];
+    P.keys[o] = (py / P.ets) * P.ntx + px / P.ets;
+    P.vals[o] = t;
+}
+
+// ===========================================================================
 // Host-side launchers
 // ===========================================================================
 static inline uint32_t blocks_for(uint64_t n, uint32_t per) { return (uint32_t)((n + per - 1) / per); }
@@ -781,12 +1050,17 @@ void launch_strips(const StripParams& P, hipStream_t s) {
     hipLaunchKernelGGL(k_strips_final, dim3(blocks_for((uint64_t)P.W * P.H, 256)), dim3(256), 0, s, P);
 }
 
-// Lines and points: the raster kernels live further down.
-void launch_line_tiles(const LineSetupParams&, int, hipStream_t) {}
-void launch_line_plots(const PlotParams&, int, hipStream_t) {}
-void launch_line_emit(const PlotParams&, const uint32_t*, uint32_t*, uint32_t*, hipStream_t) {}
-void launch_point_setup(const PointSetupParams&, hipStream_t) {}
-void launch_raster_lines(const RasterParams&, uint32_t, hipStream_t) {}
-void launch_raster_points(const RasterParams&, uint32_t, hipStream_t) {}
+void launch_line_tiles(const LineSetupParams& P, int pass, hipStream_t s) {
+    if (P.n_lines) hipLaunchKernelGGL(k_line_tiles, dim3(blocks_for(P.n_lines, 256)), dim3(256), 0, s, P, pass);
+}
+void launch_line_walk(const WalkParams& P, int pass, hipStream_t s) {
+    if (P.n_items) hipLaunchKernelGGL(k_line_walk, dim3(blocks_for(P.n_items, 256)), dim3(256), 0, s, P, pass);
+}
+void launch_item_emit(const ItemEmitParams& P, hipStream_t s) {
+    if (P.n) hipLaunchKernelGGL(k_item_emit, dim3(blocks_for(P.n, 256)), dim3(256), 0, s, P);
+}
+void launch_point_setup(const PointSetupParams& P, hipStream_t s) {
+    if (P.n) hipLaunchKernelGGL(k_point_setup, dim3(blocks_for(P.n, 256)), dim3(256), 0, s, P);
+}
 
 }  // namespace sr

@@ -50,40 +50,47 @@ struct EmitParams {
 
 struct LineSetupParams {
     VertexBuf v;
-    const uint32_t* idx;
+    const uint32_t* idx;   // null: vertices 2l, 2l+1
     uint32_t n_lines;
     uint32_t W, H, tw, th, scols, srows;
-    int aa;
-    uint32_t* tile_counts;  // per line: number of accepted screen tiles (pass 1)
-    const uint32_t* tile_offsets;
-    LineItem* items;        // pass 2
-    uint32_t* plot_counts;  // per item
+    uint32_t* counts;      // per line: accepted screen tiles
+    const uint32_t* offsets;
+    LineItem* items;
     uint32_t* err;
 };
 
-struct PlotParams {
-    VertexBuf v;
+struct WalkParams {
     LineItem* items;
     uint32_t n_items;
-    const uint32_t* plot_offsets;
-    Plot* plots;
-    uint32_t W, H, tw, th, scols;
+    uint32_t W, H;
     int aa;
-    uint32_t ets, ntx;
-    uint32_t* bin_counts;  // per item: execution tiles touched
+    uint32_t ets, ntx, nty;
+    uint32_t* plot_counts;   // pass 0: plots per item
+    uint32_t* bin_counts;    // pass 0: execution tiles per item
+    const uint32_t* plot_offsets;  // pass 1
+    Plot* plots;             // pass 1
+    uint32_t* err;
+};
+
+struct ItemEmitParams {
+    const LineItem* items;
+    const uint32_t* counts;
+    const uint32_t* offsets;
+    uint32_t n, ntx;
+    uint32_t* keys;
+    uint32_t* vals;
 };
 
 struct PointSetupParams {
     VertexBuf v;
     const uint32_t* idx;
     uint32_t n;
-    uint32_t W, H, tw, th, scols, srows, ets, ntx;
-    uint32_t* counts;  // 0/1
-    uint32_t* keys;    // written at offsets
+    uint32_t W, H, tw, th, scols, ets, ntx;
+    uint32_t* counts;   // 0/1
     const uint32_t* offsets;
+    uint32_t* keys;
     uint32_t* vals;
     int pass;
-    uint32_t* err;
 };
 
 struct StripParams {
@@ -126,9 +133,8 @@ void launch_raster_tris_64(const RasterParams& P, uint32_t ntiles, hipStream_t s
 
 // lines / points / stencil strips
 void launch_line_tiles(const LineSetupParams& P, int pass, hipStream_t s);
-void launch_line_plots(const PlotParams& P, int pass, hipStream_t s);
-void launch_line_emit(const PlotParams& P, const uint32_t* offsets, uint32_t* keys, uint32_t* vals,
-                      hipStream_t s);
+void launch_line_walk(const WalkParams& P, int pass, hipStream_t s);
+void launch_item_emit(const ItemEmitParams& P, hipStream_t s);
 void launch_point_setup(const PointSetupParams& P, hipStream_t s);
 void launch_raster_lines(const RasterParams& P, uint32_t ntiles, hipStream_t s);
 void launch_raster_points(const RasterParams& P, uint32_t ntiles, hipStream_t s);

@@ -30,21 +30,107 @@ __device__ __forceinline__ void sem_range(uint32_t lo_px, uint32_t hi_px, uint32
 template <int ETS>
 constexpr int raster_threads() { return ETS == 8 ? 64 : 256; }
 
+// Loads through the constant address space: read-only for the kernel's
+// lifetime, so a wave-uniform address becomes an s_load into SGPRs.
+__device__ __forceinline__ uint32_t load_item(const uint32_t* base, uint32_t i) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return ((const __attribute__((address_space(4))) uint32_t*)base)[i];
+#else
+    return base[i];
+#endif
+}
+
+__device__ __forceinline__ TriRec load_rec(const TriRec* base, uint32_t i) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    const __attribute__((address_space(4))) TriRec* p =
+        (const __attribute__((address_space(4))) TriRec*)base + i;
+    TriRec r;
+    r.R = p->R;
+    r.x3 = p->x3; r.y3 = p->y3;
+    r.e23y = p->e23y; r.e23x = p->e23x; r.e31y = p->e31y; r.e31x = p->e31x;
+    r.z1 = p->z1; r.z2 = p->z2; r.z3 = p->z3;
+    r.X0 = p->X0; r.X1 = p->X1; r.Y0 = p->Y0; r.Y1 = p->Y1;
+    r.i0 = p->i0; r.i1 = p->i1; r.i2 = p->i2;
+    return r;
+#else
+    return base[i];
+#endif
+}
+
+// Fragments that pass stencil + depth wait in a per-wave LDS queue, in
+// order, until some pixel's queue is full; then the wave shades the whole
+// queue densely (64 fragments per pass, all lanes busy) and blends each
+// pixel's fragments in order.  This reorders nothing the reference can
+// observe: the depth test only reads depths written by earlier fragments,
+// and every fragment program here returns a colour (never Discard), so the
+// depth write (triangle.rs:142) does not depend on shading; blending, the
+// only colour-order dependence, runs in queue (= reference) order.
+template <int ETS>
+constexpr int queue_depth() { return ETS <= 16 ? 8 : (ETS == 32 ? 2 : 1); }
+
+struct QRec {
+    uint32_t tri;  // triangle id; after shading, the fragment colour
+    float u, v;    // barycentrics (w = (1 - u) - v is recomputed exactly)
+};
+
+__device__ __forceinline__ void wave_sync_lds() {
+    // LDS written by one lane and read by another within the same wave
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__device__ __forceinline__ uint32_t wave_excl_scan(uint32_t v, uint32_t& total) {
+    const int lane = threadIdx.x & 63;
+    uint32_t x = v;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t y = __shfl_up(x, o, 64);
+        if (lane >= o) x += y;
+    }
+    total = __shfl(x, 63, 64);
+    return x - v;
+}
+
+// Shade one queued fragment: interpolate the varyings (interpolate.rs:43-64
+// order) and run the fragment program.
+template <int FP>
+__device__ __forceinline__ uint32_t shade_rec(const RasterParams& P, const QRec& q, Lut lut) {
+    constexpr int KV = fp_varyings<FP>();
+    float k[KV > 0 ? KV : 1];
+    if constexpr (KV > 0) {
+        const TriRec& r = P.tris[q.tri];
+        const uint32_t i0 = r.i0, i1 = r.i1, i2 = r.i2;
+        const float u = q.u, v = q.v, w = (1.0f - u) - v;
+#pragma unroll
+        for (int c = 0; c < KV; c++) {
+            const float* vc = P.verts.comp(4 + c);
+            k[c] = ((vc[i0] * u) + (vc[i1] * v)) + (vc[i2] * w);
+        }
+    }
+    return fragment<FP>(P, k, lut);
+}
+
 template <int ETS, int FP, int BLEND, int SMODE>
 __global__ __launch_bounds__(256) void k_raster_tris(RasterParams P) {
     constexpr int NT = raster_threads<ETS>();
-    constexpr int PPT = (ETS * ETS) / NT;
-    constexpr int KV = fp_varyings<FP>();
-    constexpr int CH = NT;  // triangle records staged per chunk
+    constexpr int NW = NT / 64;
+    constexpr int S = ETS == 8 ? 8 : ETS / 2;  // each wave owns an S x S quadrant
+    constexpr int PPT = (S * S) / 64;          // pixels per lane
+    constexpr int RPP = 64 / S;                // quadrant rows covered by one pixel pass
+    constexpr int WPIX = 64 * PPT;             // pixels per wave
+    constexpr int Q = BLEND == SR_BLEND_OVER ? queue_depth<ETS>() : 1;
 
-    __shared__ TriRec srec[CH];
     __shared__ float slut[256];
+    __shared__ QRec qbuf[NW][WPIX * Q];
+    __shared__ uint16_t flat[NW][WPIX * Q];
 
-    const int tid = threadIdx.x;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const uint32_t tile = xcd_remap(blockIdx.x, gridDim.x);
     const uint32_t etx = tile % P.ntx, ety = tile / P.ntx;
+    const bool exh = P.exhaustive_n != 0;
     uint32_t beg, end;
-    if (P.exhaustive_n) {
+    if (exh) {
         beg = 0;
         end = P.exhaustive_n;
     } else {
@@ -54,94 +140,104 @@ __global__ __launch_bounds__(256) void k_raster_tris(RasterParams P) {
     if (beg >= end && SMODE != 1) return;
 
     for (int i = tid; i < 256; i += NT) slut[i] = (float)i / 255.0f;
+    __syncthreads();
     const Lut lut{slut};
 
-    // tile pixel span (clipped to the framebuffer)
+    // tile and wave pixel spans (clipped to the framebuffer)
     const uint32_t tx0 = etx * ETS, ty0 = ety * ETS;
-    const uint32_t tx1 = min(tx0 + ETS, P.W) - 1, ty1 = min(ty0 + ETS, P.H) - 1;
+    const int tx1 = (int)min(tx0 + ETS, P.W) - 1, ty1 = (int)min(ty0 + ETS, P.H) - 1;
+    const int qx = ETS == 8 ? 0 : (wave & 1) * S, qy = ETS == 8 ? 0 : (wave >> 1) * S;
+    const int wx0 = (int)tx0 + qx, wy0 = (int)ty0 + qy;
+    const int wx1 = min(wx0 + S - 1, tx1), wy1 = min(wy0 + S - 1, ty1);
 
-    // this thread's pixels
     int px[PPT], py[PPT];
     bool live[PPT];
-    uint32_t col[PPT], cnt[PPT], st[PPT];
-    float dep[PPT];
+    uint32_t col[PPT], cnt[PPT], st[PPT], qn[PPT];
+    float dep[PPT], fx[PPT], fy[PPT];
 #pragma unroll
     for (int j = 0; j < PPT; j++) {
-        int lx, ly;
-        if constexpr (ETS == 8) {
-            lx = tid & 7; ly = tid >> 3;
-        } else {
-            const int wave = tid >> 6, lane = tid & 63;
-            constexpr int RPW = ETS / 4;                 // rows per wave
-            constexpr int RPP = 64 / ETS > 0 ? 64 / ETS : 1;  // rows per pass
-            if constexpr (ETS <= 64) {
-                lx = lane % ETS;
-                ly = wave * RPW + (lane / ETS) + j * RPP;
-            }
-        }
-        px[j] = (int)tx0 + lx;
-        py[j] = (int)ty0 + ly;
+        px[j] = wx0 + lane % S;
+        py[j] = wy0 + lane / S + j * RPP;
         live[j] = (uint32_t)px[j] < P.W && (uint32_t)py[j] < P.H;
         const size_t idx = (size_t)py[j] * P.W + px[j];
         col[j] = live[j] ? P.color[idx] : 0u;
         dep[j] = live[j] ? P.depth[idx] : 0.0f;
         st[j] = (SMODE == 2 && live[j]) ? P.stencil[idx] : 0u;
         cnt[j] = 0;
+        qn[j] = 0;
+        fx[j] = (float)px[j] + 0.5f;  // pixel centre, triangle.rs:104-105
+        fy[j] = (float)py[j] + 0.5f;
     }
-    // this wave's pixel rows
-    int wy0, wy1;
-    if constexpr (ETS == 8) {
-        wy0 = (int)ty0; wy1 = (int)ty1;
-    } else {
-        const int wave = tid >> 6;
-        wy0 = (int)ty0 + wave * (ETS / 4);
-        wy1 = min((int)ty0 + (wave + 1) * (ETS / 4) - 1, (int)ty1);
-    }
-    const int wx0 = (int)tx0, wx1 = (int)tx1;
+    QRec* wq = qbuf[wave];
+    uint16_t* wflat = flat[wave];
 
-    // screen tiles meeting this execution tile, in the reference's row-major order
-    uint32_t jlo, jhi, ilo, ihi;
-    sem_range(tx0, tx1, P.tw, P.scols, jlo, jhi);
-    sem_range(ty0, ty1, P.th, P.srows, ilo, ihi);
-    const uint32_t xmax = P.W - 1, ymax = P.H - 1;
-
-    for (uint32_t si = ilo; si <= ihi; si++) {
-        const int Ty0 = (int)(si * P.th), Ty1 = (int)min(si * P.th + P.th, ymax);
-        for (uint32_t sj = jlo; sj <= jhi; sj++) {
-            const int Tx0 = (int)(sj * P.tw), Tx1 = (int)min(sj * P.tw + P.tw, xmax);
-            // this wave's rows/cols inside the screen tile
-            const int rx0 = max(wx0, Tx0), rx1 = min(wx1, Tx1);
-            const int ry0 = max(wy0, Ty0), ry1 = min(wy1, Ty1);
-            const bool wave_in = rx0 <= rx1 && ry0 <= ry1;
-            bool inT[PPT];
+    // Shade every queued fragment of this wave and blend them in order.
+    auto flush = [&]() {
+        wave_sync_lds();
+        uint32_t mine = 0;
 #pragma unroll
-            for (int j = 0; j < PPT; j++)
-                inT[j] = live[j] && px[j] >= Tx0 && px[j] <= Tx1 && py[j] >= Ty0 && py[j] <= Ty1;
+        for (int j = 0; j < PPT; j++) mine += qn[j];
+        uint32_t total;
+        uint32_t o = wave_excl_scan(mine, total);
+#pragma unroll
+        for (int j = 0; j < PPT; j++)
+            for (uint32_t k = 0; k < qn[j]; k++) wflat[o++] = (uint16_t)((lane * PPT + j) * Q + k);
+        wave_sync_lds();
+        for (uint32_t b = 0; b < total; b += 64) {
+            const uint32_t r = b + lane;
+            if (r < total) {
+                const uint32_t s = wflat[r];
+                const QRec q = wq[s];
+                wq[s].tri = shade_rec<FP>(P, q, lut);
+            }
+        }
+        wave_sync_lds();
+#pragma unroll
+        for (int j = 0; j < PPT; j++) {
+            for (uint32_t k = 0; k < qn[j]; k++)
+                col[j] = blend<BLEND>(wq[(lane * PPT + j) * Q + k].tri, col[j], lut);
+            qn[j] = 0;
+        }
+        wave_sync_lds();
+    };
 
-            for (uint32_t c0 = beg; c0 < end; c0 += CH) {
-                const uint32_t nc = min((uint32_t)CH, end - c0);
-                __syncthreads();
-                if ((uint32_t)tid < nc) {
-                    const uint32_t item = P.exhaustive_n ? c0 + tid : P.items[c0 + tid];
-                    srec[tid] = P.tris[item];
-                }
-                __syncthreads();
-                if (!wave_in) continue;
-                for (uint32_t q = 0; q < nc; q++) {
-                    const TriRec& r = srec[q];
+
+    // screen tiles meeting this wave's pixels, in the reference's row-major order
+    if (beg < end && wx0 <= wx1 && wy0 <= wy1) {
+        uint32_t jlo, jhi, ilo, ihi;
+        sem_range((uint32_t)wx0, (uint32_t)wx1, P.tw, P.scols, jlo, jhi);
+        sem_range((uint32_t)wy0, (uint32_t)wy1, P.th, P.srows, ilo, ihi);
+        const uint32_t xmax = P.W - 1, ymax = P.H - 1;
+        for (uint32_t si = ilo; si <= ihi; si++) {
+            const int Ty0 = (int)(si * P.th), Ty1 = (int)min(si * P.th + P.th, ymax);
+            for (uint32_t sj = jlo; sj <= jhi; sj++) {
+                const int Tx0 = (int)(sj * P.tw), Tx1 = (int)min(sj * P.tw + P.tw, xmax);
+                const int rx0 = max(wx0, Tx0), rx1 = min(wx1, Tx1);
+                const int ry0 = max(wy0, Ty0), ry1 = min(wy1, Ty1);
+                if (rx0 > rx1 || ry0 > ry1) continue;
+                bool inT[PPT];
+#pragma unroll
+                for (int j = 0; j < PPT; j++)
+                    inT[j] = live[j] && px[j] >= Tx0 && px[j] <= Tx1 && py[j] >= Ty0 && py[j] <= Ty1;
+
+                // walk the tile's triangles in order; records arrive through the
+                // scalar cache one iteration ahead of their use
+                uint32_t item = exh ? beg : load_item(P.items, beg);
+                TriRec r = load_rec(P.tris, item);
+                for (uint32_t c = beg; c < end; c++) {
+                    const uint32_t cur = item;
+                    const TriRec rc = r;
+                    if (c + 1 < end) {
+                        item = exh ? c + 1 : load_item(P.items, c + 1);
+                        r = load_rec(P.tris, item);
+                    }
                     // tile-clamped bbox, clamp_as_int!, triangle.rs:66-78
-                    const int X0 = __builtin_amdgcn_readfirstlane(r.X0);
-                    const int X1 = __builtin_amdgcn_readfirstlane(r.X1);
-                    const int Y0 = __builtin_amdgcn_readfirstlane(r.Y0);
-                    const int Y1 = __builtin_amdgcn_readfirstlane(r.Y1);
+                    const int X0 = rc.X0, X1 = rc.X1, Y0 = rc.Y0, Y1 = rc.Y1;
                     if (X0 > X1) continue;  // culled or invalid
                     const int cx0 = min(max(X0, Tx0), Tx1), cx1 = min(max(X1, Tx0), Tx1);
                     const int cy0 = min(max(Y0, Ty0), Ty1), cy1 = min(max(Y1, Ty0), Ty1);
                     if (cx1 < rx0 || cx0 > rx1 || cy1 < ry0 || cy0 > ry1) continue;
-                    const float x3 = r.x3, y3 = r.y3, e23y = r.e23y, e23x = r.e23x,
-                                e31y = r.e31y, e31x = r.e31x;
-                    const double R = r.R;
-                    const float z1 = r.z1, z2 = r.z2, z3 = r.z3;
+                    bool full = false;
 #pragma unroll
                     for (int j = 0; j < PPT; j++) {
                         if (!(inT[j] && px[j] >= cx0 && px[j] <= cx1 && py[j] >= cy0 &&
@@ -153,32 +249,33 @@ __global__ __launch_bounds__(256) void k_raster_tris(RasterParams P) {
                         } else if constexpr (SMODE == 1) {
                             if (px[j] >= X0 && px[j] <= X1 && py[j] >= Y0 && py[j] <= Y1) cnt[j]++;
                         }
-                        const float x = (float)px[j] + 0.5f, y = (float)py[j] + 0.5f;
-                        const float dx = x - x3, dy = y - y3;
-                        const float un = (e23y * dx) + (e23x * dy);  // triangle.rs:108
-                        const float vn = (e31y * dx) + (e31x * dy);  // triangle.rs:109
-                        const float u = quot(un, R), v = quot(vn, R);
-                        const float w = (1.0f - u) - v;              // triangle.rs:110
-                        if (u < 0.0f || v < 0.0f || w < 0.0f) continue;  // :113
-                        const float z = ((z1 * u) + (z2 * v)) + (z3 * w);  // :115-117
-                        if (!(z < 0.0f && z >= dep[j])) continue;          // :120-126
-                        float k[KV > 0 ? KV : 1];
-                        if constexpr (KV > 0) {
-                            const uint32_t i0 = r.i0, i1 = r.i1, i2 = r.i2;
-#pragma unroll
-                            for (int c = 0; c < KV; c++) {
-                                const float* vc = P.verts.comp(4 + c);
-                                k[c] = ((vc[i0] * u) + (vc[i1] * v)) + (vc[i2] * w);
-                            }
-                        }
-                        const uint32_t fc = fragment<FP>(P, k, lut);
-                        col[j] = blend<BLEND>(fc, col[j], lut);
+                        const float dx = fx[j] - rc.x3, dy = fy[j] - rc.y3;
+                        const float un = (rc.e23y * dx) + (rc.e23x * dy);  // triangle.rs:108
+                        const float vn = (rc.e31y * dx) + (rc.e31x * dy);  // triangle.rs:109
+                        const float u = quot(un, rc.R), v = quot(vn, rc.R);
+                        const float w = (1.0f - u) - v;                    // triangle.rs:110
+                        if (u < 0.0f || v < 0.0f || w < 0.0f) continue;   // :113
+                        const float z = ((rc.z1 * u) + (rc.z2 * v)) + (rc.z3 * w);  // :115-117
+                        if (!(z < 0.0f && z >= dep[j])) continue;                   // :120-126
                         dep[j] = z;
+                        const int slot = (lane * PPT + j) * Q;
+                        if constexpr (BLEND == SR_BLEND_OVER) {
+                            wq[slot + qn[j]] = QRec{cur, u, v};
+                            qn[j]++;
+                            full |= qn[j] == (uint32_t)Q;
+                        } else {
+                            wq[slot] = QRec{cur, u, v};  // `()` blend: the last fragment wins
+                            qn[j] = 1;
+                        }
+                    }
+                    if constexpr (BLEND == SR_BLEND_OVER) {
+                        if (__ballot(full)) flush();
                     }
                 }
             }
         }
     }
+    flush();
     // resolve: the tile goes back to HBM once
 #pragma unroll
     for (int j = 0; j < PPT; j++) {

@@ -1,0 +1,241 @@
+// sr_raster_lp.hip — raster work-groups for line plots and points.
+//
+// One work-group per 16x16 execution tile; the tile's colour, depth and
+// stencil live in LDS.  Each plot (or point) is one stencil/depth/shade/
+// blend step on one pixel.  Per pixel those steps must run in the
+// reference's order: screen tile by screen tile (row-major,
+// fragment.rs:250-253), then line by line, then plot by plot.  Plots of a
+// chunk run in parallel; two plots on the same pixel are ordered by claim
+// rounds (atomicMin on the plot's rank: the earliest pending plot of each
+// pixel goes first).
+#include "sr_device.hpp"
+#include "sr_kernels.hpp"
+
+namespace sr {
+
+constexpr int LP_ETS = 16;
+
+__device__ __forceinline__ float lp_hypot(float a, float b) {
+    // glibc hypotf: sqrt(x*x + y*y) in double, rounded once to float
+    if (__builtin_isinf(a) || __builtin_isinf(b)) return __builtin_inff();
+    const double x = (double)a, y = (double)b;
+    return (float)sqrt(x * x + y * y);
+}
+
+__device__ __forceinline__ uint32_t block_min_256(uint32_t v, uint32_t* lds4) {
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) v = min(v, (uint32_t)__shfl_xor(v, o, 64));
+    if ((threadIdx.x & 63) == 0) lds4[threadIdx.x >> 6] = v;
+    __syncthreads();
+    const uint32_t r = min(min(lds4[0], lds4[1]), min(lds4[2], lds4[3]));
+    __syncthreads();
+    return r;
+}
+
+struct PixState {
+    uint32_t* col;
+    float* dep;
+    uint32_t* st;
+};
+
+__device__ __forceinline__ void load_tile(const RasterParams& P, uint32_t etx, uint32_t ety,
+                                          PixState s, int& gx, int& gy, bool& live) {
+    const int t = threadIdx.x;
+    gx = (int)(etx * LP_ETS) + (t % LP_ETS);
+    gy = (int)(ety * LP_ETS) + (t / LP_ETS);
+    live = (uint32_t)gx < P.W && (uint32_t)gy < P.H;
+    const size_t idx = (size_t)gy * P.W + gx;
+    s.col[t] = live ? P.color[idx] : 0u;
+    s.dep[t] = live ? P.depth[idx] : 0.0f;
+    s.st[t] = live ? P.stencil[idx] : 0u;
+}
+
+__device__ __forceinline__ void store_tile(const RasterParams& P, PixState s, int gx, int gy, bool live) {
+    if (!live) return;
+    const int t = threadIdx.x;
+    const size_t idx = (size_t)gy * P.W + gx;
+    P.color[idx] = s.col[t];
+    P.depth[idx] = s.dep[t];
+    P.stencil[idx] = (uint8_t)s.st[t];
+}
+
+// Runs `apply(rank)` for every pending rank in this chunk, same-pixel ranks
+// in increasing order.
+template <class F>
+__device__ __forceinline__ void ordered_rounds(bool pending, int lp, uint32_t rank, uint32_t* claim,
+                                               F&& apply) {
+    while (__syncthreads_or(pending)) {
+        if (pending) atomicMin(&claim[lp], rank);
+        __syncthreads();
+        const bool mine = pending && claim[lp] == rank;
+        if (mine) {
+            apply();
+            pending = false;
+        }
+        __syncthreads();
+        if (mine) claim[lp] = 0xFFFFFFFFu;
+        __syncthreads();
+    }
+}
+
+template <int FP, int BLEND>
+__global__ __launch_bounds__(256) void k_raster_lines(RasterParams P) {
+    constexpr int KV = fp_varyings<FP>();
+    __shared__ uint32_t s_col[256], s_st[256], claim[256], lds4[4];
+    __shared__ float s_dep[256], slut[256];
+    const int t = threadIdx.x;
+    const uint32_t tile = xcd_remap(blockIdx.x, gridDim.x);
+    const uint32_t etx = tile % P.ntx, ety = tile / P.ntx;
+    const uint32_t beg = P.tile_start[tile], end = P.tile_end[tile];
+    if (beg >= end) return;
+    const PixState S{s_col, s_dep, s_st};
+    int gx, gy;
+    bool live;
+    load_tile(P, etx, ety, S, gx, gy, live);
+    slut[t] = (float)t / 255.0f;
+    claim[t] = 0xFFFFFFFFu;
+    __syncthreads();
+    const Lut lut{slut};
+    const int ex0 = (int)(etx * LP_ETS), ey0 = (int)(ety * LP_ETS);
+
+    // screen tiles in ascending (row-major) order
+    bool first = true;
+    uint32_t prev = 0;
+    for (;;) {
+        uint32_t best = 0xFFFFFFFFu;
+        for (uint32_t i = beg + t; i < end; i += 256) {
+            const uint32_t T = P.line_items[P.items[i]].tile;
+            if ((first || T > prev) && T < best) best = T;
+        }
+        best = block_min_256(best, lds4);
+        if (best == 0xFFFFFFFFu) break;
+        for (uint32_t i = beg; i < end; i++) {
+            const LineItem it = P.line_items[P.items[i]];
+            if (it.tile != best) continue;
+            const uint32_t v0 = P.line_v ? P.line_v[2 * it.line] : 2 * it.line;
+            const uint32_t v1 = P.line_v ? P.line_v[2 * it.line + 1] : 2 * it.line + 1;
+            const float zs = P.verts.comp(2)[v0], ze = P.verts.comp(2)[v1];
+            for (uint32_t b = 0; b < it.plot_cnt; b += 256) {
+                const uint32_t k = b + t;
+                bool pending = false;
+                int lp = 0;
+                Plot pl{};
+                if (k < it.plot_cnt) {
+                    pl = P.plots[it.plot_off + k];
+                    const int px = (int)(pl.index % P.W) - ex0, py = (int)(pl.index / P.W) - ey0;
+                    if (px >= 0 && px < LP_ETS && py >= 0 && py < LP_ETS) {
+                        pending = true;
+                        lp = py * LP_ETS + px;
+                    }
+                }
+                ordered_rounds(pending, lp, k, claim, [&]() {
+                    // the rasterize_fragment closure, line.rs:54-109
+                    const uint32_t sv = s_st[lp];
+                    if (!stencil_test(P.s_test, sv, P.s_ref)) return;
+                    s_st[lp] = stencil_op(P.s_op, P.s_wrap, sv, P.s_ref);
+                    const float xf = (float)pl.x + 0.5f, yf = (float)pl.y + 0.5f;
+                    const float tt = lp_hypot(it.cx1 - xf, it.cy1 - yf) / it.dlen;  // :75
+                    const float omt = 1.0f - tt;
+                    const float z = (omt * zs) + (tt * ze);                         // :77
+                    if (!(z < 0.0f && z >= s_dep[lp])) return;
+                    float kv[KV > 0 ? KV : 1];
+#pragma unroll
+                    for (int c = 0; c < KV; c++) {
+                        const float* vc = P.verts.comp(4 + c);
+                        kv[c] = (omt * vc[v0]) + (tt * vc[v1]);
+                    }
+                    uint32_t c = fragment<FP>(P, kv, lut);
+                    c = mul_alpha(P.alpha_mode, c, pl.alpha);                     // :100
+                    s_col[lp] = blend<BLEND>(c, s_col[lp], lut);
+                    s_dep[lp] = z;
+                });
+            }
+        }
+        prev = best;
+        first = false;
+    }
+    __syncthreads();
+    store_tile(P, S, gx, gy, live);
+}
+
+template <int FP, int BLEND>
+__global__ __launch_bounds__(256) void k_raster_points(RasterParams P) {
+    constexpr int KV = fp_varyings<FP>();
+    __shared__ uint32_t s_col[256], s_st[256], claim[256];
+    __shared__ float s_dep[256], slut[256];
+    const int t = threadIdx.x;
+    const uint32_t tile = xcd_remap(blockIdx.x, gridDim.x);
+    const uint32_t etx = tile % P.ntx, ety = tile / P.ntx;
+    const uint32_t beg = P.tile_start[tile], end = P.tile_end[tile];
+    if (beg >= end) return;
+    const PixState S{s_col, s_dep, s_st};
+    int gx, gy;
+    bool live;
+    load_tile(P, etx, ety, S, gx, gy, live);
+    slut[t] = (float)t / 255.0f;
+    claim[t] = 0xFFFFFFFFu;
+    __syncthreads();
+    const Lut lut{slut};
+    const int ex0 = (int)(etx * LP_ETS), ey0 = (int)(ety * LP_ETS);
+    // Every point on a pixel belongs to the same screen tile, so list
+    // (= point) order is the reference's per-pixel order.
+    for (uint32_t b = beg; b < end; b += 256) {
+        const uint32_t k = b + t;
+        bool pending = false;
+        int lp = 0;
+        uint32_t v = 0;
+        float x = 0.f, y = 0.f;
+        if (k < end) {
+            const uint32_t pt = P.items[k];
+            v = P.point_v ? P.point_v[pt] : pt;
+            x = P.verts.comp(0)[v];
+            y = P.verts.comp(1)[v];
+            lp = ((int)(uint32_t)y - ey0) * LP_ETS + ((int)(uint32_t)x - ex0);  // point.rs:47
+            pending = true;
+        }
+        ordered_rounds(pending, lp, k, claim, [&]() {
+            const uint32_t sv = s_st[lp];
+            if (!stencil_test(P.s_test, sv, P.s_ref)) return;  // point.rs:293
+            s_st[lp] = stencil_op(P.s_op, P.s_wrap, sv, P.s_ref);
+            const float z = P.verts.comp(2)[v];
+            if (!(z < 0.0f && z >= s_dep[lp])) return;           // point.rs:300-306
+            float kv[KV > 0 ? KV : 1];
+#pragma unroll
+            for (int c = 0; c < KV; c++) kv[c] = P.verts.comp(4 + c)[v];
+            const uint32_t c = fragment<FP>(P, kv, lut);
+            s_col[lp] = blend<BLEND>(c, s_col[lp], lut);
+            s_dep[lp] = z;
+        });
+    }
+    __syncthreads();
+    store_tile(P, S, gx, gy, live);
+}
+
+#define SR_LP_DISPATCH(KERNEL)                                                                   \
+    template <int FP>                                                                            \
+    static void KERNEL##_blend(const RasterParams& P, uint32_t n, hipStream_t s) {              \
+        if (P.blend == SR_BLEND_OVER)                                                            \
+            hipLaunchKernelGGL((KERNEL<FP, SR_BLEND_OVER>), dim3(n), dim3(256), 0, s, P);        \
+        else                                                                                     \
+            hipLaunchKernelGGL((KERNEL<FP, SR_BLEND_REPLACE>), dim3(n), dim3(256), 0, s, P);     \
+    }                                                                                            \
+    static void KERNEL##_fp(const RasterParams& P, uint32_t n, hipStream_t s) {                 \
+        switch (P.fp) {                                                                          \
+            case SR_FP_BLINN_PHONG: KERNEL##_blend<SR_FP_BLINN_PHONG>(P, n, s); break;           \
+            case SR_FP_TEX_LAMBERT: KERNEL##_blend<SR_FP_TEX_LAMBERT>(P, n, s); break;           \
+            case SR_FP_UV_COLOR: KERNEL##_blend<SR_FP_UV_COLOR>(P, n, s); break;                 \
+            default: KERNEL##_blend<SR_FP_FLAT>(P, n, s); break;                                 \
+        }                                                                                        \
+    }
+
+SR_LP_DISPATCH(k_raster_lines)
+SR_LP_DISPATCH(k_raster_points)
+
+void launch_raster_lines(const RasterParams& P, uint32_t ntiles, hipStream_t s) {
+    if (ntiles) k_raster_lines_fp(P, ntiles, s);
+}
+void launch_raster_points(const RasterParams& P, uint32_t ntiles, hipStream_t s) {
+    if (ntiles) k_raster_points_fp(P, ntiles, s);
+}
+
+}  // namespace sr

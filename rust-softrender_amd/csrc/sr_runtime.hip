@@ -35,7 +35,7 @@ struct sr_context_t {
     hipStream_t stream = nullptr;
     std::string err;
     DevBuf verts, gverts, recs, counts, offsets, keys, vals, keys2, vals2, tstart, tend, scan_tmp,
-        hist, misc, strips, strip_aux, litems, plots, lcounts, loffsets, pcounts;
+        hist, misc, strips, strip_aux, litems, plots, lcounts, loffsets, pcounts, poffsets;
     uint32_t* h_misc = nullptr;  // pinned readback words
     hipEvent_t ev[ST_COUNT + 1] = {};
     float stage_ms[ST_COUNT] = {};
@@ -119,12 +119,43 @@ int read_words(sr_context ctx, int n) {
 
 int varyings_of(int vp) { return vp == SR_VP_MVP_NORMAL_WORLD ? 6 : 5; }
 
-// exclusive scan of counts[0..n) into offsets, total -> misc[1]
-int scan_counts(sr_context ctx, const uint32_t* counts, uint32_t* offsets, uint32_t n) {
+// exclusive scan of counts[0..n) into offsets, total -> misc[slot]
+int scan_counts(sr_context ctx, const uint32_t* counts, uint32_t* offsets, uint32_t n, int slot = 1) {
     int rc = ensure(ctx, ctx->scan_tmp, sizeof(uint32_t) * (scan_tmp_words(n) + 16));
     if (rc) return rc;
-    launch_scan(counts, offsets, n, P<uint32_t>(ctx->scan_tmp), P<uint32_t>(ctx->misc) + 1,
+    launch_scan(counts, offsets, n, P<uint32_t>(ctx->scan_tmp), P<uint32_t>(ctx->misc) + slot,
                 ctx->stream);
+    return SR_OK;
+}
+
+// Sorts (execution tile, item) pairs stably by tile and fills the per-tile
+// ranges.  keys/vals hold npairs entries; returns the sorted item array.
+int bin_pairs(sr_context ctx, uint32_t npairs, uint32_t ntiles, const uint32_t** sorted_vals) {
+    int rc;
+    hipStream_t s = ctx->stream;
+    if ((rc = ensure(ctx, ctx->hist, sizeof(uint32_t) * (radix_hist_words(npairs) + 256)))) return rc;
+    if ((rc = ensure(ctx, ctx->scan_tmp, sizeof(uint32_t) * (scan_tmp_words(radix_hist_words(npairs) + 256) + 16))))
+        return rc;
+    if ((rc = ensure(ctx, ctx->tstart, sizeof(uint32_t) * ntiles)) ||
+        (rc = ensure(ctx, ctx->tend, sizeof(uint32_t) * ntiles)))
+        return rc;
+    const int swapped = radix_sort(P<uint32_t>(ctx->keys), P<uint32_t>(ctx->vals), P<uint32_t>(ctx->keys2),
+                                   P<uint32_t>(ctx->vals2), npairs, bits_for(ntiles), P<uint32_t>(ctx->hist),
+                                   P<uint32_t>(ctx->scan_tmp), P<uint32_t>(ctx->misc) + 8, s);
+    const uint32_t* skeys = swapped ? P<uint32_t>(ctx->keys2) : P<uint32_t>(ctx->keys);
+    *sorted_vals = swapped ? P<uint32_t>(ctx->vals2) : P<uint32_t>(ctx->vals);
+    launch_fill(P<uint32_t>(ctx->tstart), 0u, ntiles, s);
+    launch_fill(P<uint32_t>(ctx->tend), 0u, ntiles, s);
+    launch_ranges(skeys, npairs, 0u, P<uint32_t>(ctx->tstart), P<uint32_t>(ctx->tend), s);
+    return SR_OK;
+}
+
+int ensure_pairs(sr_context ctx, uint32_t npairs) {
+    int rc;
+    const size_t pb = sizeof(uint32_t) * std::max<uint32_t>(npairs, 1);
+    if ((rc = ensure(ctx, ctx->keys, pb)) || (rc = ensure(ctx, ctx->vals, pb)) ||
+        (rc = ensure(ctx, ctx->keys2, pb)) || (rc = ensure(ctx, ctx->vals2, pb)))
+        return rc;
     return SR_OK;
 }
 
@@ -292,7 +323,7 @@ int sr_ctx_destroy(sr_context ctx) {
     DevBuf* bufs[] = {&ctx->verts, &ctx->gverts, &ctx->recs, &ctx->counts, &ctx->offsets, &ctx->keys,
                       &ctx->vals, &ctx->keys2, &ctx->vals2, &ctx->tstart, &ctx->tend, &ctx->scan_tmp,
                       &ctx->hist, &ctx->misc, &ctx->strips, &ctx->strip_aux, &ctx->litems,
-                      &ctx->plots, &ctx->lcounts, &ctx->loffsets, &ctx->pcounts};
+                      &ctx->plots, &ctx->lcounts, &ctx->loffsets, &ctx->pcounts, &ctx->poffsets};
     for (DevBuf* b : bufs)
         if (b->p) hipFree(b->p);
     for (auto& e : ctx->ev)
@@ -609,13 +640,124 @@ int sr_batch_draw(sr_context ctx, const sr_batch_item* items, uint32_t n) {
 
 }  // extern "C"
 
-// Lines and points share the context plumbing; they live in this file's
-// namespace so they can use the helpers above.
-int draw_lines_impl(sr_context ctx, sr_framebuffer, const DrawPlan&, const sr_draw_desc*, sr_texture,
-                    VertexBuf, const uint32_t*, uint32_t) {
-    return fail(ctx, SR_ERR_UNSUPPORTED, "lines: not yet available");
+// ---------------------------------------------------------------------------
+// Lines: (line, screen tile) items -> plot records -> binned by execution
+// tile -> raster.  Points: one record each -> binned -> raster.  Both use
+// 16x16 execution tiles.
+// ---------------------------------------------------------------------------
+static constexpr uint32_t LP_ETS = 16;
+
+int draw_lines_impl(sr_context ctx, sr_framebuffer fb, const DrawPlan& pl, const sr_draw_desc* d,
+                    sr_texture tex, VertexBuf vb, const uint32_t* idx, uint32_t nlines) {
+    int rc;
+    hipStream_t s = ctx->stream;
+    const uint32_t ntx = (pl.W + LP_ETS - 1) / LP_ETS, nty = (pl.H + LP_ETS - 1) / LP_ETS;
+    if ((rc = ensure(ctx, ctx->counts, sizeof(uint32_t) * (nlines + 1))) ||
+        (rc = ensure(ctx, ctx->offsets, sizeof(uint32_t) * (nlines + 1))))
+        return rc;
+    LineSetupParams lp;
+    memset(&lp, 0, sizeof(lp));
+    lp.v = vb; lp.idx = idx; lp.n_lines = nlines; lp.W = pl.W; lp.H = pl.H;
+    lp.tw = pl.tw; lp.th = pl.th; lp.scols = pl.scols; lp.srows = pl.srows;
+    lp.counts = P<uint32_t>(ctx->counts); lp.offsets = P<uint32_t>(ctx->offsets);
+    lp.err = P<uint32_t>(ctx->misc);
+    launch_line_tiles(lp, 0, s);
+    mark(ctx, pl.timing, ST_SETUP);
+    if ((rc = scan_counts(ctx, lp.counts, P<uint32_t>(ctx->offsets), nlines, 1))) return rc;
+    if ((rc = read_words(ctx, 2))) return rc;
+    if (ctx->h_misc[0]) return fail(ctx, SR_ERR_INVALID_COORDINATE, "non-finite line coordinate");
+    const uint32_t nitems = ctx->h_misc[1];
+    if (nitems == 0) return SR_OK;
+    if ((rc = ensure(ctx, ctx->litems, sizeof(LineItem) * nitems)) ||
+        (rc = ensure(ctx, ctx->lcounts, sizeof(uint32_t) * (nitems + 1))) ||
+        (rc = ensure(ctx, ctx->loffsets, sizeof(uint32_t) * (nitems + 1))) ||
+        (rc = ensure(ctx, ctx->pcounts, sizeof(uint32_t) * (nitems + 1))) ||
+        (rc = ensure(ctx, ctx->poffsets, sizeof(uint32_t) * (nitems + 1))))
+        return rc;
+    lp.items = P<LineItem>(ctx->litems);
+    launch_line_tiles(lp, 1, s);
+    WalkParams wp;
+    memset(&wp, 0, sizeof(wp));
+    wp.items = P<LineItem>(ctx->litems); wp.n_items = nitems; wp.W = pl.W; wp.H = pl.H;
+    wp.aa = d->aa_lines ? 1 : 0; wp.ets = LP_ETS; wp.ntx = ntx; wp.nty = nty;
+    wp.plot_counts = P<uint32_t>(ctx->lcounts); wp.bin_counts = P<uint32_t>(ctx->pcounts);
+    wp.plot_offsets = P<uint32_t>(ctx->loffsets); wp.err = P<uint32_t>(ctx->misc);
+    launch_line_walk(wp, 0, s);
+    if ((rc = scan_counts(ctx, wp.plot_counts, P<uint32_t>(ctx->loffsets), nitems, 2))) return rc;
+    if ((rc = scan_counts(ctx, wp.bin_counts, P<uint32_t>(ctx->poffsets), nitems, 3))) return rc;
+    mark(ctx, pl.timing, ST_SCAN);
+    if ((rc = read_words(ctx, 4))) return rc;
+    const uint32_t nplots = ctx->h_misc[2], npairs = ctx->h_misc[3];
+    ctx->last.bin_pairs = npairs;
+    if (npairs == 0) return SR_OK;
+    if ((rc = ensure(ctx, ctx->plots, sizeof(Plot) * std::max<uint32_t>(nplots, 1)))) return rc;
+    wp.plots = P<Plot>(ctx->plots);
+    launch_line_walk(wp, 1, s);
+    if ((rc = ensure_pairs(ctx, npairs))) return rc;
+    ItemEmitParams ip;
+    ip.items = P<LineItem>(ctx->litems); ip.counts = P<uint32_t>(ctx->pcounts);
+    ip.offsets = P<uint32_t>(ctx->poffsets); ip.n = nitems; ip.ntx = ntx;
+    ip.keys = P<uint32_t>(ctx->keys); ip.vals = P<uint32_t>(ctx->vals);
+    launch_item_emit(ip, s);
+    mark(ctx, pl.timing, ST_EMIT);
+    const uint32_t* items = nullptr;
+    if ((rc = bin_pairs(ctx, npairs, ntx * nty, &items))) return rc;
+    mark(ctx, pl.timing, ST_SORT);
+    mark(ctx, pl.timing, ST_RANGES);
+    mark(ctx, pl.timing, ST_STRIPS);
+    RasterParams R;
+    fill_raster_params(pl, fb, d, tex, R);
+    R.ets = LP_ETS; R.ntx = ntx; R.nty = nty;
+    R.tile_start = P<uint32_t>(ctx->tstart); R.tile_end = P<uint32_t>(ctx->tend);
+    R.items = items;
+    R.line_items = P<LineItem>(ctx->litems);
+    R.plots = P<Plot>(ctx->plots);
+    R.verts = vb;
+    R.line_v = idx;
+    launch_raster_lines(R, ntx * nty, s);
+    mark(ctx, pl.timing, ST_RASTER);
+    return SR_OK;
 }
-int draw_points_impl(sr_context ctx, sr_framebuffer, const DrawPlan&, const sr_draw_desc*, sr_texture,
-                     VertexBuf, const uint32_t*, uint32_t) {
-    return fail(ctx, SR_ERR_UNSUPPORTED, "points: not yet available");
+
+int draw_points_impl(sr_context ctx, sr_framebuffer fb, const DrawPlan& pl, const sr_draw_desc* d,
+                     sr_texture tex, VertexBuf vb, const uint32_t* idx, uint32_t npoints) {
+    int rc;
+    hipStream_t s = ctx->stream;
+    const uint32_t ntx = (pl.W + LP_ETS - 1) / LP_ETS, nty = (pl.H + LP_ETS - 1) / LP_ETS;
+    if ((rc = ensure(ctx, ctx->counts, sizeof(uint32_t) * (npoints + 1))) ||
+        (rc = ensure(ctx, ctx->offsets, sizeof(uint32_t) * (npoints + 1))))
+        return rc;
+    PointSetupParams pp;
+    memset(&pp, 0, sizeof(pp));
+    pp.v = vb; pp.idx = idx; pp.n = npoints; pp.W = pl.W; pp.H = pl.H; pp.tw = pl.tw; pp.th = pl.th;
+    pp.scols = pl.scols; pp.ets = LP_ETS; pp.ntx = ntx;
+    pp.counts = P<uint32_t>(ctx->counts); pp.offsets = P<uint32_t>(ctx->offsets);
+    pp.pass = 0;
+    launch_point_setup(pp, s);
+    mark(ctx, pl.timing, ST_SETUP);
+    if ((rc = scan_counts(ctx, pp.counts, P<uint32_t>(ctx->offsets), npoints, 1))) return rc;
+    mark(ctx, pl.timing, ST_SCAN);
+    if ((rc = read_words(ctx, 2))) return rc;
+    const uint32_t npairs = ctx->h_misc[1];
+    ctx->last.bin_pairs = npairs;
+    if (npairs == 0) return SR_OK;
+    if ((rc = ensure_pairs(ctx, npairs))) return rc;
+    pp.keys = P<uint32_t>(ctx->keys); pp.vals = P<uint32_t>(ctx->vals); pp.pass = 1;
+    launch_point_setup(pp, s);
+    mark(ctx, pl.timing, ST_EMIT);
+    const uint32_t* items = nullptr;
+    if ((rc = bin_pairs(ctx, npairs, ntx * nty, &items))) return rc;
+    mark(ctx, pl.timing, ST_SORT);
+    mark(ctx, pl.timing, ST_RANGES);
+    mark(ctx, pl.timing, ST_STRIPS);
+    RasterParams R;
+    fill_raster_params(pl, fb, d, tex, R);
+    R.ets = LP_ETS; R.ntx = ntx; R.nty = nty;
+    R.tile_start = P<uint32_t>(ctx->tstart); R.tile_end = P<uint32_t>(ctx->tend);
+    R.items = items;
+    R.verts = vb;
+    R.point_v = idx;
+    launch_raster_points(R, ntx * nty, s);
+    mark(ctx, pl.timing, ST_RASTER);
+    return SR_OK;
 }
