@@ -225,10 +225,25 @@ typedef struct sr_batch_item {
     sr_mesh mesh;
     const sr_draw_desc* desc;
     sr_texture tex;
+    int32_t clear;       /* non-zero: Framebuffer::clear(clear_rgba) first (depth far, stencil 0) */
+    uint32_t clear_rgba;
 } sr_batch_item;
 /* Renders n independent (framebuffer, mesh) draws; each framebuffer must
- * appear at most once.  Results equal n separate sr_draw calls. */
+ * appear at most once.  Results equal, item by item, an sr_fb_clear (if
+ * `clear`) followed by sr_draw.  Triangle draws without a geometry program
+ * and with stencil Always/Keep run in one pass over the whole batch
+ * (captured in a hipGraph and replayed while the batch is unchanged);
+ * other items are drawn one by one.  Synchronous. */
 int sr_batch_draw(sr_context ctx, const sr_batch_item* items, uint32_t n);
+
+/* Execution tile used when sr_draw_desc.exec_tile == 0: entry i applies to
+ * framebuffers of at most max_pixels[i] pixels (first match wins).  Set by
+ * the autotuner (rust-softrender_amd/autotune.py). */
+int sr_ctx_set_exec_tile_policy(sr_context ctx, uint32_t n, const uint64_t* max_pixels,
+                                const int32_t* exec_tile);
+/* Initial (execution tile, triangle) pair capacity of the batch path; it
+ * grows by itself when a batch overflows it. */
+int sr_ctx_set_batch_capacity(sr_context ctx, uint32_t pairs);
 
 #ifdef __cplusplus
 }

@@ -98,6 +98,25 @@ struct RasterParams {
     uint32_t tex_border;
     float fsu[32];
     uint32_t* error_flags;
+    // batch: this scene's first global execution tile, and a fused clear
+    // (Framebuffer::clear, renderbuffer/mod.rs:125-133) applied on load
+    uint32_t tile_base;
+    int clear;
+    uint32_t clear_rgba;
+};
+
+// Raster launch: one scene given inline (one work-group per tile), or a
+// table of scenes whose execution tiles are numbered globally
+// [scene_tile_end[s-1], scene_tile_end[s]) and pulled from per-XCD queues
+// by a persistent grid.
+struct RasterBatch {
+    RasterParams one;
+    const RasterParams* scenes;      // null: use `one`
+    const uint32_t* scene_tile_end;  // inclusive prefix of tiles per scene
+    uint32_t nscenes;
+    uint32_t tile0, ntiles;          // global tile range of this launch
+    uint32_t* queue;                 // 8 counters (one per XCD), zeroed before launch
+    const uint32_t* overflow;        // non-zero: binning overflowed, draw nothing
 };
 
 }  // namespace sr

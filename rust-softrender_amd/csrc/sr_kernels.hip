@@ -39,14 +39,13 @@ __device__ __forceinline__ void normalize_pos(float x, float y, float z, float w
     out[3] = 1.0f / w;
 }
 
-// One vertex program invocation per thread.  Inputs are coalesced SoA
-// streams; outputs are SoA (x, y, z, w, varyings).
-__global__ __launch_bounds__(256) void k_vertex(VSParams P) {
-    const uint32_t i = blockIdx.x * 256u + threadIdx.x;
-    if (i >= P.n) return;
-    const float* U = P.U;
-    const float pin[4] = {P.px[i], P.py[i], P.pz[i], 1.0f};
-    const float nin[3] = {P.nx ? P.nx[i] : 0.0f, P.ny ? P.ny[i] : 0.0f, P.nz ? P.nz[i] : 0.0f};
+// One vertex program invocation (the VertexShader::run closure stand-in,
+// vertex.rs:97-109), optionally fused with the viewport transform
+// (run_to_fragment, vertex.rs:139).  Writes slot `o` of the SoA output.
+__device__ __forceinline__ void vertex_one(const float* U, int program, int K, int normalize,
+                                           const sr_viewport& viewport, const float pin[4],
+                                           const float nin[3], float u, float v,
+                                           const VertexBuf& out, uint32_t o) {
     float clip[4];
 #pragma unroll
     for (int r = 0; r < 4; r++) {
@@ -66,7 +65,7 @@ __global__ __launch_bounds__(256) void k_vertex(VSParams P) {
     normalize3(n[0], n[1], n[2]);
     float var[6];
     var[0] = n[0]; var[1] = n[1]; var[2] = n[2];
-    if (P.program == SR_VP_MVP_NORMAL_WORLD) {
+    if (program == SR_VP_MVP_NORMAL_WORLD) {
 #pragma unroll
         for (int r = 0; r < 3; r++) {
             float acc = 0.0f;
@@ -75,22 +74,54 @@ __global__ __launch_bounds__(256) void k_vertex(VSParams P) {
             var[3 + r] = acc;
         }
     } else {
-        var[3] = P.u ? P.u[i] : 0.0f;
-        var[4] = P.v ? P.v[i] : 0.0f;
+        var[3] = u;
+        var[4] = v;
         var[5] = 0.0f;
     }
-    float out[4];
-    if (P.normalize) {
-        normalize_pos(clip[0], clip[1], clip[2], clip[3], P.viewport, out);
+    float outp[4];
+    if (normalize) {
+        normalize_pos(clip[0], clip[1], clip[2], clip[3], viewport, outp);
     } else {
-        out[0] = clip[0]; out[1] = clip[1]; out[2] = clip[2]; out[3] = clip[3];
+        outp[0] = clip[0]; outp[1] = clip[1]; outp[2] = clip[2]; outp[3] = clip[3];
     }
-    const VertexBuf& o = P.out;
-    o.comp(0)[i] = out[0];
-    o.comp(1)[i] = out[1];
-    o.comp(2)[i] = out[2];
-    o.comp(3)[i] = out[3];
-    for (int k = 0; k < P.K; k++) o.comp(4 + k)[i] = var[k];
+    out.comp(0)[o] = outp[0];
+    out.comp(1)[o] = outp[1];
+    out.comp(2)[o] = outp[2];
+    out.comp(3)[o] = outp[3];
+    for (int k = 0; k < K; k++) out.comp(4 + k)[o] = var[k];
+}
+
+// One vertex program invocation per thread.  Inputs are coalesced SoA
+// streams; outputs are SoA (x, y, z, w, varyings).
+__global__ __launch_bounds__(256) void k_vertex(VSParams P) {
+    const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+    if (i >= P.n) return;
+    const float pin[4] = {P.px[i], P.py[i], P.pz[i], 1.0f};
+    const float nin[3] = {P.nx ? P.nx[i] : 0.0f, P.ny ? P.ny[i] : 0.0f, P.nz ? P.nz[i] : 0.0f};
+    vertex_one(P.U, P.program, P.K, P.normalize, P.viewport, pin, nin, P.u ? P.u[i] : 0.0f,
+               P.v ? P.v[i] : 0.0f, P.out, i);
+}
+
+// scene owning element i of a batch: first s with i < end[s]
+__device__ __forceinline__ uint32_t scene_of(const uint32_t* end, uint32_t n, uint32_t i) {
+    uint32_t a = 0, b = n;
+    while (a < b) {
+        const uint32_t m = (a + b) >> 1;
+        if (end[m] <= i) a = m + 1;
+        else b = m;
+    }
+    return a;
+}
+
+__global__ __launch_bounds__(256) void k_vertex_batch(BatchTables T, VertexBuf out) {
+    const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+    if (i >= T.nv) return;
+    const SceneGeom& S = T.scenes[scene_of(T.vend, T.nscenes, i)];
+    const uint32_t l = i - S.vbase;
+    const float pin[4] = {S.px[l], S.py[l], S.pz[l], 1.0f};
+    const float nin[3] = {S.nx ? S.nx[l] : 0.0f, S.ny ? S.ny[l] : 0.0f, S.nz ? S.nz[l] : 0.0f};
+    vertex_one(S.U, S.program, S.K, 1, S.viewport, pin, nin, S.u ? S.u[l] : 0.0f,
+               S.v ? S.v[l] : 0.0f, out, i);
 }
 
 // ===========================================================================
@@ -290,58 +321,98 @@ __global__ __launch_bounds__(256) void k_gs_emit(GSParams P) {
 // Triangle setup: cull, det, bounding box, execution-tile counts
 // (rasterize_triangle's per-triangle part, triangle.rs:48-78)
 // ===========================================================================
-__global__ __launch_bounds__(256) void k_tri_setup(SetupParams P) {
-    const uint32_t t = blockIdx.x * 256u + threadIdx.x;
-    if (t >= P.n) return;
-    const uint32_t i0 = prim_vertex(P.idx, 3 * t), i1 = prim_vertex(P.idx, 3 * t + 1),
-                   i2 = prim_vertex(P.idx, 3 * t + 2);
-    const float* sx = P.v.comp(0);
-    const float* sy = P.v.comp(1);
-    const float* sz = P.v.comp(2);
+// Returns the number of execution tiles the triangle's box covers and
+// writes its record (X0 > X1 marks culled / invalid).
+__device__ __forceinline__ uint32_t tri_setup_one(const VertexBuf& v, uint32_t i0, uint32_t i1,
+                                                  uint32_t i2, int cull, uint32_t Wi, uint32_t Hi,
+                                                  uint32_t ets, TriRec& r, uint32_t* err) {
+    const float* sx = v.comp(0);
+    const float* sy = v.comp(1);
+    const float* sz = v.comp(2);
     const float x1 = sx[i0], y1 = sy[i0], x2 = sx[i1], y2 = sy[i1], x3 = sx[i2], y3 = sy[i2];
-    uint32_t count = 0;
-    if (P.cull != SR_CULL_NONE) {
+    r.i0 = i0; r.i1 = i1; r.i2 = i2;
+    if (cull != SR_CULL_NONE) {
         // shoelace, triangle.rs:56 (left-to-right evaluation)
         const float A = x1 * y2 + x2 * y3 + x3 * y1 - x2 * y1 - x3 * y2 - x1 * y3;
         const int wind = signbit(A) ? SR_CULL_CW : SR_CULL_CCW;
-        if (wind == P.cull) {
-            P.counts[t] = 0;
-            P.recs[t].X0 = 1;  // marks "culled" for the strip pass: X0 > X1
-            P.recs[t].X1 = 0;
-            return;
+        if (wind == cull) {
+            r.X0 = 1; r.X1 = 0; r.Y0 = 1; r.Y1 = 0;
+            return 0;
         }
     }
     const float det = (y2 - y3) * (x1 - x3) + (x3 - x2) * (y1 - y3);  // triangle.rs:64
     const float mnx = fminf(fminf(x1, x2), x3), mxx = fmaxf(fmaxf(x1, x2), x3);
     const float mny = fminf(fminf(y1, y2), y3), mxy = fmaxf(fmaxf(y1, y2), y3);
-    TriRec r;
     r.R = 1.0 / (double)det;
     r.x3 = x3; r.y3 = y3;
     r.e23y = y2 - y3; r.e23x = x3 - x2;
     r.e31y = y3 - y1; r.e31x = x1 - x3;
     r.z1 = sz[i0]; r.z2 = sz[i1]; r.z3 = sz[i2];
-    r.i0 = i0; r.i1 = i1; r.i2 = i2;
     if (mnx != mnx || mxx != mxx || mny != mny || mxy != mxy) {
         // cast(NaN).unwrap() panics in clamp_as_int!, triangle.rs:70
-        atomicOr(&P.err[0], 1u);
+        atomicOr(&err[0], 1u);
         r.X0 = 1; r.X1 = 0; r.Y0 = 1; r.Y1 = 0;
-        P.recs[t] = r;
-        P.counts[t] = 0;
-        return;
+        return 0;
     }
-    const float W = (float)P.W, H = (float)P.H;
+    const float W = (float)Wi, H = (float)Hi;
     r.X0 = (int32_t)floorf(clampf(mnx, -1.0f, W));
     r.X1 = (int32_t)floorf(clampf(mxx, -1.0f, W));
     r.Y0 = (int32_t)floorf(clampf(mny, -1.0f, H));
     r.Y1 = (int32_t)floorf(clampf(mxy, -1.0f, H));
+    const int32_t px0 = r.X0 > 0 ? r.X0 : 0, px1 = r.X1 < (int32_t)Wi - 1 ? r.X1 : (int32_t)Wi - 1;
+    const int32_t py0 = r.Y0 > 0 ? r.Y0 : 0, py1 = r.Y1 < (int32_t)Hi - 1 ? r.Y1 : (int32_t)Hi - 1;
+    if (px0 > px1 || py0 > py1) return 0;
+    const uint32_t ex0 = px0 / ets, ex1 = px1 / ets, ey0 = py0 / ets, ey1 = py1 / ets;
+    return (ex1 - ex0 + 1) * (ey1 - ey0 + 1);
+}
+
+__global__ __launch_bounds__(256) void k_tri_setup(SetupParams P) {
+    const uint32_t t = blockIdx.x * 256u + threadIdx.x;
+    if (t >= P.n) return;
+    TriRec r;
+    P.counts[t] = tri_setup_one(P.v, prim_vertex(P.idx, 3 * t), prim_vertex(P.idx, 3 * t + 1),
+                                prim_vertex(P.idx, 3 * t + 2), P.cull, P.W, P.H, P.ets, r, P.err);
     P.recs[t] = r;
-    const int32_t px0 = r.X0 > 0 ? r.X0 : 0, px1 = r.X1 < (int32_t)P.W - 1 ? r.X1 : (int32_t)P.W - 1;
-    const int32_t py0 = r.Y0 > 0 ? r.Y0 : 0, py1 = r.Y1 < (int32_t)P.H - 1 ? r.Y1 : (int32_t)P.H - 1;
-    if (px0 <= px1 && py0 <= py1) {
-        const uint32_t ex0 = px0 / P.ets, ex1 = px1 / P.ets, ey0 = py0 / P.ets, ey1 = py1 / P.ets;
-        count = (ex1 - ex0 + 1) * (ey1 - ey0 + 1);
+}
+
+__global__ __launch_bounds__(256) void k_tri_setup_batch(BatchTables T, VertexBuf v, TriRec* recs,
+                                                        uint32_t* counts, uint32_t* err) {
+    const uint32_t t = blockIdx.x * 256u + threadIdx.x;
+    if (t >= T.nt) return;
+    const SceneGeom& S = T.scenes[scene_of(T.tend, T.nscenes, t)];
+    const uint32_t l = t - S.tbase;
+    TriRec r;
+    counts[t] = tri_setup_one(v, S.vbase + prim_vertex(S.idx, 3 * l), S.vbase + prim_vertex(S.idx, 3 * l + 1),
+                              S.vbase + prim_vertex(S.idx, 3 * l + 2), S.cull, S.W, S.H, S.ets, r, err);
+    recs[t] = r;
+}
+
+// (execution tile, triangle) pairs of a batch; keys are global tile ids.
+// Pairs past the capacity are not written and raise the overflow flag.
+__global__ __launch_bounds__(256) void k_tri_emit_batch(BatchTables T, const TriRec* recs,
+                                                       const uint32_t* counts, const uint32_t* offsets,
+                                                       uint32_t cap, uint32_t* keys, uint32_t* vals,
+                                                       uint32_t* overflow) {
+    const uint32_t t = blockIdx.x * 256u + threadIdx.x;
+    if (t >= T.nt) return;
+    const uint32_t c = counts[t];
+    if (c == 0) return;
+    uint32_t o = offsets[t];
+    if ((uint64_t)o + c > cap) {
+        atomicOr(overflow, 1u);
+        return;
     }
-    P.counts[t] = count;
+    const SceneGeom& S = T.scenes[scene_of(T.tend, T.nscenes, t)];
+    const TriRec& r = recs[t];
+    const int32_t px0 = r.X0 > 0 ? r.X0 : 0, px1 = r.X1 < (int32_t)S.W - 1 ? r.X1 : (int32_t)S.W - 1;
+    const int32_t py0 = r.Y0 > 0 ? r.Y0 : 0, py1 = r.Y1 < (int32_t)S.H - 1 ? r.Y1 : (int32_t)S.H - 1;
+    const uint32_t ex0 = px0 / S.ets, ex1 = px1 / S.ets, ey0 = py0 / S.ets, ey1 = py1 / S.ets;
+    for (uint32_t ey = ey0; ey <= ey1; ey++)
+        for (uint32_t ex = ex0; ex <= ex1; ex++) {
+            keys[o] = S.tile_base + ey * S.ntx + ex;
+            vals[o] = t;
+            o++;
+        }
 }
 
 __global__ __launch_bounds__(256) void k_tri_emit(EmitParams P) {
@@ -433,8 +504,10 @@ __global__ __launch_bounds__(256) void k_scan_down(const uint32_t* in, uint32_t*
 // Stable LSD radix sort of (key, value) by key, 8-bit digits.
 // ===========================================================================
 __global__ __launch_bounds__(256) void k_radix_hist(const uint32_t* keys, uint32_t n, int shift,
-                                                   uint32_t* hist, uint32_t nblocks) {
+                                                   uint32_t* hist, uint32_t nblocks,
+                                                   const uint32_t* n_dev) {
     __shared__ uint32_t h[256];
+    if (n_dev) n = min(n, *n_dev);
     h[threadIdx.x] = 0;
     __syncthreads();
     const uint32_t base = blockIdx.x * SCAN_TILE;
@@ -454,8 +527,9 @@ __global__ __launch_bounds__(256) void k_radix_scatter(const uint32_t* keys_in,
                                                       const uint32_t* vals_in, uint32_t n,
                                                       int shift, const uint32_t* offsets,
                                                       uint32_t nblocks, uint32_t* keys_out,
-                                                      uint32_t* vals_out) {
+                                                      uint32_t* vals_out, const uint32_t* n_dev) {
     __shared__ uint32_t run[256];
+    if (n_dev) n = min(n, *n_dev);
     __shared__ uint32_t wcnt[4][256];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     run[tid] = offsets[tid * nblocks + blockIdx.x];
@@ -500,7 +574,8 @@ __global__ __launch_bounds__(256) void k_fill_u32(uint32_t* p, uint32_t v, uint3
 }
 
 __global__ __launch_bounds__(256) void k_ranges(const uint32_t* keys, uint32_t n, uint32_t key_base,
-                                               uint32_t* start, uint32_t* end) {
+                                               uint32_t* start, uint32_t* end, const uint32_t* n_dev) {
+    if (n_dev) n = min(n, *n_dev);
     const uint32_t i = blockIdx.x * 256u + threadIdx.x;
     if (i >= n) return;
     const uint32_t k = keys[i];
@@ -966,6 +1041,20 @@ void launch_gs_emit(int mode, int prim, const GSParams& P, hipStream_t s) {
 void launch_tri_setup(const SetupParams& P, hipStream_t s) {
     if (P.n) hipLaunchKernelGGL(k_tri_setup, dim3(blocks_for(P.n, 256)), dim3(256), 0, s, P);
 }
+void launch_vertex_batch(const BatchTables& T, const VertexBuf& out, hipStream_t s) {
+    if (T.nv) hipLaunchKernelGGL(k_vertex_batch, dim3(blocks_for(T.nv, 256)), dim3(256), 0, s, T, out);
+}
+void launch_tri_setup_batch(const BatchTables& T, const VertexBuf& v, TriRec* recs, uint32_t* counts,
+                            uint32_t* err, hipStream_t s) {
+    if (T.nt) hipLaunchKernelGGL(k_tri_setup_batch, dim3(blocks_for(T.nt, 256)), dim3(256), 0, s, T, v,
+                                 recs, counts, err);
+}
+void launch_tri_emit_batch(const BatchTables& T, const TriRec* recs, const uint32_t* counts,
+                           const uint32_t* offsets, uint32_t cap, uint32_t* keys, uint32_t* vals,
+                           uint32_t* overflow, hipStream_t s) {
+    if (T.nt) hipLaunchKernelGGL(k_tri_emit_batch, dim3(blocks_for(T.nt, 256)), dim3(256), 0, s, T, recs,
+                                 counts, offsets, cap, keys, vals, overflow);
+}
 void launch_tri_emit(const EmitParams& P, hipStream_t s) {
     if (P.n) hipLaunchKernelGGL(k_tri_emit, dim3(blocks_for(P.n, 256)), dim3(256), 0, s, P);
 }
@@ -987,17 +1076,18 @@ void launch_scan(const uint32_t* in, uint32_t* out, uint32_t n, uint32_t* tmp, u
 uint32_t radix_hist_words(uint32_t n) { return 256u * blocks_for(n, SCAN_TILE); }
 
 int radix_sort(uint32_t* keys, uint32_t* vals, uint32_t* keys_tmp, uint32_t* vals_tmp, uint32_t n,
-               int key_bits, uint32_t* hist, uint32_t* scan_tmp, uint32_t* total, hipStream_t s) {
+               int key_bits, uint32_t* hist, uint32_t* scan_tmp, uint32_t* total, hipStream_t s,
+               const uint32_t* n_dev) {
     if (n <= 1) return 0;
     const uint32_t nb = blocks_for(n, SCAN_TILE);
     const int passes = (key_bits + 7) / 8;
     uint32_t *ki = keys, *vi = vals, *ko = keys_tmp, *vo = vals_tmp;
     for (int p = 0; p < passes; p++) {
         const int shift = 8 * p;
-        hipLaunchKernelGGL(k_radix_hist, dim3(nb), dim3(256), 0, s, ki, n, shift, hist, nb);
+        hipLaunchKernelGGL(k_radix_hist, dim3(nb), dim3(256), 0, s, ki, n, shift, hist, nb, n_dev);
         launch_scan(hist, hist, 256u * nb, scan_tmp, total, s);
         hipLaunchKernelGGL(k_radix_scatter, dim3(nb), dim3(256), 0, s, ki, vi, n, shift,
-                           (const uint32_t*)hist, nb, ko, vo);
+                           (const uint32_t*)hist, nb, ko, vo, n_dev);
         uint32_t* t;
         t = ki; ki = ko; ko = t;
         t = vi; vi = vo; vo = t;
@@ -1010,9 +1100,9 @@ void launch_fill(uint32_t* p, uint32_t v, uint32_t n, hipStream_t s) {
 }
 
 void launch_ranges(const uint32_t* keys, uint32_t n, uint32_t key_base, uint32_t* start,
-                   uint32_t* end, hipStream_t s) {
+                   uint32_t* end, hipStream_t s, const uint32_t* n_dev) {
     if (n) hipLaunchKernelGGL(k_ranges, dim3(blocks_for(n, 256)), dim3(256), 0, s, keys, n, key_base,
-                              start, end);
+                              start, end, n_dev);
 }
 
 void launch_fb_clear(uint32_t* color, float* depth, uint8_t* stencil, uint32_t n, uint32_t rgba,
@@ -1021,8 +1111,8 @@ void launch_fb_clear(uint32_t* color, float* depth, uint8_t* stencil, uint32_t n
                               stencil, n, rgba, d, st);
 }
 
-void launch_raster_tris(const RasterParams& P, uint32_t ntiles, hipStream_t s) {
-    switch (P.ets) {
+void launch_raster_tris(const RasterBatch& P, uint32_t ntiles, hipStream_t s) {
+    switch (P.one.ets) {
         case 8: launch_raster_tris_8(P, ntiles, s); break;
         case 32: launch_raster_tris_32(P, ntiles, s); break;
         case 64: launch_raster_tris_64(P, ntiles, s); break;

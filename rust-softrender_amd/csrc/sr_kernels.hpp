@@ -48,6 +48,26 @@ struct EmitParams {
     uint32_t* vals;
 };
 
+// One scene of a batch: its vertex program, mesh streams and placement in
+// the batch's vertex / triangle / execution-tile numbering.
+struct SceneGeom {
+    const float *px, *py, *pz, *nx, *ny, *nz, *u, *v;
+    const uint32_t* idx;      // null: vertex 3t+k
+    uint32_t nv, vbase;       // vertices, first batch vertex
+    uint32_t nt, tbase;       // triangles, first batch triangle
+    int program, K, cull;
+    uint32_t W, H, ets, ntx, tile_base;
+    sr_viewport viewport;
+    float U[48];
+};
+
+struct BatchTables {
+    const SceneGeom* scenes;
+    const uint32_t* vend;  // inclusive prefix of vertices per scene
+    const uint32_t* tend;  // inclusive prefix of triangles per scene
+    uint32_t nscenes, nv, nt;
+};
+
 struct LineSetupParams {
     VertexBuf v;
     const uint32_t* idx;   // null: vertices 2l, 2l+1
@@ -113,23 +133,33 @@ void launch_gs_count(int mode, int prim, const GSParams& P, hipStream_t s);
 void launch_gs_emit(int mode, int prim, const GSParams& P, hipStream_t s);
 void launch_tri_setup(const SetupParams& P, hipStream_t s);
 void launch_tri_emit(const EmitParams& P, hipStream_t s);
+void launch_vertex_batch(const BatchTables& T, const VertexBuf& out, hipStream_t s);
+void launch_tri_setup_batch(const BatchTables& T, const VertexBuf& v, TriRec* recs, uint32_t* counts,
+                            uint32_t* err, hipStream_t s);
+void launch_tri_emit_batch(const BatchTables& T, const TriRec* recs, const uint32_t* counts,
+                           const uint32_t* offsets, uint32_t cap, uint32_t* keys, uint32_t* vals,
+                           uint32_t* overflow, hipStream_t s);
 uint32_t scan_tmp_words(uint32_t n);
 void launch_scan(const uint32_t* in, uint32_t* out, uint32_t n, uint32_t* tmp, uint32_t* total,
                  hipStream_t s);
 uint32_t radix_hist_words(uint32_t n);
 // returns 1 if the sorted data ended up in the *_tmp buffers
+// n_dev (optional): the element count in device memory, at most n
 int radix_sort(uint32_t* keys, uint32_t* vals, uint32_t* keys_tmp, uint32_t* vals_tmp, uint32_t n,
-               int key_bits, uint32_t* hist, uint32_t* scan_tmp, uint32_t* total, hipStream_t s);
+               int key_bits, uint32_t* hist, uint32_t* scan_tmp, uint32_t* total, hipStream_t s,
+               const uint32_t* n_dev = nullptr);
 void launch_fill(uint32_t* p, uint32_t v, uint32_t n, hipStream_t s);
 void launch_ranges(const uint32_t* keys, uint32_t n, uint32_t key_base, uint32_t* start,
-                   uint32_t* end, hipStream_t s);
+                   uint32_t* end, hipStream_t s, const uint32_t* n_dev = nullptr);
 void launch_fb_clear(uint32_t* color, float* depth, uint8_t* stencil, uint32_t n, uint32_t rgba,
                      float d, uint32_t st, hipStream_t s);
-void launch_raster_tris(const RasterParams& P, uint32_t ntiles, hipStream_t s);
-void launch_raster_tris_8(const RasterParams& P, uint32_t ntiles, hipStream_t s);
-void launch_raster_tris_16(const RasterParams& P, uint32_t ntiles, hipStream_t s);
-void launch_raster_tris_32(const RasterParams& P, uint32_t ntiles, hipStream_t s);
-void launch_raster_tris_64(const RasterParams& P, uint32_t ntiles, hipStream_t s);
+// grid = ntiles work-groups (inline scene) or a persistent grid (scene table);
+// the batch's `one` carries the launch-wide program choice (fp, blend, smode, ets)
+void launch_raster_tris(const RasterBatch& P, uint32_t grid, hipStream_t s);
+void launch_raster_tris_8(const RasterBatch& P, uint32_t grid, hipStream_t s);
+void launch_raster_tris_16(const RasterBatch& P, uint32_t grid, hipStream_t s);
+void launch_raster_tris_32(const RasterBatch& P, uint32_t grid, hipStream_t s);
+void launch_raster_tris_64(const RasterBatch& P, uint32_t grid, hipStream_t s);
 
 // lines / points / stencil strips
 void launch_line_tiles(const LineSetupParams& P, int pass, hipStream_t s);

@@ -111,22 +111,29 @@ __device__ __forceinline__ uint32_t shade_rec(const RasterParams& P, const QRec&
     return fragment<FP>(P, k, lut);
 }
 
-template <int ETS, int FP, int BLEND, int SMODE>
-__global__ __launch_bounds__(256) void k_raster_tris(RasterParams P) {
-    constexpr int NT = raster_threads<ETS>();
-    constexpr int NW = NT / 64;
-    constexpr int S = ETS == 8 ? 8 : ETS / 2;  // each wave owns an S x S quadrant
-    constexpr int PPT = (S * S) / 64;          // pixels per lane
-    constexpr int RPP = 64 / S;                // quadrant rows covered by one pixel pass
-    constexpr int WPIX = 64 * PPT;             // pixels per wave
-    constexpr int Q = BLEND == SR_BLEND_OVER ? queue_depth<ETS>() : 1;
+template <int ETS, int FP, int BLEND>
+struct RasterShared {
+    static constexpr int NT = raster_threads<ETS>();
+    static constexpr int NW = NT / 64;
+    static constexpr int S = ETS == 8 ? 8 : ETS / 2;  // each wave owns an S x S quadrant
+    static constexpr int PPT = (S * S) / 64;          // pixels per lane
+    static constexpr int WPIX = 64 * PPT;             // pixels per wave
+    static constexpr int Q = BLEND == SR_BLEND_OVER ? queue_depth<ETS>() : 1;
+    float lut[256];
+    QRec qbuf[NW][WPIX * Q];
+    uint16_t flat[NW][WPIX * Q];
+};
 
-    __shared__ float slut[256];
-    __shared__ QRec qbuf[NW][WPIX * Q];
-    __shared__ uint16_t flat[NW][WPIX * Q];
+// One execution tile: `tile` is the scene-local tile index, `gtile` the
+// global one that indexes the bins.
+template <int ETS, int FP, int BLEND, int SMODE>
+__device__ __forceinline__ void raster_tile(const RasterParams& P, uint32_t tile, uint32_t gtile,
+                                            RasterShared<ETS, FP, BLEND>& sh) {
+    using SH = RasterShared<ETS, FP, BLEND>;
+    constexpr int S = SH::S, PPT = SH::PPT, Q = SH::Q;
+    constexpr int RPP = 64 / S;  // quadrant rows covered by one pixel pass
 
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const uint32_t tile = xcd_remap(blockIdx.x, gridDim.x);
     const uint32_t etx = tile % P.ntx, ety = tile / P.ntx;
     const bool exh = P.exhaustive_n != 0;
     uint32_t beg, end;
@@ -134,14 +141,11 @@ __global__ __launch_bounds__(256) void k_raster_tris(RasterParams P) {
         beg = 0;
         end = P.exhaustive_n;
     } else {
-        beg = P.tile_start[tile];
-        end = P.tile_end[tile];
+        beg = P.tile_start[gtile];
+        end = P.tile_end[gtile];
     }
-    if (beg >= end && SMODE != 1) return;
-
-    for (int i = tid; i < 256; i += NT) slut[i] = (float)i / 255.0f;
-    __syncthreads();
-    const Lut lut{slut};
+    if (beg >= end && SMODE != 1 && !P.clear) return;
+    const Lut lut{sh.lut};
 
     // tile and wave pixel spans (clipped to the framebuffer)
     const uint32_t tx0 = etx * ETS, ty0 = ety * ETS;
@@ -160,16 +164,22 @@ __global__ __launch_bounds__(256) void k_raster_tris(RasterParams P) {
         py[j] = wy0 + lane / S + j * RPP;
         live[j] = (uint32_t)px[j] < P.W && (uint32_t)py[j] < P.H;
         const size_t idx = (size_t)py[j] * P.W + px[j];
-        col[j] = live[j] ? P.color[idx] : 0u;
-        dep[j] = live[j] ? P.depth[idx] : 0.0f;
-        st[j] = (SMODE == 2 && live[j]) ? P.stencil[idx] : 0u;
+        if (P.clear) {  // Framebuffer::clear: colour, depth far, stencil 0
+            col[j] = P.clear_rgba;
+            dep[j] = -3.40282347e+38f;
+            st[j] = 0u;
+        } else {
+            col[j] = live[j] ? P.color[idx] : 0u;
+            dep[j] = live[j] ? P.depth[idx] : 0.0f;
+            st[j] = (SMODE == 2 && live[j]) ? P.stencil[idx] : 0u;
+        }
         cnt[j] = 0;
         qn[j] = 0;
         fx[j] = (float)px[j] + 0.5f;  // pixel centre, triangle.rs:104-105
         fy[j] = (float)py[j] + 0.5f;
     }
-    QRec* wq = qbuf[wave];
-    uint16_t* wflat = flat[wave];
+    QRec* wq = sh.qbuf[wave];
+    uint16_t* wflat = sh.flat[wave];
 
     // Shade every queued fragment of this wave and blend them in order.
     auto flush = [&]() {
@@ -281,7 +291,7 @@ __global__ __launch_bounds__(256) void k_raster_tris(RasterParams P) {
     for (int j = 0; j < PPT; j++) {
         if (!live[j]) continue;
         const size_t idx = (size_t)py[j] * P.W + px[j];
-        if (beg < end) {
+        if (beg < end || P.clear) {
             P.color[idx] = col[j];
             P.depth[idx] = dep[j];
         }
@@ -289,28 +299,91 @@ __global__ __launch_bounds__(256) void k_raster_tris(RasterParams P) {
             P.stencil[idx] = (uint8_t)st[j];
         } else if constexpr (SMODE == 1) {
             const uint32_t n = cnt[j] + (P.strip_counts ? P.strip_counts[idx] : 0u);
-            if (n) P.stencil[idx] = (uint8_t)stencil_op_n(P.s_op, P.s_wrap, P.stencil[idx], P.s_ref, n);
+            const uint32_t s0 = P.clear ? 0u : P.stencil[idx];
+            if (n || P.clear) P.stencil[idx] = (uint8_t)stencil_op_n(P.s_op, P.s_wrap, s0, P.s_ref, n);
+        } else {
+            if (P.clear) P.stencil[idx] = 0;
+        }
+    }
+}
+
+
+// Scene parameters through the scalar cache (a uniform index).
+__device__ __forceinline__ RasterParams load_params(const RasterParams* base, uint32_t s) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    constexpr int N = sizeof(RasterParams) / 4;
+    const __attribute__((address_space(4))) uint32_t* src =
+        (const __attribute__((address_space(4))) uint32_t*)(base + s);
+    uint32_t w[N];
+#pragma unroll
+    for (int i = 0; i < N; i++) w[i] = src[i];
+    RasterParams r;
+    __builtin_memcpy(&r, w, sizeof(r));
+    return r;
+#else
+    return base[s];
+#endif
+}
+
+template <int ETS, int FP, int BLEND, int SMODE>
+__global__ __launch_bounds__(256) void k_raster_tris(RasterBatch B) {
+    using SH = RasterShared<ETS, FP, BLEND>;
+    __shared__ SH sh;
+    __shared__ uint32_t s_next;
+    const int tid = threadIdx.x;
+    if (B.overflow && *B.overflow) return;
+    for (int i = tid; i < 256; i += SH::NT) sh.lut[i] = (float)i / 255.0f;
+    __syncthreads();
+    if (!B.scenes) {  // one scene, one work-group per tile
+        const uint32_t tile = xcd_remap(blockIdx.x, gridDim.x);
+        raster_tile<ETS, FP, BLEND, SMODE>(B.one, tile, tile, sh);
+        return;
+    }
+    // Persistent: work-groups on XCD x first drain that XCD's contiguous
+    // share of the tiles, then help the other XCDs.
+    const uint32_t x0 = blockIdx.x & 7u;
+    const uint32_t q = B.ntiles >> 3, r = B.ntiles & 7u;
+    for (uint32_t k = 0; k < 8; k++) {
+        const uint32_t x = (x0 + k) & 7u;
+        const uint32_t lo = x * q + min(x, r), n = q + (x < r ? 1u : 0u);
+        for (;;) {
+            if (tid == 0) s_next = atomicAdd(&B.queue[x], 1u);
+            __syncthreads();
+            const uint32_t i = __builtin_amdgcn_readfirstlane(s_next);
+            __syncthreads();
+            if (i >= n) break;
+            const uint32_t g = B.tile0 + lo + i;
+            // scene owning global tile g: first s with g < scene_tile_end[s]
+            uint32_t a = 0, b = B.nscenes;
+            while (a < b) {
+                const uint32_t m = (a + b) >> 1;
+                if (load_item(B.scene_tile_end, m) <= g) a = m + 1;
+                else b = m;
+            }
+            const RasterParams P = load_params(B.scenes, a);
+            raster_tile<ETS, FP, BLEND, SMODE>(P, g - P.tile_base, g, sh);
         }
     }
 }
 
 template <int ETS, int FP, int BLEND>
-static void raster_tris_smode(const RasterParams& P, uint32_t ntiles, hipStream_t s) {
+static void raster_tris_smode(const RasterBatch& P, uint32_t ntiles, hipStream_t s) {
     const dim3 g(ntiles), b(raster_threads<ETS>());
-    if (P.smode == 2) hipLaunchKernelGGL((k_raster_tris<ETS, FP, BLEND, 2>), g, b, 0, s, P);
-    else if (P.smode == 1) hipLaunchKernelGGL((k_raster_tris<ETS, FP, BLEND, 1>), g, b, 0, s, P);
+    const int smode = P.scenes ? 0 : P.one.smode;
+    if (smode == 2) hipLaunchKernelGGL((k_raster_tris<ETS, FP, BLEND, 2>), g, b, 0, s, P);
+    else if (smode == 1) hipLaunchKernelGGL((k_raster_tris<ETS, FP, BLEND, 1>), g, b, 0, s, P);
     else hipLaunchKernelGGL((k_raster_tris<ETS, FP, BLEND, 0>), g, b, 0, s, P);
 }
 
 template <int ETS, int FP>
-static void raster_tris_blend(const RasterParams& P, uint32_t ntiles, hipStream_t s) {
-    if (P.blend == SR_BLEND_OVER) raster_tris_smode<ETS, FP, SR_BLEND_OVER>(P, ntiles, s);
+static void raster_tris_blend(const RasterBatch& P, uint32_t ntiles, hipStream_t s) {
+    if (P.one.blend == SR_BLEND_OVER) raster_tris_smode<ETS, FP, SR_BLEND_OVER>(P, ntiles, s);
     else raster_tris_smode<ETS, FP, SR_BLEND_REPLACE>(P, ntiles, s);
 }
 
 template <int ETS>
-static void raster_tris_fp(const RasterParams& P, uint32_t ntiles, hipStream_t s) {
-    switch (P.fp) {
+static void raster_tris_fp(const RasterBatch& P, uint32_t ntiles, hipStream_t s) {
+    switch (P.one.fp) {
         case SR_FP_BLINN_PHONG: raster_tris_blend<ETS, SR_FP_BLINN_PHONG>(P, ntiles, s); break;
         case SR_FP_TEX_LAMBERT: raster_tris_blend<ETS, SR_FP_TEX_LAMBERT>(P, ntiles, s); break;
         case SR_FP_UV_COLOR: raster_tris_blend<ETS, SR_FP_UV_COLOR>(P, ntiles, s); break;
@@ -320,7 +393,7 @@ static void raster_tris_fp(const RasterParams& P, uint32_t ntiles, hipStream_t s
 
 #define SR_CAT2(a, b) a##b
 #define SR_CAT(a, b) SR_CAT2(a, b)
-void SR_CAT(launch_raster_tris_, SR_ETS)(const RasterParams& P, uint32_t ntiles, hipStream_t s) {
+void SR_CAT(launch_raster_tris_, SR_ETS)(const RasterBatch& P, uint32_t ntiles, hipStream_t s) {
     if (ntiles) raster_tris_fp<SR_ETS>(P, ntiles, s);
 }
 

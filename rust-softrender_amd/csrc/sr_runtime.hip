@@ -13,6 +13,7 @@
 #include <algorithm>
 #include <new>
 #include <string>
+#include <vector>
 
 #include "sr_kernels.hpp"
 
@@ -41,6 +42,10 @@ struct sr_context_t {
     float stage_ms[ST_COUNT] = {};
     int n_stage = 0;
     sr_draw_stats last = {};
+    // execution tile chosen when sr_draw_desc.exec_tile == 0: the first
+    // entry whose pixel bound holds the framebuffer (set by the autotuner)
+    std::vector<std::pair<uint64_t, int>> tile_policy{{~0ull, 8}};
+    struct Batch* batch = nullptr;
 };
 
 struct sr_mesh_t {
@@ -118,6 +123,13 @@ int read_words(sr_context ctx, int n) {
 }
 
 int varyings_of(int vp) { return vp == SR_VP_MVP_NORMAL_WORLD ? 6 : 5; }
+
+int policy_tile(sr_context ctx, uint32_t w, uint32_t h) {
+    const uint64_t px = (uint64_t)w * h;
+    for (const auto& e : ctx->tile_policy)
+        if (px <= e.first) return e.second;
+    return 8;
+}
 
 // exclusive scan of counts[0..n) into offsets, total -> misc[slot]
 int scan_counts(sr_context ctx, const uint32_t* counts, uint32_t* offsets, uint32_t n, int slot = 1) {
@@ -276,7 +288,11 @@ int draw_tris(sr_context ctx, sr_framebuffer fb, const DrawPlan& pl, const sr_dr
         R.strip_counts = P<uint32_t>(ctx->strips);
     }
     mark(ctx, pl.timing, ST_STRIPS);
-    launch_raster_tris(R, pl.ntiles, s);
+    RasterBatch RB;
+    memset(&RB, 0, sizeof(RB));
+    RB.one = R;
+    RB.ntiles = pl.ntiles;
+    launch_raster_tris(RB, pl.ntiles, s);
     mark(ctx, pl.timing, ST_RASTER);
     return SR_OK;
 }
@@ -287,6 +303,327 @@ int draw_lines_impl(sr_context ctx, sr_framebuffer fb, const DrawPlan& pl, const
                     sr_texture tex, VertexBuf vb, const uint32_t* idx, uint32_t nlines);
 int draw_points_impl(sr_context ctx, sr_framebuffer fb, const DrawPlan& pl, const sr_draw_desc* d,
                      sr_texture tex, VertexBuf vb, const uint32_t* idx, uint32_t npoints);
+
+int validate_draw(sr_context ctx, sr_framebuffer fb, sr_mesh mesh, const sr_draw_desc* d, sr_texture tex) {
+    if (!ctx || !fb || !mesh || !d) return fail(ctx, SR_ERR_INVALID_ARGUMENT, "draw: null argument");
+    if (fb->ctx != ctx || mesh->ctx != ctx) return fail(ctx, SR_ERR_INVALID_ARGUMENT, "draw: handle from another context");
+    if (d->vertex_program != SR_VP_MVP_NORMAL_UV && d->vertex_program != SR_VP_MVP_NORMAL_WORLD)
+        return fail(ctx, SR_ERR_INVALID_ARGUMENT, "draw: unknown vertex program");
+    if (d->geometry_program < SR_GS_NONE || d->geometry_program > SR_GS_CLIP_NEAR)
+        return fail(ctx, SR_ERR_INVALID_ARGUMENT, "draw: unknown geometry program");
+    if (d->fragment_program < SR_FP_FLAT || d->fragment_program > SR_FP_UV_COLOR)
+        return fail(ctx, SR_ERR_INVALID_ARGUMENT, "draw: unknown fragment program");
+    if (d->fragment_program == SR_FP_TEX_LAMBERT && !tex)
+        return fail(ctx, SR_ERR_INVALID_ARGUMENT, "draw: fragment program samples a texture");
+    if (d->fragment_program == SR_FP_BLINN_PHONG && d->vertex_program != SR_VP_MVP_NORMAL_WORLD)
+        return fail(ctx, SR_ERR_INVALID_ARGUMENT, "draw: Blinn-Phong needs world-space varyings");
+    if ((d->fragment_program == SR_FP_TEX_LAMBERT || d->fragment_program == SR_FP_UV_COLOR) &&
+        d->vertex_program != SR_VP_MVP_NORMAL_UV)
+        return fail(ctx, SR_ERR_INVALID_ARGUMENT, "draw: program needs uv varyings");
+    if (d->stencil_test < 0 || d->stencil_test > 7 || d->stencil_op < 0 || d->stencil_op > 5)
+        return fail(ctx, SR_ERR_INVALID_ARGUMENT, "draw: bad stencil config");
+    if (d->exec_tile != 0 && d->exec_tile != 8 && d->exec_tile != 16 && d->exec_tile != 32 && d->exec_tile != 64)
+        return fail(ctx, SR_ERR_INVALID_ARGUMENT, "draw: exec_tile must be 0, 8, 16, 32 or 64");
+    return SR_OK;
+}
+
+
+// ===========================================================================
+// Batched rendering: many independent (framebuffer, mesh) triangle draws in
+// one pass of the pipeline.  Every stage runs once over the whole batch:
+// vertices and triangles are numbered across scenes, execution tiles get
+// global ids so one stable sort bins all scenes, and the raster is one
+// persistent launch per (tile size, program, blend) class pulling tiles
+// from per-XCD queues.  Sizes that depend on the geometry (bin pairs) use a
+// capacity checked on the device: on overflow nothing is drawn, the
+// capacity grows and the batch runs again, so the sequence has no host
+// round trip and is captured once into a hipGraph and replayed while the
+// batch is unchanged.
+// ===========================================================================
+struct Batch {
+    DevBuf verts, recs, counts, offsets, keys, vals, keys2, vals2, tstart, tend, hist, scan, tables, misc;
+    uint32_t cap = 0;          // bin pair capacity
+    std::vector<uint8_t> key;  // what the cached graph renders
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    uint32_t* h_stage = nullptr;  // pinned staging for the scene tables
+    size_t h_stage_bytes = 0;
+};
+
+static void invalidate_batch_graph(sr_context ctx) {
+    Batch* B = ctx->batch;
+    if (!B) return;
+    if (B->exec) hipGraphExecDestroy(B->exec);
+    if (B->graph) hipGraphDestroy(B->graph);
+    B->exec = nullptr;
+    B->graph = nullptr;
+    B->key.clear();
+}
+
+static int set_batch_capacity(sr_context ctx, uint32_t pairs) {
+    if (!ctx->batch) ctx->batch = new (std::nothrow) Batch();
+    if (!ctx->batch) return SR_ERR_OUT_OF_MEMORY;
+    if (pairs > ctx->batch->cap) {
+        ctx->batch->cap = pairs;
+        invalidate_batch_graph(ctx);
+    }
+    return SR_OK;
+}
+
+static void destroy_batch(sr_context ctx) {
+    Batch* B = ctx->batch;
+    if (!B) return;
+    invalidate_batch_graph(ctx);
+    DevBuf* bufs[] = {&B->verts, &B->recs, &B->counts, &B->offsets, &B->keys, &B->vals, &B->keys2,
+                      &B->vals2, &B->tstart, &B->tend, &B->hist, &B->scan, &B->tables, &B->misc};
+    for (DevBuf* b : bufs)
+        if (b->p) hipFree(b->p);
+    if (B->h_stage) hipHostFree(B->h_stage);
+    delete B;
+    ctx->batch = nullptr;
+}
+
+// grows a batch buffer; a reallocation invalidates the cached graph
+static int bensure(sr_context ctx, DevBuf& b, size_t bytes) {
+    if (bytes <= b.bytes) return SR_OK;
+    invalidate_batch_graph(ctx);
+    return ensure(ctx, b, bytes);
+}
+
+static bool batchable(sr_mesh m, const sr_draw_desc* d) {
+    return m->prim == SR_PRIM_TRIANGLES && d->geometry_program == SR_GS_NONE &&
+           d->stencil_test == SR_STENCIL_ALWAYS && d->stencil_op == SR_STENCIL_KEEP &&
+           !(d->exec_flags & SR_EXEC_EXHAUSTIVE);
+}
+
+struct BatchScene {
+    uint32_t item;
+    uint32_t ets;
+    uint64_t cls;  // (ets, fp, blend)
+};
+
+static int batch_draw(sr_context ctx, const sr_batch_item* items, uint32_t n) {
+    int rc;
+    hipSetDevice(ctx->device);
+    ctx->n_stage = 0;
+    std::vector<BatchScene> fast;
+    std::vector<sr_framebuffer> seen;
+    for (uint32_t i = 0; i < n; i++) {
+        const sr_batch_item& it = items[i];
+        if ((rc = validate_draw(ctx, it.fb, it.mesh, it.desc, it.tex))) return rc;
+        for (sr_framebuffer f : seen)
+            if (f == it.fb) return fail(ctx, SR_ERR_INVALID_ARGUMENT, "batch: a framebuffer appears twice");
+        seen.push_back(it.fb);
+    }
+    for (uint32_t i = 0; i < n; i++) {
+        const sr_batch_item& it = items[i];
+        const sr_draw_desc* d = it.desc;
+        const uint32_t per = (uint32_t)it.mesh->prim;
+        const uint32_t nprim = (it.mesh->ni ? it.mesh->ni : it.mesh->nv) / per;
+        const bool drawable = it.fb->w > 1 && it.fb->h > 1 && nprim > 0 && d->stencil_test != SR_STENCIL_NEVER;
+        if (batchable(it.mesh, d)) {
+            const uint32_t ets = d->exec_tile ? (uint32_t)d->exec_tile : (uint32_t)policy_tile(ctx, it.fb->w, it.fb->h);
+            fast.push_back(BatchScene{i, ets, ((uint64_t)ets << 16) | ((uint64_t)d->fragment_program << 8) |
+                                                   (uint64_t)d->blend});
+            (void)drawable;
+        } else {
+            if (it.clear && (rc = sr_fb_clear(it.fb, it.clear_rgba, -3.40282347e+38f, 0))) return rc;
+            if ((rc = sr_draw(ctx, it.fb, it.mesh, d, it.tex))) return rc;
+        }
+    }
+    if (fast.empty()) return SR_OK;
+    std::stable_sort(fast.begin(), fast.end(), [](const BatchScene& a, const BatchScene& b) { return a.cls < b.cls; });
+    if (!ctx->batch && (rc = set_batch_capacity(ctx, 0))) return rc;
+    Batch* B = ctx->batch;
+    hipStream_t s = ctx->stream;
+
+    // ---- host tables
+    const uint32_t ns = (uint32_t)fast.size();
+    std::vector<SceneGeom> geom(ns);
+    std::vector<RasterParams> rp(ns);
+    std::vector<uint32_t> vend(ns), tend(ns), tile_end(ns);
+    struct Class { uint64_t cls; uint32_t ets, fp, blend, tile0, ntiles; };
+    std::vector<Class> classes;
+    uint64_t NV = 0, NT = 0, NTILES = 0;
+    for (uint32_t k = 0; k < ns; k++) {
+        const sr_batch_item& it = items[fast[k].item];
+        const sr_draw_desc* d = it.desc;
+        const sr_mesh m = it.mesh;
+        const uint32_t W = it.fb->w, H = it.fb->h, ets = fast[k].ets;
+        const uint32_t tw = d->tile_w > 0 ? (uint32_t)d->tile_w : 128u, th = d->tile_h > 0 ? (uint32_t)d->tile_h : 128u;
+        const bool drawable = W > 1 && H > 1;  // W or H == 1: no tiles (fragment.rs:196,201)
+        const uint32_t ntri = drawable ? (m->ni ? m->ni : m->nv) / 3 : 0;
+        SceneGeom& g = geom[k];
+        memset(&g, 0, sizeof(g));
+        g.px = m->soa[0]; g.py = m->soa[1]; g.pz = m->soa[2]; g.nx = m->soa[3]; g.ny = m->soa[4];
+        g.nz = m->soa[5]; g.u = m->soa[6]; g.v = m->soa[7]; g.idx = m->idx;
+        g.nv = ntri ? m->nv : 0; g.vbase = (uint32_t)NV;
+        g.nt = ntri; g.tbase = (uint32_t)NT;
+        g.program = d->vertex_program; g.K = varyings_of(d->vertex_program); g.cull = d->cull;
+        g.W = W; g.H = H; g.ets = ets; g.ntx = (W + ets - 1) / ets;
+        g.tile_base = (uint32_t)NTILES;
+        g.viewport = d->viewport;
+        memcpy(g.U, d->vs_uniforms, sizeof(g.U));
+        const uint32_t ntiles = g.ntx * ((H + ets - 1) / ets);
+        NV += g.nv; NT += g.nt; NTILES += ntiles;
+        vend[k] = (uint32_t)NV; tend[k] = (uint32_t)NT; tile_end[k] = (uint32_t)NTILES;
+        if (classes.empty() || classes.back().cls != fast[k].cls)
+            classes.push_back(Class{fast[k].cls, ets, (uint32_t)d->fragment_program, (uint32_t)d->blend,
+                                    g.tile_base, 0});
+        classes.back().ntiles += ntiles;
+    }
+    if (NV >= (1ull << 31) || NT >= (1ull << 31) || NTILES >= (1ull << 31))
+        return fail(ctx, SR_ERR_OUT_OF_MEMORY, "batch: too large");
+    if (B->cap == 0) B->cap = (uint32_t)std::min<uint64_t>(4 * NT + (1u << 16), 0x7FFFFFFFull);
+
+    for (int attempt = 0; attempt < 4; attempt++) {
+        const uint32_t cap = B->cap;
+        const uint32_t nvb = (uint32_t)std::max<uint64_t>(NV, 1);
+        if ((rc = bensure(ctx, B->verts, sizeof(float) * 10ull * nvb)) ||
+            (rc = bensure(ctx, B->recs, sizeof(TriRec) * std::max<uint64_t>(NT, 1))) ||
+            (rc = bensure(ctx, B->counts, sizeof(uint32_t) * (NT + 1))) ||
+            (rc = bensure(ctx, B->offsets, sizeof(uint32_t) * (NT + 1))) ||
+            (rc = bensure(ctx, B->keys, sizeof(uint32_t) * (size_t)cap)) ||
+            (rc = bensure(ctx, B->vals, sizeof(uint32_t) * (size_t)cap)) ||
+            (rc = bensure(ctx, B->keys2, sizeof(uint32_t) * (size_t)cap)) ||
+            (rc = bensure(ctx, B->vals2, sizeof(uint32_t) * (size_t)cap)) ||
+            (rc = bensure(ctx, B->tstart, sizeof(uint32_t) * std::max<uint64_t>(NTILES, 1))) ||
+            (rc = bensure(ctx, B->tend, sizeof(uint32_t) * std::max<uint64_t>(NTILES, 1))) ||
+            (rc = bensure(ctx, B->hist, sizeof(uint32_t) * (radix_hist_words(cap) + 256))) ||
+            (rc = bensure(ctx, B->scan, sizeof(uint32_t) * (scan_tmp_words(std::max<uint32_t>(radix_hist_words(cap) + 256, (uint32_t)NT + 1)) + 16))) ||
+            (rc = bensure(ctx, B->misc, sizeof(uint32_t) * (8 + 8 * classes.size()))))
+            return rc;
+        const size_t tbytes = sizeof(SceneGeom) * ns + sizeof(RasterParams) * ns + sizeof(uint32_t) * 3 * ns + 64;
+        if ((rc = bensure(ctx, B->tables, tbytes))) return rc;
+
+        // scene tables (device copies, pointers into the batch buffers)
+        uint8_t* tb = P<uint8_t>(B->tables);
+        SceneGeom* d_geom = (SceneGeom*)tb;
+        RasterParams* d_rp = (RasterParams*)(tb + sizeof(SceneGeom) * ns);
+        uint32_t* d_vend = (uint32_t*)(tb + sizeof(SceneGeom) * ns + sizeof(RasterParams) * ns);
+        uint32_t* d_tend = d_vend + ns;
+        uint32_t* d_tile_end = d_tend + ns;
+        uint32_t* misc = P<uint32_t>(B->misc);  // [0] overflow [1] pairs [2] err [3] scratch [8..) queues
+        const VertexBuf vb{P<float>(B->verts), nvb};
+        const int swapped_passes = (bits_for((uint32_t)std::max<uint64_t>(NTILES, 2)) + 7) / 8;
+        const uint32_t* sorted_vals = (swapped_passes & 1) ? P<uint32_t>(B->vals2) : P<uint32_t>(B->vals);
+        for (uint32_t k = 0; k < ns; k++) {
+            const sr_batch_item& it = items[fast[k].item];
+            const sr_draw_desc* d = it.desc;
+            RasterParams& R = rp[k];
+            memset(&R, 0, sizeof(R));
+            R.W = it.fb->w; R.H = it.fb->h;
+            R.color = it.fb->color; R.depth = it.fb->depth; R.stencil = it.fb->stencil;
+            R.tw = d->tile_w > 0 ? (uint32_t)d->tile_w : 128u;
+            R.th = d->tile_h > 0 ? (uint32_t)d->tile_h : 128u;
+            R.scols = R.W > 1 ? (R.W - 1 + R.tw - 1) / R.tw : 0;
+            R.srows = R.H > 1 ? (R.H - 1 + R.th - 1) / R.th : 0;
+            R.ets = geom[k].ets; R.ntx = geom[k].ntx; R.nty = (R.H + R.ets - 1) / R.ets;
+            R.tile_start = P<uint32_t>(B->tstart); R.tile_end = P<uint32_t>(B->tend);
+            R.items = sorted_vals;
+            R.tris = P<TriRec>(B->recs);
+            R.verts = vb;
+            R.K = geom[k].K;
+            R.fp = d->fragment_program; R.blend = d->blend; R.alpha_mode = d->alpha_mode;
+            R.s_test = SR_STENCIL_ALWAYS; R.s_op = SR_STENCIL_KEEP;
+            R.smode = 0;
+            if (it.tex) { R.tex = it.tex->d; R.texW = it.tex->w; R.texH = it.tex->h; }
+            R.tex_filter = d->tex_filter; R.tex_edge = d->tex_edge; R.tex_border = d->tex_border;
+            memcpy(R.fsu, d->fs_uniforms, sizeof(R.fsu));
+            R.tile_base = geom[k].tile_base;
+            R.clear = it.clear ? 1 : 0;
+            R.clear_rgba = it.clear_rgba;
+        }
+        // cache key: everything the captured sequence depends on
+        std::vector<uint8_t> key;
+        auto put = [&](const void* p, size_t b) {
+            const uint8_t* q = (const uint8_t*)p;
+            key.insert(key.end(), q, q + b);
+        };
+        put(&cap, sizeof(cap));
+        put(geom.data(), sizeof(SceneGeom) * ns);
+        put(rp.data(), sizeof(RasterParams) * ns);
+        const bool replay = B->exec && key == B->key;
+        if (!replay) {
+            invalidate_batch_graph(ctx);
+            if (B->h_stage_bytes < tbytes) {
+                if (B->h_stage) hipHostFree(B->h_stage);
+                B->h_stage = nullptr;
+                B->h_stage_bytes = 0;
+                if (hipHostMalloc((void**)&B->h_stage, tbytes, hipHostMallocDefault) != hipSuccess)
+                    return fail(ctx, SR_ERR_OUT_OF_MEMORY, "batch: staging alloc failed");
+                B->h_stage_bytes = tbytes;
+            }
+            uint8_t* hs = (uint8_t*)B->h_stage;
+            memcpy(hs, geom.data(), sizeof(SceneGeom) * ns);
+            memcpy(hs + sizeof(SceneGeom) * ns, rp.data(), sizeof(RasterParams) * ns);
+            uint32_t* hv = (uint32_t*)(hs + sizeof(SceneGeom) * ns + sizeof(RasterParams) * ns);
+            memcpy(hv, vend.data(), 4 * ns);
+            memcpy(hv + ns, tend.data(), 4 * ns);
+            memcpy(hv + 2 * ns, tile_end.data(), 4 * ns);
+            HIP_OK(ctx, hipMemcpyAsync(tb, hs, tbytes - 64, hipMemcpyHostToDevice, s));
+            HIP_OK(ctx, hipStreamSynchronize(s));
+
+            // ---- capture the batch
+            HIP_OK(ctx, hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+            BatchTables T{d_geom, d_vend, d_tend, ns, (uint32_t)NV, (uint32_t)NT};
+            launch_fill(misc, 0u, (uint32_t)(8 + 8 * classes.size()), s);
+            launch_vertex_batch(T, vb, s);
+            launch_tri_setup_batch(T, vb, P<TriRec>(B->recs), P<uint32_t>(B->counts), misc + 2, s);
+            launch_scan(P<uint32_t>(B->counts), P<uint32_t>(B->offsets), (uint32_t)NT, P<uint32_t>(B->scan), misc + 1, s);
+            launch_tri_emit_batch(T, P<TriRec>(B->recs), P<uint32_t>(B->counts), P<uint32_t>(B->offsets), cap,
+                                  P<uint32_t>(B->keys), P<uint32_t>(B->vals), misc + 0, s);
+            const int sw = radix_sort(P<uint32_t>(B->keys), P<uint32_t>(B->vals), P<uint32_t>(B->keys2),
+                                      P<uint32_t>(B->vals2), cap, bits_for((uint32_t)std::max<uint64_t>(NTILES, 2)),
+                                      P<uint32_t>(B->hist), P<uint32_t>(B->scan), misc + 3, s, misc + 1);
+            const uint32_t* skeys = sw ? P<uint32_t>(B->keys2) : P<uint32_t>(B->keys);
+            launch_fill(P<uint32_t>(B->tstart), 0u, (uint32_t)NTILES, s);
+            launch_fill(P<uint32_t>(B->tend), 0u, (uint32_t)NTILES, s);
+            launch_ranges(skeys, cap, 0u, P<uint32_t>(B->tstart), P<uint32_t>(B->tend), s, misc + 1);
+            for (size_t c = 0; c < classes.size(); c++) {
+                const Class& C = classes[c];
+                if (C.ntiles == 0) continue;
+                RasterBatch RB;
+                memset(&RB, 0, sizeof(RB));
+                RB.one.fp = (int)C.fp; RB.one.blend = (int)C.blend; RB.one.ets = C.ets; RB.one.smode = 0;
+                RB.scenes = d_rp;
+                RB.scene_tile_end = d_tile_end;
+                RB.nscenes = ns;
+                RB.tile0 = C.tile0;
+                RB.ntiles = C.ntiles;
+                RB.queue = misc + 8 + 8 * c;
+                RB.overflow = misc;
+                const uint32_t grid = std::min<uint32_t>(C.ntiles, C.ets == 8 ? 256u * 20u : 256u * 5u);
+                launch_raster_tris(RB, grid, s);
+            }
+            hipGraph_t g = nullptr;
+            hipError_t e = hipStreamEndCapture(s, &g);
+            if (e != hipSuccess) return fail(ctx, SR_ERR_DEVICE, std::string("batch capture: ") + hipGetErrorString(e));
+            B->graph = g;
+            HIP_OK(ctx, hipGraphInstantiate(&B->exec, g, nullptr, nullptr, 0));
+            B->key = key;
+        }
+        HIP_OK(ctx, hipEventRecord(ctx->ev[0], s));
+        HIP_OK(ctx, hipGraphLaunch(B->exec, s));
+        HIP_OK(ctx, hipEventRecord(ctx->ev[1], s));
+        HIP_OK(ctx, hipMemcpyAsync(ctx->h_misc, misc, 4 * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+        HIP_OK(ctx, hipStreamSynchronize(s));
+        float ms = 0.f;
+        hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[1]);
+        ctx->stage_ms[0] = ms;
+        ctx->n_stage = 1;
+        ctx->last.bin_pairs = ctx->h_misc[1];
+        ctx->last.triangles_in = NT;
+        if (ctx->h_misc[2]) return fail(ctx, SR_ERR_INVALID_COORDINATE, "batch: NaN screen coordinate");
+        if (!ctx->h_misc[0]) return SR_OK;
+        // binning overflowed: nothing was drawn; grow and run again
+        const uint64_t want = (uint64_t)ctx->h_misc[1] + ctx->h_misc[1] / 4 + 4096;
+        if (want >= 0x7FFFFFFFull) return fail(ctx, SR_ERR_OUT_OF_MEMORY, "batch: bin pairs exceed 2^31");
+        B->cap = (uint32_t)want;
+        invalidate_batch_graph(ctx);
+    }
+    return fail(ctx, SR_ERR_DEVICE, "batch: capacity did not converge");
+}
 
 // ===========================================================================
 // C ABI
@@ -328,6 +665,7 @@ int sr_ctx_destroy(sr_context ctx) {
         if (b->p) hipFree(b->p);
     for (auto& e : ctx->ev)
         if (e) hipEventDestroy(e);
+    destroy_batch(ctx);
     if (ctx->h_misc) hipHostFree(ctx->h_misc);
     hipStreamDestroy(ctx->stream);
     delete ctx;
@@ -504,25 +842,7 @@ int sr_fb_size(sr_framebuffer fb, uint32_t* w, uint32_t* h) {
 }
 
 int sr_draw(sr_context ctx, sr_framebuffer fb, sr_mesh mesh, const sr_draw_desc* d, sr_texture tex) {
-    if (!ctx || !fb || !mesh || !d) return fail(ctx, SR_ERR_INVALID_ARGUMENT, "draw: null argument");
-    if (fb->ctx != ctx || mesh->ctx != ctx) return fail(ctx, SR_ERR_INVALID_ARGUMENT, "draw: handle from another context");
-    if (d->vertex_program != SR_VP_MVP_NORMAL_UV && d->vertex_program != SR_VP_MVP_NORMAL_WORLD)
-        return fail(ctx, SR_ERR_INVALID_ARGUMENT, "draw: unknown vertex program");
-    if (d->geometry_program < SR_GS_NONE || d->geometry_program > SR_GS_CLIP_NEAR)
-        return fail(ctx, SR_ERR_INVALID_ARGUMENT, "draw: unknown geometry program");
-    if (d->fragment_program < SR_FP_FLAT || d->fragment_program > SR_FP_UV_COLOR)
-        return fail(ctx, SR_ERR_INVALID_ARGUMENT, "draw: unknown fragment program");
-    if (d->fragment_program == SR_FP_TEX_LAMBERT && !tex)
-        return fail(ctx, SR_ERR_INVALID_ARGUMENT, "draw: fragment program samples a texture");
-    if (d->fragment_program == SR_FP_BLINN_PHONG && d->vertex_program != SR_VP_MVP_NORMAL_WORLD)
-        return fail(ctx, SR_ERR_INVALID_ARGUMENT, "draw: Blinn-Phong needs world-space varyings");
-    if ((d->fragment_program == SR_FP_TEX_LAMBERT || d->fragment_program == SR_FP_UV_COLOR) &&
-        d->vertex_program != SR_VP_MVP_NORMAL_UV)
-        return fail(ctx, SR_ERR_INVALID_ARGUMENT, "draw: program needs uv varyings");
-    if (d->stencil_test < 0 || d->stencil_test > 7 || d->stencil_op < 0 || d->stencil_op > 5)
-        return fail(ctx, SR_ERR_INVALID_ARGUMENT, "draw: bad stencil config");
-    if (d->exec_tile != 0 && d->exec_tile != 8 && d->exec_tile != 16 && d->exec_tile != 32 && d->exec_tile != 64)
-        return fail(ctx, SR_ERR_INVALID_ARGUMENT, "draw: exec_tile must be 0, 8, 16, 32 or 64");
+    if (int rc = validate_draw(ctx, fb, mesh, d, tex)) return rc;
     hipSetDevice(ctx->device);
     memset(&ctx->last, 0, sizeof(ctx->last));
     ctx->n_stage = 0;
@@ -531,7 +851,7 @@ int sr_draw(sr_context ctx, sr_framebuffer fb, sr_mesh mesh, const sr_draw_desc*
     pl.W = fb->w; pl.H = fb->h;
     pl.tw = d->tile_w > 0 ? (uint32_t)d->tile_w : 128u;
     pl.th = d->tile_h > 0 ? (uint32_t)d->tile_h : 128u;
-    pl.ets = d->exec_tile ? (uint32_t)d->exec_tile : 16u;
+    pl.ets = d->exec_tile ? (uint32_t)d->exec_tile : (uint32_t)policy_tile(ctx, fb->w, fb->h);
     pl.ntx = (pl.W + pl.ets - 1) / pl.ets;
     pl.nty = (pl.H + pl.ets - 1) / pl.ets;
     pl.ntiles = pl.ntx * pl.nty;
@@ -631,11 +951,27 @@ int sr_draw(sr_context ctx, sr_framebuffer fb, sr_mesh mesh, const sr_draw_desc*
 
 int sr_batch_draw(sr_context ctx, const sr_batch_item* items, uint32_t n) {
     if (!ctx || (!items && n)) return SR_ERR_INVALID_ARGUMENT;
+    return batch_draw(ctx, items, n);
+}
+
+int sr_ctx_set_exec_tile_policy(sr_context ctx, uint32_t n, const uint64_t* max_pixels,
+                                const int32_t* exec_tile) {
+    if (!ctx || !n || !max_pixels || !exec_tile) return SR_ERR_INVALID_ARGUMENT;
+    std::vector<std::pair<uint64_t, int>> p;
     for (uint32_t i = 0; i < n; i++) {
-        int rc = sr_draw(ctx, items[i].fb, items[i].mesh, items[i].desc, items[i].tex);
-        if (rc) return rc;
+        const int e = exec_tile[i];
+        if (e != 8 && e != 16 && e != 32 && e != 64)
+            return fail(ctx, SR_ERR_INVALID_ARGUMENT, "policy: exec tile must be 8, 16, 32 or 64");
+        p.emplace_back(max_pixels[i], e);
     }
+    ctx->tile_policy = p;
+    if (ctx->batch) invalidate_batch_graph(ctx);
     return SR_OK;
+}
+
+int sr_ctx_set_batch_capacity(sr_context ctx, uint32_t pairs) {
+    if (!ctx) return SR_ERR_INVALID_ARGUMENT;
+    return set_batch_capacity(ctx, pairs);
 }
 
 }  // extern "C"

@@ -27,6 +27,7 @@ EXPORTED = [
     "sr_ctx_stage_times", "sr_ctx_last_stats", "sr_mesh_upload", "sr_mesh_free",
     "sr_texture_upload", "sr_texture_free", "sr_fb_create", "sr_fb_free", "sr_fb_clear",
     "sr_fb_read", "sr_fb_write", "sr_fb_size", "sr_draw", "sr_batch_draw",
+    "sr_ctx_set_exec_tile_policy", "sr_ctx_set_batch_capacity",
 ]
 
 
@@ -60,7 +61,7 @@ class DrawStats(C.Structure):
 
 class BatchItem(C.Structure):
     _fields_ = [("fb", C.c_void_p), ("mesh", C.c_void_p), ("desc", C.POINTER(DrawDesc)),
-                ("tex", C.c_void_p)]
+                ("tex", C.c_void_p), ("clear", C.c_int32), ("clear_rgba", C.c_uint32)]
 
 
 def make_desc(**kw) -> DrawDesc:
@@ -132,6 +133,9 @@ def lib():
         L.sr_fb_size.argtypes = [vp, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]
         L.sr_draw.argtypes = [vp, vp, vp, C.POINTER(DrawDesc), vp]
         L.sr_batch_draw.argtypes = [vp, C.POINTER(BatchItem), C.c_uint32]
+        L.sr_ctx_set_exec_tile_policy.argtypes = [vp, C.c_uint32, C.POINTER(C.c_uint64),
+                                                  C.POINTER(C.c_int32)]
+        L.sr_ctx_set_batch_capacity.argtypes = [vp, C.c_uint32]
         for name in EXPORTED:
             f = getattr(L, name)
             if f.restype is None and name not in ("sr_last_error", "sr_ctx_stream"):
@@ -202,13 +206,25 @@ class Context:
         self._check(lib().sr_draw(self.h, fb.h, mesh.h, C.byref(desc), tex.h if tex else None))
 
     def batch_draw(self, items):
-        """items: list of (fb, mesh, desc, tex)."""
+        """items: list of (fb, mesh, desc, tex[, clear_rgba or None])."""
         arr = (BatchItem * len(items))()
         keep = []
-        for i, (fb, mesh, desc, tex) in enumerate(items):
+        for i, it in enumerate(items):
+            fb, mesh, desc, tex = it[:4]
+            clear = it[4] if len(it) > 4 else None
             keep.append(desc)
-            arr[i] = BatchItem(fb.h, mesh.h, C.pointer(desc), tex.h if tex else None)
+            arr[i] = BatchItem(fb.h, mesh.h, C.pointer(desc), tex.h if tex else None,
+                               0 if clear is None else 1, 0 if clear is None else clear)
         self._check(lib().sr_batch_draw(self.h, arr, len(items)))
+
+    def set_exec_tile_policy(self, entries):
+        """entries: [(max_pixels, exec_tile), ...] (first match wins)."""
+        mp = (C.c_uint64 * len(entries))(*[int(e[0]) for e in entries])
+        et = (C.c_int32 * len(entries))(*[int(e[1]) for e in entries])
+        self._check(lib().sr_ctx_set_exec_tile_policy(self.h, len(entries), mp, et))
+
+    def set_batch_capacity(self, pairs: int):
+        self._check(lib().sr_ctx_set_batch_capacity(self.h, pairs))
 
 
 class Mesh:

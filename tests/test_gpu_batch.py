@@ -1,0 +1,117 @@
+"""Batched rendering (sr_batch_draw): every framebuffer of the batch equals
+that scene rendered alone by the oracle, bit for bit — across graph replays,
+capacity overflow, mixed classes and items that fall back to sr_draw."""
+import numpy as np
+import pytest
+
+import scenes
+import softrender_hip as sr
+from parity import assert_parity, render_ref
+
+pytestmark = pytest.mark.gpu
+
+
+def make_items(ctx, specs, tex):
+    items, keep = [], []
+    for s in specs:
+        fb = ctx.framebuffer(s["width"], s["height"])
+        fb.clear(0xFF00FF00)  # a stale frame the fused clear must erase
+        mesh = ctx.mesh(s)
+        desc = sr.make_desc(**s["desc"])
+        keep.append((fb, mesh, desc))
+        items.append((fb, mesh, desc, tex if s.get("texture") else None, s.get("clear", 0xFF000000)))
+    return items, keep
+
+
+def check_all(items, specs):
+    for (fb, mesh, desc, tex, clear), s in zip(items, specs):
+        ref, _ = render_ref(dict(s, clear=clear))
+        assert_parity(fb.read(), ref)
+
+
+def mix(seed=50, n=10):
+    out = []
+    rng = scenes.SplitMix64(seed)
+    res = [(256, 256), (512, 512), (320, 180), (97, 61), (640, 360)]
+    for i in range(n):
+        w, h = res[i % len(res)]
+        nt = int(200 + 4000 * rng.uniform(1)[0])
+        s = scenes.soup_scene(nt, w, h, seed=seed + i)
+        if i % 3 == 1:
+            s["desc"]["blend"] = 0
+        if i % 4 == 2:
+            s["desc"]["exec_tile"] = 16
+        out.append(s)
+    return out
+
+
+def test_batch_matches_oracle(ctx):
+    specs = mix()
+    tw, th, rgba = specs[0]["texture"]
+    tex = ctx.texture(tw, th, rgba)
+    items, keep = make_items(ctx, specs, tex)
+    ctx.batch_draw(items)
+    check_all(items, specs)
+    # replay (the cached graph) after dirtying the framebuffers
+    for fb, *_ in items:
+        fb.clear(0xFF123456)
+    ctx.batch_draw(items)
+    ctx.batch_draw(items)
+    check_all(items, specs)
+
+
+def test_batch_capacity_overflow_regrows(ctx):
+    specs = mix(seed=60, n=6)
+    tw, th, rgba = specs[0]["texture"]
+    tex = ctx.texture(tw, th, rgba)
+    c2 = sr.Context(0)
+    try:
+        tex2 = c2.texture(tw, th, rgba)
+        c2.set_batch_capacity(64)
+        items, keep = make_items(c2, specs, tex2)
+        c2.batch_draw(items)
+        assert c2.last_stats()["bin_pairs"] > 64
+        check_all(items, specs)
+    finally:
+        c2.close()
+
+
+def test_batch_alone_equals_in_the_mix(ctx):
+    specs = mix(seed=70, n=8)
+    tw, th, rgba = specs[0]["texture"]
+    tex = ctx.texture(tw, th, rgba)
+    items, keep = make_items(ctx, specs, tex)
+    ctx.batch_draw(items)
+    mixed = [fb.read() for fb, *_ in items]
+    for k, s in enumerate(specs):
+        alone, keep1 = make_items(ctx, [s], tex)
+        ctx.batch_draw(alone)
+        a = alone[0][0].read()
+        for x, y in zip(a, mixed[k]):
+            assert np.array_equal(x, y)
+
+
+def test_batch_with_fallback_items(ctx):
+    tri = scenes.soup_scene(1500, 200, 120, seed=80)
+    sphere = scenes.sphere_scene(150, 150)  # Blinn-Phong, culled
+    stencil = scenes.soup_scene(800, 128, 96, seed=81)
+    stencil["desc"].update(stencil_test=0, stencil_op=4, stencil_wrap=0)  # falls back to sr_draw
+    clip = scenes.soup_scene(800, 160, 90, seed=82)
+    clip["desc"]["geometry_program"] = 2  # falls back
+    thin = scenes.soup_scene(50, 1, 40, seed=83)  # no tiles at all
+    specs = [tri, sphere, stencil, clip, thin]
+    tw, th, rgba = tri["texture"]
+    tex = ctx.texture(tw, th, rgba)
+    items, keep = make_items(ctx, specs, tex)
+    ctx.batch_draw(items)
+    check_all(items, specs)
+
+
+def test_batch_rejects_repeated_framebuffer(ctx):
+    s = scenes.soup_scene(100, 64, 64, seed=84)
+    tex = ctx.texture(*s["texture"])
+    fb = ctx.framebuffer(64, 64)
+    mesh = ctx.mesh(s)
+    d = sr.make_desc(**s["desc"])
+    with pytest.raises(sr.SoftrenderError):
+        ctx.batch_draw([(fb, mesh, d, tex, 0), (fb, mesh, d, tex, 0)])
