@@ -28,7 +28,28 @@ FP32_PEAK_TFLOPS = 157.3
 # expressions; a divide or sqrt counts as one operation).
 FLOPS_PER_TEST = 14      # triangle.rs:104-113: x,y (2), edge numerators (8), 2 divides, w (2)
 FLOPS_PER_COVERED = 5    # z = ((z1*u)+(z2*v))+(z3*w)
-FLOPS_PER_SHADED = 145   # 5 varyings x 5, tex+Lambert program, over-blend
+FLOPS_PER_SHADED = 145   # 5 varyings x 5 (interpolation), bilinear sample + tex x Lambert, over-blend
+
+# Raster work units of the default workload (1920x1080, 1M-triangle soup,
+# seed 0x5EED0002) as the oracle counts them: pixel tests inside the
+# tile-clamped boxes, covered pixels, shaded fragments.  The CPU leg
+# recounts them and checks these.
+DEFAULT_UNITS = {"bbox_tests": 86093801, "covered": 33448710, "shaded": 6997987}
+
+# HBM bytes per k_raster_tris launch from the rocprofv3 FETCH_SIZE /
+# WRITE_SIZE summary of this workload (scaled by the copy-kernel factor).
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_raster_tris_soup1080p_1M.md")
+
+
+def profiled_traffic(exec_tile):
+    try:
+        with open(TRAFFIC_FILE) as f:
+            for line in f:
+                if line.startswith(f"traffic_bytes_per_launch_ets{exec_tile}:"):
+                    return int(line.split(":", 1)[1].strip())
+    except OSError:
+        pass
+    return None
 
 
 def parse():
@@ -40,7 +61,38 @@ def parse():
     ap.add_argument("--exec-tile", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--no-extras", action="store_true",
+                    help="skip the batched (256 scenes) and serving (Poisson) measurements")
+    ap.add_argument("--serve-seconds", type=float, default=2.0, help="seconds per offered rate")
+    ap.add_argument("--copy-calib", type=int, default=0, metavar="MIB",
+                    help="only run device-to-device copies of MIB MiB (calibrates PMC byte counters)")
     return ap.parse_args()
+
+
+def copy_calibration(mib):
+    """Five device-to-device hipMemcpy of `mib` MiB (a blit copy kernel that
+    reads and writes exactly that many bytes), for scaling FETCH_SIZE and
+    WRITE_SIZE into bytes."""
+    import ctypes as C
+    hip = None
+    for name in ("libamdhip64.so", "/opt/rocm/lib/libamdhip64.so"):
+        try:
+            hip = C.CDLL(name)
+            break
+        except OSError:
+            continue
+    n = mib << 20
+    a, b = C.c_void_p(), C.c_void_p()
+    hip.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
+    hip.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+    hip.hipFree.argtypes = [C.c_void_p]
+    assert hip.hipMalloc(C.byref(a), n) == 0 and hip.hipMalloc(C.byref(b), n) == 0
+    for _ in range(5):
+        assert hip.hipMemcpy(b, a, n, 3) == 0  # hipMemcpyDeviceToDevice
+    assert hip.hipDeviceSynchronize() == 0
+    hip.hipFree(a)
+    hip.hipFree(b)
+    print(json.dumps({"copy_calibration_bytes": n, "copies": 5}), flush=True)
 
 
 def cpu_cores():
@@ -56,8 +108,11 @@ def main():
     a = parse()
     if a.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {a.gpus} is not supported "
-              "(multi-GPU sharding is design-only, see DESIGN.md)", file=sys.stderr)
+              "(sharding across GPUs is designed but not built)", file=sys.stderr)
         sys.exit(2)
+    if a.copy_calib:
+        copy_calibration(a.copy_calib)
+        return
     import numpy as np
 
     import scenes
@@ -79,24 +134,22 @@ def main():
         fb.clear(scene["clear"])
         ctx.draw(fb, mesh, d, tex)
 
+    # Timed region: every step records HIP events between the pipeline
+    # stages on the context stream, so the raster kernel's duration is
+    # measured on the very launches that are timed.
     for _ in range(a.warmup):
-        step(desc)
+        step(tdesc)
     ctx.finish()
+    stages = {}
     t0 = time.perf_counter()
     for _ in range(a.steps):
-        step(desc)
+        step(tdesc)
+        for k, v in ctx.stage_times().items():
+            stages[k] = stages.get(k, 0.0) + v / a.steps
     ctx.finish()
     dt = time.perf_counter() - t0
     ms = 1000.0 * dt / a.steps
     fps = a.steps / dt
-
-    # per-stage device times (HIP events on the context stream)
-    stages = {}
-    n_t = max(3, min(10, a.steps))
-    for _ in range(n_t):
-        step(tdesc)
-        for k, v in ctx.stage_times().items():
-            stages[k] = stages.get(k, 0.0) + v / n_t
     stats = ctx.last_stats()
 
     out = {
@@ -108,7 +161,7 @@ def main():
                                "log-normal area (median 20 px), bilinear checkerboard, "
                                "20% over-blend, screen tile 128",
                    "resolution": [1920, 1080], "triangles": a.tris,
-                   "exec_tile": desc.exec_tile or 16},
+                   "exec_tile": desc.exec_tile or 8},
         "mtris_per_s": round(fps * a.tris / 1e6, 2),
         "stage_ms": {k: round(v, 4) for k, v in stages.items()},
         "bin_pairs": stats.get("bin_pairs"),
@@ -127,7 +180,8 @@ def main():
         out["roofline_sort"] = {"bound": "hbm", "achieved": round(sort_bytes / stages["sort"] / 1e6, 1),
                                 "peak": HBM_PEAK_GBS, "unit": "GB/s"}
 
-    cpu = None
+    default_workload = a.tris == 1_000_000
+    units = dict(DEFAULT_UNITS) if default_workload else None
     if not a.no_cpu:
         # oracle as the CPU baseline: one full frame, reference scheduling
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
@@ -138,23 +192,60 @@ def main():
         t0 = time.perf_counter()
         rc, st = oracle.draw(ofb, scene, desc, scene["texture"], n)
         cdt = time.perf_counter() - t0
-        cpu = {"value": round(1.0 / cdt, 4), "unit": "frames/s", "cores": n, "kind": "port",
-               "sample": f"1 full frame of the same workload, {n} threads, tile queue "
-                         f"(every 128x128 tile visits all {ntris} triangles); {cdt:.2f} s"}
-        out["cpu_baseline"] = cpu
-        inbox = st["bbox_tests"] - st["strip_tests"]
-        flops = FLOPS_PER_TEST * inbox + FLOPS_PER_COVERED * st["covered"] + FLOPS_PER_SHADED * st["shaded"]
-        out["raster_units"] = {"bbox_tests": inbox, "covered": st["covered"], "shaded": st["shaded"]}
-        if stages.get("raster"):
-            ach = flops / (stages["raster"] * 1e-3) / 1e12
-            out["roofline"] = {"bound": "valu", "achieved": round(ach, 3), "peak": FP32_PEAK_TFLOPS,
-                               "unit": "TFLOP/s", "frac": round(ach / FP32_PEAK_TFLOPS, 4),
-                               "traffic": None, "kernel": "k_raster_tris"}
-    print(json.dumps(out), flush=True)
+        out["cpu_baseline"] = {
+            "value": round(1.0 / cdt, 4), "unit": "frames/s", "cores": n, "kind": "port",
+            "sample": f"1 full frame of the same workload, {n} threads, tile queue "
+                      f"(every 128x128 tile visits all {ntris} triangles); {cdt:.2f} s"}
+        counted = {"bbox_tests": st["bbox_tests"] - st["strip_tests"], "covered": st["covered"],
+                   "shaded": st["shaded"]}
+        if units is not None and counted != units:
+            raise SystemExit(f"bench.py: raster units changed: {counted} != {units}")
+        units = counted
+    if units and stages.get("raster"):
+        flops = (FLOPS_PER_TEST * units["bbox_tests"] + FLOPS_PER_COVERED * units["covered"]
+                 + FLOPS_PER_SHADED * units["shaded"])
+        ach = flops / (stages["raster"] * 1e-3) / 1e12
+        ets = desc.exec_tile or 8
+        out["raster_units"] = units
+        out["roofline"] = {"bound": "valu", "achieved": round(ach, 3), "peak": FP32_PEAK_TFLOPS,
+                           "unit": "TFLOP/s", "frac": round(ach / FP32_PEAK_TFLOPS, 4),
+                           "traffic": profiled_traffic(ets) if default_workload else None,
+                           "kernel": "k_raster_tris", "avg_launch_ms": round(stages["raster"], 4)}
     mesh.free()
     tex.free()
     fb.free()
     ctx.close()
+    if not a.no_extras:
+        out.update(extras(a))
+    print(json.dumps(out), flush=True)
+
+
+def extras(a):
+    """The batched path (256 scenes of uneven size in one launch vs one
+    launch per scene) and the request server under Poisson load (p50/p99
+    latency, frames/s) against the plain per-request path."""
+    import batchbench
+    import serve
+
+    t = time.time()
+    b = batchbench.run(256, reps=3)
+    batch = {k: (round(v, 4) if isinstance(v, float) else v) for k, v in b.items()}
+    batch["wall_s"] = round(time.time() - t, 1)
+    t = time.time()
+    sv = serve.run(rates=(200, 1000, 2000), duration=a.serve_seconds, capacity_n=384)
+    keep = ("fps", "p50_ms", "p99_ms", "max_ms", "mean_batch", "completed", "truncated")
+    srv = {"meshes": sv["meshes"], "mean_mesh_tris": round(sv["mean_mesh_tris"]),
+           "capacity_served_fps": round(sv["capacity_served"]["fps"], 1),
+           "capacity_plain_fps": round(sv["capacity_plain"]["fps"], 1),
+           "capacity_ratio": round(sv["capacity_ratio"], 2), "rates": {}}
+    for r, row in sv["rates"].items():
+        srv["rates"][r] = {
+            "served": {k: round(v, 2) if isinstance(v, float) else v
+                       for k, v in row["served"].items() if k in keep},
+            "plain": {k: round(v, 2) if isinstance(v, float) else v
+                      for k, v in row["plain"].items() if k in keep}}
+    srv["wall_s"] = round(time.time() - t, 1)
+    return {"batch": batch, "serve": srv}
 
 
 if __name__ == "__main__":

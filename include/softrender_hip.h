@@ -49,6 +49,7 @@ extern "C" {
 #define SR_ERR_BUDGET (-6)             /* serving: request does not fit the HBM budget */
 #define SR_ERR_NOT_READY (-7)          /* serving: ticket still in flight */
 #define SR_ERR_NO_DEVICE (-8)
+#define SR_ERR_CANCELLED (-9)          /* serving: the server was destroyed first */
 
 /* ---- enums (values are part of the ABI) -------------------------------- */
 /* primitive.rs:7-167: zero-sized type tags selecting index assembly */
@@ -244,6 +245,71 @@ int sr_ctx_set_exec_tile_policy(sr_context ctx, uint32_t n, const uint64_t* max_
 /* Initial (execution tile, triangle) pair capacity of the batch path; it
  * grows by itself when a batch overflows it. */
 int sr_ctx_set_batch_capacity(sr_context ctx, uint32_t pairs);
+
+/* ---- serving: a request loop with continuous batching -------------------
+ * No counterpart in the reference (it renders one frame per call on the
+ * calling thread, pipeline/mod.rs:115-155); this is the serving layer built
+ * over the same draw.  Meshes and textures stay resident in the context
+ * that created them; the server owns a pool of device framebuffers and of
+ * pinned host result buffers.  A dispatcher thread admits pending requests
+ * in arrival order while their framebuffers plus bin workspaces fit the HBM
+ * budget, and whenever one of `max_inflight` batch slots is free it renders
+ * everything admitted so far as one batch on that slot's stream, so work
+ * that arrives while other batches run joins the next one and overlaps on
+ * the device.  Every result equals an sr_fb_clear + sr_draw of the same
+ * request on its own.  submit / poll / release / stats are thread-safe. */
+typedef struct sr_server_config {
+    uint64_t hbm_budget;       /* bytes of framebuffers + bin workspaces in flight; 0: 16 GiB */
+    uint32_t max_inflight;     /* batches on the device at once, one stream each; 0: 2 */
+    uint32_t max_batch_frames; /* requests per batch; 0: 256 */
+    uint64_t max_batch_tris;   /* triangles per batch (a larger request goes alone); 0: 32M */
+    float pairs_per_tri;       /* bin pairs per triangle used to size workspaces; 0: 4 */
+    int32_t reserved[7];
+} sr_server_config;
+
+typedef struct sr_request {
+    sr_mesh mesh;              /* triangles; from the server's context */
+    sr_texture tex;            /* NULL unless the fragment program samples */
+    sr_draw_desc desc;         /* copied; triangle draw without geometry program, stencil Always/Keep */
+    uint32_t width, height;    /* framebuffer size */
+    uint32_t clear_rgba;       /* the frame starts as Framebuffer::clear(clear_rgba) */
+    int32_t want_depth;        /* non-zero: the depth plane is returned as well */
+    uint64_t tag;              /* caller's id, returned in the completion */
+    double arrival;            /* sr_server_clock() time of arrival; 0: the submit time */
+} sr_request;
+
+typedef struct sr_completion {
+    uint64_t ticket, tag;
+    int32_t status;            /* SR_OK or SR_ERR_* */
+    uint32_t width, height;
+    uint32_t batch_frames;     /* requests in the batch that drew this one */
+    uint64_t batch_id;
+    const uint32_t* rgba;      /* pinned host memory, valid until sr_server_release(ticket) */
+    const float* depth;        /* NULL unless want_depth */
+    double t_arrival, t_admit, t_done; /* sr_server_clock() seconds; t_done = pixels on the host */
+} sr_completion;
+
+typedef struct sr_server_stats {
+    uint64_t submitted, completed, failed, batches, overflow_reruns;
+    uint64_t budget_bytes, in_flight_bytes, peak_in_flight_bytes, device_bytes;
+    uint32_t pending, inflight_batches, max_batch_frames_seen, reserved;
+} sr_server_stats;
+
+int sr_server_create(sr_context ctx, const sr_server_config* cfg, sr_server* out);
+/* Waits for batches on the device; requests not yet admitted complete with
+ * SR_ERR_CANCELLED.  Result pointers die with the server. */
+int sr_server_destroy(sr_server srv);
+/* Queues one request (never blocks on the device).  SR_ERR_BUDGET when the
+ * request alone exceeds the budget. */
+int sr_server_submit(sr_server srv, const sr_request* req, uint64_t* ticket);
+/* Up to `max` completions; waits up to timeout_ms (0: no wait, <0: forever)
+ * for the first one. */
+int sr_server_poll(sr_server srv, sr_completion* out, uint32_t max, uint32_t* n, int32_t timeout_ms);
+/* Returns a completion's host buffers to the pool. */
+int sr_server_release(sr_server srv, uint64_t ticket);
+int sr_server_get_stats(sr_server srv, sr_server_stats* out);
+const char* sr_server_last_error(sr_server srv);
+double sr_server_clock(void); /* monotonic seconds */
 
 #ifdef __cplusplus
 }

@@ -77,7 +77,6 @@ struct RasterParams {
     const uint32_t* tile_start;
     const uint32_t* tile_end;
     const uint32_t* items;  // sorted item ids per execution tile
-    uint32_t exhaustive_n;  // >0: every tile walks items 0..n-1 (no bins)
     // primitives
     const TriRec* tris;
     const LineItem* line_items;

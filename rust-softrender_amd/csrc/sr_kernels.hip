@@ -573,6 +573,11 @@ __global__ __launch_bounds__(256) void k_fill_u32(uint32_t* p, uint32_t v, uint3
     if (i < n) p[i] = v;
 }
 
+__global__ __launch_bounds__(256) void k_iota_u32(uint32_t* p, uint32_t n) {
+    const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+    if (i < n) p[i] = i;
+}
+
 __global__ __launch_bounds__(256) void k_ranges(const uint32_t* keys, uint32_t n, uint32_t key_base,
                                                uint32_t* start, uint32_t* end, const uint32_t* n_dev) {
     if (n_dev) n = min(n, *n_dev);
@@ -1097,6 +1102,10 @@ int radix_sort(uint32_t* keys, uint32_t* vals, uint32_t* keys_tmp, uint32_t* val
 
 void launch_fill(uint32_t* p, uint32_t v, uint32_t n, hipStream_t s) {
     if (n) hipLaunchKernelGGL(k_fill_u32, dim3(blocks_for(n, 256)), dim3(256), 0, s, p, v, n);
+}
+
+void launch_iota(uint32_t* p, uint32_t n, hipStream_t s) {
+    if (n) hipLaunchKernelGGL(k_iota_u32, dim3(blocks_for(n, 256)), dim3(256), 0, s, p, n);
 }
 
 void launch_ranges(const uint32_t* keys, uint32_t n, uint32_t key_base, uint32_t* start,

@@ -149,6 +149,7 @@ int radix_sort(uint32_t* keys, uint32_t* vals, uint32_t* keys_tmp, uint32_t* val
                int key_bits, uint32_t* hist, uint32_t* scan_tmp, uint32_t* total, hipStream_t s,
                const uint32_t* n_dev = nullptr);
 void launch_fill(uint32_t* p, uint32_t v, uint32_t n, hipStream_t s);
+void launch_iota(uint32_t* p, uint32_t n, hipStream_t s);  // p[i] = i
 void launch_ranges(const uint32_t* keys, uint32_t n, uint32_t key_base, uint32_t* start,
                    uint32_t* end, hipStream_t s, const uint32_t* n_dev = nullptr);
 void launch_fb_clear(uint32_t* color, float* depth, uint8_t* stencil, uint32_t n, uint32_t rgba,

@@ -40,6 +40,78 @@ __device__ __forceinline__ uint32_t load_item(const uint32_t* base, uint32_t i) 
 #endif
 }
 
+// 64 consecutive item ids, lane k holding the id at position c0 + k, the
+// position clamped into [beg, end) (beg < end) so every lane loads.
+__device__ __forceinline__ uint32_t load_ids(const uint32_t* items, uint32_t c0, uint32_t beg,
+                                             uint32_t end, int lane) {
+    const uint32_t c = min(max(c0 + (uint32_t)lane, beg), end - 1);
+#if defined(__HIP_DEVICE_COMPILE__)
+    return ((const __attribute__((address_space(1))) uint32_t*)items)[c];
+#else
+    return items[c];
+#endif
+}
+
+// A block of 64 triangle records, lane k holding the first 60 bytes of
+// record k (R, x3, y3, edges, z, bbox).
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+typedef uint32_t u32x3 __attribute__((ext_vector_type(3)));
+struct RecBlock {
+    u32x4 a, b, c;
+    u32x3 d;
+};
+
+__device__ __forceinline__ RecBlock load_block(const TriRec* tris, uint32_t id) {
+    RecBlock r;
+#if defined(__HIP_DEVICE_COMPILE__)
+    typedef const __attribute__((address_space(1))) u32x4 G4;
+    typedef const __attribute__((address_space(1))) u32x3 G3;
+#else
+    typedef const u32x4 G4;
+    typedef const u32x3 G3;
+#endif
+    const char* base = (const char*)(tris + id);
+    r.a = *(G4*)(base);
+    r.b = *(G4*)(base + 16);
+    r.c = *(G4*)(base + 32);
+    r.d = *(G3*)(base + 48);
+    return r;
+}
+
+// The fields of record k of a block, in SGPRs (k is wave-uniform).
+struct RecLane {
+    double R;
+    float x3, y3, e23y, e23x, e31y, e31x, z1, z2, z3;
+    int X0, X1, Y0, Y1;
+};
+
+__device__ __forceinline__ uint32_t lane_u32(uint32_t v, int k) {
+    return (uint32_t)__builtin_amdgcn_readlane((int)v, k);
+}
+__device__ __forceinline__ float lane_f32(uint32_t v, int k) {
+    return __builtin_bit_cast(float, lane_u32(v, k));
+}
+
+__device__ __forceinline__ RecLane rec_at(const RecBlock& b, int k) {
+    RecLane r;
+    const uint64_t lo = lane_u32(b.a.x, k), hi = lane_u32(b.a.y, k);
+    r.R = __builtin_bit_cast(double, lo | (hi << 32));
+    r.x3 = lane_f32(b.a.z, k);
+    r.y3 = lane_f32(b.a.w, k);
+    r.e23y = lane_f32(b.b.x, k);
+    r.e23x = lane_f32(b.b.y, k);
+    r.e31y = lane_f32(b.b.z, k);
+    r.e31x = lane_f32(b.b.w, k);
+    r.z1 = lane_f32(b.c.x, k);
+    r.z2 = lane_f32(b.c.y, k);
+    r.z3 = lane_f32(b.c.z, k);
+    r.X0 = (int)lane_u32(b.c.w, k);
+    r.X1 = (int)lane_u32(b.d.x, k);
+    r.Y0 = (int)lane_u32(b.d.y, k);
+    r.Y1 = (int)lane_u32(b.d.z, k);
+    return r;
+}
+
 __device__ __forceinline__ TriRec load_rec(const TriRec* base, uint32_t i) {
 #if defined(__HIP_DEVICE_COMPILE__)
     const __attribute__((address_space(4))) TriRec* p =
@@ -88,7 +160,7 @@ __device__ __forceinline__ uint32_t wave_excl_scan(uint32_t v, uint32_t& total) 
         const uint32_t y = __shfl_up(x, o, 64);
         if (lane >= o) x += y;
     }
-    total = __shfl(x, 63, 64);
+    total = __builtin_amdgcn_readlane(x, 63);
     return x - v;
 }
 
@@ -133,17 +205,13 @@ __device__ __forceinline__ void raster_tile(const RasterParams& P, uint32_t tile
     constexpr int S = SH::S, PPT = SH::PPT, Q = SH::Q;
     constexpr int RPP = 64 / S;  // quadrant rows covered by one pixel pass
 
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     const uint32_t etx = tile % P.ntx, ety = tile / P.ntx;
-    const bool exh = P.exhaustive_n != 0;
-    uint32_t beg, end;
-    if (exh) {
-        beg = 0;
-        end = P.exhaustive_n;
-    } else {
-        beg = P.tile_start[gtile];
-        end = P.tile_end[gtile];
-    }
+    // constant-space loads: a flat load counts as divergent, which would turn
+    // the whole triangle loop into per-lane control flow
+    const uint32_t beg = load_item(P.tile_start, gtile);
+    const uint32_t end = load_item(P.tile_end, gtile);
     if (beg >= end && SMODE != 1 && !P.clear) return;
     const Lut lut{sh.lut};
 
@@ -230,17 +298,26 @@ __device__ __forceinline__ void raster_tile(const RasterParams& P, uint32_t tile
                 for (int j = 0; j < PPT; j++)
                     inT[j] = live[j] && px[j] >= Tx0 && px[j] <= Tx1 && py[j] >= Ty0 && py[j] <= Ty1;
 
-                // walk the tile's triangles in order; records arrive through the
-                // scalar cache one iteration ahead of their use
-                uint32_t item = exh ? beg : load_item(P.items, beg);
-                TriRec r = load_rec(P.tris, item);
-                for (uint32_t c = beg; c < end; c++) {
-                    const uint32_t cur = item;
-                    const TriRec rc = r;
-                    if (c + 1 < end) {
-                        item = exh ? c + 1 : load_item(P.items, c + 1);
-                        r = load_rec(P.tris, item);
-                    }
+                // walk the tile's triangles in order, 64 at a time: lane k
+                // gathers the record of item c0 + k one block ahead of its use
+                // (item ids two blocks ahead) and the loop hands record fields
+                // to the whole wave with v_readlane, so the random record
+                // fetches overlap instead of stalling each triangle.  Loads
+                // are unconditional, with positions clamped into [beg, end),
+                // so no loaded register is copied (which would wait for it).
+                const uint32_t b0 = beg & ~63u;
+                uint32_t id = load_ids(P.items, b0, beg, end, lane);
+                RecBlock blk = load_block(P.tris, id);
+                uint32_t idn = load_ids(P.items, b0 + 64, beg, end, lane);
+                for (uint32_t c0 = b0; c0 < end; c0 += 64) {
+                    const RecBlock nxt = load_block(P.tris, idn);
+                    const uint32_t idc = idn;
+                    idn = load_ids(P.items, c0 + 128, beg, end, lane);
+                    const uint32_t lo = max(c0, beg), hi = min(c0 + 64, end);
+                    for (uint32_t c = lo; c < hi; c++) {
+                    const int k = (int)(c - c0);
+                    const RecLane rc = rec_at(blk, k);
+                    const uint32_t cur = lane_u32(id, k);
                     // tile-clamped bbox, clamp_as_int!, triangle.rs:66-78
                     const int X0 = rc.X0, X1 = rc.X1, Y0 = rc.Y0, Y1 = rc.Y1;
                     if (X0 > X1) continue;  // culled or invalid
@@ -281,6 +358,9 @@ __device__ __forceinline__ void raster_tile(const RasterParams& P, uint32_t tile
                     if constexpr (BLEND == SR_BLEND_OVER) {
                         if (__ballot(full)) flush();
                     }
+                    }
+                    blk = nxt;
+                    id = idc;
                 }
             }
         }

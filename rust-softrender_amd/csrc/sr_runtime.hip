@@ -8,86 +8,22 @@
 // executed as: vertex kernel -> [clip kernels] -> setup -> scan -> pair
 // emission -> stable radix sort by execution tile -> tile ranges ->
 // [stencil strip counts] -> raster (one work-group per execution tile).
-#include <string.h>
-
-#include <algorithm>
-#include <new>
-#include <string>
-#include <vector>
-
-#include "sr_kernels.hpp"
+#include "sr_runtime.hpp"
 
 using namespace sr;
+using namespace sr_rt;
 
-namespace {
-
-struct DevBuf {
-    void* p = nullptr;
-    size_t bytes = 0;
-};
-
-enum Stage { ST_VERTEX, ST_GEOMETRY, ST_SETUP, ST_SCAN, ST_EMIT, ST_SORT, ST_RANGES, ST_STRIPS,
-             ST_RASTER, ST_COUNT };
-
-}  // namespace
-
-struct sr_context_t {
-    int device = 0;
-    hipStream_t stream = nullptr;
-    std::string err;
-    DevBuf verts, gverts, recs, counts, offsets, keys, vals, keys2, vals2, tstart, tend, scan_tmp,
-        hist, misc, strips, strip_aux, litems, plots, lcounts, loffsets, pcounts, poffsets;
-    uint32_t* h_misc = nullptr;  // pinned readback words
-    hipEvent_t ev[ST_COUNT + 1] = {};
-    float stage_ms[ST_COUNT] = {};
-    int n_stage = 0;
-    sr_draw_stats last = {};
-    // execution tile chosen when sr_draw_desc.exec_tile == 0: the first
-    // entry whose pixel bound holds the framebuffer (set by the autotuner)
-    std::vector<std::pair<uint64_t, int>> tile_policy{{~0ull, 8}};
-    struct Batch* batch = nullptr;
-};
-
-struct sr_mesh_t {
-    sr_context ctx;
-    float* soa[8] = {};  // px py pz nx ny nz u v (null if absent)
-    uint32_t* idx = nullptr;
-    uint32_t nv = 0, ni = 0;
-    int prim = 0;
-};
-
-struct sr_texture_t {
-    sr_context ctx;
-    uint32_t w = 0, h = 0;
-    uint32_t* d = nullptr;
-};
-
-struct sr_framebuffer_t {
-    sr_context ctx;
-    uint32_t w = 0, h = 0;
-    uint32_t* color = nullptr;
-    float* depth = nullptr;
-    uint8_t* stencil = nullptr;
-};
-
-namespace {
+namespace sr_rt {
 
 int fail(sr_context ctx, int code, const std::string& msg) {
     if (ctx) ctx->err = msg;
     return code;
 }
 
-#define HIP_OK(ctx, call)                                                              \
-    do {                                                                               \
-        hipError_t e_ = (call);                                                        \
-        if (e_ != hipSuccess)                                                          \
-            return fail((ctx), SR_ERR_DEVICE, std::string(#call ": ") + hipGetErrorString(e_)); \
-    } while (0)
-
-int ensure(sr_context ctx, DevBuf& b, size_t bytes) {
+int ensure(sr_context ctx, DevBuf& b, size_t bytes, hipStream_t s) {
     if (bytes <= b.bytes) return SR_OK;
     if (b.p) {
-        HIP_OK(ctx, hipStreamSynchronize(ctx->stream));
+        HIP_OK(ctx, hipStreamSynchronize(s));
         HIP_OK(ctx, hipFree(b.p));
         b.p = nullptr;
         b.bytes = 0;
@@ -102,14 +38,24 @@ int ensure(sr_context ctx, DevBuf& b, size_t bytes) {
     return SR_OK;
 }
 
-template <class T>
-T* P(DevBuf& b) { return reinterpret_cast<T*>(b.p); }
-
-int bits_for(uint32_t n) {  // bits to hold values 0..n-1
+int bits_for(uint32_t n) {
     int b = 1;
     while (b < 32 && (1ull << b) < n) b++;
     return b;
 }
+
+int varyings_of(int vp) { return vp == SR_VP_MVP_NORMAL_WORLD ? 6 : 5; }
+
+int policy_tile(sr_context ctx, uint32_t w, uint32_t h) {
+    const uint64_t px = (uint64_t)w * h;
+    for (const auto& e : ctx->tile_policy)
+        if (px <= e.first) return e.second;
+    return 8;
+}
+
+}  // namespace sr_rt
+
+namespace {
 
 void mark(sr_context ctx, bool timing, int stage) {
     if (timing) hipEventRecord(ctx->ev[stage + 1], ctx->stream);
@@ -120,15 +66,6 @@ int read_words(sr_context ctx, int n) {
                                ctx->stream));
     HIP_OK(ctx, hipStreamSynchronize(ctx->stream));
     return SR_OK;
-}
-
-int varyings_of(int vp) { return vp == SR_VP_MVP_NORMAL_WORLD ? 6 : 5; }
-
-int policy_tile(sr_context ctx, uint32_t w, uint32_t h) {
-    const uint64_t px = (uint64_t)w * h;
-    for (const auto& e : ctx->tile_policy)
-        if (px <= e.first) return e.second;
-    return 8;
 }
 
 // exclusive scan of counts[0..n) into offsets, total -> misc[slot]
@@ -220,7 +157,18 @@ int draw_tris(sr_context ctx, sr_framebuffer fb, const DrawPlan& pl, const sr_dr
     if (pl.exhaustive) {
         if ((rc = read_words(ctx, 1))) return rc;
         if (ctx->h_misc[0]) return fail(ctx, SR_ERR_INVALID_COORDINATE, "NaN screen coordinate");
-        R.exhaustive_n = ntris;
+        // every execution tile walks all triangles in order: identity item
+        // list, every tile's range [0, ntris)
+        if ((rc = ensure_pairs(ctx, ntris))) return rc;
+        if ((rc = ensure(ctx, ctx->tstart, sizeof(uint32_t) * pl.ntiles)) ||
+            (rc = ensure(ctx, ctx->tend, sizeof(uint32_t) * pl.ntiles)))
+            return rc;
+        launch_iota(P<uint32_t>(ctx->vals), ntris, s);
+        launch_fill(P<uint32_t>(ctx->tstart), 0u, pl.ntiles, s);
+        launch_fill(P<uint32_t>(ctx->tend), ntris, pl.ntiles, s);
+        R.tile_start = P<uint32_t>(ctx->tstart);
+        R.tile_end = P<uint32_t>(ctx->tend);
+        R.items = P<uint32_t>(ctx->vals);
         for (int st = ST_SCAN; st <= ST_RANGES; st++) mark(ctx, pl.timing, st);
     } else {
         if ((rc = scan_counts(ctx, P<uint32_t>(ctx->counts), P<uint32_t>(ctx->offsets), ntris)))
@@ -304,7 +252,7 @@ int draw_lines_impl(sr_context ctx, sr_framebuffer fb, const DrawPlan& pl, const
 int draw_points_impl(sr_context ctx, sr_framebuffer fb, const DrawPlan& pl, const sr_draw_desc* d,
                      sr_texture tex, VertexBuf vb, const uint32_t* idx, uint32_t npoints);
 
-int validate_draw(sr_context ctx, sr_framebuffer fb, sr_mesh mesh, const sr_draw_desc* d, sr_texture tex) {
+int sr_rt::validate_draw(sr_context ctx, sr_framebuffer fb, sr_mesh mesh, const sr_draw_desc* d, sr_texture tex) {
     if (!ctx || !fb || !mesh || !d) return fail(ctx, SR_ERR_INVALID_ARGUMENT, "draw: null argument");
     if (fb->ctx != ctx || mesh->ctx != ctx) return fail(ctx, SR_ERR_INVALID_ARGUMENT, "draw: handle from another context");
     if (d->vertex_program != SR_VP_MVP_NORMAL_UV && d->vertex_program != SR_VP_MVP_NORMAL_WORLD)
@@ -337,76 +285,299 @@ int validate_draw(sr_context ctx, sr_framebuffer fb, sr_mesh mesh, const sr_draw
 // from per-XCD queues.  Sizes that depend on the geometry (bin pairs) use a
 // capacity checked on the device: on overflow nothing is drawn, the
 // capacity grows and the batch runs again, so the sequence has no host
-// round trip and is captured once into a hipGraph and replayed while the
-// batch is unchanged.
+// round trip; it can be captured once into a hipGraph and replayed while
+// the batch is unchanged, or queued directly on a stream of its own.
 // ===========================================================================
-struct Batch {
-    DevBuf verts, recs, counts, offsets, keys, vals, keys2, vals2, tstart, tend, hist, scan, tables, misc;
-    uint32_t cap = 0;          // bin pair capacity
-    std::vector<uint8_t> key;  // what the cached graph renders
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t exec = nullptr;
-    uint32_t* h_stage = nullptr;  // pinned staging for the scene tables
-    size_t h_stage_bytes = 0;
-};
+namespace sr_rt {
 
-static void invalidate_batch_graph(sr_context ctx) {
-    Batch* B = ctx->batch;
-    if (!B) return;
-    if (B->exec) hipGraphExecDestroy(B->exec);
-    if (B->graph) hipGraphDestroy(B->graph);
-    B->exec = nullptr;
-    B->graph = nullptr;
-    B->key.clear();
-}
-
-static int set_batch_capacity(sr_context ctx, uint32_t pairs) {
-    if (!ctx->batch) ctx->batch = new (std::nothrow) Batch();
-    if (!ctx->batch) return SR_ERR_OUT_OF_MEMORY;
-    if (pairs > ctx->batch->cap) {
-        ctx->batch->cap = pairs;
-        invalidate_batch_graph(ctx);
-    }
+int BatchWork::init(sr_context ctx, bool new_stream) {
+    own_stream = new_stream;
+    if (new_stream) HIP_OK(ctx, hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+    else stream = ctx->stream;
+    HIP_OK(ctx, hipHostMalloc((void**)&h_misc, 16 * sizeof(uint32_t), hipHostMallocDefault));
+    HIP_OK(ctx, hipEventCreate(&ev0));
+    HIP_OK(ctx, hipEventCreate(&ev1));
+    cap = ctx->batch_cap_hint;
     return SR_OK;
 }
 
-static void destroy_batch(sr_context ctx) {
-    Batch* B = ctx->batch;
-    if (!B) return;
-    invalidate_batch_graph(ctx);
-    DevBuf* bufs[] = {&B->verts, &B->recs, &B->counts, &B->offsets, &B->keys, &B->vals, &B->keys2,
-                      &B->vals2, &B->tstart, &B->tend, &B->hist, &B->scan, &B->tables, &B->misc};
-    for (DevBuf* b : bufs)
+void BatchWork::drop_graph() {
+    if (exec) hipGraphExecDestroy(exec);
+    if (graph) hipGraphDestroy(graph);
+    exec = nullptr;
+    graph = nullptr;
+    key.clear();
+}
+
+void BatchWork::release() {
+    if (stream) hipStreamSynchronize(stream);
+    drop_graph();
+    DevBuf* bufs[] = {&verts, &recs, &counts, &offsets, &keys, &vals, &keys2,
+                      &vals2, &tstart, &tend, &hist, &scan, &tables, &misc};
+    for (DevBuf* b : bufs) {
         if (b->p) hipFree(b->p);
-    if (B->h_stage) hipHostFree(B->h_stage);
-    delete B;
-    ctx->batch = nullptr;
+        b->p = nullptr;
+        b->bytes = 0;
+    }
+    if (h_stage) hipHostFree(h_stage);
+    if (h_misc) hipHostFree(h_misc);
+    h_stage = nullptr;
+    h_misc = nullptr;
+    h_stage_bytes = 0;
+    if (ev0) hipEventDestroy(ev0);
+    if (ev1) hipEventDestroy(ev1);
+    ev0 = ev1 = nullptr;
+    if (own_stream && stream) hipStreamDestroy(stream);
+    stream = nullptr;
 }
 
-// grows a batch buffer; a reallocation invalidates the cached graph
-static int bensure(sr_context ctx, DevBuf& b, size_t bytes) {
-    if (bytes <= b.bytes) return SR_OK;
-    invalidate_batch_graph(ctx);
-    return ensure(ctx, b, bytes);
+size_t BatchWork::device_bytes() const {
+    const DevBuf* bufs[] = {&verts, &recs, &counts, &offsets, &keys, &vals, &keys2,
+                            &vals2, &tstart, &tend, &hist, &scan, &tables, &misc};
+    size_t t = 0;
+    for (const DevBuf* b : bufs) t += b->bytes;
+    return t;
 }
 
-static bool batchable(sr_mesh m, const sr_draw_desc* d) {
+size_t batch_bytes_estimate(uint64_t tris, uint64_t verts, double pairs_per_tri) {
+    // screen vertices (10 words), triangle records, counts + offsets, four
+    // pair arrays of the sort, radix histograms; 25% growth headroom
+    const double pairs = (double)tris * pairs_per_tri + 65536.0;
+    const double b = 40.0 * (double)verts + (double)sizeof(TriRec) * (double)tris + 8.0 * (double)tris +
+                     16.0 * pairs + 4.0 * (double)radix_hist_words((uint32_t)std::min(pairs, 2e9));
+    return (size_t)(b * 1.25);
+}
+
+bool batchable(sr_mesh m, const sr_draw_desc* d) {
     return m->prim == SR_PRIM_TRIANGLES && d->geometry_program == SR_GS_NONE &&
            d->stencil_test == SR_STENCIL_ALWAYS && d->stencil_op == SR_STENCIL_KEEP &&
            !(d->exec_flags & SR_EXEC_EXHAUSTIVE);
 }
 
-struct BatchScene {
-    uint32_t item;
-    uint32_t ets;
-    uint64_t cls;  // (ets, fp, blend)
-};
+int batch_plan(sr_context ctx, const sr_batch_item* items, uint32_t n, BatchPlan& pl) {
+    struct Order {
+        uint32_t item;
+        uint32_t ets;
+        uint64_t cls;  // (ets, fp, blend)
+    };
+    std::vector<Order> order(n);
+    for (uint32_t i = 0; i < n; i++) {
+        const sr_draw_desc* d = items[i].desc;
+        const uint32_t ets = d->exec_tile ? (uint32_t)d->exec_tile
+                                          : (uint32_t)policy_tile(ctx, items[i].fb->w, items[i].fb->h);
+        order[i] = Order{i, ets, ((uint64_t)ets << 16) | ((uint64_t)d->fragment_program << 8) | (uint64_t)d->blend};
+    }
+    std::stable_sort(order.begin(), order.end(), [](const Order& a, const Order& b) { return a.cls < b.cls; });
+    pl = BatchPlan();
+    pl.items.resize(n);
+    pl.descs.resize(n);
+    pl.geom.resize(n);
+    pl.vend.resize(n);
+    pl.tend.resize(n);
+    pl.tile_end.resize(n);
+    for (uint32_t k = 0; k < n; k++) {
+        pl.items[k] = items[order[k].item];
+        pl.descs[k] = *pl.items[k].desc;
+        pl.items[k].desc = nullptr;  // the copy in descs[k] is the one used
+        const sr_draw_desc* d = &pl.descs[k];
+        const sr_mesh m = pl.items[k].mesh;
+        const uint32_t W = pl.items[k].fb->w, H = pl.items[k].fb->h, ets = order[k].ets;
+        const bool drawable = W > 1 && H > 1;  // W or H == 1: no tiles (fragment.rs:196,201)
+        const uint32_t ntri = drawable ? (m->ni ? m->ni : m->nv) / 3 : 0;
+        SceneGeom& g = pl.geom[k];
+        memset(&g, 0, sizeof(g));
+        g.px = m->soa[0]; g.py = m->soa[1]; g.pz = m->soa[2]; g.nx = m->soa[3]; g.ny = m->soa[4];
+        g.nz = m->soa[5]; g.u = m->soa[6]; g.v = m->soa[7]; g.idx = m->idx;
+        g.nv = ntri ? m->nv : 0; g.vbase = (uint32_t)pl.NV;
+        g.nt = ntri; g.tbase = (uint32_t)pl.NT;
+        g.program = d->vertex_program; g.K = varyings_of(d->vertex_program); g.cull = d->cull;
+        g.W = W; g.H = H; g.ets = ets; g.ntx = (W + ets - 1) / ets;
+        g.tile_base = (uint32_t)pl.NTILES;
+        g.viewport = d->viewport;
+        memcpy(g.U, d->vs_uniforms, sizeof(g.U));
+        const uint32_t ntiles = g.ntx * ((H + ets - 1) / ets);
+        pl.NV += g.nv; pl.NT += g.nt; pl.NTILES += ntiles;
+        pl.vend[k] = (uint32_t)pl.NV; pl.tend[k] = (uint32_t)pl.NT; pl.tile_end[k] = (uint32_t)pl.NTILES;
+        if (pl.classes.empty() || pl.classes.back().cls != order[k].cls)
+            pl.classes.push_back(BatchClass{order[k].cls, ets, (uint32_t)d->fragment_program, (uint32_t)d->blend,
+                                            g.tile_base, 0});
+        pl.classes.back().ntiles += ntiles;
+    }
+    if (pl.NV >= (1ull << 31) || pl.NT >= (1ull << 31) || pl.NTILES >= (1ull << 31))
+        return fail(ctx, SR_ERR_OUT_OF_MEMORY, "batch: too large");
+    return SR_OK;
+}
 
-static int batch_draw(sr_context ctx, const sr_batch_item* items, uint32_t n) {
+int batch_enqueue(sr_context ctx, BatchWork& W, const BatchPlan& pl, bool use_graph) {
+    int rc;
+    hipStream_t s = W.stream;
+    const uint32_t ns = (uint32_t)pl.items.size();
+    const uint64_t NV = pl.NV, NT = pl.NT, NTILES = pl.NTILES;
+    if (W.cap == 0) W.cap = (uint32_t)std::min<uint64_t>(4 * NT + (1u << 16), 0x7FFFFFFFull);
+    const uint32_t cap = W.cap;
+    const uint32_t nvb = (uint32_t)std::max<uint64_t>(NV, 1);
+    const size_t before = W.device_bytes();
+    if ((rc = ensure(ctx, W.verts, sizeof(float) * 10ull * nvb, s)) ||
+        (rc = ensure(ctx, W.recs, sizeof(TriRec) * std::max<uint64_t>(NT, 1), s)) ||
+        (rc = ensure(ctx, W.counts, sizeof(uint32_t) * (NT + 1), s)) ||
+        (rc = ensure(ctx, W.offsets, sizeof(uint32_t) * (NT + 1), s)) ||
+        (rc = ensure(ctx, W.keys, sizeof(uint32_t) * (size_t)cap, s)) ||
+        (rc = ensure(ctx, W.vals, sizeof(uint32_t) * (size_t)cap, s)) ||
+        (rc = ensure(ctx, W.keys2, sizeof(uint32_t) * (size_t)cap, s)) ||
+        (rc = ensure(ctx, W.vals2, sizeof(uint32_t) * (size_t)cap, s)) ||
+        (rc = ensure(ctx, W.tstart, sizeof(uint32_t) * std::max<uint64_t>(NTILES, 1), s)) ||
+        (rc = ensure(ctx, W.tend, sizeof(uint32_t) * std::max<uint64_t>(NTILES, 1), s)) ||
+        (rc = ensure(ctx, W.hist, sizeof(uint32_t) * (radix_hist_words(cap) + 256), s)) ||
+        (rc = ensure(ctx, W.scan, sizeof(uint32_t) * (scan_tmp_words(std::max<uint32_t>(radix_hist_words(cap) + 256, (uint32_t)NT + 1)) + 16), s)) ||
+        (rc = ensure(ctx, W.misc, sizeof(uint32_t) * (8 + 8 * pl.classes.size()), s)))
+        return rc;
+    const size_t tbytes = sizeof(SceneGeom) * ns + sizeof(RasterParams) * ns + sizeof(uint32_t) * 3 * ns + 64;
+    if ((rc = ensure(ctx, W.tables, tbytes, s))) return rc;
+    if (W.device_bytes() != before) W.drop_graph();  // a buffer moved
+
+    // scene tables (device copies, pointers into the workspace)
+    uint8_t* tb = P<uint8_t>(W.tables);
+    SceneGeom* d_geom = (SceneGeom*)tb;
+    RasterParams* d_rp = (RasterParams*)(tb + sizeof(SceneGeom) * ns);
+    uint32_t* d_vend = (uint32_t*)(tb + sizeof(SceneGeom) * ns + sizeof(RasterParams) * ns);
+    uint32_t* d_tend = d_vend + ns;
+    uint32_t* d_tile_end = d_tend + ns;
+    uint32_t* misc = P<uint32_t>(W.misc);  // [0] overflow [1] pairs [2] err [3] scratch [8..) queues
+    const VertexBuf vb{P<float>(W.verts), nvb};
+    const int swapped_passes = (bits_for((uint32_t)std::max<uint64_t>(NTILES, 2)) + 7) / 8;
+    const uint32_t* sorted_vals = (swapped_passes & 1) ? P<uint32_t>(W.vals2) : P<uint32_t>(W.vals);
+    std::vector<RasterParams> rp(ns);
+    for (uint32_t k = 0; k < ns; k++) {
+        const sr_batch_item& it = pl.items[k];
+        const sr_draw_desc* d = &pl.descs[k];
+        RasterParams& R = rp[k];
+        memset(&R, 0, sizeof(R));
+        R.W = it.fb->w; R.H = it.fb->h;
+        R.color = it.fb->color; R.depth = it.fb->depth; R.stencil = it.fb->stencil;
+        R.tw = d->tile_w > 0 ? (uint32_t)d->tile_w : 128u;
+        R.th = d->tile_h > 0 ? (uint32_t)d->tile_h : 128u;
+        R.scols = R.W > 1 ? (R.W - 1 + R.tw - 1) / R.tw : 0;
+        R.srows = R.H > 1 ? (R.H - 1 + R.th - 1) / R.th : 0;
+        R.ets = pl.geom[k].ets; R.ntx = pl.geom[k].ntx; R.nty = (R.H + R.ets - 1) / R.ets;
+        R.tile_start = P<uint32_t>(W.tstart); R.tile_end = P<uint32_t>(W.tend);
+        R.items = sorted_vals;
+        R.tris = P<TriRec>(W.recs);
+        R.verts = vb;
+        R.K = pl.geom[k].K;
+        R.fp = d->fragment_program; R.blend = d->blend; R.alpha_mode = d->alpha_mode;
+        R.s_test = SR_STENCIL_ALWAYS; R.s_op = SR_STENCIL_KEEP;
+        R.smode = 0;
+        if (it.tex) { R.tex = it.tex->d; R.texW = it.tex->w; R.texH = it.tex->h; }
+        R.tex_filter = d->tex_filter; R.tex_edge = d->tex_edge; R.tex_border = d->tex_border;
+        memcpy(R.fsu, d->fs_uniforms, sizeof(R.fsu));
+        R.tile_base = pl.geom[k].tile_base;
+        R.clear = it.clear ? 1 : 0;
+        R.clear_rgba = it.clear_rgba;
+    }
+    // the captured sequence depends on the capacity and the tables
+    std::vector<uint8_t> key;
+    if (use_graph) {
+        auto put = [&](const void* p, size_t b) {
+            const uint8_t* q = (const uint8_t*)p;
+            key.insert(key.end(), q, q + b);
+        };
+        put(&cap, sizeof(cap));
+        put(pl.geom.data(), sizeof(SceneGeom) * ns);
+        put(rp.data(), sizeof(RasterParams) * ns);
+    }
+    const bool replay = use_graph && W.exec && key == W.key;
+    if (!replay) {
+        W.drop_graph();
+        if (W.h_stage_bytes < tbytes) {
+            if (W.h_stage) hipHostFree(W.h_stage);
+            W.h_stage = nullptr;
+            W.h_stage_bytes = 0;
+            if (hipHostMalloc((void**)&W.h_stage, tbytes, hipHostMallocDefault) != hipSuccess)
+                return fail(ctx, SR_ERR_OUT_OF_MEMORY, "batch: staging alloc failed");
+            W.h_stage_bytes = tbytes;
+        }
+        uint8_t* hs = W.h_stage;
+        memcpy(hs, pl.geom.data(), sizeof(SceneGeom) * ns);
+        memcpy(hs + sizeof(SceneGeom) * ns, rp.data(), sizeof(RasterParams) * ns);
+        uint32_t* hv = (uint32_t*)(hs + sizeof(SceneGeom) * ns + sizeof(RasterParams) * ns);
+        memcpy(hv, pl.vend.data(), 4 * ns);
+        memcpy(hv + ns, pl.tend.data(), 4 * ns);
+        memcpy(hv + 2 * ns, pl.tile_end.data(), 4 * ns);
+        if (!use_graph) HIP_OK(ctx, hipEventRecord(W.ev0, s));
+        HIP_OK(ctx, hipMemcpyAsync(tb, hs, tbytes - 64, hipMemcpyHostToDevice, s));
+        if (use_graph) {
+            HIP_OK(ctx, hipStreamSynchronize(s));
+            HIP_OK(ctx, hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        }
+        BatchTables T{d_geom, d_vend, d_tend, ns, (uint32_t)NV, (uint32_t)NT};
+        launch_fill(misc, 0u, (uint32_t)(8 + 8 * pl.classes.size()), s);
+        launch_vertex_batch(T, vb, s);
+        launch_tri_setup_batch(T, vb, P<TriRec>(W.recs), P<uint32_t>(W.counts), misc + 2, s);
+        launch_scan(P<uint32_t>(W.counts), P<uint32_t>(W.offsets), (uint32_t)NT, P<uint32_t>(W.scan), misc + 1, s);
+        launch_tri_emit_batch(T, P<TriRec>(W.recs), P<uint32_t>(W.counts), P<uint32_t>(W.offsets), cap,
+                              P<uint32_t>(W.keys), P<uint32_t>(W.vals), misc + 0, s);
+        const int sw = radix_sort(P<uint32_t>(W.keys), P<uint32_t>(W.vals), P<uint32_t>(W.keys2),
+                                  P<uint32_t>(W.vals2), cap, bits_for((uint32_t)std::max<uint64_t>(NTILES, 2)),
+                                  P<uint32_t>(W.hist), P<uint32_t>(W.scan), misc + 3, s, misc + 1);
+        const uint32_t* skeys = sw ? P<uint32_t>(W.keys2) : P<uint32_t>(W.keys);
+        launch_fill(P<uint32_t>(W.tstart), 0u, (uint32_t)NTILES, s);
+        launch_fill(P<uint32_t>(W.tend), 0u, (uint32_t)NTILES, s);
+        launch_ranges(skeys, cap, 0u, P<uint32_t>(W.tstart), P<uint32_t>(W.tend), s, misc + 1);
+        for (size_t c = 0; c < pl.classes.size(); c++) {
+            const BatchClass& C = pl.classes[c];
+            if (C.ntiles == 0) continue;
+            RasterBatch RB;
+            memset(&RB, 0, sizeof(RB));
+            RB.one.fp = (int)C.fp; RB.one.blend = (int)C.blend; RB.one.ets = C.ets; RB.one.smode = 0;
+            RB.scenes = d_rp;
+            RB.scene_tile_end = d_tile_end;
+            RB.nscenes = ns;
+            RB.tile0 = C.tile0;
+            RB.ntiles = C.ntiles;
+            RB.queue = misc + 8 + 8 * c;
+            RB.overflow = misc;
+            const uint32_t grid = std::min<uint32_t>(C.ntiles, C.ets == 8 ? 256u * 20u : 256u * 5u);
+            launch_raster_tris(RB, grid, s);
+        }
+        if (use_graph) {
+            hipGraph_t g = nullptr;
+            hipError_t e = hipStreamEndCapture(s, &g);
+            if (e != hipSuccess) return fail(ctx, SR_ERR_DEVICE, std::string("batch capture: ") + hipGetErrorString(e));
+            W.graph = g;
+            HIP_OK(ctx, hipGraphInstantiate(&W.exec, g, nullptr, nullptr, 0));
+            W.key.swap(key);
+        }
+    }
+    if (use_graph) {
+        HIP_OK(ctx, hipEventRecord(W.ev0, s));
+        HIP_OK(ctx, hipGraphLaunch(W.exec, s));
+    }
+    HIP_OK(ctx, hipMemcpyAsync(W.h_misc, misc, 4 * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+    HIP_OK(ctx, hipEventRecord(W.ev1, s));
+    return SR_OK;
+}
+
+int batch_result(sr_context ctx, BatchWork& W, const BatchPlan& pl, bool* again) {
+    *again = false;
+    if (W.h_misc[2]) return fail(ctx, SR_ERR_INVALID_COORDINATE, "batch: NaN screen coordinate");
+    if (!W.h_misc[0]) return SR_OK;
+    // binning overflowed: nothing was drawn; grow and run again
+    const uint64_t want = (uint64_t)W.h_misc[1] + W.h_misc[1] / 4 + 4096;
+    if (want >= 0x7FFFFFFFull) return fail(ctx, SR_ERR_OUT_OF_MEMORY, "batch: bin pairs exceed 2^31");
+    W.cap = (uint32_t)want;
+    W.drop_graph();
+    *again = true;
+    return SR_OK;
+}
+
+}  // namespace sr_rt
+
+namespace {
+
+int batch_draw(sr_context ctx, const sr_batch_item* items, uint32_t n) {
     int rc;
     hipSetDevice(ctx->device);
     ctx->n_stage = 0;
-    std::vector<BatchScene> fast;
     std::vector<sr_framebuffer> seen;
     for (uint32_t i = 0; i < n; i++) {
         const sr_batch_item& it = items[i];
@@ -415,215 +586,54 @@ static int batch_draw(sr_context ctx, const sr_batch_item* items, uint32_t n) {
             if (f == it.fb) return fail(ctx, SR_ERR_INVALID_ARGUMENT, "batch: a framebuffer appears twice");
         seen.push_back(it.fb);
     }
+    std::vector<sr_batch_item> fast;
     for (uint32_t i = 0; i < n; i++) {
         const sr_batch_item& it = items[i];
-        const sr_draw_desc* d = it.desc;
-        const uint32_t per = (uint32_t)it.mesh->prim;
-        const uint32_t nprim = (it.mesh->ni ? it.mesh->ni : it.mesh->nv) / per;
-        const bool drawable = it.fb->w > 1 && it.fb->h > 1 && nprim > 0 && d->stencil_test != SR_STENCIL_NEVER;
-        if (batchable(it.mesh, d)) {
-            const uint32_t ets = d->exec_tile ? (uint32_t)d->exec_tile : (uint32_t)policy_tile(ctx, it.fb->w, it.fb->h);
-            fast.push_back(BatchScene{i, ets, ((uint64_t)ets << 16) | ((uint64_t)d->fragment_program << 8) |
-                                                   (uint64_t)d->blend});
-            (void)drawable;
-        } else {
+        if (batchable(it.mesh, it.desc)) {
+            fast.push_back(it);
+        } else {  // drawn on its own, in item order, before the batch
             if (it.clear && (rc = sr_fb_clear(it.fb, it.clear_rgba, -3.40282347e+38f, 0))) return rc;
-            if ((rc = sr_draw(ctx, it.fb, it.mesh, d, it.tex))) return rc;
+            if ((rc = sr_draw(ctx, it.fb, it.mesh, it.desc, it.tex))) return rc;
         }
     }
     if (fast.empty()) return SR_OK;
-    std::stable_sort(fast.begin(), fast.end(), [](const BatchScene& a, const BatchScene& b) { return a.cls < b.cls; });
-    if (!ctx->batch && (rc = set_batch_capacity(ctx, 0))) return rc;
-    Batch* B = ctx->batch;
-    hipStream_t s = ctx->stream;
-
-    // ---- host tables
-    const uint32_t ns = (uint32_t)fast.size();
-    std::vector<SceneGeom> geom(ns);
-    std::vector<RasterParams> rp(ns);
-    std::vector<uint32_t> vend(ns), tend(ns), tile_end(ns);
-    struct Class { uint64_t cls; uint32_t ets, fp, blend, tile0, ntiles; };
-    std::vector<Class> classes;
-    uint64_t NV = 0, NT = 0, NTILES = 0;
-    for (uint32_t k = 0; k < ns; k++) {
-        const sr_batch_item& it = items[fast[k].item];
-        const sr_draw_desc* d = it.desc;
-        const sr_mesh m = it.mesh;
-        const uint32_t W = it.fb->w, H = it.fb->h, ets = fast[k].ets;
-        const uint32_t tw = d->tile_w > 0 ? (uint32_t)d->tile_w : 128u, th = d->tile_h > 0 ? (uint32_t)d->tile_h : 128u;
-        const bool drawable = W > 1 && H > 1;  // W or H == 1: no tiles (fragment.rs:196,201)
-        const uint32_t ntri = drawable ? (m->ni ? m->ni : m->nv) / 3 : 0;
-        SceneGeom& g = geom[k];
-        memset(&g, 0, sizeof(g));
-        g.px = m->soa[0]; g.py = m->soa[1]; g.pz = m->soa[2]; g.nx = m->soa[3]; g.ny = m->soa[4];
-        g.nz = m->soa[5]; g.u = m->soa[6]; g.v = m->soa[7]; g.idx = m->idx;
-        g.nv = ntri ? m->nv : 0; g.vbase = (uint32_t)NV;
-        g.nt = ntri; g.tbase = (uint32_t)NT;
-        g.program = d->vertex_program; g.K = varyings_of(d->vertex_program); g.cull = d->cull;
-        g.W = W; g.H = H; g.ets = ets; g.ntx = (W + ets - 1) / ets;
-        g.tile_base = (uint32_t)NTILES;
-        g.viewport = d->viewport;
-        memcpy(g.U, d->vs_uniforms, sizeof(g.U));
-        const uint32_t ntiles = g.ntx * ((H + ets - 1) / ets);
-        NV += g.nv; NT += g.nt; NTILES += ntiles;
-        vend[k] = (uint32_t)NV; tend[k] = (uint32_t)NT; tile_end[k] = (uint32_t)NTILES;
-        if (classes.empty() || classes.back().cls != fast[k].cls)
-            classes.push_back(Class{fast[k].cls, ets, (uint32_t)d->fragment_program, (uint32_t)d->blend,
-                                    g.tile_base, 0});
-        classes.back().ntiles += ntiles;
-    }
-    if (NV >= (1ull << 31) || NT >= (1ull << 31) || NTILES >= (1ull << 31))
-        return fail(ctx, SR_ERR_OUT_OF_MEMORY, "batch: too large");
-    if (B->cap == 0) B->cap = (uint32_t)std::min<uint64_t>(4 * NT + (1u << 16), 0x7FFFFFFFull);
-
-    for (int attempt = 0; attempt < 4; attempt++) {
-        const uint32_t cap = B->cap;
-        const uint32_t nvb = (uint32_t)std::max<uint64_t>(NV, 1);
-        if ((rc = bensure(ctx, B->verts, sizeof(float) * 10ull * nvb)) ||
-            (rc = bensure(ctx, B->recs, sizeof(TriRec) * std::max<uint64_t>(NT, 1))) ||
-            (rc = bensure(ctx, B->counts, sizeof(uint32_t) * (NT + 1))) ||
-            (rc = bensure(ctx, B->offsets, sizeof(uint32_t) * (NT + 1))) ||
-            (rc = bensure(ctx, B->keys, sizeof(uint32_t) * (size_t)cap)) ||
-            (rc = bensure(ctx, B->vals, sizeof(uint32_t) * (size_t)cap)) ||
-            (rc = bensure(ctx, B->keys2, sizeof(uint32_t) * (size_t)cap)) ||
-            (rc = bensure(ctx, B->vals2, sizeof(uint32_t) * (size_t)cap)) ||
-            (rc = bensure(ctx, B->tstart, sizeof(uint32_t) * std::max<uint64_t>(NTILES, 1))) ||
-            (rc = bensure(ctx, B->tend, sizeof(uint32_t) * std::max<uint64_t>(NTILES, 1))) ||
-            (rc = bensure(ctx, B->hist, sizeof(uint32_t) * (radix_hist_words(cap) + 256))) ||
-            (rc = bensure(ctx, B->scan, sizeof(uint32_t) * (scan_tmp_words(std::max<uint32_t>(radix_hist_words(cap) + 256, (uint32_t)NT + 1)) + 16))) ||
-            (rc = bensure(ctx, B->misc, sizeof(uint32_t) * (8 + 8 * classes.size()))))
+    if (!ctx->batch) {
+        BatchWork* W = new (std::nothrow) BatchWork();
+        if (!W) return fail(ctx, SR_ERR_OUT_OF_MEMORY, "batch: workspace");
+        if ((rc = W->init(ctx, false))) {
+            W->release();
+            delete W;
             return rc;
-        const size_t tbytes = sizeof(SceneGeom) * ns + sizeof(RasterParams) * ns + sizeof(uint32_t) * 3 * ns + 64;
-        if ((rc = bensure(ctx, B->tables, tbytes))) return rc;
-
-        // scene tables (device copies, pointers into the batch buffers)
-        uint8_t* tb = P<uint8_t>(B->tables);
-        SceneGeom* d_geom = (SceneGeom*)tb;
-        RasterParams* d_rp = (RasterParams*)(tb + sizeof(SceneGeom) * ns);
-        uint32_t* d_vend = (uint32_t*)(tb + sizeof(SceneGeom) * ns + sizeof(RasterParams) * ns);
-        uint32_t* d_tend = d_vend + ns;
-        uint32_t* d_tile_end = d_tend + ns;
-        uint32_t* misc = P<uint32_t>(B->misc);  // [0] overflow [1] pairs [2] err [3] scratch [8..) queues
-        const VertexBuf vb{P<float>(B->verts), nvb};
-        const int swapped_passes = (bits_for((uint32_t)std::max<uint64_t>(NTILES, 2)) + 7) / 8;
-        const uint32_t* sorted_vals = (swapped_passes & 1) ? P<uint32_t>(B->vals2) : P<uint32_t>(B->vals);
-        for (uint32_t k = 0; k < ns; k++) {
-            const sr_batch_item& it = items[fast[k].item];
-            const sr_draw_desc* d = it.desc;
-            RasterParams& R = rp[k];
-            memset(&R, 0, sizeof(R));
-            R.W = it.fb->w; R.H = it.fb->h;
-            R.color = it.fb->color; R.depth = it.fb->depth; R.stencil = it.fb->stencil;
-            R.tw = d->tile_w > 0 ? (uint32_t)d->tile_w : 128u;
-            R.th = d->tile_h > 0 ? (uint32_t)d->tile_h : 128u;
-            R.scols = R.W > 1 ? (R.W - 1 + R.tw - 1) / R.tw : 0;
-            R.srows = R.H > 1 ? (R.H - 1 + R.th - 1) / R.th : 0;
-            R.ets = geom[k].ets; R.ntx = geom[k].ntx; R.nty = (R.H + R.ets - 1) / R.ets;
-            R.tile_start = P<uint32_t>(B->tstart); R.tile_end = P<uint32_t>(B->tend);
-            R.items = sorted_vals;
-            R.tris = P<TriRec>(B->recs);
-            R.verts = vb;
-            R.K = geom[k].K;
-            R.fp = d->fragment_program; R.blend = d->blend; R.alpha_mode = d->alpha_mode;
-            R.s_test = SR_STENCIL_ALWAYS; R.s_op = SR_STENCIL_KEEP;
-            R.smode = 0;
-            if (it.tex) { R.tex = it.tex->d; R.texW = it.tex->w; R.texH = it.tex->h; }
-            R.tex_filter = d->tex_filter; R.tex_edge = d->tex_edge; R.tex_border = d->tex_border;
-            memcpy(R.fsu, d->fs_uniforms, sizeof(R.fsu));
-            R.tile_base = geom[k].tile_base;
-            R.clear = it.clear ? 1 : 0;
-            R.clear_rgba = it.clear_rgba;
         }
-        // cache key: everything the captured sequence depends on
-        std::vector<uint8_t> key;
-        auto put = [&](const void* p, size_t b) {
-            const uint8_t* q = (const uint8_t*)p;
-            key.insert(key.end(), q, q + b);
-        };
-        put(&cap, sizeof(cap));
-        put(geom.data(), sizeof(SceneGeom) * ns);
-        put(rp.data(), sizeof(RasterParams) * ns);
-        const bool replay = B->exec && key == B->key;
-        if (!replay) {
-            invalidate_batch_graph(ctx);
-            if (B->h_stage_bytes < tbytes) {
-                if (B->h_stage) hipHostFree(B->h_stage);
-                B->h_stage = nullptr;
-                B->h_stage_bytes = 0;
-                if (hipHostMalloc((void**)&B->h_stage, tbytes, hipHostMallocDefault) != hipSuccess)
-                    return fail(ctx, SR_ERR_OUT_OF_MEMORY, "batch: staging alloc failed");
-                B->h_stage_bytes = tbytes;
-            }
-            uint8_t* hs = (uint8_t*)B->h_stage;
-            memcpy(hs, geom.data(), sizeof(SceneGeom) * ns);
-            memcpy(hs + sizeof(SceneGeom) * ns, rp.data(), sizeof(RasterParams) * ns);
-            uint32_t* hv = (uint32_t*)(hs + sizeof(SceneGeom) * ns + sizeof(RasterParams) * ns);
-            memcpy(hv, vend.data(), 4 * ns);
-            memcpy(hv + ns, tend.data(), 4 * ns);
-            memcpy(hv + 2 * ns, tile_end.data(), 4 * ns);
-            HIP_OK(ctx, hipMemcpyAsync(tb, hs, tbytes - 64, hipMemcpyHostToDevice, s));
-            HIP_OK(ctx, hipStreamSynchronize(s));
-
-            // ---- capture the batch
-            HIP_OK(ctx, hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-            BatchTables T{d_geom, d_vend, d_tend, ns, (uint32_t)NV, (uint32_t)NT};
-            launch_fill(misc, 0u, (uint32_t)(8 + 8 * classes.size()), s);
-            launch_vertex_batch(T, vb, s);
-            launch_tri_setup_batch(T, vb, P<TriRec>(B->recs), P<uint32_t>(B->counts), misc + 2, s);
-            launch_scan(P<uint32_t>(B->counts), P<uint32_t>(B->offsets), (uint32_t)NT, P<uint32_t>(B->scan), misc + 1, s);
-            launch_tri_emit_batch(T, P<TriRec>(B->recs), P<uint32_t>(B->counts), P<uint32_t>(B->offsets), cap,
-                                  P<uint32_t>(B->keys), P<uint32_t>(B->vals), misc + 0, s);
-            const int sw = radix_sort(P<uint32_t>(B->keys), P<uint32_t>(B->vals), P<uint32_t>(B->keys2),
-                                      P<uint32_t>(B->vals2), cap, bits_for((uint32_t)std::max<uint64_t>(NTILES, 2)),
-                                      P<uint32_t>(B->hist), P<uint32_t>(B->scan), misc + 3, s, misc + 1);
-            const uint32_t* skeys = sw ? P<uint32_t>(B->keys2) : P<uint32_t>(B->keys);
-            launch_fill(P<uint32_t>(B->tstart), 0u, (uint32_t)NTILES, s);
-            launch_fill(P<uint32_t>(B->tend), 0u, (uint32_t)NTILES, s);
-            launch_ranges(skeys, cap, 0u, P<uint32_t>(B->tstart), P<uint32_t>(B->tend), s, misc + 1);
-            for (size_t c = 0; c < classes.size(); c++) {
-                const Class& C = classes[c];
-                if (C.ntiles == 0) continue;
-                RasterBatch RB;
-                memset(&RB, 0, sizeof(RB));
-                RB.one.fp = (int)C.fp; RB.one.blend = (int)C.blend; RB.one.ets = C.ets; RB.one.smode = 0;
-                RB.scenes = d_rp;
-                RB.scene_tile_end = d_tile_end;
-                RB.nscenes = ns;
-                RB.tile0 = C.tile0;
-                RB.ntiles = C.ntiles;
-                RB.queue = misc + 8 + 8 * c;
-                RB.overflow = misc;
-                const uint32_t grid = std::min<uint32_t>(C.ntiles, C.ets == 8 ? 256u * 20u : 256u * 5u);
-                launch_raster_tris(RB, grid, s);
-            }
-            hipGraph_t g = nullptr;
-            hipError_t e = hipStreamEndCapture(s, &g);
-            if (e != hipSuccess) return fail(ctx, SR_ERR_DEVICE, std::string("batch capture: ") + hipGetErrorString(e));
-            B->graph = g;
-            HIP_OK(ctx, hipGraphInstantiate(&B->exec, g, nullptr, nullptr, 0));
-            B->key = key;
-        }
-        HIP_OK(ctx, hipEventRecord(ctx->ev[0], s));
-        HIP_OK(ctx, hipGraphLaunch(B->exec, s));
-        HIP_OK(ctx, hipEventRecord(ctx->ev[1], s));
-        HIP_OK(ctx, hipMemcpyAsync(ctx->h_misc, misc, 4 * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
-        HIP_OK(ctx, hipStreamSynchronize(s));
+        ctx->batch = W;
+    }
+    BatchWork& W = *ctx->batch;
+    BatchPlan plan;
+    if ((rc = batch_plan(ctx, fast.data(), (uint32_t)fast.size(), plan))) return rc;
+    for (int attempt = 0; attempt < 4; attempt++) {
+        if ((rc = batch_enqueue(ctx, W, plan, true))) return rc;
+        HIP_OK(ctx, hipEventSynchronize(W.ev1));
         float ms = 0.f;
-        hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[1]);
+        hipEventElapsedTime(&ms, W.ev0, W.ev1);
         ctx->stage_ms[0] = ms;
         ctx->n_stage = 1;
-        ctx->last.bin_pairs = ctx->h_misc[1];
-        ctx->last.triangles_in = NT;
-        if (ctx->h_misc[2]) return fail(ctx, SR_ERR_INVALID_COORDINATE, "batch: NaN screen coordinate");
-        if (!ctx->h_misc[0]) return SR_OK;
-        // binning overflowed: nothing was drawn; grow and run again
-        const uint64_t want = (uint64_t)ctx->h_misc[1] + ctx->h_misc[1] / 4 + 4096;
-        if (want >= 0x7FFFFFFFull) return fail(ctx, SR_ERR_OUT_OF_MEMORY, "batch: bin pairs exceed 2^31");
-        B->cap = (uint32_t)want;
-        invalidate_batch_graph(ctx);
+        ctx->last.bin_pairs = W.h_misc[1];
+        ctx->last.triangles_in = plan.NT;
+        bool again = false;
+        if ((rc = batch_result(ctx, W, plan, &again))) return rc;
+        if (!again) return SR_OK;
     }
     return fail(ctx, SR_ERR_DEVICE, "batch: capacity did not converge");
 }
+
+void destroy_batch(sr_context ctx) {
+    if (!ctx->batch) return;
+    ctx->batch->release();
+    delete ctx->batch;
+    ctx->batch = nullptr;
+}
+
+}  // namespace
 
 // ===========================================================================
 // C ABI
@@ -965,13 +975,18 @@ int sr_ctx_set_exec_tile_policy(sr_context ctx, uint32_t n, const uint64_t* max_
         p.emplace_back(max_pixels[i], e);
     }
     ctx->tile_policy = p;
-    if (ctx->batch) invalidate_batch_graph(ctx);
+    if (ctx->batch) ctx->batch->drop_graph();
     return SR_OK;
 }
 
 int sr_ctx_set_batch_capacity(sr_context ctx, uint32_t pairs) {
     if (!ctx) return SR_ERR_INVALID_ARGUMENT;
-    return set_batch_capacity(ctx, pairs);
+    ctx->batch_cap_hint = std::max(ctx->batch_cap_hint, pairs);
+    if (ctx->batch && pairs > ctx->batch->cap) {
+        ctx->batch->cap = pairs;
+        ctx->batch->drop_graph();
+    }
+    return SR_OK;
 }
 
 }  // extern "C"

@@ -1,0 +1,161 @@
+// sr_runtime.hpp — host-side runtime internals shared by the C ABI
+// translation units: the context and handle types, device workspaces, and
+// the batched pipeline split into plan / enqueue / result so a caller can
+// keep several batches in flight on their own streams (the request server)
+// or run one batch synchronously through a cached hipGraph (sr_batch_draw).
+#pragma once
+
+#include <string.h>
+
+#include <algorithm>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "sr_kernels.hpp"
+
+namespace sr_rt {
+using namespace sr;
+
+struct DevBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+};
+
+enum Stage { ST_VERTEX, ST_GEOMETRY, ST_SETUP, ST_SCAN, ST_EMIT, ST_SORT, ST_RANGES, ST_STRIPS,
+             ST_RASTER, ST_COUNT };
+
+struct BatchWork;
+
+}  // namespace sr_rt
+
+struct sr_context_t {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    sr_rt::DevBuf verts, gverts, recs, counts, offsets, keys, vals, keys2, vals2, tstart, tend, scan_tmp,
+        hist, misc, strips, strip_aux, litems, plots, lcounts, loffsets, pcounts, poffsets;
+    uint32_t* h_misc = nullptr;  // pinned readback words
+    hipEvent_t ev[sr_rt::ST_COUNT + 1] = {};
+    float stage_ms[sr_rt::ST_COUNT] = {};
+    int n_stage = 0;
+    sr_draw_stats last = {};
+    // execution tile chosen when sr_draw_desc.exec_tile == 0: the first
+    // entry whose pixel bound holds the framebuffer (set by the autotuner)
+    std::vector<std::pair<uint64_t, int>> tile_policy{{~0ull, 8}};
+    uint32_t batch_cap_hint = 0;             // initial bin-pair capacity of new batch workspaces
+    sr_rt::BatchWork* batch = nullptr;       // workspace of sr_batch_draw (on ctx->stream)
+};
+
+struct sr_mesh_t {
+    sr_context ctx;
+    float* soa[8] = {};  // px py pz nx ny nz u v (null if absent)
+    uint32_t* idx = nullptr;
+    uint32_t nv = 0, ni = 0;
+    int prim = 0;
+};
+
+struct sr_texture_t {
+    sr_context ctx;
+    uint32_t w = 0, h = 0;
+    uint32_t* d = nullptr;
+};
+
+struct sr_framebuffer_t {
+    sr_context ctx;
+    uint32_t w = 0, h = 0;
+    uint32_t* color = nullptr;
+    float* depth = nullptr;
+    uint8_t* stencil = nullptr;
+};
+
+namespace sr_rt {
+using namespace sr;
+
+int fail(sr_context ctx, int code, const std::string& msg);
+
+#define HIP_OK(ctx, call)                                                                        \
+    do {                                                                                         \
+        hipError_t e_ = (call);                                                                  \
+        if (e_ != hipSuccess)                                                                    \
+            return sr_rt::fail((ctx), SR_ERR_DEVICE, std::string(#call ": ") + hipGetErrorString(e_)); \
+    } while (0)
+
+// Grows `b` to hold `bytes` (with 25% headroom).  Work queued on `s` that
+// may still read the old buffer is drained before it is freed.
+int ensure(sr_context ctx, DevBuf& b, size_t bytes, hipStream_t s);
+inline int ensure(sr_context ctx, DevBuf& b, size_t bytes) { return ensure(ctx, b, bytes, ctx->stream); }
+
+template <class T>
+T* P(DevBuf& b) { return reinterpret_cast<T*>(b.p); }
+
+int bits_for(uint32_t n);  // bits to hold values 0..n-1
+int varyings_of(int vp);
+int policy_tile(sr_context ctx, uint32_t w, uint32_t h);
+int validate_draw(sr_context ctx, sr_framebuffer fb, sr_mesh mesh, const sr_draw_desc* d, sr_texture tex);
+
+// True when a draw can join a batch: triangles, no geometry program,
+// stencil Always/Keep, not the exhaustive (unbinned) debug mode.
+bool batchable(sr_mesh m, const sr_draw_desc* d);
+
+// ---------------------------------------------------------------------------
+// Batched pipeline
+// ---------------------------------------------------------------------------
+// One batch workspace: device buffers sized by the batch, pinned staging for
+// the scene tables and the readback words, a stream, and the cached graph of
+// the last synchronously drawn batch.
+struct BatchWork {
+    DevBuf verts, recs, counts, offsets, keys, vals, keys2, vals2, tstart, tend, hist, scan, tables, misc;
+    uint32_t cap = 0;          // bin pair capacity
+    std::vector<uint8_t> key;  // what the cached graph renders
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    uint8_t* h_stage = nullptr;  // pinned staging for the scene tables
+    size_t h_stage_bytes = 0;
+    uint32_t* h_misc = nullptr;  // pinned: [0] overflow [1] bin pairs [2] NaN flag
+    hipStream_t stream = nullptr;
+    bool own_stream = false;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+
+    int init(sr_context ctx, bool new_stream);
+    void release();
+    void drop_graph();
+    size_t device_bytes() const;
+};
+
+struct BatchClass {
+    uint64_t cls;
+    uint32_t ets, fp, blend, tile0, ntiles;
+};
+
+// Host tables of one batch.  `items` are copies (descriptors included), so a
+// plan outlives the caller's arrays.
+struct BatchPlan {
+    std::vector<sr_batch_item> items;
+    std::vector<sr_draw_desc> descs;
+    std::vector<SceneGeom> geom;
+    std::vector<uint32_t> vend, tend, tile_end;
+    std::vector<BatchClass> classes;
+    uint64_t NV = 0, NT = 0, NTILES = 0;
+};
+
+// Builds the tables for `items` (all batchable, validated, distinct
+// framebuffers); scenes are ordered by (execution tile, program, blend).
+int batch_plan(sr_context ctx, const sr_batch_item* items, uint32_t n, BatchPlan& plan);
+
+// Sizes the workspace, uploads the tables and queues the whole batch on
+// W.stream, ending with the readback of W.h_misc and W.ev1.  With
+// `use_graph` the sequence is captured once and replayed while the batch is
+// unchanged (the host waits for the table upload before capturing).
+int batch_enqueue(sr_context ctx, BatchWork& W, const BatchPlan& plan, bool use_graph);
+
+// After W.ev1 has completed: SR_OK with *again = false when the batch was
+// drawn; *again = true when the bins overflowed (nothing was drawn and the
+// capacity has grown: enqueue the same plan again).
+int batch_result(sr_context ctx, BatchWork& W, const BatchPlan& plan, bool* again);
+
+// Device bytes a batch of `tris` triangles and `verts` vertices needs in a
+// workspace with `pairs_per_tri` bin pairs per triangle.
+size_t batch_bytes_estimate(uint64_t tris, uint64_t verts, double pairs_per_tri);
+
+}  // namespace sr_rt

@@ -20,6 +20,7 @@ Workloads (sizes as the reference's benchmark plan names them):
 from __future__ import annotations
 
 import math
+import os
 
 import numpy as np
 
@@ -33,29 +34,77 @@ SEED_BATCH = 0x5EED0004
 SEED_SERVE = 0x5EED0005
 
 
+_POOL = None
+_CHUNK = 1 << 19
+
+
+def _pool():
+    global _POOL
+    if _POOL is None:
+        from concurrent.futures import ThreadPoolExecutor
+        _POOL = ThreadPoolExecutor(max_workers=max(1, min(16, os.cpu_count() or 1)))
+    return _POOL
+
+
+def _chunked(n: int, fn, dtype):
+    """fn(lo, hi) -> array of hi - lo values, evaluated in parallel chunks
+    (numpy releases the GIL inside each operation)."""
+    if n <= 2 * _CHUNK:
+        return fn(0, n)
+    out = np.empty(n, dtype)
+    bounds = [(lo, min(n, lo + _CHUNK)) for lo in range(0, n, _CHUNK)]
+
+    def run(b):
+        out[b[0]:b[1]] = fn(b[0], b[1])
+
+    list(_pool().map(run, bounds))
+    return out
+
+
 class SplitMix64:
-    """Vectorised SplitMix64 stream."""
+    """Vectorised SplitMix64 stream: draw k of the stream is
+    mix(state0 + k * GOLDEN), so any range of draws can be computed
+    independently (large requests run in parallel chunks)."""
 
     def __init__(self, seed: int):
         self.state = np.uint64(seed & 0xFFFFFFFFFFFFFFFF)
 
-    def next_u64(self, n: int) -> np.ndarray:
+    @staticmethod
+    def _mix(base, lo: int, hi: int) -> np.ndarray:
         with np.errstate(over="ignore"):
-            idx = np.arange(1, n + 1, dtype=np.uint64)
-            z = self.state + idx * GOLDEN
-            self.state = np.uint64(self.state + np.uint64(n) * GOLDEN)
+            idx = np.arange(lo + 1, hi + 1, dtype=np.uint64)
+            z = base + idx * GOLDEN
             z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
             z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
             return z ^ (z >> np.uint64(31))
 
+    def _advance(self, n: int):
+        base = self.state
+        with np.errstate(over="ignore"):
+            self.state = np.uint64(self.state + np.uint64(n) * GOLDEN)
+        return base
+
+    def next_u64(self, n: int) -> np.ndarray:
+        base = self._advance(n)
+        return _chunked(n, lambda lo, hi: self._mix(base, lo, hi), np.uint64)
+
     def uniform(self, n: int, lo: float = 0.0, hi: float = 1.0) -> np.ndarray:
-        u = (self.next_u64(n) >> np.uint64(40)).astype(np.float64) * (2.0 ** -24)
-        return lo + (hi - lo) * u
+        base = self._advance(n)
+
+        def f(a, b):
+            u = (self._mix(base, a, b) >> np.uint64(40)).astype(np.float64) * (2.0 ** -24)
+            return lo + (hi - lo) * u
+        return _chunked(n, f, np.float64)
 
     def normal(self, n: int) -> np.ndarray:
-        u1 = self.uniform(n)
-        u2 = self.uniform(n)
-        return np.sqrt(-2.0 * np.log1p(-u1)) * np.cos(2.0 * math.pi * u2)
+        b1 = self._advance(n)
+        b2 = self._advance(n)
+
+        def f(a, b):
+            u1 = (self._mix(b1, a, b) >> np.uint64(40)).astype(np.float64) * (2.0 ** -24)
+            u2 = (self._mix(b2, a, b) >> np.uint64(40)).astype(np.float64) * (2.0 ** -24)
+            return np.sqrt(-2.0 * np.log1p(-u1)) * np.cos(2.0 * math.pi * u2)
+        return _chunked(n, f, np.float64)
 
 
 # ---------------------------------------------------------------------------
@@ -178,6 +227,12 @@ def sphere_scene(width: int = 512, height: int = 512):
 # ---------------------------------------------------------------------------
 # Clip-space triangle soup (the 1080p / 1M-triangle frame)
 # ---------------------------------------------------------------------------
+def _unit3(x, y, z):
+    """(x, y, z) / max(|(x, y, z)|, 1e-12), componentwise."""
+    n = np.maximum(np.sqrt((x * x + y * y) + z * z), 1e-12)
+    return x / n, y / n, z / n
+
+
 def _soup_arrays(rng: SplitMix64, n: int, width: int, height: int,
                  median_area: float = 20.0, sigma: float = 1.0,
                  z_range=(-0.99, 0.99), cover=1.0):
@@ -193,11 +248,13 @@ def _soup_arrays(rng: SplitMix64, n: int, width: int, height: int,
     y = cy[:, None] + r[:, None] * np.sin(ang) * (2.0 / height)
     z = np.repeat(cz[:, None], 3, 1)
     nv = 3 * n
-    nrm = np.stack([rng.normal(nv), rng.normal(nv), rng.normal(nv)], -1)
-    nrm /= np.maximum(np.linalg.norm(nrm, axis=1, keepdims=True), 1e-12)
-    uv = np.stack([rng.uniform(nv), rng.uniform(nv)], -1)
+    nx, ny, nz = _unit3(rng.normal(nv), rng.normal(nv), rng.normal(nv))
+    nrm = np.empty((nv, 3), np.float32)
+    nrm[:, 0], nrm[:, 1], nrm[:, 2] = nx, ny, nz
+    uv = np.empty((nv, 2), np.float32)
+    uv[:, 0], uv[:, 1] = rng.uniform(nv), rng.uniform(nv)
     return (x.reshape(-1).astype(np.float32), y.reshape(-1).astype(np.float32),
-            z.reshape(-1).astype(np.float32), nrm.astype(np.float32), uv.astype(np.float32))
+            z.reshape(-1).astype(np.float32), nrm, uv)
 
 
 def tex_lambert_uniforms(alpha: float = 0.8) -> np.ndarray:
@@ -326,20 +383,20 @@ def serve_mesh(n_tris: int, seed: int):
     """A closed-ish object-space blob: soup triangles scattered in a unit ball
     so an orbit camera at distance 2..4 sees all of it."""
     rng = SplitMix64(seed)
-    c = np.stack([rng.normal(n_tris), rng.normal(n_tris), rng.normal(n_tris)], -1)
-    c /= np.maximum(np.linalg.norm(c, axis=1, keepdims=True), 1e-12)
-    c *= np.cbrt(rng.uniform(n_tris))[:, None] * 0.8
+    cx, cy, cz = _unit3(rng.normal(n_tris), rng.normal(n_tris), rng.normal(n_tris))
+    rad = np.cbrt(rng.uniform(n_tris)) * 0.8
+    cx, cy, cz = cx * rad, cy * rad, cz * rad
     size = 0.02 * np.exp(0.5 * rng.normal(n_tris)) * (1e4 / max(n_tris, 1e4)) ** 0.5
-    offs = np.stack([rng.normal(3 * n_tris), rng.normal(3 * n_tris), rng.normal(3 * n_tris)], -1)
-    offs = offs.reshape(n_tris, 3, 3) * size[:, None, None]
-    p = (c[:, None, :] + offs).reshape(-1, 3)
-    nrm = np.stack([rng.normal(3 * n_tris), rng.normal(3 * n_tris), rng.normal(3 * n_tris)], -1)
-    nrm /= np.maximum(np.linalg.norm(nrm, axis=1, keepdims=True), 1e-12)
-    uv = np.stack([rng.uniform(3 * n_tris), rng.uniform(3 * n_tris)], -1)
-    return dict(px=p[:, 0].astype(np.float32), py=p[:, 1].astype(np.float32),
-                pz=p[:, 2].astype(np.float32), nx=nrm[:, 0].astype(np.float32),
-                ny=nrm[:, 1].astype(np.float32), nz=nrm[:, 2].astype(np.float32),
-                u=uv[:, 0].astype(np.float32), v=uv[:, 1].astype(np.float32), indices=None,
+    size3 = np.repeat(size, 3)
+    ox, oy, oz = rng.normal(3 * n_tris), rng.normal(3 * n_tris), rng.normal(3 * n_tris)
+    px = np.repeat(cx, 3) + ox * size3
+    py = np.repeat(cy, 3) + oy * size3
+    pz = np.repeat(cz, 3) + oz * size3
+    nx, ny, nz = _unit3(rng.normal(3 * n_tris), rng.normal(3 * n_tris), rng.normal(3 * n_tris))
+    u, v = rng.uniform(3 * n_tris), rng.uniform(3 * n_tris)
+    f = np.float32
+    return dict(px=px.astype(f), py=py.astype(f), pz=pz.astype(f), nx=nx.astype(f),
+                ny=ny.astype(f), nz=nz.astype(f), u=u.astype(f), v=v.astype(f), indices=None,
                 primitive=3)
 
 
@@ -351,3 +408,38 @@ def orbit_camera(rng: SplitMix64, width: int, height: int):
     view = look_at(eye, (0, 0, 0), (0, 1, 0))
     proj = perspective_zero_to_one(60.0, width / height, 0.1, 100.0)
     return proj @ view
+
+
+SERVE_RESOLUTIONS = [(512, 512), (1920, 1080)]
+
+
+def serve_desc(mvp: np.ndarray, width: int, height: int, alpha: float = 0.8) -> dict:
+    """Draw state of a served frame: the soup shading (checkerboard x
+    Lambert, over-blend at `alpha`), no culling, full-frame viewport."""
+    return dict(vertex_program=0, fragment_program=2, blend=1, cull=0,
+                vs_uniforms=vs_uniforms(mvp), fs_uniforms=tex_lambert_uniforms(alpha),
+                tex_filter=1, tex_edge=0,
+                viewport=(0.0, 0.0, float(width), float(height), 0.0, 1.0))
+
+
+def serve_requests(n: int, n_meshes: int = 64, seed: int = SEED_SERVE + 0x10000,
+                   resolutions=SERVE_RESOLUTIONS):
+    """Per request: mesh id U[0, n_meshes), resolution uniform over
+    `resolutions`, orbit camera (azimuth U(0, 2pi), elevation U(-60, 60) deg,
+    distance U(2, 4))."""
+    rng = SplitMix64(seed)
+    out = []
+    for _ in range(n):
+        m = int(float(rng.uniform(1)[0]) * n_meshes) % n_meshes
+        w, h = resolutions[int(float(rng.uniform(1)[0]) * len(resolutions)) % len(resolutions)]
+        out.append(dict(mesh=m, width=w, height=h, desc=serve_desc(orbit_camera(rng, w, h), w, h)))
+    return out
+
+
+def poisson_arrivals(rate: float, duration: float, seed: int) -> np.ndarray:
+    """Arrival times in [0, duration) of a Poisson process of `rate` per second."""
+    rng = SplitMix64(seed)
+    n = int(rate * duration * 1.5) + 64
+    gaps = -np.log1p(-rng.uniform(n)) / rate
+    t = np.cumsum(gaps)
+    return t[t < duration]

@@ -1,0 +1,235 @@
+"""Serving workload: Poisson request arrivals against a device-resident cache
+of seeded meshes, served by the continuous-batching request server
+(sr_server_*) and, for comparison, by the plain per-request path (one
+sr_fb_clear + sr_draw + colour read-back per request, in arrival order).
+
+Each request is one mesh of the cache (64 meshes, log-uniform 1e3..1e6
+triangles), an orbit camera and a resolution from {512^2, 1920x1080}, drawn
+with the soup shading (checkerboard x Lambert, 20% translucent over-blend).
+Latency is arrival -> pixels in host memory.  For every rate the result
+reports p50/p99/max latency and completed frames/s; `capacity` is frames/s
+with every request queued at once.
+
+    python serve.py [--rates 200,500,1000,2000] [--duration 10] [--meshes 64]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import threading
+import time
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+if HERE not in sys.path:
+    sys.path.insert(0, HERE)
+
+import ctypes as C  # noqa: E402
+
+import numpy as np  # noqa: E402
+
+import scenes  # noqa: E402
+import softrender_hip as sr  # noqa: E402
+
+
+class World:
+    """Resident meshes and texture, plus prebuilt draw descriptors."""
+
+    def __init__(self, ctx, n_meshes=64, lo=1e3, hi=1e6, seed=scenes.SEED_SERVE):
+        self.ctx = ctx
+        self.specs = scenes.serve_mesh_specs(n_meshes, lo, hi, seed)
+        self.meshes = [ctx.mesh(scenes.serve_mesh(n, sd)) for n, sd in self.specs]
+        self.tris = [n for n, _ in self.specs]
+        self.tex = ctx.texture(1024, 1024, scenes.checkerboard(1024, 32))
+
+    def close(self):
+        for m in self.meshes:
+            m.free()
+        self.tex.free()
+
+
+def latency_summary(lat, t_first, t_last, n):
+    lat = np.asarray(lat) * 1e3
+    span = max(t_last - t_first, 1e-9)
+    return dict(completed=int(n), fps=n / span, p50_ms=float(np.percentile(lat, 50)),
+                p99_ms=float(np.percentile(lat, 99)), max_ms=float(lat.max()),
+                mean_ms=float(lat.mean()))
+
+
+def _wait_until(clock, t):
+    while True:
+        d = t - clock()
+        if d <= 0:
+            return
+        time.sleep(d - 0.0002 if d > 0.0004 else 0)
+
+
+def run_served(srv, world, reqs, arrivals):
+    """Submits reqs[i] at arrivals[i] (seconds from now) to the running
+    server and collects every completion."""
+    st0 = srv.stats()
+    descs = [sr.make_desc(**r["desc"]) for r in reqs]
+    clock = srv.clock
+    t0 = clock() + 0.05
+    err = []
+
+    def submitter():
+        try:
+            for i, a in enumerate(arrivals):
+                t = t0 + float(a)
+                _wait_until(clock, t)
+                r = reqs[i]
+                srv.submit(world.meshes[r["mesh"]], descs[i], r["width"], r["height"],
+                           tex=world.tex, tag=i, arrival=t)
+        except Exception as e:  # surfaced after the poll loop
+            err.append(e)
+
+    th = threading.Thread(target=submitter)
+    th.start()
+    got = []
+    deadline = time.time() + 120 + float(arrivals[-1] if len(arrivals) else 0)
+    while len(got) < len(arrivals) and not err:
+        got += srv.poll(256, timeout_ms=20, copy=False)
+        if time.time() > deadline:
+            break
+    th.join()
+    st = srv.stats()
+    if err:
+        raise err[0]
+    ok = [g for g in got if g["status"] == 0]
+    lat = [g["t_done"] - g["t_arrival"] for g in ok]
+    out = latency_summary(lat, t0, max(g["t_done"] for g in ok), len(ok))
+    out.update(failed=len(got) - len(ok), lost=len(arrivals) - len(got),
+               mean_batch=float(np.mean([g["batch_frames"] for g in ok])),
+               batches=int(st["batches"] - st0["batches"]),
+               overflow_reruns=int(st["overflow_reruns"] - st0["overflow_reruns"]),
+               peak_in_flight_gb=st["peak_in_flight_bytes"] / 2**30,
+               device_gb=st["device_bytes"] / 2**30)
+    return out
+
+
+class _HostBufs:
+    """Pinned host buffers (hipHostMalloc) for the plain path's read-back,
+    like the server's own result buffers."""
+
+    def __init__(self):
+        self.bufs = {}
+        self.hip = None
+        for name in ("libamdhip64.so", "/opt/rocm/lib/libamdhip64.so"):
+            try:
+                self.hip = C.CDLL(name)
+                break
+            except OSError:
+                continue
+        if self.hip is None:
+            raise RuntimeError("libamdhip64.so not found")
+        self.hip.hipHostMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t, C.c_uint]
+        self.hip.hipHostFree.argtypes = [C.c_void_p]
+
+    def get(self, px):
+        if px not in self.bufs:
+            p = C.c_void_p()
+            if self.hip.hipHostMalloc(C.byref(p), px * 4, 0) != 0:
+                raise RuntimeError("hipHostMalloc failed")
+            self.bufs[px] = p.value
+        return self.bufs[px]
+
+    def free(self):
+        for p in self.bufs.values():
+            self.hip.hipHostFree(p)
+        self.bufs = {}
+
+
+def run_plain(ctx, world, reqs, arrivals, max_seconds=60.0):
+    """One request at a time in arrival order; stops after max_seconds of
+    wall time (the queue grows without bound above the path's capacity)."""
+    fbs, hb = {}, _HostBufs()
+    descs = [sr.make_desc(**r["desc"]) for r in reqs]
+    for r in reqs:
+        k = (r["width"], r["height"])
+        if k not in fbs:
+            fbs[k] = ctx.framebuffer(*k)
+            hb.get(k[0] * k[1])
+    clock = sr.Server.clock
+    t0 = clock() + 0.05
+    lat, t_last = [], t0
+    for i, a in enumerate(arrivals):
+        t = t0 + float(a)
+        _wait_until(clock, t)
+        r = reqs[i]
+        fb = fbs[(r["width"], r["height"])]
+        fb.clear(0xFF000000)
+        ctx.draw(fb, world.meshes[r["mesh"]], descs[i], world.tex)
+        fb.read_color_into(hb.get(r["width"] * r["height"]))
+        t_last = clock()
+        lat.append(t_last - t)
+        if t_last - t0 > max_seconds:
+            break
+    for fb in fbs.values():
+        fb.free()
+    hb.free()
+    out = latency_summary(lat, t0, t_last, len(lat))
+    out["truncated"] = len(lat) < len(arrivals)
+    return out
+
+
+def run(rates=(200, 500, 1000, 2000), duration=10.0, n_meshes=64, capacity_n=512,
+        server_cfg=None, plain=True, tune=None, seed=scenes.SEED_SERVE):
+    ctx = sr.Context(0)
+    if tune:
+        import autotune
+        autotune.apply(ctx, tune)
+    t = time.time()
+    world = World(ctx, n_meshes)
+    build_s = time.time() - t
+    cfg = dict(hbm_budget=64 << 30, max_inflight=2, max_batch_frames=256, max_batch_tris=48 << 20,
+               pairs_per_tri=(tune or {}).get("pairs_per_tri", 4.0))
+    cfg.update(server_cfg or {})
+    res = dict(meshes=n_meshes, mesh_tris_total=int(sum(world.tris)),
+               mean_mesh_tris=float(np.mean(world.tris)), build_s=build_s, server=cfg, rates={})
+    # capacity: everything queued at once
+    reqs = scenes.serve_requests(capacity_n, n_meshes=n_meshes, seed=seed + 0x20000)
+    zero = np.zeros(capacity_n)
+    srv = sr.Server(ctx, **cfg)
+    run_served(srv, world, reqs[:64], zero[:64])  # warm: pools and workspaces
+    res["capacity_served"] = run_served(srv, world, reqs, zero)
+    if plain:
+        run_plain(ctx, world, reqs[:16], zero[:16])
+        res["capacity_plain"] = run_plain(ctx, world, reqs, zero)
+        res["capacity_ratio"] = res["capacity_served"]["fps"] / res["capacity_plain"]["fps"]
+    for k, rate in enumerate(rates):
+        arr = scenes.poisson_arrivals(rate, duration, seed + 0x30000 + k)
+        reqs = scenes.serve_requests(len(arr), n_meshes=n_meshes, seed=seed + 0x40000 + k)
+        row = dict(offered=rate, requests=len(arr), served=run_served(srv, world, reqs, arr))
+        if plain:
+            row["plain"] = run_plain(ctx, world, reqs, arr, max_seconds=min(duration * 1.5, 30.0))
+        res["rates"][str(rate)] = row
+        print(json.dumps({"rate": rate, **{k2: v for k2, v in row["served"].items()
+                                           if k2 in ("fps", "p50_ms", "p99_ms", "mean_batch")}}),
+              file=sys.stderr, flush=True)
+    srv.close()
+    world.close()
+    ctx.close()
+    return res
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--rates", default="200,500,1000,2000")
+    ap.add_argument("--duration", type=float, default=10.0)
+    ap.add_argument("--meshes", type=int, default=64)
+    ap.add_argument("--capacity-n", type=int, default=512)
+    ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--inflight", type=int, default=2)
+    ap.add_argument("--tune", default="", help="autotuner JSON to apply")
+    a = ap.parse_args()
+    tune = json.load(open(a.tune)) if a.tune else None
+    rates = [float(x) for x in a.rates.split(",") if x]
+    out = run(rates, a.duration, a.meshes, a.capacity_n, dict(max_inflight=a.inflight),
+              plain=not a.no_plain, tune=tune)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

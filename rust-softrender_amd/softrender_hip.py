@@ -19,7 +19,7 @@ SR_OK = 0
 ERRORS = {
     -1: "SR_ERR_INVALID_ARGUMENT", -2: "SR_ERR_OUT_OF_MEMORY", -3: "SR_ERR_DEVICE",
     -4: "SR_ERR_INVALID_COORDINATE", -5: "SR_ERR_UNSUPPORTED", -6: "SR_ERR_BUDGET",
-    -7: "SR_ERR_NOT_READY", -8: "SR_ERR_NO_DEVICE",
+    -7: "SR_ERR_NOT_READY", -8: "SR_ERR_NO_DEVICE", -9: "SR_ERR_CANCELLED",
 }
 
 EXPORTED = [
@@ -28,6 +28,8 @@ EXPORTED = [
     "sr_texture_upload", "sr_texture_free", "sr_fb_create", "sr_fb_free", "sr_fb_clear",
     "sr_fb_read", "sr_fb_write", "sr_fb_size", "sr_draw", "sr_batch_draw",
     "sr_ctx_set_exec_tile_policy", "sr_ctx_set_batch_capacity",
+    "sr_server_create", "sr_server_destroy", "sr_server_submit", "sr_server_poll",
+    "sr_server_release", "sr_server_get_stats", "sr_server_last_error", "sr_server_clock",
 ]
 
 
@@ -62,6 +64,34 @@ class DrawStats(C.Structure):
 class BatchItem(C.Structure):
     _fields_ = [("fb", C.c_void_p), ("mesh", C.c_void_p), ("desc", C.POINTER(DrawDesc)),
                 ("tex", C.c_void_p), ("clear", C.c_int32), ("clear_rgba", C.c_uint32)]
+
+
+class ServerConfig(C.Structure):
+    _fields_ = [("hbm_budget", C.c_uint64), ("max_inflight", C.c_uint32),
+                ("max_batch_frames", C.c_uint32), ("max_batch_tris", C.c_uint64),
+                ("pairs_per_tri", C.c_float), ("reserved", C.c_int32 * 7)]
+
+
+class Request(C.Structure):
+    _fields_ = [("mesh", C.c_void_p), ("tex", C.c_void_p), ("desc", DrawDesc),
+                ("width", C.c_uint32), ("height", C.c_uint32), ("clear_rgba", C.c_uint32),
+                ("want_depth", C.c_int32), ("tag", C.c_uint64), ("arrival", C.c_double)]
+
+
+class Completion(C.Structure):
+    _fields_ = [("ticket", C.c_uint64), ("tag", C.c_uint64), ("status", C.c_int32),
+                ("width", C.c_uint32), ("height", C.c_uint32), ("batch_frames", C.c_uint32),
+                ("batch_id", C.c_uint64), ("rgba", C.POINTER(C.c_uint32)),
+                ("depth", C.POINTER(C.c_float)), ("t_arrival", C.c_double),
+                ("t_admit", C.c_double), ("t_done", C.c_double)]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [(n, C.c_uint64) for n in (
+        "submitted", "completed", "failed", "batches", "overflow_reruns", "budget_bytes",
+        "in_flight_bytes", "peak_in_flight_bytes", "device_bytes")] + [
+        (n, C.c_uint32) for n in ("pending", "inflight_batches", "max_batch_frames_seen",
+                                  "reserved")]
 
 
 def make_desc(**kw) -> DrawDesc:
@@ -136,6 +166,17 @@ def lib():
         L.sr_ctx_set_exec_tile_policy.argtypes = [vp, C.c_uint32, C.POINTER(C.c_uint64),
                                                   C.POINTER(C.c_int32)]
         L.sr_ctx_set_batch_capacity.argtypes = [vp, C.c_uint32]
+        L.sr_server_create.argtypes = [vp, C.POINTER(ServerConfig), C.POINTER(vp)]
+        L.sr_server_destroy.argtypes = [vp]
+        L.sr_server_submit.argtypes = [vp, C.POINTER(Request), C.POINTER(C.c_uint64)]
+        L.sr_server_poll.argtypes = [vp, C.POINTER(Completion), C.c_uint32, C.POINTER(C.c_uint32),
+                                     C.c_int32]
+        L.sr_server_release.argtypes = [vp, C.c_uint64]
+        L.sr_server_get_stats.argtypes = [vp, C.POINTER(ServerStats)]
+        L.sr_server_last_error.argtypes = [vp]
+        L.sr_server_last_error.restype = C.c_char_p
+        L.sr_server_clock.argtypes = []
+        L.sr_server_clock.restype = C.c_double
         for name in EXPORTED:
             f = getattr(L, name)
             if f.restype is None and name not in ("sr_last_error", "sr_ctx_stream"):
@@ -227,6 +268,78 @@ class Context:
         self._check(lib().sr_ctx_set_batch_capacity(self.h, pairs))
 
 
+class Server:
+    """Continuous-batching request server over a context's resident meshes.
+
+    submit() queues a request and returns its ticket; poll() returns
+    completions as dicts whose "rgba" (and "depth") are flat numpy copies of
+    the pinned result buffers (the buffers go back to the pool right away).
+    """
+
+    def __init__(self, ctx: Context, hbm_budget=0, max_inflight=0, max_batch_frames=0,
+                 max_batch_tris=0, pairs_per_tri=0.0):
+        self.ctx = ctx
+        self.h = C.c_void_p()
+        cfg = ServerConfig(hbm_budget, max_inflight, max_batch_frames, max_batch_tris,
+                           pairs_per_tri)
+        rc = lib().sr_server_create(ctx.h, C.byref(cfg), C.byref(self.h))
+        if rc != SR_OK:
+            ctx._check(rc)
+
+    def _check(self, rc):
+        if rc != SR_OK:
+            raise SoftrenderError(rc, (lib().sr_server_last_error(self.h) or b"").decode())
+        return rc
+
+    @staticmethod
+    def clock() -> float:
+        return lib().sr_server_clock()
+
+    def submit(self, mesh: "Mesh", desc: DrawDesc, width: int, height: int, tex: "Texture" = None,
+               clear_rgba: int = 0xFF000000, want_depth: bool = False, tag: int = 0,
+               arrival: float = 0.0) -> int:
+        r = Request(mesh.h, tex.h if tex else None, desc, width, height, clear_rgba,
+                    1 if want_depth else 0, tag, arrival)
+        t = C.c_uint64()
+        self._check(lib().sr_server_submit(self.h, C.byref(r), C.byref(t)))
+        return t.value
+
+    def poll(self, max_n: int = 64, timeout_ms: int = 0, copy: bool = True):
+        buf = (Completion * max_n)()
+        n = C.c_uint32()
+        self._check(lib().sr_server_poll(self.h, buf, max_n, C.byref(n), timeout_ms))
+        out = []
+        for i in range(n.value):
+            c = buf[i]
+            d = dict(ticket=c.ticket, tag=c.tag, status=c.status, width=c.width, height=c.height,
+                     batch_frames=c.batch_frames, batch_id=c.batch_id, t_arrival=c.t_arrival,
+                     t_admit=c.t_admit, t_done=c.t_done)
+            if copy and c.status == SR_OK:
+                px = c.width * c.height
+                d["rgba"] = np.ctypeslib.as_array(c.rgba, shape=(px,)).copy()
+                if c.depth:
+                    d["depth"] = np.ctypeslib.as_array(c.depth, shape=(px,)).copy()
+            self._check(lib().sr_server_release(self.h, c.ticket))
+            out.append(d)
+        return out
+
+    def stats(self) -> dict:
+        st = ServerStats()
+        self._check(lib().sr_server_get_stats(self.h, C.byref(st)))
+        return {k: getattr(st, k) for k, _ in ServerStats._fields_ if k != "reserved"}
+
+    def close(self):
+        if self.h:
+            lib().sr_server_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 class Mesh:
     def __init__(self, ctx: Context, s: dict):
         self.ctx = ctx
@@ -279,6 +392,10 @@ class Framebuffer:
                                          dep.ctypes.data_as(C.POINTER(C.c_float)),
                                          st.ctypes.data_as(C.POINTER(C.c_uint8))))
         return col, dep, st
+
+    def read_color_into(self, ptr):
+        """Copies the colour plane to host memory at `ptr` (w*h uint32)."""
+        self.ctx._check(lib().sr_fb_read(self.h, C.cast(ptr, C.POINTER(C.c_uint32)), None, None))
 
     def write(self, col, dep, st):
         col = np.ascontiguousarray(col, np.uint32)
