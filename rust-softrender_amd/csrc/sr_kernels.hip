@@ -323,9 +323,72 @@ __global__ __launch_bounds__(256) void k_gs_emit(GSParams P) {
 // ===========================================================================
 // Returns the number of execution tiles the triangle's box covers and
 // writes its record (X0 > X1 marks culled / invalid).
+// Conservative coverage test of the pixel rectangle [x0,x1] x [y0,y1]:
+// false only when no pixel centre in it can pass the inside test of
+// rasterize_triangle (triangle.rs:104-113).  The barycentrics u, v, w are
+// linear in the pixel centre, so their maxima over the rectangle are at its
+// corners; they are evaluated there in f64 and the rectangle is rejected
+// only when one of them stays below minus sixteen times the error bound of
+// the f32 evaluation (edge products, sums and the correctly rounded
+// quotient: |u_f32 - u| <= 2^-22 * m_u, m_u = (|e23y dx| + |e23x dy|) / |det|).
+// A triangle with det == 0 never draws (its barycentrics are infinite or
+// NaN, so either a sign test or the depth test fails).  Only valid when the
+// draw has no stencil effects, which apply to the whole clamped box.
+__device__ __forceinline__ bool tile_may_cover(const TriRec& r, int x0, int x1, int y0, int y1) {
+    const double R = r.R;
+    if (!(fabs(R) < __builtin_huge_val())) return false;
+    const double ey23 = r.e23y, ex23 = r.e23x, ey31 = r.e31y, ex31 = r.e31x;
+    const double x3 = r.x3, y3 = r.y3;
+    double umax = -__builtin_huge_val(), vmax = umax, wmax = umax, mu = 0.0, mv = 0.0, au = 0.0, av = 0.0;
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        const double dx = ((k & 1) ? (double)x1 : (double)x0) + 0.5 - x3;
+        const double dy = ((k & 2) ? (double)y1 : (double)y0) + 0.5 - y3;
+        const double u = (ey23 * dx + ex23 * dy) * R;
+        const double v = (ey31 * dx + ex31 * dy) * R;
+        const double w = (1.0 - u) - v;
+        umax = fmax(umax, u);
+        vmax = fmax(vmax, v);
+        wmax = fmax(wmax, w);
+        mu = fmax(mu, (fabs(ey23 * dx) + fabs(ex23 * dy)) * fabs(R));
+        mv = fmax(mv, (fabs(ey31 * dx) + fabs(ex31 * dy)) * fabs(R));
+        au = fmax(au, fabs(u));
+        av = fmax(av, fabs(v));
+    }
+    const double s = 0x1p-18;
+    const double du = s * (mu + 1.0), dv = s * (mv + 1.0);
+    const double dw = du + dv + s * (2.0 + au + av);
+    return !(umax < -du || vmax < -dv || wmax < -dw);
+}
+
+// Execution tiles of a triangle's clamped box: [ex0,ex1] x [ey0,ey1] and
+// the pixel box [px0,px1] x [py0,py1]; false if the box misses the
+// framebuffer.
+struct TileSpan {
+    int32_t px0, px1, py0, py1;
+    uint32_t ex0, ex1, ey0, ey1;
+};
+
+__device__ __forceinline__ bool tile_span(const TriRec& r, uint32_t Wi, uint32_t Hi, uint32_t ets, TileSpan& s) {
+    s.px0 = r.X0 > 0 ? r.X0 : 0;
+    s.px1 = r.X1 < (int32_t)Wi - 1 ? r.X1 : (int32_t)Wi - 1;
+    s.py0 = r.Y0 > 0 ? r.Y0 : 0;
+    s.py1 = r.Y1 < (int32_t)Hi - 1 ? r.Y1 : (int32_t)Hi - 1;
+    if (s.px0 > s.px1 || s.py0 > s.py1) return false;
+    s.ex0 = s.px0 / ets; s.ex1 = s.px1 / ets; s.ey0 = s.py0 / ets; s.ey1 = s.py1 / ets;
+    return true;
+}
+
+__device__ __forceinline__ bool span_tile_covered(const TriRec& r, const TileSpan& s, uint32_t ets,
+                                                  uint32_t ex, uint32_t ey) {
+    const int tx0 = max(s.px0, (int)(ex * ets)), tx1 = min(s.px1, (int)(ex * ets + ets - 1));
+    const int ty0 = max(s.py0, (int)(ey * ets)), ty1 = min(s.py1, (int)(ey * ets + ets - 1));
+    return tile_may_cover(r, tx0, tx1, ty0, ty1);
+}
+
 __device__ __forceinline__ uint32_t tri_setup_one(const VertexBuf& v, uint32_t i0, uint32_t i1,
                                                   uint32_t i2, int cull, uint32_t Wi, uint32_t Hi,
-                                                  uint32_t ets, TriRec& r, uint32_t* err) {
+                                                  uint32_t ets, TriRec& r, uint32_t* err, int empty_cull) {
     const float* sx = v.comp(0);
     const float* sy = v.comp(1);
     const float* sz = v.comp(2);
@@ -359,11 +422,13 @@ __device__ __forceinline__ uint32_t tri_setup_one(const VertexBuf& v, uint32_t i
     r.X1 = (int32_t)floorf(clampf(mxx, -1.0f, W));
     r.Y0 = (int32_t)floorf(clampf(mny, -1.0f, H));
     r.Y1 = (int32_t)floorf(clampf(mxy, -1.0f, H));
-    const int32_t px0 = r.X0 > 0 ? r.X0 : 0, px1 = r.X1 < (int32_t)Wi - 1 ? r.X1 : (int32_t)Wi - 1;
-    const int32_t py0 = r.Y0 > 0 ? r.Y0 : 0, py1 = r.Y1 < (int32_t)Hi - 1 ? r.Y1 : (int32_t)Hi - 1;
-    if (px0 > px1 || py0 > py1) return 0;
-    const uint32_t ex0 = px0 / ets, ex1 = px1 / ets, ey0 = py0 / ets, ey1 = py1 / ets;
-    return (ex1 - ex0 + 1) * (ey1 - ey0 + 1);
+    TileSpan sp;
+    if (!tile_span(r, Wi, Hi, ets, sp)) return 0;
+    if (!empty_cull) return (sp.ex1 - sp.ex0 + 1) * (sp.ey1 - sp.ey0 + 1);
+    uint32_t n = 0;
+    for (uint32_t ey = sp.ey0; ey <= sp.ey1; ey++)
+        for (uint32_t ex = sp.ex0; ex <= sp.ex1; ex++) n += span_tile_covered(r, sp, ets, ex, ey) ? 1u : 0u;
+    return n;
 }
 
 __global__ __launch_bounds__(256) void k_tri_setup(SetupParams P) {
@@ -371,7 +436,8 @@ __global__ __launch_bounds__(256) void k_tri_setup(SetupParams P) {
     if (t >= P.n) return;
     TriRec r;
     P.counts[t] = tri_setup_one(P.v, prim_vertex(P.idx, 3 * t), prim_vertex(P.idx, 3 * t + 1),
-                                prim_vertex(P.idx, 3 * t + 2), P.cull, P.W, P.H, P.ets, r, P.err);
+                                prim_vertex(P.idx, 3 * t + 2), P.cull, P.W, P.H, P.ets, r, P.err,
+                                P.empty_cull);
     P.recs[t] = r;
 }
 
@@ -383,7 +449,8 @@ __global__ __launch_bounds__(256) void k_tri_setup_batch(BatchTables T, VertexBu
     const uint32_t l = t - S.tbase;
     TriRec r;
     counts[t] = tri_setup_one(v, S.vbase + prim_vertex(S.idx, 3 * l), S.vbase + prim_vertex(S.idx, 3 * l + 1),
-                              S.vbase + prim_vertex(S.idx, 3 * l + 2), S.cull, S.W, S.H, S.ets, r, err);
+                              S.vbase + prim_vertex(S.idx, 3 * l + 2), S.cull, S.W, S.H, S.ets, r, err,
+                              S.empty_cull);
     recs[t] = r;
 }
 
@@ -403,12 +470,12 @@ __global__ __launch_bounds__(256) void k_tri_emit_batch(BatchTables T, const Tri
         return;
     }
     const SceneGeom& S = T.scenes[scene_of(T.tend, T.nscenes, t)];
-    const TriRec& r = recs[t];
-    const int32_t px0 = r.X0 > 0 ? r.X0 : 0, px1 = r.X1 < (int32_t)S.W - 1 ? r.X1 : (int32_t)S.W - 1;
-    const int32_t py0 = r.Y0 > 0 ? r.Y0 : 0, py1 = r.Y1 < (int32_t)S.H - 1 ? r.Y1 : (int32_t)S.H - 1;
-    const uint32_t ex0 = px0 / S.ets, ex1 = px1 / S.ets, ey0 = py0 / S.ets, ey1 = py1 / S.ets;
-    for (uint32_t ey = ey0; ey <= ey1; ey++)
-        for (uint32_t ex = ex0; ex <= ex1; ex++) {
+    const TriRec r = recs[t];
+    TileSpan sp;
+    if (!tile_span(r, S.W, S.H, S.ets, sp)) return;
+    for (uint32_t ey = sp.ey0; ey <= sp.ey1; ey++)
+        for (uint32_t ex = sp.ex0; ex <= sp.ex1; ex++) {
+            if (S.empty_cull && !span_tile_covered(r, sp, S.ets, ex, ey)) continue;
             keys[o] = S.tile_base + ey * S.ntx + ex;
             vals[o] = t;
             o++;
@@ -419,13 +486,13 @@ __global__ __launch_bounds__(256) void k_tri_emit(EmitParams P) {
     const uint32_t t = blockIdx.x * 256u + threadIdx.x;
     if (t >= P.n) return;
     if (P.counts[t] == 0) return;
-    const TriRec& r = P.recs[t];
-    const int32_t px0 = r.X0 > 0 ? r.X0 : 0, px1 = r.X1 < (int32_t)P.W - 1 ? r.X1 : (int32_t)P.W - 1;
-    const int32_t py0 = r.Y0 > 0 ? r.Y0 : 0, py1 = r.Y1 < (int32_t)P.H - 1 ? r.Y1 : (int32_t)P.H - 1;
-    const uint32_t ex0 = px0 / P.ets, ex1 = px1 / P.ets, ey0 = py0 / P.ets, ey1 = py1 / P.ets;
+    const TriRec r = P.recs[t];
+    TileSpan sp;
+    if (!tile_span(r, P.W, P.H, P.ets, sp)) return;
     uint32_t o = P.offsets[t];
-    for (uint32_t ey = ey0; ey <= ey1; ey++)
-        for (uint32_t ex = ex0; ex <= ex1; ex++) {
+    for (uint32_t ey = sp.ey0; ey <= sp.ey1; ey++)
+        for (uint32_t ex = sp.ex0; ex <= sp.ex1; ex++) {
+            if (P.empty_cull && !span_tile_covered(r, sp, P.ets, ex, ey)) continue;
             P.keys[o] = P.key_base + ey * P.ntx + ex;
             P.vals[o] = P.val_base + t;
             o++;

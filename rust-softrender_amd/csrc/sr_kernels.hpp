@@ -36,6 +36,7 @@ struct SetupParams {
     TriRec* recs;
     uint32_t* counts;
     uint32_t* err;
+    int empty_cull;  // bin only execution tiles the triangle may cover (no stencil effects)
 };
 
 struct EmitParams {
@@ -46,6 +47,7 @@ struct EmitParams {
     uint32_t key_base, val_base;
     uint32_t* keys;
     uint32_t* vals;
+    int empty_cull;
 };
 
 // One scene of a batch: its vertex program, mesh streams and placement in
@@ -57,6 +59,7 @@ struct SceneGeom {
     uint32_t nt, tbase;       // triangles, first batch triangle
     int program, K, cull;
     uint32_t W, H, ets, ntx, tile_base;
+    int empty_cull;
     sr_viewport viewport;
     float U[48];
 };

@@ -146,6 +146,9 @@ int draw_tris(sr_context ctx, sr_framebuffer fb, const DrawPlan& pl, const sr_dr
     sp.v = vb; sp.idx = idx; sp.n = ntris; sp.cull = d->cull; sp.W = pl.W; sp.H = pl.H;
     sp.ets = pl.ets; sp.recs = P<TriRec>(ctx->recs); sp.counts = P<uint32_t>(ctx->counts);
     sp.err = P<uint32_t>(ctx->misc);
+    // with no stencil effects, tiles whose pixels the triangle cannot cover
+    // are left out of the bins
+    sp.empty_cull = pl.smode == 0 && !pl.exhaustive;
     launch_tri_setup(sp, s);
     mark(ctx, pl.timing, ST_SETUP);
 
@@ -195,6 +198,7 @@ int draw_tris(sr_context ctx, sr_framebuffer fb, const DrawPlan& pl, const sr_dr
         ep.offsets = P<uint32_t>(ctx->offsets); ep.n = ntris; ep.W = pl.W; ep.H = pl.H;
         ep.ets = pl.ets; ep.ntx = pl.ntx; ep.key_base = 0; ep.val_base = 0;
         ep.keys = P<uint32_t>(ctx->keys); ep.vals = P<uint32_t>(ctx->vals);
+        ep.empty_cull = sp.empty_cull;
         launch_tri_emit(ep, s);
         mark(ctx, pl.timing, ST_EMIT);
         const int swapped = radix_sort(P<uint32_t>(ctx->keys), P<uint32_t>(ctx->vals),
@@ -391,6 +395,7 @@ int batch_plan(sr_context ctx, const sr_batch_item* items, uint32_t n, BatchPlan
         g.nv = ntri ? m->nv : 0; g.vbase = (uint32_t)pl.NV;
         g.nt = ntri; g.tbase = (uint32_t)pl.NT;
         g.program = d->vertex_program; g.K = varyings_of(d->vertex_program); g.cull = d->cull;
+        g.empty_cull = 1;  // batched draws have no stencil effects
         g.W = W; g.H = H; g.ets = ets; g.ntx = (W + ets - 1) / ets;
         g.tile_base = (uint32_t)pl.NTILES;
         g.viewport = d->viewport;

@@ -103,6 +103,39 @@ def test_degenerate_and_tiny(ctx):
         assert_parity(render_gpu(ctx, s, ov), ref)
 
 
+def _lattice_scene(W=64, H=64, n=600, seed=23):
+    """Triangles whose vertices sit on the half-pixel lattice (pixel centres
+    and corners), so edges pass exactly through pixel centres (barycentric
+    exactly 0), plus slivers from nearly collinear lattice points and
+    degenerate ones: the cases where binning's coverage test has no slack."""
+    rng = scenes.SplitMix64(seed)
+    q = np.floor(rng.uniform(6 * n, 0, 2 * W + 1)).reshape(n, 3, 2) / 2.0  # screen coords, step 0.5
+    q[:, :, 1] *= H / W
+    k = n // 3
+    # slivers: third vertex one half-step off the line through the first two
+    q[k:2 * k, 2] = (q[k:2 * k, 0] + q[k:2 * k, 1]) / 2.0
+    q[k:2 * k, 2, 0] += 0.5
+    q[2 * k:2 * k + 20, 1] = q[2 * k:2 * k + 20, 0]  # repeated vertex (det 0)
+    sx = q[:, :, 0].reshape(-1)
+    sy = q[:, :, 1].reshape(-1)
+    s = scenes.soup_scene(n, W, H, seed=seed)
+    # screen -> NDC through the exact inverse of the viewport transform (W, H powers of two)
+    s["px"] = (sx * (2.0 / W) - 1.0).astype(np.float32)
+    s["py"] = (1.0 - sy * (2.0 / H)).astype(np.float32)
+    s["pz"] = np.repeat(rng.uniform(n, 0.05, 0.95), 3).astype(np.float32)
+    return s
+
+
+@pytest.mark.parametrize("ets", [8, 16])
+@pytest.mark.parametrize("blend", [0, 1])
+def test_lattice_edges_through_pixel_centres(ctx, ets, blend):
+    s = _lattice_scene()
+    ov = dict(exec_tile=ets, blend=blend)
+    ref, st = render_ref(s, ov)
+    assert st[0]["covered"] > 1000, st
+    assert_parity(render_gpu(ctx, s, ov), ref)
+
+
 @pytest.mark.parametrize("wh", [(1, 50), (50, 1), (2, 2), (3, 200)])
 def test_thin_framebuffers(ctx, wh):
     s = scenes.soup_scene(400, wh[0], wh[1], seed=19)
