@@ -228,6 +228,7 @@ def extras(a):
     import batchbench
     import serve
 
+    out = {"clip_frame": clip_frame(a)}
     t = time.time()
     b = batchbench.run(256, reps=3)
     batch = {k: (round(v, 4) if isinstance(v, float) else v) for k, v in b.items()}
@@ -246,7 +247,42 @@ def extras(a):
             "plain": {k: round(v, 2) if isinstance(v, float) else v
                       for k, v in row["plain"].items() if k in keep}}
     srv["wall_s"] = round(time.time() - t, 1)
-    return {"batch": batch, "serve": srv}
+    out.update(batch=batch, serve=srv)
+    return out
+
+
+def clip_frame(a, steps=10):
+    """The 3840x2160 clipping- and primitive-heavy frame: 500k triangles (30%
+    crossing the near plane, clipped by the geometry program), 200k Wu lines,
+    100k points, stencil increment; frames/s of clear + the three draws."""
+    import scenes
+    import softrender_hip as sr
+
+    s = scenes.clip_scene()
+    ctx = sr.Context(0)
+    fb = ctx.framebuffer(s["width"], s["height"])
+    tex = ctx.texture(*s["texture"])
+    draws = [(ctx.mesh(d), sr.make_desc(**d["desc"])) for d in s["draws"]]
+
+    def frame():
+        fb.clear(s["clear"])
+        for m, d in draws:
+            ctx.draw(fb, m, d, tex)
+
+    frame()
+    ctx.finish()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        frame()
+    ctx.finish()
+    dt = (time.perf_counter() - t0) / steps
+    for m, _ in draws:
+        m.free()
+    tex.free()
+    fb.free()
+    ctx.close()
+    return {"workload": "clip_4k: 3840x2160, 500k tris (30% near-clipped), 200k Wu lines, 100k points, "
+                        "stencil increment", "fps": round(1.0 / dt, 2), "ms_per_frame": round(dt * 1e3, 3)}
 
 
 if __name__ == "__main__":

@@ -198,6 +198,14 @@ int sr_mesh_free(sr_mesh mesh);
 int sr_texture_upload(sr_context ctx, uint32_t width, uint32_t height,
                       const uint32_t* rgba8, sr_texture* out);
 int sr_texture_free(sr_texture tex);
+/* Render to texture: a texture that reads the framebuffer's colour plane
+ * (RGBA8, row-major, no copy), for a later draw into another framebuffer.
+ * It stays valid while the framebuffer lives and sees its current contents;
+ * a draw may not sample the framebuffer it draws into
+ * (SR_ERR_INVALID_ARGUMENT).  Replaces a texture buffer declared with
+ * declare_texture_buffer! and read through TextureRead::sample
+ * (src/framebuffer/texturebuffer.rs:15-231, src/texture.rs:14-62). */
+int sr_texture_from_fb(sr_framebuffer fb, sr_texture* out);
 
 /* ---- framebuffer (renderbuffer/mod.rs:14-133) --------------------------- */
 /* Width and height must be non-zero (pipeline/mod.rs:129-130).  Planes are

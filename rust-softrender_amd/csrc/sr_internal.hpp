@@ -31,6 +31,9 @@ struct alignas(16) TriRec {
     float z1, z2, z3;
     int32_t X0, X1, Y0, Y1;  // floor of the f32 bbox, clamped to [-1, W] x [-1, H]
     uint32_t i0, i1, i2;     // vertex ids in the screen vertex buffer
+    uint32_t tmask;          // binning: bit i = i-th execution tile of the box (row-major)
+                             // may be covered; all ones when the box has over 32 tiles
+    uint32_t pad;
 };
 static_assert(sizeof(TriRec) == 80, "TriRec layout");
 
@@ -106,15 +109,15 @@ struct RasterParams {
 
 // Raster launch: one scene given inline (one work-group per tile), or a
 // table of scenes whose execution tiles are numbered globally
-// [scene_tile_end[s-1], scene_tile_end[s]) and pulled from per-XCD queues
-// by a persistent grid.
+// [scene_tile_end[s-1], scene_tile_end[s]) and split into runs of
+// near-equal cost, one per work-group of a persistent grid.
 struct RasterBatch {
     RasterParams one;
     const RasterParams* scenes;      // null: use `one`
     const uint32_t* scene_tile_end;  // inclusive prefix of tiles per scene
     uint32_t nscenes;
     uint32_t tile0, ntiles;          // global tile range of this launch
-    uint32_t* queue;                 // 8 counters (one per XCD), zeroed before launch
+    const uint32_t* wg_start;        // run p = tiles [tile0 + wg_start[p], tile0 + wg_start[p+1])
     const uint32_t* overflow;        // non-zero: binning overflowed, draw nothing
 };
 

@@ -12,10 +12,16 @@
 //   k_raster_*      one work-group per execution tile: colour/depth/stencil
 //                   of its pixels stay on-chip, every item is applied in the
 //                   reference's per-pixel order, the tile is written once.
+#include <algorithm>
+
 #include "sr_device.hpp"
 #include "sr_kernels.hpp"
 
 namespace sr {
+
+__host__ __device__ static inline uint32_t blocks_for(uint64_t n, uint32_t per) {
+    return (uint32_t)((n + per - 1) / per);
+}
 
 // ===========================================================================
 // Vertex stage
@@ -323,44 +329,6 @@ __global__ __launch_bounds__(256) void k_gs_emit(GSParams P) {
 // ===========================================================================
 // Returns the number of execution tiles the triangle's box covers and
 // writes its record (X0 > X1 marks culled / invalid).
-// Conservative coverage test of the pixel rectangle [x0,x1] x [y0,y1]:
-// false only when no pixel centre in it can pass the inside test of
-// rasterize_triangle (triangle.rs:104-113).  The barycentrics u, v, w are
-// linear in the pixel centre, so their maxima over the rectangle are at its
-// corners; they are evaluated there in f64 and the rectangle is rejected
-// only when one of them stays below minus sixteen times the error bound of
-// the f32 evaluation (edge products, sums and the correctly rounded
-// quotient: |u_f32 - u| <= 2^-22 * m_u, m_u = (|e23y dx| + |e23x dy|) / |det|).
-// A triangle with det == 0 never draws (its barycentrics are infinite or
-// NaN, so either a sign test or the depth test fails).  Only valid when the
-// draw has no stencil effects, which apply to the whole clamped box.
-__device__ __forceinline__ bool tile_may_cover(const TriRec& r, int x0, int x1, int y0, int y1) {
-    const double R = r.R;
-    if (!(fabs(R) < __builtin_huge_val())) return false;
-    const double ey23 = r.e23y, ex23 = r.e23x, ey31 = r.e31y, ex31 = r.e31x;
-    const double x3 = r.x3, y3 = r.y3;
-    double umax = -__builtin_huge_val(), vmax = umax, wmax = umax, mu = 0.0, mv = 0.0, au = 0.0, av = 0.0;
-#pragma unroll
-    for (int k = 0; k < 4; k++) {
-        const double dx = ((k & 1) ? (double)x1 : (double)x0) + 0.5 - x3;
-        const double dy = ((k & 2) ? (double)y1 : (double)y0) + 0.5 - y3;
-        const double u = (ey23 * dx + ex23 * dy) * R;
-        const double v = (ey31 * dx + ex31 * dy) * R;
-        const double w = (1.0 - u) - v;
-        umax = fmax(umax, u);
-        vmax = fmax(vmax, v);
-        wmax = fmax(wmax, w);
-        mu = fmax(mu, (fabs(ey23 * dx) + fabs(ex23 * dy)) * fabs(R));
-        mv = fmax(mv, (fabs(ey31 * dx) + fabs(ex31 * dy)) * fabs(R));
-        au = fmax(au, fabs(u));
-        av = fmax(av, fabs(v));
-    }
-    const double s = 0x1p-18;
-    const double du = s * (mu + 1.0), dv = s * (mv + 1.0);
-    const double dw = du + dv + s * (2.0 + au + av);
-    return !(umax < -du || vmax < -dv || wmax < -dw);
-}
-
 // Execution tiles of a triangle's clamped box: [ex0,ex1] x [ey0,ey1] and
 // the pixel box [px0,px1] x [py0,py1]; false if the box misses the
 // framebuffer.
@@ -379,11 +347,71 @@ __device__ __forceinline__ bool tile_span(const TriRec& r, uint32_t Wi, uint32_t
     return true;
 }
 
-__device__ __forceinline__ bool span_tile_covered(const TriRec& r, const TileSpan& s, uint32_t ets,
+// Conservative coverage test for binning.  The barycentrics of
+// rasterize_triangle (triangle.rs:104-113) are linear in the pixel
+// coordinates: u(x, y) = ua*x + ub*y + uc at pixel (x, y) (centre x+0.5,
+// y+0.5), likewise v, and w = 1 - u - v.  Over a pixel rectangle each
+// maximum sits at the corner picked by the signs of the two coefficients,
+// so a rectangle is rejected when one maximum stays below minus its margin:
+// sixteen times the error bound of the f32 evaluation the raster performs
+// (edge products, sums and the correctly rounded quotient:
+// |u_f32 - u| <= 2^-22 * (m_u + |u|), m_u = (|e23y dx| + |e23x dy|) / |det|,
+// bounded over the whole clamped box), plus a bound on the rounding of this
+// f64 evaluation itself.  A triangle with det == 0 never draws (its
+// barycentrics are infinite or NaN, so a sign test or the depth test fails).
+// Only valid when the draw has no stencil effects, which apply to the whole
+// clamped box.
+struct CoverPlanes {
+    double ua, ub, uc, va, vb, vc, wa, wb, wc;
+    double du, dv, dw;
+    bool never;
+};
+
+__device__ __forceinline__ CoverPlanes cover_planes(const TriRec& r, const TileSpan& sp) {
+    CoverPlanes c;
+    const double R = r.R;
+    c.never = !(fabs(R) < __builtin_huge_val());
+    const double ey23 = r.e23y, ex23 = r.e23x, ey31 = r.e31y, ex31 = r.e31x;
+    const double ox = 0.5 - (double)r.x3, oy = 0.5 - (double)r.y3;  // dx = x + ox
+    c.ua = ey23 * R; c.ub = ex23 * R; c.uc = (ey23 * ox + ex23 * oy) * R;
+    c.va = ey31 * R; c.vb = ex31 * R; c.vc = (ey31 * ox + ex31 * oy) * R;
+    c.wa = -(c.ua + c.va); c.wb = -(c.ub + c.vb); c.wc = (1.0 - c.uc) - c.vc;
+    // f32-evaluation margins, bounded over the clamped box's corners
+    double mu = 0.0, mv = 0.0, au = 0.0, av = 0.0;
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        const double dx = (double)((k & 1) ? sp.px1 : sp.px0) + ox;
+        const double dy = (double)((k & 2) ? sp.py1 : sp.py0) + oy;
+        const double u = (ey23 * dx + ex23 * dy) * R, v = (ey31 * dx + ex31 * dy) * R;
+        mu = fmax(mu, (fabs(ey23 * dx) + fabs(ex23 * dy)) * fabs(R));
+        mv = fmax(mv, (fabs(ey31 * dx) + fabs(ex31 * dy)) * fabs(R));
+        au = fmax(au, fabs(u));
+        av = fmax(av, fabs(v));
+    }
+    // rounding of the plane evaluation (products, sums, the folded terms)
+    const double xm = (double)sp.px1 + 1.0, ym = (double)sp.py1 + 1.0;
+    const double eu = 0x1p-48 * (fabs(c.ua) * xm + fabs(c.ub) * ym + fabs(c.uc) + mu + 1.0);
+    const double ev = 0x1p-48 * (fabs(c.va) * xm + fabs(c.vb) * ym + fabs(c.vc) + mv + 1.0);
+    const double ew = 0x1p-48 * (fabs(c.wa) * xm + fabs(c.wb) * ym + fabs(c.wc)) + eu + ev;
+    const double s = 0x1p-18;
+    c.du = s * (mu + au + 1.0) + eu;
+    c.dv = s * (mv + av + 1.0) + ev;
+    c.dw = s * (mu + mv + au + av + 3.0) + ew;
+    return c;
+}
+
+__device__ __forceinline__ double plane_max(double a, double b, double cst, int x0, int x1, int y0, int y1) {
+    return a * (double)(a > 0.0 ? x1 : x0) + b * (double)(b > 0.0 ? y1 : y0) + cst;
+}
+
+__device__ __forceinline__ bool span_tile_covered(const CoverPlanes& c, const TileSpan& s, uint32_t ets,
                                                   uint32_t ex, uint32_t ey) {
-    const int tx0 = max(s.px0, (int)(ex * ets)), tx1 = min(s.px1, (int)(ex * ets + ets - 1));
-    const int ty0 = max(s.py0, (int)(ey * ets)), ty1 = min(s.py1, (int)(ey * ets + ets - 1));
-    return tile_may_cover(r, tx0, tx1, ty0, ty1);
+    if (c.never) return false;
+    const int x0 = max(s.px0, (int)(ex * ets)), x1 = min(s.px1, (int)(ex * ets + ets - 1));
+    const int y0 = max(s.py0, (int)(ey * ets)), y1 = min(s.py1, (int)(ey * ets + ets - 1));
+    return !(plane_max(c.ua, c.ub, c.uc, x0, x1, y0, y1) < -c.du ||
+             plane_max(c.va, c.vb, c.vc, x0, x1, y0, y1) < -c.dv ||
+             plane_max(c.wa, c.wb, c.wc, x0, x1, y0, y1) < -c.dw);
 }
 
 __device__ __forceinline__ uint32_t tri_setup_one(const VertexBuf& v, uint32_t i0, uint32_t i1,
@@ -394,6 +422,8 @@ __device__ __forceinline__ uint32_t tri_setup_one(const VertexBuf& v, uint32_t i
     const float* sz = v.comp(2);
     const float x1 = sx[i0], y1 = sy[i0], x2 = sx[i1], y2 = sy[i1], x3 = sx[i2], y3 = sy[i2];
     r.i0 = i0; r.i1 = i1; r.i2 = i2;
+    r.tmask = 0u;
+    r.pad = 0u;
     if (cull != SR_CULL_NONE) {
         // shoelace, triangle.rs:56 (left-to-right evaluation)
         const float A = x1 * y2 + x2 * y3 + x3 * y1 - x2 * y1 - x3 * y2 - x1 * y3;
@@ -422,13 +452,53 @@ __device__ __forceinline__ uint32_t tri_setup_one(const VertexBuf& v, uint32_t i
     r.X1 = (int32_t)floorf(clampf(mxx, -1.0f, W));
     r.Y0 = (int32_t)floorf(clampf(mny, -1.0f, H));
     r.Y1 = (int32_t)floorf(clampf(mxy, -1.0f, H));
+    r.tmask = ~0u;
     TileSpan sp;
     if (!tile_span(r, Wi, Hi, ets, sp)) return 0;
-    if (!empty_cull) return (sp.ex1 - sp.ex0 + 1) * (sp.ey1 - sp.ey0 + 1);
-    uint32_t n = 0;
+    const uint32_t ntile = (sp.ex1 - sp.ex0 + 1) * (sp.ey1 - sp.ey0 + 1);
+    if (!empty_cull) return ntile;
+    const CoverPlanes cp = cover_planes(r, sp);
+    if (cp.never) {
+        r.tmask = 0u;
+        return 0;
+    }
+    uint32_t n = 0, m = 0, i = 0;
     for (uint32_t ey = sp.ey0; ey <= sp.ey1; ey++)
-        for (uint32_t ex = sp.ex0; ex <= sp.ex1; ex++) n += span_tile_covered(r, sp, ets, ex, ey) ? 1u : 0u;
+        for (uint32_t ex = sp.ex0; ex <= sp.ex1; ex++, i++) {
+            const bool c = span_tile_covered(cp, sp, ets, ex, ey);
+            n += c ? 1u : 0u;
+            if (c && i < 32) m |= 1u << i;
+        }
+    if (ntile <= 32) r.tmask = m;
     return n;
+}
+
+// Writes the (tile, item) pairs of one triangle: every tile of its box, or
+// with culling the tiles setup found coverable (its tile mask, or the test
+// again for boxes of more than 32 tiles), in row-major order.
+__device__ __forceinline__ void emit_pairs(const TriRec& r, const TileSpan& sp, uint32_t ets, uint32_t ntx,
+                                           int empty_cull, uint32_t key_base, uint32_t val, uint32_t* keys,
+                                           uint32_t* vals) {
+    const uint32_t w = sp.ex1 - sp.ex0 + 1, h = sp.ey1 - sp.ey0 + 1;
+    uint32_t o = 0;
+    if (empty_cull && w * h <= 32) {
+        for (uint32_t m = r.tmask; m; m &= m - 1) {
+            const uint32_t i = __builtin_ctz(m);
+            keys[o] = key_base + (sp.ey0 + i / w) * ntx + sp.ex0 + i % w;
+            vals[o] = val;
+            o++;
+        }
+        return;
+    }
+    CoverPlanes cp;
+    if (empty_cull) cp = cover_planes(r, sp);
+    for (uint32_t ey = sp.ey0; ey <= sp.ey1; ey++)
+        for (uint32_t ex = sp.ex0; ex <= sp.ex1; ex++) {
+            if (empty_cull && !span_tile_covered(cp, sp, ets, ex, ey)) continue;
+            keys[o] = key_base + ey * ntx + ex;
+            vals[o] = val;
+            o++;
+        }
 }
 
 __global__ __launch_bounds__(256) void k_tri_setup(SetupParams P) {
@@ -473,13 +543,7 @@ __global__ __launch_bounds__(256) void k_tri_emit_batch(BatchTables T, const Tri
     const TriRec r = recs[t];
     TileSpan sp;
     if (!tile_span(r, S.W, S.H, S.ets, sp)) return;
-    for (uint32_t ey = sp.ey0; ey <= sp.ey1; ey++)
-        for (uint32_t ex = sp.ex0; ex <= sp.ex1; ex++) {
-            if (S.empty_cull && !span_tile_covered(r, sp, S.ets, ex, ey)) continue;
-            keys[o] = S.tile_base + ey * S.ntx + ex;
-            vals[o] = t;
-            o++;
-        }
+    emit_pairs(r, sp, S.ets, S.ntx, S.empty_cull, S.tile_base, t, keys + o, vals + o);
 }
 
 __global__ __launch_bounds__(256) void k_tri_emit(EmitParams P) {
@@ -489,14 +553,8 @@ __global__ __launch_bounds__(256) void k_tri_emit(EmitParams P) {
     const TriRec r = P.recs[t];
     TileSpan sp;
     if (!tile_span(r, P.W, P.H, P.ets, sp)) return;
-    uint32_t o = P.offsets[t];
-    for (uint32_t ey = sp.ey0; ey <= sp.ey1; ey++)
-        for (uint32_t ex = sp.ex0; ex <= sp.ex1; ex++) {
-            if (P.empty_cull && !span_tile_covered(r, sp, P.ets, ex, ey)) continue;
-            P.keys[o] = P.key_base + ey * P.ntx + ex;
-            P.vals[o] = P.val_base + t;
-            o++;
-        }
+    const uint32_t o = P.offsets[t];
+    emit_pairs(r, sp, P.ets, P.ntx, P.empty_cull, P.key_base, P.val_base + t, P.keys + o, P.vals + o);
 }
 
 // ===========================================================================
@@ -524,8 +582,18 @@ __device__ __forceinline__ uint32_t block_scan_256(uint32_t v, uint32_t& total, 
     return x - v + off;
 }
 
-__global__ __launch_bounds__(256) void k_scan_reduce(const uint32_t* in, uint32_t n, uint32_t* sums) {
+// The scans take an optional device-side element count (`count`, bounded
+// by `n`): blocks past it exit, so a launch sized for a capacity costs
+// little when the live count is small.
+__device__ __forceinline__ uint32_t live_count(uint32_t n, const uint32_t* count) {
+    return count ? min(n, *count) : n;
+}
+
+__global__ __launch_bounds__(256) void k_scan_reduce(const uint32_t* in, uint32_t n, uint32_t* sums,
+                                                    const uint32_t* count) {
     __shared__ uint32_t lds4[4];
+    n = live_count(n, count);
+    if (blockIdx.x >= blocks_for(n, SCAN_TILE)) return;
     const uint32_t base = blockIdx.x * SCAN_TILE;
     uint32_t acc = 0;
     for (int r = 0; r < 16; r++) {
@@ -538,8 +606,10 @@ __global__ __launch_bounds__(256) void k_scan_reduce(const uint32_t* in, uint32_
 }
 
 // Single work-group scan of the block sums; also writes the grand total.
-__global__ __launch_bounds__(256) void k_scan_top(uint32_t* sums, uint32_t nb, uint32_t* total_out) {
+__global__ __launch_bounds__(256) void k_scan_top(uint32_t* sums, uint32_t n, uint32_t* total_out,
+                                                 const uint32_t* count) {
     __shared__ uint32_t lds4[4];
+    const uint32_t nb = blocks_for(live_count(n, count), SCAN_TILE);
     uint32_t run = 0;
     for (uint32_t b = 0; b < nb; b += 256) {
         const uint32_t e = b + threadIdx.x;
@@ -553,8 +623,10 @@ __global__ __launch_bounds__(256) void k_scan_top(uint32_t* sums, uint32_t nb, u
 }
 
 __global__ __launch_bounds__(256) void k_scan_down(const uint32_t* in, uint32_t* out, uint32_t n,
-                                                  const uint32_t* block_off) {
+                                                  const uint32_t* block_off, const uint32_t* count) {
     __shared__ uint32_t lds4[4];
+    n = live_count(n, count);
+    if (blockIdx.x >= blocks_for(n, SCAN_TILE)) return;
     const uint32_t base = blockIdx.x * SCAN_TILE;
     uint32_t run = block_off[blockIdx.x];
     for (int r = 0; r < 16; r++) {
@@ -570,11 +642,20 @@ __global__ __launch_bounds__(256) void k_scan_down(const uint32_t* in, uint32_t*
 // ===========================================================================
 // Stable LSD radix sort of (key, value) by key, 8-bit digits.
 // ===========================================================================
+// Digit histograms, digit-major: hist[d * nb + block].  With a device-side
+// element count the layout uses the live block count nb = ceil(count /
+// SCAN_TILE) (block 0 stores 256 * nb in hist_count for the scan) and
+// blocks past it exit.
 __global__ __launch_bounds__(256) void k_radix_hist(const uint32_t* keys, uint32_t n, int shift,
                                                    uint32_t* hist, uint32_t nblocks,
-                                                   const uint32_t* n_dev) {
+                                                   const uint32_t* n_dev, uint32_t* hist_count) {
     __shared__ uint32_t h[256];
-    if (n_dev) n = min(n, *n_dev);
+    if (n_dev) {
+        n = min(n, *n_dev);
+        nblocks = blocks_for(n, SCAN_TILE);
+        if (blockIdx.x == 0 && threadIdx.x == 0) *hist_count = 256u * nblocks;
+    }
+    if (blockIdx.x >= nblocks) return;
     h[threadIdx.x] = 0;
     __syncthreads();
     const uint32_t base = blockIdx.x * SCAN_TILE;
@@ -588,25 +669,52 @@ __global__ __launch_bounds__(256) void k_radix_hist(const uint32_t* keys, uint32
 
 // Elements are taken in 16 rounds of 256 in index order.  Within a round a
 // lane's rank among equal digits comes from 8 ballots (a wave-wide match on
-// the digit), the waves before it from per-wave digit counts in LDS.  The
-// output is therefore stable: equal keys keep their input order.
+// the digit), the waves before it from per-wave digit counts in LDS; the
+// element goes to its digit's run in an LDS copy of the block, which is
+// therefore the block stably sorted by digit.  The copy is then written
+// out with consecutive threads on consecutive addresses of each digit's
+// run (the digit's global offset for this block comes from the scanned
+// histograms), so the writes coalesce instead of scattering 4-byte words.
 __global__ __launch_bounds__(256) void k_radix_scatter(const uint32_t* keys_in,
                                                       const uint32_t* vals_in, uint32_t n,
                                                       int shift, const uint32_t* offsets,
                                                       uint32_t nblocks, uint32_t* keys_out,
                                                       uint32_t* vals_out, const uint32_t* n_dev) {
-    __shared__ uint32_t run[256];
-    if (n_dev) n = min(n, *n_dev);
+    __shared__ uint32_t lkeys[SCAN_TILE], lvals[SCAN_TILE];
+    __shared__ uint32_t run[256], lstart[256], gbase[256];
     __shared__ uint32_t wcnt[4][256];
+    __shared__ uint32_t lds4[4];
+    if (n_dev) {
+        n = min(n, *n_dev);
+        nblocks = blocks_for(n, SCAN_TILE);
+    }
+    if (blockIdx.x >= nblocks) return;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    run[tid] = offsets[tid * nblocks + blockIdx.x];
-    const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
     const uint32_t base = blockIdx.x * SCAN_TILE;
+    const uint32_t cnt_blk = min(n - base, (uint32_t)SCAN_TILE);
+    gbase[tid] = offsets[tid * nblocks + blockIdx.x];
+    run[tid] = 0;
+    __syncthreads();
+    uint32_t K[16], V[16];
+#pragma unroll
+    for (int r = 0; r < 16; r++) {
+        const uint32_t e = base + r * 256u + tid;
+        K[r] = e < n ? keys_in[e] : 0u;
+        V[r] = e < n ? vals_in[e] : 0u;
+        if (e < n) atomicAdd(&run[(K[r] >> shift) & 255u], 1u);
+    }
+    __syncthreads();
+    uint32_t tot;
+    const uint32_t st = block_scan_256(run[tid], tot, lds4);
+    lstart[tid] = st;
+    __syncthreads();
+    run[tid] = st;
+    const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+#pragma unroll
     for (int r = 0; r < 16; r++) {
         const uint32_t e = base + r * 256u + tid;
         const bool valid = e < n;
-        const uint32_t k = valid ? keys_in[e] : 0u;
-        const uint32_t v = valid ? vals_in[e] : 0u;
+        const uint32_t k = K[r];
         const uint32_t d = (k >> shift) & 255u;
         uint64_t mask = __ballot(valid);
 #pragma unroll
@@ -626,12 +734,19 @@ __global__ __launch_bounds__(256) void k_radix_scatter(const uint32_t* keys_in,
             uint32_t pre = 0;
             for (int w = 0; w < wave; w++) pre += wcnt[w][d];
             const uint32_t pos = run[d] + pre + rank;
-            keys_out[pos] = k;
-            vals_out[pos] = v;
+            lkeys[pos] = k;
+            lvals[pos] = V[r];
         }
         __syncthreads();
         run[tid] += wcnt[0][tid] + wcnt[1][tid] + wcnt[2][tid] + wcnt[3][tid];
         __syncthreads();
+    }
+    for (uint32_t i = tid; i < cnt_blk; i += 256u) {
+        const uint32_t k = lkeys[i];
+        const uint32_t d = (k >> shift) & 255u;
+        const uint32_t pos = gbase[d] + (i - lstart[d]);
+        keys_out[pos] = k;
+        vals_out[pos] = lvals[i];
     }
 }
 
@@ -645,14 +760,39 @@ __global__ __launch_bounds__(256) void k_iota_u32(uint32_t* p, uint32_t n) {
     if (i < n) p[i] = i;
 }
 
+// Persistent-raster partition.  Each tile costs a fixed charge (load or
+// clear, resolve) plus one per binned item; the costs are prefix-summed and
+// tiles [tile0, tile0 + n) are cut into G runs of near-equal cost: run p
+// holds the tiles whose cost prefix lies in [p*total/G, (p+1)*total/G), so
+// wg_start[p] is its first tile (local index) and wg_start[G] = n (filled
+// beforehand, like the starts of empty runs).
+__global__ __launch_bounds__(256) void k_tile_cost(const uint32_t* start, const uint32_t* end, uint32_t n,
+                                                  uint32_t fixed, uint32_t* cost) {
+    const uint32_t t = blockIdx.x * 256u + threadIdx.x;
+    if (t > n) return;
+    cost[t] = t < n ? fixed + (end[t] - start[t]) : 0u;
+}
+
+__global__ __launch_bounds__(256) void k_partition(const uint32_t* prefix, uint32_t tile0, uint32_t n,
+                                                  uint32_t G, uint32_t* wg_start) {
+    const uint32_t t = blockIdx.x * 256u + threadIdx.x;
+    if (t >= n) return;
+    const uint64_t base = prefix[tile0], tot = (uint64_t)prefix[tile0 + n] - base;
+    const uint32_t w = (uint32_t)(((uint64_t)prefix[tile0 + t] - base) * G / tot);
+    const uint32_t w0 = t == 0 ? 0u : (uint32_t)(((uint64_t)prefix[tile0 + t - 1] - base) * G / tot) + 1u;
+    for (uint32_t q = w0; q <= w; q++) wg_start[q] = t;
+}
+
+// Grid-stride over the sorted keys (the grid is bounded; the live count
+// may be far below the capacity the launch was sized for).
 __global__ __launch_bounds__(256) void k_ranges(const uint32_t* keys, uint32_t n, uint32_t key_base,
                                                uint32_t* start, uint32_t* end, const uint32_t* n_dev) {
     if (n_dev) n = min(n, *n_dev);
-    const uint32_t i = blockIdx.x * 256u + threadIdx.x;
-    if (i >= n) return;
-    const uint32_t k = keys[i];
-    if (i == 0 || keys[i - 1] != k) start[k - key_base] = i;
-    if (i == n - 1 || keys[i + 1] != k) end[k - key_base] = i + 1;
+    for (uint32_t i = blockIdx.x * 256u + threadIdx.x; i < n; i += gridDim.x * 256u) {
+        const uint32_t k = keys[i];
+        if (i == 0 || keys[i - 1] != k) start[k - key_base] = i;
+        if (i == n - 1 || keys[i + 1] != k) end[k - key_base] = i + 1;
+    }
 }
 
 // ===========================================================================
@@ -1084,7 +1224,6 @@ __global__ __launch_bounds__(256) void k_point_setup(PointSetupParams P) {
 // ===========================================================================
 // Host-side launchers
 // ===========================================================================
-static inline uint32_t blocks_for(uint64_t n, uint32_t per) { return (uint32_t)((n + per - 1) / per); }
 
 void launch_vertex(const VSParams& P, hipStream_t s) {
     if (P.n) hipLaunchKernelGGL(k_vertex, dim3(blocks_for(P.n, 256)), dim3(256), 0, s, P);
@@ -1134,15 +1273,15 @@ void launch_tri_emit(const EmitParams& P, hipStream_t s) {
 uint32_t scan_tmp_words(uint32_t n) { return blocks_for(n, SCAN_TILE) + 1; }
 
 void launch_scan(const uint32_t* in, uint32_t* out, uint32_t n, uint32_t* tmp, uint32_t* total,
-                 hipStream_t s) {
+                 hipStream_t s, const uint32_t* count) {
     const uint32_t nb = blocks_for(n, SCAN_TILE);
     if (n == 0) {
         hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(256), 0, s, total, 0u, 1u);
         return;
     }
-    hipLaunchKernelGGL(k_scan_reduce, dim3(nb), dim3(256), 0, s, in, n, tmp);
-    hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(256), 0, s, tmp, nb, total);
-    hipLaunchKernelGGL(k_scan_down, dim3(nb), dim3(256), 0, s, in, out, n, (const uint32_t*)tmp);
+    hipLaunchKernelGGL(k_scan_reduce, dim3(nb), dim3(256), 0, s, in, n, tmp, count);
+    hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(256), 0, s, tmp, n, total, count);
+    hipLaunchKernelGGL(k_scan_down, dim3(nb), dim3(256), 0, s, in, out, n, (const uint32_t*)tmp, count);
 }
 
 uint32_t radix_hist_words(uint32_t n) { return 256u * blocks_for(n, SCAN_TILE); }
@@ -1156,8 +1295,8 @@ int radix_sort(uint32_t* keys, uint32_t* vals, uint32_t* keys_tmp, uint32_t* val
     uint32_t *ki = keys, *vi = vals, *ko = keys_tmp, *vo = vals_tmp;
     for (int p = 0; p < passes; p++) {
         const int shift = 8 * p;
-        hipLaunchKernelGGL(k_radix_hist, dim3(nb), dim3(256), 0, s, ki, n, shift, hist, nb, n_dev);
-        launch_scan(hist, hist, 256u * nb, scan_tmp, total, s);
+        hipLaunchKernelGGL(k_radix_hist, dim3(nb), dim3(256), 0, s, ki, n, shift, hist, nb, n_dev, total + 1);
+        launch_scan(hist, hist, 256u * nb, scan_tmp, total, s, n_dev ? total + 1 : nullptr);
         hipLaunchKernelGGL(k_radix_scatter, dim3(nb), dim3(256), 0, s, ki, vi, n, shift,
                            (const uint32_t*)hist, nb, ko, vo, n_dev);
         uint32_t* t;
@@ -1177,8 +1316,21 @@ void launch_iota(uint32_t* p, uint32_t n, hipStream_t s) {
 
 void launch_ranges(const uint32_t* keys, uint32_t n, uint32_t key_base, uint32_t* start,
                    uint32_t* end, hipStream_t s, const uint32_t* n_dev) {
-    if (n) hipLaunchKernelGGL(k_ranges, dim3(blocks_for(n, 256)), dim3(256), 0, s, keys, n, key_base,
-                              start, end, n_dev);
+    if (n) hipLaunchKernelGGL(k_ranges, dim3(std::min<uint32_t>(blocks_for(n, 256), 8192u)), dim3(256), 0, s,
+                              keys, n, key_base, start, end, n_dev);
+}
+
+void launch_tile_cost(const uint32_t* start, const uint32_t* end, uint32_t n, uint32_t fixed, uint32_t* cost,
+                      hipStream_t s) {
+    hipLaunchKernelGGL(k_tile_cost, dim3(blocks_for((uint64_t)n + 1, 256)), dim3(256), 0, s, start, end, n,
+                       fixed, cost);
+}
+
+void launch_partition(const uint32_t* prefix, uint32_t tile0, uint32_t n, uint32_t G, uint32_t* wg_start,
+                      hipStream_t s) {
+    launch_fill(wg_start, n, G + 1, s);
+    if (n) hipLaunchKernelGGL(k_partition, dim3(blocks_for(n, 256)), dim3(256), 0, s, prefix, tile0, n, G,
+                              wg_start);
 }
 
 void launch_fb_clear(uint32_t* color, float* depth, uint8_t* stencil, uint32_t n, uint32_t rgba,

@@ -143,11 +143,13 @@ void launch_tri_emit_batch(const BatchTables& T, const TriRec* recs, const uint3
                            const uint32_t* offsets, uint32_t cap, uint32_t* keys, uint32_t* vals,
                            uint32_t* overflow, hipStream_t s);
 uint32_t scan_tmp_words(uint32_t n);
+// exclusive scan of in[0..n) (or of the first *count elements, count <= n)
 void launch_scan(const uint32_t* in, uint32_t* out, uint32_t n, uint32_t* tmp, uint32_t* total,
-                 hipStream_t s);
+                 hipStream_t s, const uint32_t* count = nullptr);
 uint32_t radix_hist_words(uint32_t n);
 // returns 1 if the sorted data ended up in the *_tmp buffers
-// n_dev (optional): the element count in device memory, at most n
+// n_dev (optional): the element count in device memory, at most n; then
+// total[1] is used as scratch
 int radix_sort(uint32_t* keys, uint32_t* vals, uint32_t* keys_tmp, uint32_t* vals_tmp, uint32_t n,
                int key_bits, uint32_t* hist, uint32_t* scan_tmp, uint32_t* total, hipStream_t s,
                const uint32_t* n_dev = nullptr);
@@ -155,6 +157,13 @@ void launch_fill(uint32_t* p, uint32_t v, uint32_t n, hipStream_t s);
 void launch_iota(uint32_t* p, uint32_t n, hipStream_t s);  // p[i] = i
 void launch_ranges(const uint32_t* keys, uint32_t n, uint32_t key_base, uint32_t* start,
                    uint32_t* end, hipStream_t s, const uint32_t* n_dev = nullptr);
+// persistent-raster partition: cost[t] = fixed + items of tile t for
+// t < n, cost[n] = 0 (scan it into a prefix of n + 1 entries), then G runs
+// of near-equal cost over tiles [tile0, tile0 + n): wg_start[0..G]
+void launch_tile_cost(const uint32_t* start, const uint32_t* end, uint32_t n, uint32_t fixed, uint32_t* cost,
+                      hipStream_t s);
+void launch_partition(const uint32_t* prefix, uint32_t tile0, uint32_t n, uint32_t G, uint32_t* wg_start,
+                      hipStream_t s);
 void launch_fb_clear(uint32_t* color, float* depth, uint8_t* stencil, uint32_t n, uint32_t rgba,
                      float d, uint32_t st, hipStream_t s);
 // grid = ntiles work-groups (inline scene) or a persistent grid (scene table);

@@ -8,6 +8,17 @@
 #define SR_ETS 16
 #endif
 
+// Waves per SIMD the raster kernels are compiled for (the register budget:
+// 512 / waves VGPRs).  0 leaves the choice to the compiler.
+#ifndef SR_RASTER_WAVES
+#define SR_RASTER_WAVES 0
+#endif
+#if SR_RASTER_WAVES > 0
+#define SR_RASTER_OCCUPANCY __attribute__((amdgpu_waves_per_eu(SR_RASTER_WAVES, 8)))
+#else
+#define SR_RASTER_OCCUPANCY
+#endif
+
 namespace sr {
 
 // ===========================================================================
@@ -205,7 +216,13 @@ __device__ __forceinline__ void raster_tile(const RasterParams& P, uint32_t tile
     constexpr int S = SH::S, PPT = SH::PPT, Q = SH::Q;
     constexpr int RPP = 64 / S;  // quadrant rows covered by one pixel pass
 
-    const int tid = threadIdx.x, lane = tid & 63;
+    // The empty asm makes the thread index opaque per tile: in the
+    // persistent batch kernel the compiler would otherwise hoist every
+    // lane-derived value out of the tile loop and keep it live across the
+    // whole loop (133 instead of 115 VGPRs: one wave less per SIMD).
+    int tid = threadIdx.x;
+    asm volatile("" : "+v"(tid));
+    const int lane = tid & 63;
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     const uint32_t etx = tile % P.ntx, ety = tile / P.ntx;
     // constant-space loads: a flat load counts as divergent, which would turn
@@ -323,7 +340,9 @@ __device__ __forceinline__ void raster_tile(const RasterParams& P, uint32_t tile
                     if (X0 > X1) continue;  // culled or invalid
                     const int cx0 = min(max(X0, Tx0), Tx1), cx1 = min(max(X1, Tx0), Tx1);
                     const int cy0 = min(max(Y0, Ty0), Ty1), cy1 = min(max(Y1, Ty0), Ty1);
-                    if (cx1 < rx0 || cx0 > rx1 || cy1 < ry0 || cy0 > ry1) continue;
+                    // (no wave-level early-out on the clamped box: binned
+                    // triangles nearly always reach the wave's pixels, and
+                    // the scalar test cost more than the rare skip saved)
                     bool full = false;
 #pragma unroll
                     for (int j = 0; j < PPT; j++) {
@@ -405,34 +424,35 @@ __device__ __forceinline__ RasterParams load_params(const RasterParams* base, ui
 #endif
 }
 
+// One scene: one work-group per execution tile.
 template <int ETS, int FP, int BLEND, int SMODE>
-__global__ __launch_bounds__(256) void k_raster_tris(RasterBatch B) {
+__global__ __launch_bounds__(256) SR_RASTER_OCCUPANCY void k_raster_tris(RasterParams P) {
     using SH = RasterShared<ETS, FP, BLEND>;
     __shared__ SH sh;
-    __shared__ uint32_t s_next;
-    const int tid = threadIdx.x;
-    if (B.overflow && *B.overflow) return;
-    for (int i = tid; i < 256; i += SH::NT) sh.lut[i] = (float)i / 255.0f;
+    for (int i = threadIdx.x; i < 256; i += SH::NT) sh.lut[i] = (float)i / 255.0f;
     __syncthreads();
-    if (!B.scenes) {  // one scene, one work-group per tile
-        const uint32_t tile = xcd_remap(blockIdx.x, gridDim.x);
-        raster_tile<ETS, FP, BLEND, SMODE>(B.one, tile, tile, sh);
-        return;
-    }
-    // Persistent: work-groups on XCD x first drain that XCD's contiguous
-    // share of the tiles, then help the other XCDs.
-    const uint32_t x0 = blockIdx.x & 7u;
-    const uint32_t q = B.ntiles >> 3, r = B.ntiles & 7u;
-    for (uint32_t k = 0; k < 8; k++) {
-        const uint32_t x = (x0 + k) & 7u;
-        const uint32_t lo = x * q + min(x, r), n = q + (x < r ? 1u : 0u);
-        for (;;) {
-            if (tid == 0) s_next = atomicAdd(&B.queue[x], 1u);
-            __syncthreads();
-            const uint32_t i = __builtin_amdgcn_readfirstlane(s_next);
-            __syncthreads();
-            if (i >= n) break;
-            const uint32_t g = B.tile0 + lo + i;
+    const uint32_t tile = xcd_remap(blockIdx.x, gridDim.x);
+    raster_tile<ETS, FP, BLEND, SMODE>(P, tile, tile, sh);
+}
+
+// A batch: a persistent grid over the scene table (no stencil effects in
+// batches, so SMODE is 0).
+template <int ETS, int FP, int BLEND>
+__global__ __launch_bounds__(256) SR_RASTER_OCCUPANCY void k_raster_tris_batch(RasterBatch B) {
+    using SH = RasterShared<ETS, FP, BLEND>;
+    __shared__ SH sh;
+    if (B.overflow && *B.overflow) return;
+    for (int i = threadIdx.x; i < 256; i += SH::NT) sh.lut[i] = (float)i / 255.0f;
+    __syncthreads();
+    // Persistent: work-group b takes run p of near-equal cost (a contiguous
+    // run of tiles); runs are numbered so that each XCD holds a contiguous
+    // stretch of tiles (neighbouring tiles share records in its L2).
+    const uint32_t p = xcd_remap(blockIdx.x, gridDim.x);
+    const uint32_t t0 = load_item(B.wg_start, p), t1 = load_item(B.wg_start, p + 1);
+    uint32_t scene = 0, scene_end = 0;
+    for (uint32_t t = t0; t < t1; t++) {
+        const uint32_t g = B.tile0 + t;
+        if (g >= scene_end) {
             // scene owning global tile g: first s with g < scene_tile_end[s]
             uint32_t a = 0, b = B.nscenes;
             while (a < b) {
@@ -440,19 +460,22 @@ __global__ __launch_bounds__(256) void k_raster_tris(RasterBatch B) {
                 if (load_item(B.scene_tile_end, m) <= g) a = m + 1;
                 else b = m;
             }
-            const RasterParams P = load_params(B.scenes, a);
-            raster_tile<ETS, FP, BLEND, SMODE>(P, g - P.tile_base, g, sh);
+            scene = a;
+            scene_end = load_item(B.scene_tile_end, a);
         }
+        // reloaded per tile (scalar-cache hits) rather than held in registers
+        const RasterParams P = load_params(B.scenes, scene);
+        raster_tile<ETS, FP, BLEND, 0>(P, g - P.tile_base, g, sh);
     }
 }
 
 template <int ETS, int FP, int BLEND>
 static void raster_tris_smode(const RasterBatch& P, uint32_t ntiles, hipStream_t s) {
     const dim3 g(ntiles), b(raster_threads<ETS>());
-    const int smode = P.scenes ? 0 : P.one.smode;
-    if (smode == 2) hipLaunchKernelGGL((k_raster_tris<ETS, FP, BLEND, 2>), g, b, 0, s, P);
-    else if (smode == 1) hipLaunchKernelGGL((k_raster_tris<ETS, FP, BLEND, 1>), g, b, 0, s, P);
-    else hipLaunchKernelGGL((k_raster_tris<ETS, FP, BLEND, 0>), g, b, 0, s, P);
+    if (P.scenes) hipLaunchKernelGGL((k_raster_tris_batch<ETS, FP, BLEND>), g, b, 0, s, P);
+    else if (P.one.smode == 2) hipLaunchKernelGGL((k_raster_tris<ETS, FP, BLEND, 2>), g, b, 0, s, P.one);
+    else if (P.one.smode == 1) hipLaunchKernelGGL((k_raster_tris<ETS, FP, BLEND, 1>), g, b, 0, s, P.one);
+    else hipLaunchKernelGGL((k_raster_tris<ETS, FP, BLEND, 0>), g, b, 0, s, P.one);
 }
 
 template <int ETS, int FP>

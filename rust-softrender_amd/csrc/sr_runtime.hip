@@ -267,6 +267,8 @@ int sr_rt::validate_draw(sr_context ctx, sr_framebuffer fb, sr_mesh mesh, const 
         return fail(ctx, SR_ERR_INVALID_ARGUMENT, "draw: unknown fragment program");
     if (d->fragment_program == SR_FP_TEX_LAMBERT && !tex)
         return fail(ctx, SR_ERR_INVALID_ARGUMENT, "draw: fragment program samples a texture");
+    if (tex && tex->view && tex->fb == fb)
+        return fail(ctx, SR_ERR_INVALID_ARGUMENT, "draw: samples the framebuffer it draws into");
     if (d->fragment_program == SR_FP_BLINN_PHONG && d->vertex_program != SR_VP_MVP_NORMAL_WORLD)
         return fail(ctx, SR_ERR_INVALID_ARGUMENT, "draw: Blinn-Phong needs world-space varyings");
     if ((d->fragment_program == SR_FP_TEX_LAMBERT || d->fragment_program == SR_FP_UV_COLOR) &&
@@ -317,7 +319,7 @@ void BatchWork::release() {
     if (stream) hipStreamSynchronize(stream);
     drop_graph();
     DevBuf* bufs[] = {&verts, &recs, &counts, &offsets, &keys, &vals, &keys2,
-                      &vals2, &tstart, &tend, &hist, &scan, &tables, &misc};
+                      &vals2, &tstart, &tend, &hist, &scan, &tables, &misc, &cost, &parts};
     for (DevBuf* b : bufs) {
         if (b->p) hipFree(b->p);
         b->p = nullptr;
@@ -337,7 +339,7 @@ void BatchWork::release() {
 
 size_t BatchWork::device_bytes() const {
     const DevBuf* bufs[] = {&verts, &recs, &counts, &offsets, &keys, &vals, &keys2,
-                            &vals2, &tstart, &tend, &hist, &scan, &tables, &misc};
+                            &vals2, &tstart, &tend, &hist, &scan, &tables, &misc, &cost, &parts};
     size_t t = 0;
     for (const DevBuf* b : bufs) t += b->bytes;
     return t;
@@ -413,6 +415,21 @@ int batch_plan(sr_context ctx, const sr_batch_item* items, uint32_t n, BatchPlan
     return SR_OK;
 }
 
+// Persistent raster: runs (work-groups) per class, about twice the
+// work-groups the chip holds at once (ETS 8: one wave each, 16 per CU;
+// larger tiles: four waves each, 4 per CU), never more than the tiles.
+constexpr uint32_t TILE_FIXED_COST = 8;  // a tile's load/clear + resolve, in item iterations
+
+static uint32_t raster_runs(const BatchClass& C) {
+    return std::min<uint32_t>(C.ntiles, C.ets == 8 ? 256u * 16u * 2u : 256u * 4u * 2u);
+}
+
+static size_t raster_runs_total(const BatchPlan& pl) {
+    size_t n = 0;
+    for (const BatchClass& C : pl.classes) n += raster_runs(C) + 1;
+    return n;
+}
+
 int batch_enqueue(sr_context ctx, BatchWork& W, const BatchPlan& pl, bool use_graph) {
     int rc;
     hipStream_t s = W.stream;
@@ -433,8 +450,11 @@ int batch_enqueue(sr_context ctx, BatchWork& W, const BatchPlan& pl, bool use_gr
         (rc = ensure(ctx, W.tstart, sizeof(uint32_t) * std::max<uint64_t>(NTILES, 1), s)) ||
         (rc = ensure(ctx, W.tend, sizeof(uint32_t) * std::max<uint64_t>(NTILES, 1), s)) ||
         (rc = ensure(ctx, W.hist, sizeof(uint32_t) * (radix_hist_words(cap) + 256), s)) ||
-        (rc = ensure(ctx, W.scan, sizeof(uint32_t) * (scan_tmp_words(std::max<uint32_t>(radix_hist_words(cap) + 256, (uint32_t)NT + 1)) + 16), s)) ||
-        (rc = ensure(ctx, W.misc, sizeof(uint32_t) * (8 + 8 * pl.classes.size()), s)))
+        (rc = ensure(ctx, W.scan, sizeof(uint32_t) * (scan_tmp_words(std::max<uint64_t>(
+                                      std::max<uint64_t>(radix_hist_words(cap) + 256, NT + 1), NTILES + 1)) + 16), s)) ||
+        (rc = ensure(ctx, W.misc, sizeof(uint32_t) * 8, s)) ||
+        (rc = ensure(ctx, W.cost, sizeof(uint32_t) * (NTILES + 1), s)) ||
+        (rc = ensure(ctx, W.parts, sizeof(uint32_t) * (raster_runs_total(pl) + pl.classes.size()), s)))
         return rc;
     const size_t tbytes = sizeof(SceneGeom) * ns + sizeof(RasterParams) * ns + sizeof(uint32_t) * 3 * ns + 64;
     if ((rc = ensure(ctx, W.tables, tbytes, s))) return rc;
@@ -447,7 +467,7 @@ int batch_enqueue(sr_context ctx, BatchWork& W, const BatchPlan& pl, bool use_gr
     uint32_t* d_vend = (uint32_t*)(tb + sizeof(SceneGeom) * ns + sizeof(RasterParams) * ns);
     uint32_t* d_tend = d_vend + ns;
     uint32_t* d_tile_end = d_tend + ns;
-    uint32_t* misc = P<uint32_t>(W.misc);  // [0] overflow [1] pairs [2] err [3] scratch [8..) queues
+    uint32_t* misc = P<uint32_t>(W.misc);  // [0] overflow [1] pairs [2] err [3..4] sort [5] tile cost total
     const VertexBuf vb{P<float>(W.verts), nvb};
     const int swapped_passes = (bits_for((uint32_t)std::max<uint64_t>(NTILES, 2)) + 7) / 8;
     const uint32_t* sorted_vals = (swapped_passes & 1) ? P<uint32_t>(W.vals2) : P<uint32_t>(W.vals);
@@ -515,7 +535,7 @@ int batch_enqueue(sr_context ctx, BatchWork& W, const BatchPlan& pl, bool use_gr
             HIP_OK(ctx, hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
         }
         BatchTables T{d_geom, d_vend, d_tend, ns, (uint32_t)NV, (uint32_t)NT};
-        launch_fill(misc, 0u, (uint32_t)(8 + 8 * pl.classes.size()), s);
+        launch_fill(misc, 0u, 8u, s);
         launch_vertex_batch(T, vb, s);
         launch_tri_setup_batch(T, vb, P<TriRec>(W.recs), P<uint32_t>(W.counts), misc + 2, s);
         launch_scan(P<uint32_t>(W.counts), P<uint32_t>(W.offsets), (uint32_t)NT, P<uint32_t>(W.scan), misc + 1, s);
@@ -528,9 +548,16 @@ int batch_enqueue(sr_context ctx, BatchWork& W, const BatchPlan& pl, bool use_gr
         launch_fill(P<uint32_t>(W.tstart), 0u, (uint32_t)NTILES, s);
         launch_fill(P<uint32_t>(W.tend), 0u, (uint32_t)NTILES, s);
         launch_ranges(skeys, cap, 0u, P<uint32_t>(W.tstart), P<uint32_t>(W.tend), s, misc + 1);
+        // cost-balanced runs of tiles for the persistent raster
+        launch_tile_cost(P<uint32_t>(W.tstart), P<uint32_t>(W.tend), (uint32_t)NTILES, TILE_FIXED_COST,
+                         P<uint32_t>(W.cost), s);
+        launch_scan(P<uint32_t>(W.cost), P<uint32_t>(W.cost), (uint32_t)NTILES + 1, P<uint32_t>(W.scan), misc + 5, s);
+        uint32_t* runs = P<uint32_t>(W.parts);
         for (size_t c = 0; c < pl.classes.size(); c++) {
             const BatchClass& C = pl.classes[c];
             if (C.ntiles == 0) continue;
+            const uint32_t G = raster_runs(C);
+            launch_partition(P<uint32_t>(W.cost), C.tile0, C.ntiles, G, runs, s);
             RasterBatch RB;
             memset(&RB, 0, sizeof(RB));
             RB.one.fp = (int)C.fp; RB.one.blend = (int)C.blend; RB.one.ets = C.ets; RB.one.smode = 0;
@@ -539,10 +566,10 @@ int batch_enqueue(sr_context ctx, BatchWork& W, const BatchPlan& pl, bool use_gr
             RB.nscenes = ns;
             RB.tile0 = C.tile0;
             RB.ntiles = C.ntiles;
-            RB.queue = misc + 8 + 8 * c;
+            RB.wg_start = runs;
             RB.overflow = misc;
-            const uint32_t grid = std::min<uint32_t>(C.ntiles, C.ets == 8 ? 256u * 20u : 256u * 5u);
-            launch_raster_tris(RB, grid, s);
+            launch_raster_tris(RB, G, s);
+            runs += G + 1;
         }
         if (use_graph) {
             hipGraph_t g = nullptr;
@@ -590,6 +617,14 @@ int batch_draw(sr_context ctx, const sr_batch_item* items, uint32_t n) {
         for (sr_framebuffer f : seen)
             if (f == it.fb) return fail(ctx, SR_ERR_INVALID_ARGUMENT, "batch: a framebuffer appears twice");
         seen.push_back(it.fb);
+    }
+    for (uint32_t i = 0; i < n; i++) {
+        const sr_texture t = items[i].tex;
+        if (t && t->view)
+            for (sr_framebuffer f : seen)
+                if (f == t->fb)
+                    return fail(ctx, SR_ERR_INVALID_ARGUMENT,
+                                "batch: an item samples a framebuffer the batch draws into");
     }
     std::vector<sr_batch_item> fast;
     for (uint32_t i = 0; i < n; i++) {
@@ -783,8 +818,20 @@ int sr_texture_upload(sr_context ctx, uint32_t w, uint32_t h, const uint32_t* rg
 int sr_texture_free(sr_texture t) {
     if (!t) return SR_ERR_INVALID_ARGUMENT;
     hipStreamSynchronize(t->ctx->stream);
-    hipFree(t->d);
+    if (!t->view) hipFree(t->d);
     delete t;
+    return SR_OK;
+}
+
+int sr_texture_from_fb(sr_framebuffer fb, sr_texture* out) {
+    if (!fb || !out) return SR_ERR_INVALID_ARGUMENT;
+    sr_texture t = new (std::nothrow) sr_texture_t();
+    if (!t) return SR_ERR_OUT_OF_MEMORY;
+    t->ctx = fb->ctx; t->w = fb->w; t->h = fb->h;
+    t->d = fb->color;  // RGBA8 row-major: the texture layout already
+    t->view = true;
+    t->fb = fb;
+    *out = t;
     return SR_OK;
 }
 

@@ -59,6 +59,8 @@ struct sr_texture_t {
     sr_context ctx;
     uint32_t w = 0, h = 0;
     uint32_t* d = nullptr;
+    bool view = false;              // aliases a framebuffer's colour plane (not owned)
+    sr_framebuffer fb = nullptr;    // that framebuffer
 };
 
 struct sr_framebuffer_t {
@@ -105,7 +107,8 @@ bool batchable(sr_mesh m, const sr_draw_desc* d);
 // the scene tables and the readback words, a stream, and the cached graph of
 // the last synchronously drawn batch.
 struct BatchWork {
-    DevBuf verts, recs, counts, offsets, keys, vals, keys2, vals2, tstart, tend, hist, scan, tables, misc;
+    DevBuf verts, recs, counts, offsets, keys, vals, keys2, vals2, tstart, tend, hist, scan, tables, misc,
+        cost, parts;
     uint32_t cap = 0;          // bin pair capacity
     std::vector<uint8_t> key;  // what the cached graph renders
     hipGraph_t graph = nullptr;

@@ -25,7 +25,7 @@ ERRORS = {
 EXPORTED = [
     "sr_ctx_create", "sr_ctx_destroy", "sr_last_error", "sr_finish", "sr_ctx_stream",
     "sr_ctx_stage_times", "sr_ctx_last_stats", "sr_mesh_upload", "sr_mesh_free",
-    "sr_texture_upload", "sr_texture_free", "sr_fb_create", "sr_fb_free", "sr_fb_clear",
+    "sr_texture_upload", "sr_texture_free", "sr_texture_from_fb", "sr_fb_create", "sr_fb_free", "sr_fb_clear",
     "sr_fb_read", "sr_fb_write", "sr_fb_size", "sr_draw", "sr_batch_draw",
     "sr_ctx_set_exec_tile_policy", "sr_ctx_set_batch_capacity",
     "sr_server_create", "sr_server_destroy", "sr_server_submit", "sr_server_poll",
@@ -153,6 +153,7 @@ def lib():
         L.sr_texture_upload.argtypes = [vp, C.c_uint32, C.c_uint32, C.POINTER(C.c_uint32),
                                         C.POINTER(vp)]
         L.sr_texture_free.argtypes = [vp]
+        L.sr_texture_from_fb.argtypes = [vp, C.POINTER(vp)]
         L.sr_fb_create.argtypes = [vp, C.c_uint32, C.c_uint32, C.POINTER(vp)]
         L.sr_fb_free.argtypes = [vp]
         L.sr_fb_clear.argtypes = [vp, C.c_uint32, C.c_float, C.c_uint8]
@@ -242,6 +243,9 @@ class Context:
 
     def framebuffer(self, w, h) -> "Framebuffer":
         return Framebuffer(self, w, h)
+
+    def texture_from_fb(self, fb: "Framebuffer") -> "Texture":
+        return Texture(self, fb.w, fb.h_, fb=fb)
 
     def draw(self, fb: "Framebuffer", mesh: "Mesh", desc: DrawDesc, tex: "Texture" = None):
         self._check(lib().sr_draw(self.h, fb.h, mesh.h, C.byref(desc), tex.h if tex else None))
@@ -362,8 +366,13 @@ class Mesh:
 
 
 class Texture:
-    def __init__(self, ctx: Context, w: int, h: int, rgba):
+    def __init__(self, ctx: Context, w: int, h: int, rgba=None, fb: "Framebuffer" = None):
+        """An uploaded RGBA8 image, or (fb=...) a view of a framebuffer's
+        colour plane for render-to-texture."""
         self.h = C.c_void_p()
+        if fb is not None:
+            ctx._check(lib().sr_texture_from_fb(fb.h, C.byref(self.h)))
+            return
         a = np.ascontiguousarray(rgba, np.uint32)
         ctx._check(lib().sr_texture_upload(ctx.h, w, h, a.ctypes.data_as(C.POINTER(C.c_uint32)),
                                            C.byref(self.h)))

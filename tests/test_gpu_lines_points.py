@@ -139,3 +139,13 @@ def test_clip_frame_reduced(ctx):
     ref, st = render_ref(s)
     assert st[0]["gen_tris"] > 20_000
     assert_parity(render_gpu(ctx, s), ref)
+
+
+def test_clip_frame_full_size(ctx):
+    # the full 3840x2160 frame: 500k triangles (30% through the near clip),
+    # 200k Wu lines with f32 alpha, 100k points, stencil increment; the
+    # single-threaded oracle takes ~30 s on one core
+    s = scenes.clip_scene()
+    ref, st = render_ref(s)
+    assert st[0]["gen_tris"] > 500_000 and st[1]["line_plots"] > 10_000_000
+    assert_parity(render_gpu(ctx, s), ref)

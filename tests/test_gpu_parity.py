@@ -88,6 +88,14 @@ def test_soup_1080p_100k(ctx):
     assert_parity(render_gpu(ctx, s), ref)
 
 
+def test_soup_1080p_full_size(ctx):
+    # the 1920x1080, 1M-triangle frame the benchmark draws (oracle: ~15 s)
+    s = scenes.soup_scene(1_000_000, 1920, 1080)
+    ref, st = render_ref(s)
+    assert st[0]["shaded"] == 6973823, st[0]["shaded"]
+    assert_parity(render_gpu(ctx, s), ref)
+
+
 def test_degenerate_and_tiny(ctx):
     # repeated vertices, collinear triangles, zero-area, and sub-pixel ones
     p = np.array([[-0.5, -0.5, 0.3], [-0.5, -0.5, 0.3], [0.2, 0.4, 0.3],
@@ -166,3 +174,43 @@ def test_accumulating_draws(ctx):
     both = dict(width=200, height=130, clear=0xFF101010, texture=a["texture"], draws=[a, b])
     ref, _ = render_ref(both)
     assert_parity(render_gpu(ctx, both), ref)
+
+
+@pytest.mark.parametrize("flt,edge", [(1, 0), (0, 1), (1, 2)])
+def test_render_to_texture_two_passes(ctx, flt, edge):
+    # pass 1: the lit sphere into a 512x512 framebuffer; pass 2: a soup that
+    # samples pass 1's colour plane through a texture view (no copy).  The
+    # oracle runs both passes, feeding its own pass-1 image to pass 2.
+    import softrender_hip as sr
+
+    s1 = scenes.sphere_scene()
+    ref1, _ = render_ref(s1)
+    fb1 = ctx.framebuffer(s1["width"], s1["height"])
+    fb1.clear(s1.get("clear", 0))
+    m1 = ctx.mesh(s1)
+    ctx.draw(fb1, m1, sr.make_desc(**s1["desc"]))
+    tex = ctx.texture_from_fb(fb1)
+    s2 = scenes.soup_scene(20_000, 320, 240, seed=61)
+    s2["desc"] = dict(s2["desc"], tex_filter=flt, tex_edge=edge, tex_border=0xFF3366CC)
+    s2["u"] = (s2["u"] * 1.5 - 0.25).astype(np.float32)  # reach past the edges
+    s2["texture"] = (s1["width"], s1["height"], ref1.color.copy())
+    ref2, st = render_ref(s2)
+    assert st[0]["shaded"] > 10_000
+    fb2 = ctx.framebuffer(s2["width"], s2["height"])
+    fb2.clear(s2["clear"])
+    m2 = ctx.mesh(s2)
+    ctx.draw(fb2, m2, sr.make_desc(**s2["desc"]), tex)
+    assert_parity(fb2.read(), ref2)
+    # sampling the framebuffer being drawn is refused
+    with pytest.raises(sr.SoftrenderError) as e:
+        ctx.draw(fb1, m2, sr.make_desc(**s2["desc"]), tex)
+    assert e.value.code == -1
+    # so is a batch item sampling a framebuffer the same batch draws
+    with pytest.raises(sr.SoftrenderError):
+        ctx.batch_draw([(fb1, m1, sr.make_desc(**s1["desc"]), None, 0xFF000000),
+                        (fb2, m2, sr.make_desc(**s2["desc"]), tex, 0xFF000000)])
+    tex.free()
+    for m in (m1, m2):
+        m.free()
+    for fb in (fb1, fb2):
+        fb.free()
