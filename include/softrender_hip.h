@@ -221,6 +221,25 @@ int sr_fb_write(sr_framebuffer fb, const uint32_t* rgba, const float* depth,
                 const uint8_t* stencil);
 int sr_fb_size(sr_framebuffer fb, uint32_t* width, uint32_t* height);
 
+/* ---- ingestion: Wavefront OBJ -------------------------------------------
+ * The reference's examples load meshes with tobj 0.1.3's load_obj
+ * (examples/suzanne.rs:23-47): each distinct v/vt/vn corner becomes a
+ * vertex (in order of first use), polygons become triangle fans from their
+ * first corner, and an `o` statement starts a new model. */
+typedef struct sr_obj_info {
+    uint32_t models;      /* models in the file */
+    uint32_t vertices;    /* of the requested model */
+    uint32_t indices;     /* 3 per triangle */
+    int32_t has_normals, has_uvs;
+} sr_obj_info;
+/* Host-side parse (no device work).  With null arrays only `info` is
+ * filled; otherwise positions (3 floats per vertex), normals (3, when
+ * present), uvs (2, when present) and indices are copied out. */
+int sr_obj_parse(const char* path, uint32_t model, sr_obj_info* info, float* positions, float* normals,
+                 float* uvs, uint32_t* indices);
+/* Parses model `model` and uploads it as an indexed triangle mesh. */
+int sr_mesh_load_obj(sr_context ctx, const char* path, uint32_t model, sr_mesh* out);
+
 /* ---- draw --------------------------------------------------------------- */
 /* One reference draw chain (see top of file).  `tex` may be NULL unless the
  * fragment program samples.  Asynchronous on the context stream except
@@ -237,12 +256,17 @@ typedef struct sr_batch_item {
     int32_t clear;       /* non-zero: Framebuffer::clear(clear_rgba) first (depth far, stencil 0) */
     uint32_t clear_rgba;
 } sr_batch_item;
-/* Renders n independent (framebuffer, mesh) draws; each framebuffer must
- * appear at most once.  Results equal, item by item, an sr_fb_clear (if
- * `clear`) followed by sr_draw.  Triangle draws without a geometry program
- * and with stencil Always/Keep run in one pass over the whole batch
- * (captured in a hipGraph and replayed while the batch is unchanged);
- * other items are drawn one by one.  Synchronous. */
+/* Renders n (framebuffer, mesh) draws.  Results equal, item by item in
+ * order, an sr_fb_clear (if `clear`) followed by sr_draw.  A framebuffer may
+ * appear several times: its items are then drawn into it in item order,
+ * like several render_mesh chains on one pipeline (the multi-mesh case of
+ * src/lib.rs:41-51).  Triangle draws without a geometry program and with
+ * stencil Always/Keep run in one pass over the whole batch (captured in a
+ * hipGraph and replayed while the batch is unchanged) when every item on
+ * their framebuffer qualifies, shares execution tile, program and blend,
+ * and only the first of them clears; other framebuffers' items are drawn
+ * one by one.  No item may sample a framebuffer the batch draws into.
+ * Synchronous. */
 int sr_batch_draw(sr_context ctx, const sr_batch_item* items, uint32_t n);
 
 /* Execution tile used when sr_draw_desc.exec_tile == 0: entry i applies to

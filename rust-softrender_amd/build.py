@@ -30,7 +30,8 @@ COMMON = [
     "-Wno-unused-result", "-Wno-unknown-warning-option",
 ]
 
-UNITS = [("sr_kernels.hip", []), ("sr_runtime.hip", []), ("sr_server.hip", []), ("sr_raster_lp.hip", [])] + [
+UNITS = [("sr_kernels.hip", []), ("sr_runtime.hip", []), ("sr_server.hip", []), ("sr_obj.cpp", []),
+         ("sr_raster_lp.hip", [])] + [
     ("sr_raster.hip", [f"-DSR_ETS={e}"]) for e in (8, 16, 32, 64)]
 
 

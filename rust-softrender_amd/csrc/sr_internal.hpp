@@ -108,14 +108,17 @@ struct RasterParams {
 };
 
 // Raster launch: one scene given inline (one work-group per tile), or a
-// table of scenes whose execution tiles are numbered globally
-// [scene_tile_end[s-1], scene_tile_end[s]) and split into runs of
-// near-equal cost, one per work-group of a persistent grid.
+// batch: a table of draws ("scenes") grouped by framebuffer, each group's
+// draws consecutive and in item order; the groups' execution tiles are
+// numbered globally [group_tile_end[g-1], group_tile_end[g]) and split
+// into runs of near-equal cost, one per work-group of a persistent grid.
 struct RasterBatch {
     RasterParams one;
     const RasterParams* scenes;      // null: use `one`
-    const uint32_t* scene_tile_end;  // inclusive prefix of tiles per scene
-    uint32_t nscenes;
+    const uint32_t* group_tile_end;  // inclusive prefix of tiles per group
+    const uint32_t* group_first;     // first draw of each group; [ngroups] = draw count
+    const uint32_t* tri_end;         // per draw: exclusive end of its batch triangle ids
+    uint32_t ngroups;
     uint32_t tile0, ntiles;          // global tile range of this launch
     const uint32_t* wg_start;        // run p = tiles [tile0 + wg_start[p], tile0 + wg_start[p+1])
     const uint32_t* overflow;        // non-zero: binning overflowed, draw nothing

@@ -1,6 +1,8 @@
 // sr_raster.hip — the raster work-groups.  Compiled once per execution
 // tile size (-DSR_ETS=8|16|32|64) so the template instances build in
 // parallel; each object exports launch_raster_*_<ETS>.
+#include <type_traits>
+
 #include "sr_device.hpp"
 #include "sr_kernels.hpp"
 
@@ -209,9 +211,53 @@ struct RasterShared {
 
 // One execution tile: `tile` is the scene-local tile index, `gtile` the
 // global one that indexes the bins.
-template <int ETS, int FP, int BLEND, int SMODE>
+// Scene parameters through the scalar cache (a uniform index).
+__device__ __forceinline__ RasterParams load_params(const RasterParams* base, uint32_t s) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    constexpr int N = sizeof(RasterParams) / 4;
+    const __attribute__((address_space(4))) uint32_t* src =
+        (const __attribute__((address_space(4))) uint32_t*)(base + s);
+    uint32_t w[N];
+#pragma unroll
+    for (int i = 0; i < N; i++) w[i] = src[i];
+    RasterParams r;
+    __builtin_memcpy(&r, w, sizeof(r));
+    return r;
+#else
+    return base[s];
+#endif
+}
+
+// The draws a tile renders, in order: the flat kernel has one; a batch
+// group (several draws into one framebuffer) has its draws' parameters in
+// the scene table and each draw's items are the tile-list entries whose
+// triangle ids lie below the draw's end.
+struct OneDraw {
+    __device__ uint32_t count() const { return 1u; }
+};
+
+struct GroupDraws {
+    const RasterParams* table;  // scene (draw) table
+    const uint32_t* tri_end;    // exclusive end of each draw's triangle ids
+    uint32_t d0, n;             // the group's first draw and draw count
+    __device__ uint32_t count() const { return n; }
+};
+
+// First position in items[lo, hi) (ascending) whose id is >= key.
+__device__ __forceinline__ uint32_t lower_bound_items(const uint32_t* items, uint32_t lo, uint32_t hi,
+                                                      uint32_t key) {
+    while (lo < hi) {
+        const uint32_t m = (lo + hi) >> 1;
+        if (load_item(items, m) < key) lo = m + 1;
+        else hi = m;
+    }
+    return lo;
+}
+
+
+template <int ETS, int FP, int BLEND, int SMODE, class Draws>
 __device__ __forceinline__ void raster_tile(const RasterParams& P, uint32_t tile, uint32_t gtile,
-                                            RasterShared<ETS, FP, BLEND>& sh) {
+                                            RasterShared<ETS, FP, BLEND>& sh, const Draws& draws) {
     using SH = RasterShared<ETS, FP, BLEND>;
     constexpr int S = SH::S, PPT = SH::PPT, Q = SH::Q;
     constexpr int RPP = 64 / S;  // quadrant rows covered by one pixel pass
@@ -267,7 +313,7 @@ __device__ __forceinline__ void raster_tile(const RasterParams& P, uint32_t tile
     uint16_t* wflat = sh.flat[wave];
 
     // Shade every queued fragment of this wave and blend them in order.
-    auto flush = [&]() {
+    auto flush = [&](const RasterParams& PD) {
         wave_sync_lds();
         uint32_t mine = 0;
 #pragma unroll
@@ -283,7 +329,7 @@ __device__ __forceinline__ void raster_tile(const RasterParams& P, uint32_t tile
             if (r < total) {
                 const uint32_t s = wflat[r];
                 const QRec q = wq[s];
-                wq[s].tri = shade_rec<FP>(P, q, lut);
+                wq[s].tri = shade_rec<FP>(PD, q, lut);
             }
         }
         wave_sync_lds();
@@ -297,8 +343,10 @@ __device__ __forceinline__ void raster_tile(const RasterParams& P, uint32_t tile
     };
 
 
-    // screen tiles meeting this wave's pixels, in the reference's row-major order
-    if (beg < end && wx0 <= wx1 && wy0 <= wy1) {
+    // One draw's items [sb, se): the screen tiles meeting this wave's pixels
+    // in the reference's row-major order, each walking the items in order.
+    auto walk = [&](uint32_t sb, uint32_t se, const RasterParams& PD) {
+    if (sb < se && wx0 <= wx1 && wy0 <= wy1) {
         uint32_t jlo, jhi, ilo, ihi;
         sem_range((uint32_t)wx0, (uint32_t)wx1, P.tw, P.scols, jlo, jhi);
         sem_range((uint32_t)wy0, (uint32_t)wy1, P.th, P.srows, ilo, ihi);
@@ -322,15 +370,15 @@ __device__ __forceinline__ void raster_tile(const RasterParams& P, uint32_t tile
                 // fetches overlap instead of stalling each triangle.  Loads
                 // are unconditional, with positions clamped into [beg, end),
                 // so no loaded register is copied (which would wait for it).
-                const uint32_t b0 = beg & ~63u;
-                uint32_t id = load_ids(P.items, b0, beg, end, lane);
+                const uint32_t b0 = sb & ~63u;
+                uint32_t id = load_ids(P.items, b0, sb, se, lane);
                 RecBlock blk = load_block(P.tris, id);
-                uint32_t idn = load_ids(P.items, b0 + 64, beg, end, lane);
-                for (uint32_t c0 = b0; c0 < end; c0 += 64) {
+                uint32_t idn = load_ids(P.items, b0 + 64, sb, se, lane);
+                for (uint32_t c0 = b0; c0 < se; c0 += 64) {
                     const RecBlock nxt = load_block(P.tris, idn);
                     const uint32_t idc = idn;
-                    idn = load_ids(P.items, c0 + 128, beg, end, lane);
-                    const uint32_t lo = max(c0, beg), hi = min(c0 + 64, end);
+                    idn = load_ids(P.items, c0 + 128, sb, se, lane);
+                    const uint32_t lo = max(c0, sb), hi = min(c0 + 64, se);
                     for (uint32_t c = lo; c < hi; c++) {
                     const int k = (int)(c - c0);
                     const RecLane rc = rec_at(blk, k);
@@ -375,7 +423,7 @@ __device__ __forceinline__ void raster_tile(const RasterParams& P, uint32_t tile
                         }
                     }
                     if constexpr (BLEND == SR_BLEND_OVER) {
-                        if (__ballot(full)) flush();
+                        if (__ballot(full)) flush(PD);
                     }
                     }
                     blk = nxt;
@@ -384,7 +432,25 @@ __device__ __forceinline__ void raster_tile(const RasterParams& P, uint32_t tile
             }
         }
     }
-    flush();
+    };
+
+    if constexpr (std::is_same<Draws, OneDraw>::value) {
+        walk(beg, end, P);
+        flush(P);
+    } else {
+        // each draw of the group in order: its own pass over the screen
+        // tiles, its fragments shaded with its own parameters
+        uint32_t sb = beg;
+        for (uint32_t d = 0; d < draws.count(); d++) {
+            const uint32_t se = d + 1 < draws.count()
+                                    ? lower_bound_items(P.items, sb, end, load_item(draws.tri_end, draws.d0 + d))
+                                    : end;
+            const RasterParams PD = load_params(draws.table, draws.d0 + d);
+            walk(sb, se, PD);
+            flush(PD);
+            sb = se;
+        }
+    }
     // resolve: the tile goes back to HBM once
 #pragma unroll
     for (int j = 0; j < PPT; j++) {
@@ -407,23 +473,6 @@ __device__ __forceinline__ void raster_tile(const RasterParams& P, uint32_t tile
 }
 
 
-// Scene parameters through the scalar cache (a uniform index).
-__device__ __forceinline__ RasterParams load_params(const RasterParams* base, uint32_t s) {
-#if defined(__HIP_DEVICE_COMPILE__)
-    constexpr int N = sizeof(RasterParams) / 4;
-    const __attribute__((address_space(4))) uint32_t* src =
-        (const __attribute__((address_space(4))) uint32_t*)(base + s);
-    uint32_t w[N];
-#pragma unroll
-    for (int i = 0; i < N; i++) w[i] = src[i];
-    RasterParams r;
-    __builtin_memcpy(&r, w, sizeof(r));
-    return r;
-#else
-    return base[s];
-#endif
-}
-
 // One scene: one work-group per execution tile.
 template <int ETS, int FP, int BLEND, int SMODE>
 __global__ __launch_bounds__(256) SR_RASTER_OCCUPANCY void k_raster_tris(RasterParams P) {
@@ -432,7 +481,7 @@ __global__ __launch_bounds__(256) SR_RASTER_OCCUPANCY void k_raster_tris(RasterP
     for (int i = threadIdx.x; i < 256; i += SH::NT) sh.lut[i] = (float)i / 255.0f;
     __syncthreads();
     const uint32_t tile = xcd_remap(blockIdx.x, gridDim.x);
-    raster_tile<ETS, FP, BLEND, SMODE>(P, tile, tile, sh);
+    raster_tile<ETS, FP, BLEND, SMODE>(P, tile, tile, sh, OneDraw{});
 }
 
 // A batch: a persistent grid over the scene table (no stencil effects in
@@ -449,23 +498,25 @@ __global__ __launch_bounds__(256) SR_RASTER_OCCUPANCY void k_raster_tris_batch(R
     // stretch of tiles (neighbouring tiles share records in its L2).
     const uint32_t p = xcd_remap(blockIdx.x, gridDim.x);
     const uint32_t t0 = load_item(B.wg_start, p), t1 = load_item(B.wg_start, p + 1);
-    uint32_t scene = 0, scene_end = 0;
+    uint32_t group = 0, group_end = 0;
     for (uint32_t t = t0; t < t1; t++) {
         const uint32_t g = B.tile0 + t;
-        if (g >= scene_end) {
-            // scene owning global tile g: first s with g < scene_tile_end[s]
-            uint32_t a = 0, b = B.nscenes;
+        if (g >= group_end) {
+            // framebuffer group owning global tile g: first s with g < group_tile_end[s]
+            uint32_t a = 0, b = B.ngroups;
             while (a < b) {
                 const uint32_t m = (a + b) >> 1;
-                if (load_item(B.scene_tile_end, m) <= g) a = m + 1;
+                if (load_item(B.group_tile_end, m) <= g) a = m + 1;
                 else b = m;
             }
-            scene = a;
-            scene_end = load_item(B.scene_tile_end, a);
+            group = a;
+            group_end = load_item(B.group_tile_end, a);
         }
-        // reloaded per tile (scalar-cache hits) rather than held in registers
-        const RasterParams P = load_params(B.scenes, scene);
-        raster_tile<ETS, FP, BLEND, 0>(P, g - P.tile_base, g, sh);
+        // the group's draws, in item order; parameters are reloaded per tile
+        // (scalar-cache hits) rather than held in registers
+        const uint32_t d0 = load_item(B.group_first, group), d1 = load_item(B.group_first, group + 1);
+        const RasterParams P = load_params(B.scenes, d0);
+        raster_tile<ETS, FP, BLEND, 0>(P, g - P.tile_base, g, sh, GroupDraws{B.scenes, B.tri_end, d0, d1 - d0});
     }
 }
 

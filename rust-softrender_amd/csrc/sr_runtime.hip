@@ -361,17 +361,29 @@ bool batchable(sr_mesh m, const sr_draw_desc* d) {
 }
 
 int batch_plan(sr_context ctx, const sr_batch_item* items, uint32_t n, BatchPlan& pl) {
+    // groups: framebuffers in order of first appearance, items in order
+    std::vector<sr_framebuffer> fbs;
+    std::vector<std::vector<uint32_t>> members;
+    for (uint32_t i = 0; i < n; i++) {
+        size_t g = 0;
+        while (g < fbs.size() && fbs[g] != items[i].fb) g++;
+        if (g == fbs.size()) {
+            fbs.push_back(items[i].fb);
+            members.emplace_back();
+        }
+        members[g].push_back(i);
+    }
     struct Order {
-        uint32_t item;
+        uint32_t group;
         uint32_t ets;
         uint64_t cls;  // (ets, fp, blend)
     };
-    std::vector<Order> order(n);
-    for (uint32_t i = 0; i < n; i++) {
-        const sr_draw_desc* d = items[i].desc;
-        const uint32_t ets = d->exec_tile ? (uint32_t)d->exec_tile
-                                          : (uint32_t)policy_tile(ctx, items[i].fb->w, items[i].fb->h);
-        order[i] = Order{i, ets, ((uint64_t)ets << 16) | ((uint64_t)d->fragment_program << 8) | (uint64_t)d->blend};
+    std::vector<Order> order(fbs.size());
+    for (uint32_t g = 0; g < fbs.size(); g++) {
+        const sr_batch_item& it = items[members[g][0]];
+        const sr_draw_desc* d = it.desc;
+        const uint32_t ets = d->exec_tile ? (uint32_t)d->exec_tile : (uint32_t)policy_tile(ctx, it.fb->w, it.fb->h);
+        order[g] = Order{g, ets, ((uint64_t)ets << 16) | ((uint64_t)d->fragment_program << 8) | (uint64_t)d->blend};
     }
     std::stable_sort(order.begin(), order.end(), [](const Order& a, const Order& b) { return a.cls < b.cls; });
     pl = BatchPlan();
@@ -380,36 +392,46 @@ int batch_plan(sr_context ctx, const sr_batch_item* items, uint32_t n, BatchPlan
     pl.geom.resize(n);
     pl.vend.resize(n);
     pl.tend.resize(n);
-    pl.tile_end.resize(n);
-    for (uint32_t k = 0; k < n; k++) {
-        pl.items[k] = items[order[k].item];
-        pl.descs[k] = *pl.items[k].desc;
-        pl.items[k].desc = nullptr;  // the copy in descs[k] is the one used
-        const sr_draw_desc* d = &pl.descs[k];
-        const sr_mesh m = pl.items[k].mesh;
-        const uint32_t W = pl.items[k].fb->w, H = pl.items[k].fb->h, ets = order[k].ets;
-        const bool drawable = W > 1 && H > 1;  // W or H == 1: no tiles (fragment.rs:196,201)
-        const uint32_t ntri = drawable ? (m->ni ? m->ni : m->nv) / 3 : 0;
-        SceneGeom& g = pl.geom[k];
-        memset(&g, 0, sizeof(g));
-        g.px = m->soa[0]; g.py = m->soa[1]; g.pz = m->soa[2]; g.nx = m->soa[3]; g.ny = m->soa[4];
-        g.nz = m->soa[5]; g.u = m->soa[6]; g.v = m->soa[7]; g.idx = m->idx;
-        g.nv = ntri ? m->nv : 0; g.vbase = (uint32_t)pl.NV;
-        g.nt = ntri; g.tbase = (uint32_t)pl.NT;
-        g.program = d->vertex_program; g.K = varyings_of(d->vertex_program); g.cull = d->cull;
-        g.empty_cull = 1;  // batched draws have no stencil effects
-        g.W = W; g.H = H; g.ets = ets; g.ntx = (W + ets - 1) / ets;
-        g.tile_base = (uint32_t)pl.NTILES;
-        g.viewport = d->viewport;
-        memcpy(g.U, d->vs_uniforms, sizeof(g.U));
-        const uint32_t ntiles = g.ntx * ((H + ets - 1) / ets);
-        pl.NV += g.nv; pl.NT += g.nt; pl.NTILES += ntiles;
-        pl.vend[k] = (uint32_t)pl.NV; pl.tend[k] = (uint32_t)pl.NT; pl.tile_end[k] = (uint32_t)pl.NTILES;
-        if (pl.classes.empty() || pl.classes.back().cls != order[k].cls)
-            pl.classes.push_back(BatchClass{order[k].cls, ets, (uint32_t)d->fragment_program, (uint32_t)d->blend,
-                                            g.tile_base, 0});
+    uint32_t k = 0;
+    for (const Order& o : order) {
+        const sr_framebuffer fb = fbs[o.group];
+        const uint32_t W = fb->w, H = fb->h, ets = o.ets;
+        const uint32_t ntx = (W + ets - 1) / ets, ntiles = ntx * ((H + ets - 1) / ets);
+        const uint32_t tile_base = (uint32_t)pl.NTILES;
+        pl.group_first.push_back(k);
+        for (uint32_t i : members[o.group]) {
+            pl.items[k] = items[i];
+            pl.descs[k] = *items[i].desc;
+            pl.items[k].desc = nullptr;  // the copy in descs[k] is the one used
+            const sr_draw_desc* d = &pl.descs[k];
+            const sr_mesh m = pl.items[k].mesh;
+            const bool drawable = W > 1 && H > 1;  // W or H == 1: no tiles (fragment.rs:196,201)
+            const uint32_t ntri = drawable ? (m->ni ? m->ni : m->nv) / 3 : 0;
+            SceneGeom& g = pl.geom[k];
+            memset(&g, 0, sizeof(g));
+            g.px = m->soa[0]; g.py = m->soa[1]; g.pz = m->soa[2]; g.nx = m->soa[3]; g.ny = m->soa[4];
+            g.nz = m->soa[5]; g.u = m->soa[6]; g.v = m->soa[7]; g.idx = m->idx;
+            g.nv = ntri ? m->nv : 0; g.vbase = (uint32_t)pl.NV;
+            g.nt = ntri; g.tbase = (uint32_t)pl.NT;
+            g.program = d->vertex_program; g.K = varyings_of(d->vertex_program); g.cull = d->cull;
+            g.W = W; g.H = H; g.ets = ets; g.ntx = ntx;
+            g.tile_base = tile_base;
+            g.empty_cull = 1;  // batched draws have no stencil effects
+            g.viewport = d->viewport;
+            memcpy(g.U, d->vs_uniforms, sizeof(g.U));
+            pl.NV += g.nv; pl.NT += g.nt;
+            pl.vend[k] = (uint32_t)pl.NV; pl.tend[k] = (uint32_t)pl.NT;
+            k++;
+        }
+        pl.NTILES += ntiles;
+        pl.group_tile_end.push_back((uint32_t)pl.NTILES);
+        const sr_draw_desc* d0 = items[members[o.group][0]].desc;
+        if (pl.classes.empty() || pl.classes.back().cls != o.cls)
+            pl.classes.push_back(BatchClass{o.cls, ets, (uint32_t)d0->fragment_program, (uint32_t)d0->blend,
+                                            tile_base, 0});
         pl.classes.back().ntiles += ntiles;
     }
+    pl.group_first.push_back(k);
     if (pl.NV >= (1ull << 31) || pl.NT >= (1ull << 31) || pl.NTILES >= (1ull << 31))
         return fail(ctx, SR_ERR_OUT_OF_MEMORY, "batch: too large");
     return SR_OK;
@@ -456,7 +478,9 @@ int batch_enqueue(sr_context ctx, BatchWork& W, const BatchPlan& pl, bool use_gr
         (rc = ensure(ctx, W.cost, sizeof(uint32_t) * (NTILES + 1), s)) ||
         (rc = ensure(ctx, W.parts, sizeof(uint32_t) * (raster_runs_total(pl) + pl.classes.size()), s)))
         return rc;
-    const size_t tbytes = sizeof(SceneGeom) * ns + sizeof(RasterParams) * ns + sizeof(uint32_t) * 3 * ns + 64;
+    const uint32_t ng = (uint32_t)pl.group_tile_end.size();
+    const size_t tbytes = sizeof(SceneGeom) * ns + sizeof(RasterParams) * ns +
+                          sizeof(uint32_t) * (2 * ns + 2 * ng + 1) + 64;
     if ((rc = ensure(ctx, W.tables, tbytes, s))) return rc;
     if (W.device_bytes() != before) W.drop_graph();  // a buffer moved
 
@@ -466,7 +490,8 @@ int batch_enqueue(sr_context ctx, BatchWork& W, const BatchPlan& pl, bool use_gr
     RasterParams* d_rp = (RasterParams*)(tb + sizeof(SceneGeom) * ns);
     uint32_t* d_vend = (uint32_t*)(tb + sizeof(SceneGeom) * ns + sizeof(RasterParams) * ns);
     uint32_t* d_tend = d_vend + ns;
-    uint32_t* d_tile_end = d_tend + ns;
+    uint32_t* d_group_tile_end = d_tend + ns;
+    uint32_t* d_group_first = d_group_tile_end + ng;
     uint32_t* misc = P<uint32_t>(W.misc);  // [0] overflow [1] pairs [2] err [3..4] sort [5] tile cost total
     const VertexBuf vb{P<float>(W.verts), nvb};
     const int swapped_passes = (bits_for((uint32_t)std::max<uint64_t>(NTILES, 2)) + 7) / 8;
@@ -527,7 +552,8 @@ int batch_enqueue(sr_context ctx, BatchWork& W, const BatchPlan& pl, bool use_gr
         uint32_t* hv = (uint32_t*)(hs + sizeof(SceneGeom) * ns + sizeof(RasterParams) * ns);
         memcpy(hv, pl.vend.data(), 4 * ns);
         memcpy(hv + ns, pl.tend.data(), 4 * ns);
-        memcpy(hv + 2 * ns, pl.tile_end.data(), 4 * ns);
+        memcpy(hv + 2 * ns, pl.group_tile_end.data(), 4 * ng);
+        memcpy(hv + 2 * ns + ng, pl.group_first.data(), 4 * (ng + 1));
         if (!use_graph) HIP_OK(ctx, hipEventRecord(W.ev0, s));
         HIP_OK(ctx, hipMemcpyAsync(tb, hs, tbytes - 64, hipMemcpyHostToDevice, s));
         if (use_graph) {
@@ -562,8 +588,10 @@ int batch_enqueue(sr_context ctx, BatchWork& W, const BatchPlan& pl, bool use_gr
             memset(&RB, 0, sizeof(RB));
             RB.one.fp = (int)C.fp; RB.one.blend = (int)C.blend; RB.one.ets = C.ets; RB.one.smode = 0;
             RB.scenes = d_rp;
-            RB.scene_tile_end = d_tile_end;
-            RB.nscenes = ns;
+            RB.group_tile_end = d_group_tile_end;
+            RB.group_first = d_group_first;
+            RB.tri_end = d_tend;
+            RB.ngroups = ng;
             RB.tile0 = C.tile0;
             RB.ntiles = C.ntiles;
             RB.wg_start = runs;
@@ -610,13 +638,11 @@ int batch_draw(sr_context ctx, const sr_batch_item* items, uint32_t n) {
     int rc;
     hipSetDevice(ctx->device);
     ctx->n_stage = 0;
-    std::vector<sr_framebuffer> seen;
+    std::vector<sr_framebuffer> seen;  // distinct framebuffers, in order of first appearance
     for (uint32_t i = 0; i < n; i++) {
         const sr_batch_item& it = items[i];
         if ((rc = validate_draw(ctx, it.fb, it.mesh, it.desc, it.tex))) return rc;
-        for (sr_framebuffer f : seen)
-            if (f == it.fb) return fail(ctx, SR_ERR_INVALID_ARGUMENT, "batch: a framebuffer appears twice");
-        seen.push_back(it.fb);
+        if (std::find(seen.begin(), seen.end(), it.fb) == seen.end()) seen.push_back(it.fb);
     }
     for (uint32_t i = 0; i < n; i++) {
         const sr_texture t = items[i].tex;
@@ -626,12 +652,32 @@ int batch_draw(sr_context ctx, const sr_batch_item* items, uint32_t n) {
                     return fail(ctx, SR_ERR_INVALID_ARGUMENT,
                                 "batch: an item samples a framebuffer the batch draws into");
     }
+    // Per framebuffer: its items form one group of the batch when all are
+    // batchable, share execution tile, program and blend, and only the first
+    // clears; otherwise they are drawn one by one, in item order, before the
+    // batch (framebuffers are independent, so this reorders nothing visible).
     std::vector<sr_batch_item> fast;
-    for (uint32_t i = 0; i < n; i++) {
-        const sr_batch_item& it = items[i];
-        if (batchable(it.mesh, it.desc)) {
-            fast.push_back(it);
-        } else {  // drawn on its own, in item order, before the batch
+    for (sr_framebuffer fb : seen) {
+        std::vector<uint32_t> mine;
+        for (uint32_t i = 0; i < n; i++)
+            if (items[i].fb == fb) mine.push_back(i);
+        bool group = true;
+        int ets0 = -1;
+        for (size_t j = 0; j < mine.size() && group; j++) {
+            const sr_batch_item& it = items[mine[j]];
+            const sr_draw_desc* d = it.desc;
+            const int ets = d->exec_tile ? d->exec_tile : policy_tile(ctx, fb->w, fb->h);
+            const sr_draw_desc* f = items[mine[0]].desc;
+            group = batchable(it.mesh, d) && (j == 0 || !it.clear) && (j == 0 || ets == ets0) &&
+                    d->fragment_program == f->fragment_program && d->blend == f->blend;
+            if (j == 0) ets0 = ets;
+        }
+        for (uint32_t i : mine) {
+            const sr_batch_item& it = items[i];
+            if (group) {
+                fast.push_back(it);
+                continue;
+            }
             if (it.clear && (rc = sr_fb_clear(it.fb, it.clear_rgba, -3.40282347e+38f, 0))) return rc;
             if ((rc = sr_draw(ctx, it.fb, it.mesh, it.desc, it.tex))) return rc;
         }

@@ -132,18 +132,22 @@ struct BatchClass {
 };
 
 // Host tables of one batch.  `items` are copies (descriptors included), so a
-// plan outlives the caller's arrays.
+// plan outlives the caller's arrays.  Draws into the same framebuffer form a
+// group: consecutive draws (in item order) sharing one range of execution
+// tiles.
 struct BatchPlan {
-    std::vector<sr_batch_item> items;
+    std::vector<sr_batch_item> items;  // per draw, group by group
     std::vector<sr_draw_desc> descs;
     std::vector<SceneGeom> geom;
-    std::vector<uint32_t> vend, tend, tile_end;
+    std::vector<uint32_t> vend, tend;                  // per draw: prefix of vertices / triangles
+    std::vector<uint32_t> group_tile_end, group_first; // per group (+1 entry for group_first)
     std::vector<BatchClass> classes;
     uint64_t NV = 0, NT = 0, NTILES = 0;
 };
 
-// Builds the tables for `items` (all batchable, validated, distinct
-// framebuffers); scenes are ordered by (execution tile, program, blend).
+// Builds the tables for `items` (all batchable and validated; items on one
+// framebuffer share execution tile, program and blend, and only the first
+// of them may clear); groups are ordered by (execution tile, program, blend).
 int batch_plan(sr_context ctx, const sr_batch_item* items, uint32_t n, BatchPlan& plan);
 
 // Sizes the workspace, uploads the tables and queues the whole batch on

@@ -25,7 +25,8 @@ ERRORS = {
 EXPORTED = [
     "sr_ctx_create", "sr_ctx_destroy", "sr_last_error", "sr_finish", "sr_ctx_stream",
     "sr_ctx_stage_times", "sr_ctx_last_stats", "sr_mesh_upload", "sr_mesh_free",
-    "sr_texture_upload", "sr_texture_free", "sr_texture_from_fb", "sr_fb_create", "sr_fb_free", "sr_fb_clear",
+    "sr_texture_upload", "sr_texture_free", "sr_texture_from_fb", "sr_obj_parse", "sr_mesh_load_obj",
+    "sr_fb_create", "sr_fb_free", "sr_fb_clear",
     "sr_fb_read", "sr_fb_write", "sr_fb_size", "sr_draw", "sr_batch_draw",
     "sr_ctx_set_exec_tile_policy", "sr_ctx_set_batch_capacity",
     "sr_server_create", "sr_server_destroy", "sr_server_submit", "sr_server_poll",
@@ -64,6 +65,31 @@ class DrawStats(C.Structure):
 class BatchItem(C.Structure):
     _fields_ = [("fb", C.c_void_p), ("mesh", C.c_void_p), ("desc", C.POINTER(DrawDesc)),
                 ("tex", C.c_void_p), ("clear", C.c_int32), ("clear_rgba", C.c_uint32)]
+
+
+class ObjInfo(C.Structure):
+    _fields_ = [("models", C.c_uint32), ("vertices", C.c_uint32), ("indices", C.c_uint32),
+                ("has_normals", C.c_int32), ("has_uvs", C.c_int32)]
+
+
+def parse_obj(path: str, model: int = 0) -> dict:
+    """sr_obj_parse: the host-side OBJ model as numpy arrays (no device work)."""
+    info = ObjInfo()
+    rc = lib().sr_obj_parse(os.fsencode(path), model, C.byref(info), None, None, None, None)
+    if rc != SR_OK:
+        raise SoftrenderError(rc, f"sr_obj_parse({path})")
+    nv, ni = info.vertices, info.indices
+    pos = np.empty(3 * nv, np.float32)
+    nrm = np.empty(3 * nv if info.has_normals else 0, np.float32)
+    uv = np.empty(2 * nv if info.has_uvs else 0, np.float32)
+    idx = np.empty(ni, np.uint32)
+    fp = lambda a: a.ctypes.data_as(C.POINTER(C.c_float)) if a.size else None
+    rc = lib().sr_obj_parse(os.fsencode(path), model, C.byref(info), fp(pos), fp(nrm), fp(uv),
+                            idx.ctypes.data_as(C.POINTER(C.c_uint32)) if ni else None)
+    if rc != SR_OK:
+        raise SoftrenderError(rc, f"sr_obj_parse({path})")
+    return dict(models=info.models, positions=pos.reshape(-1, 3), normals=nrm.reshape(-1, 3) if nrm.size else None,
+                uvs=uv.reshape(-1, 2) if uv.size else None, indices=idx)
 
 
 class ServerConfig(C.Structure):
@@ -154,6 +180,9 @@ def lib():
                                         C.POINTER(vp)]
         L.sr_texture_free.argtypes = [vp]
         L.sr_texture_from_fb.argtypes = [vp, C.POINTER(vp)]
+        L.sr_obj_parse.argtypes = [C.c_char_p, C.c_uint32, C.POINTER(ObjInfo), C.POINTER(C.c_float),
+                                   C.POINTER(C.c_float), C.POINTER(C.c_float), C.POINTER(C.c_uint32)]
+        L.sr_mesh_load_obj.argtypes = [vp, C.c_char_p, C.c_uint32, C.POINTER(vp)]
         L.sr_fb_create.argtypes = [vp, C.c_uint32, C.c_uint32, C.POINTER(vp)]
         L.sr_fb_free.argtypes = [vp]
         L.sr_fb_clear.argtypes = [vp, C.c_uint32, C.c_float, C.c_uint8]
@@ -237,6 +266,9 @@ class Context:
 
     def mesh(self, s: dict) -> "Mesh":
         return Mesh(self, s)
+
+    def mesh_from_obj(self, path: str, model: int = 0) -> "Mesh":
+        return Mesh(self, obj=path, model=model)
 
     def texture(self, w, h, rgba) -> "Texture":
         return Texture(self, w, h, rgba)
@@ -345,9 +377,14 @@ class Server:
 
 
 class Mesh:
-    def __init__(self, ctx: Context, s: dict):
+    def __init__(self, ctx: Context, s: dict = None, obj: str = None, model: int = 0):
+        """Uploads a mesh dict (px..v streams, indices, primitive) or, with
+        obj=path, model `model` of a Wavefront OBJ file."""
         self.ctx = ctx
         self.h = C.c_void_p()
+        if obj is not None:
+            ctx._check(lib().sr_mesh_load_obj(ctx.h, os.fsencode(obj), model, C.byref(self.h)))
+            return
         soa, keep = vertex_soa(s)
         n = len(keep["px"])
         idx = s.get("indices")
