@@ -67,6 +67,14 @@ extern "C" {
 #define SR_GS_NONE 0           /* run_to_fragment / finish without a program */
 #define SR_GS_CLIP_REFERENCE 1 /* clip_primitives, geometry.rs:261-336, bit-exact */
 #define SR_GS_CLIP_NEAR 2      /* Sutherland-Hodgman against the Near plane only */
+/* Normal visualisation (the full example's programs, full_example/src/
+ * shaders.rs:35-89): each triangle emits lines, drawn by the line raster;
+ * other primitives pass through unchanged (re_emit).  They read the world
+ * position and normal varyings of SR_VP_MVP_NORMAL_WORLD, gs_uniforms[0..16)
+ * = projection * view (row-major) and gs_uniforms[16] = line length. */
+#define SR_GS_VERTEX_NORMALS 3 /* per vertex: PV*(p,1) -> PV*((p,1) + (n,0)*len) */
+#define SR_GS_FACE_NORMALS 4   /* per triangle, from the centroid c = a/3 + b/3 + c/3:
+                                  PV*c.p -> PV*(c.p + normalize(c.n)*len) */
 
 /* Fragment programs (the `Fn(&ScreenVertex,&U) -> Fragment` closure,
  * fragment.rs:168-169). */
@@ -155,6 +163,8 @@ typedef struct sr_draw_desc {
     float vs_uniforms[48];
     /* fs_uniforms: program specific, documented with each program. */
     float fs_uniforms[32];
+    /* gs_uniforms: geometry program uniforms (normal visualisation). */
+    float gs_uniforms[20];
     int32_t exec_tile;   /* execution tile edge in pixels: 0 = auto, else 8/16/32/64 */
     uint32_t exec_flags; /* SR_EXEC_* */
     int32_t reserved[6];

@@ -88,6 +88,7 @@ def lib():
         L.or_hypotf.restype = C.c_float
         L.or_div_f32.argtypes = [C.c_float, C.c_float]
         L.or_div_f32.restype = C.c_float
+        L.or_gs_normal_lines.argtypes = [C.c_int] + [fp] * 6 + [C.c_int, fp, fp, fp, C.POINTER(C.c_int)]
         _lib = L
     return _lib
 
@@ -151,3 +152,24 @@ def render_scene(scene: dict, nthreads: int = 1):
             raise RuntimeError(f"oracle draw failed: {rc}")
         stats.append(st)
     return fb, stats
+
+
+def gs_normal_lines(program: int, clip, k, G):
+    """Normal visualisation of one triangle: clip (3, 4) and varyings (3, K)
+    in, the emitted line vertices (2n, 4) and their varyings (2n, K) out."""
+    L = lib()
+    clip = np.ascontiguousarray(clip, np.float32).reshape(3, 4)
+    k = np.ascontiguousarray(k, np.float32)
+    K = k.shape[1]
+    g = np.zeros(20, np.float32)
+    flat = np.asarray(G, np.float32).ravel()
+    g[:flat.size] = flat
+    G = g
+    out_c = np.zeros((6, 4), np.float32)
+    out_k = np.zeros((6, K), np.float32)
+    n = C.c_int(0)
+    fp = C.POINTER(C.c_float)
+    ptr = lambda a: a.ctypes.data_as(fp)  # noqa: E731
+    L.or_gs_normal_lines(program, ptr(clip[0]), ptr(k[0]), ptr(clip[1]), ptr(k[1]), ptr(clip[2]), ptr(k[2]),
+                         K, ptr(G), ptr(out_c), ptr(out_k), C.byref(n))
+    return out_c[:2 * n.value].copy(), out_k[:2 * n.value].copy()

@@ -940,6 +940,85 @@ void or_fb_clear(or_fb* fb, uint32_t rgba, float depth, uint8_t stencil) {
     }
 }
 
+/* ------------------------------------------------------------------------ */
+/* Normal visualisation programs (full_example/src/shaders.rs:35-89).  The    */
+/* emitted lines carry new clip positions and the varyings of their vertex   */
+/* (vertex normals) or of the centroid (face normals).  Vector4 operands:    */
+/* position (wx,wy,wz,1), normal (nx,ny,nz,0) from SR_VP_MVP_NORMAL_WORLD's   */
+/* varyings; G[0..16) = projection * view, G[16] = length.  Matrix times      */
+/* vector accumulates k = 0..3 from +0.0 like the vertex stage.              */
+/* ------------------------------------------------------------------------ */
+static void mat4_vec4(const float* M, const float v[4], float out[4]) {
+    for (int r = 0; r < 4; r++) {
+        float acc = 0.0f;
+        for (int k = 0; k < 4; k++) acc = acc + M[r * 4 + k] * v[k];
+        out[r] = acc;
+    }
+}
+
+/* shaders.rs:35-60: per vertex, start = mv * position,
+ * end = mv * (position + normal * NORMAL_LENGTH) */
+static void gs_vertex_normals(const ovtx* a, const ovtx* b, const ovtx* c, const float* G, vlist* lines) {
+    const ovtx* vs[3] = {a, b, c};
+    for (int i = 0; i < 3; i++) {
+        const ovtx* v = vs[i];
+        const float p[4] = {v->k[3], v->k[4], v->k[5], 1.0f};
+        const float n[4] = {v->k[0], v->k[1], v->k[2], 0.0f};
+        float e[4];
+        for (int j = 0; j < 4; j++) e[j] = p[j] + n[j] * G[16];
+        ovtx s = *v, t = *v;
+        mat4_vec4(G, p, s.p);
+        mat4_vec4(G, e, t.p);
+        vl_push(lines, &s);
+        vl_push(lines, &t);
+    }
+}
+
+/* shaders.rs:63-89: center = barycentric_interpolate(1/3, a, 1/3, b, 1/3, c)
+ * per component ((a*t) + (b*t)) + (c*t); start = mv * center.position,
+ * end = mv * (center.position + center.normal.normalize() * NORMAL_LENGTH),
+ * normalize = v / sqrt(((x*x + y*y) + z*z) + w*w) */
+static void gs_face_normals(const ovtx* a, const ovtx* b, const ovtx* c, int K, const float* G,
+                            vlist* lines) {
+    const float t = 1.0f / 3.0f;
+    ovtx m = *a;
+    for (int j = 0; j < K; j++) m.k[j] = ((a->k[j] * t) + (b->k[j] * t)) + (c->k[j] * t);
+    const float pw = ((1.0f * t) + (1.0f * t)) + (1.0f * t);
+    const float nw = ((0.0f * t) + (0.0f * t)) + (0.0f * t);
+    const float p[4] = {m.k[3], m.k[4], m.k[5], pw};
+    float n[4] = {m.k[0], m.k[1], m.k[2], nw};
+    const float len = sqrtf(((n[0] * n[0] + n[1] * n[1]) + n[2] * n[2]) + n[3] * n[3]);
+    float e[4];
+    for (int j = 0; j < 4; j++) {
+        n[j] = n[j] / len;
+        e[j] = p[j] + n[j] * G[16];
+    }
+    ovtx s = m, u = m;
+    mat4_vec4(G, p, s.p);
+    mat4_vec4(G, e, u.p);
+    vl_push(lines, &s);
+    vl_push(lines, &u);
+}
+
+void or_gs_normal_lines(int program, const float* a_clip, const float* a_k, const float* b_clip,
+                        const float* b_k, const float* c_clip, const float* c_k, int K, const float* G,
+                        float* out_clip, float* out_k, int* nlines) {
+    ovtx a, b, c;
+    memcpy(a.p, a_clip, sizeof(a.p)); memcpy(a.k, a_k, sizeof(float) * K);
+    memcpy(b.p, b_clip, sizeof(b.p)); memcpy(b.k, b_k, sizeof(float) * K);
+    memcpy(c.p, c_clip, sizeof(c.p)); memcpy(c.k, c_k, sizeof(float) * K);
+    vlist l;
+    memset(&l, 0, sizeof(l));
+    if (program == SR_GS_VERTEX_NORMALS) gs_vertex_normals(&a, &b, &c, G, &l);
+    else gs_face_normals(&a, &b, &c, K, G, &l);
+    for (size_t i = 0; i < l.n; i++) {
+        memcpy(out_clip + 4 * i, l.v[i].p, sizeof(float) * 4);
+        memcpy(out_k + (size_t)K * i, l.v[i].k, sizeof(float) * K);
+    }
+    *nlines = (int)(l.n / 2);
+    free(l.v);
+}
+
 static void normalize_list(vlist* l, const sr_viewport* vp) {
     for (size_t i = 0; i < l->n; i++) {
         float sp[4];
@@ -998,12 +1077,22 @@ int or_draw(or_fb* fb, const or_mesh* mesh, const sr_draw_desc* d, const or_text
          * sequential order: i = 0, n, 2n, ...  Then finish (geometry.rs:60-129). */
         const int K = dc.rs.K;
         for (size_t i = 0; i + (size_t)nv <= nidx; i += (size_t)nv) {
+            const int normals = d->geometry_program == SR_GS_VERTEX_NORMALS ||
+                                d->geometry_program == SR_GS_FACE_NORMALS;
             if (mesh->primitive == SR_PRIM_TRIANGLES) {
                 const ovtx* a = &clip[index_at(mesh, i)];
                 const ovtx* b = &clip[index_at(mesh, i + 1)];
                 const ovtx* c = &clip[index_at(mesh, i + 2)];
-                if (d->geometry_program == SR_GS_CLIP_NEAR) clip_tri_near(a, b, c, K, &dc.tris);
+                if (d->geometry_program == SR_GS_VERTEX_NORMALS) gs_vertex_normals(a, b, c, d->gs_uniforms, &dc.lines);
+                else if (d->geometry_program == SR_GS_FACE_NORMALS)
+                    gs_face_normals(a, b, c, K, d->gs_uniforms, &dc.lines);
+                else if (d->geometry_program == SR_GS_CLIP_NEAR) clip_tri_near(a, b, c, K, &dc.tris);
                 else clip_tri_reference(a, b, c, K, &dc.tris);
+            } else if (normals && mesh->primitive == SR_PRIM_LINES) {  /* re_emit, shaders.rs:59,88 */
+                vl_push(&dc.lines, &clip[index_at(mesh, i)]);
+                vl_push(&dc.lines, &clip[index_at(mesh, i + 1)]);
+            } else if (normals) {
+                vl_push(&dc.points, &clip[index_at(mesh, i)]);
             } else if (mesh->primitive == SR_PRIM_LINES) {
                 const ovtx* s = &clip[index_at(mesh, i)];
                 const ovtx* e = &clip[index_at(mesh, i + 1)];

@@ -77,6 +77,12 @@ int or_clip_triangle_reference(const float* a, const float* b, const float* c, i
                                float* out, int max_tris);
 int or_clip_triangle_near(const float* a, const float* b, const float* c, int K,
                           float* out, int max_tris);
+/* Normal visualisation on one triangle (SR_GS_VERTEX_NORMALS or
+ * SR_GS_FACE_NORMALS): clip positions x,y,z,w and K varyings per vertex in,
+ * 2 * nlines line vertices out (clip position and K varyings each). */
+void or_gs_normal_lines(int program, const float* a_clip, const float* a_k, const float* b_clip,
+                        const float* b_k, const float* c_clip, const float* c_k, int K, const float* G,
+                        float* out_clip, float* out_k, int* nlines);
 /* clip_primitives on one line; returns 1 if emitted (out = 2 vertices). */
 int or_clip_line_reference(const float* s, const float* e, int K, float* out);
 /* Tile grid; returns tile count, writes up to max tiles as x0,y0,x1,y1. */

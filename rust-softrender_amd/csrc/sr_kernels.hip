@@ -1222,6 +1222,72 @@ __global__ __launch_bounds__(256) void k_point_setup(PointSetupParams P) {
 }
 
 // ===========================================================================
+// Geometry stage: normal visualisation (full_example/src/shaders.rs:35-89)
+// ===========================================================================
+// The world position (k[3..6), w = 1) and normal (k[0..3), w = 0) varyings
+// of SR_VP_MVP_NORMAL_WORLD become lines in clip space, G[0..16) times the
+// Vector4 (k = 0..3 accumulated from +0.0, like the vertex stage), then
+// normalised to the screen like finish().  The line vertices carry the
+// varyings of their vertex (vertex normals) or of the centroid (face normals).
+__device__ __forceinline__ void pv_times(const float* G, const float v[4], float out[4]) {
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+        float acc = 0.0f;
+#pragma unroll
+        for (int k = 0; k < 4; k++) acc = acc + G[r * 4 + k] * v[k];
+        out[r] = acc;
+    }
+}
+
+template <int MODE>
+__global__ __launch_bounds__(256) void k_gs_normals(GSParams P) {
+    const uint32_t t = blockIdx.x * 256u + threadIdx.x;
+    if (t >= P.n_in) return;
+    CV v[3];
+    for (int j = 0; j < 3; j++) load_cv(P.in, prim_vertex(P.idx, 3 * t + j), 6, v[j]);
+    const float L = P.G[16];
+    if constexpr (MODE == SR_GS_VERTEX_NORMALS) {
+        // shaders.rs:46-56: start = mv * position, end = mv * (position + normal * L)
+        for (int j = 0; j < 3; j++) {
+            const float p[4] = {v[j].k[3], v[j].k[4], v[j].k[5], 1.0f};
+            const float n[4] = {v[j].k[0], v[j].k[1], v[j].k[2], 0.0f};
+            float e[4];
+#pragma unroll
+            for (int c = 0; c < 4; c++) e[c] = p[c] + n[c] * L;
+            CV a = v[j], b = v[j];
+            pv_times(P.G, p, a.p);
+            pv_times(P.G, e, b.p);
+            store_screen(P, 6 * t + 2 * j, a);
+            store_screen(P, 6 * t + 2 * j + 1, b);
+        }
+    } else {
+        // shaders.rs:72-85: center = barycentric_interpolate(1/3, a, 1/3, b,
+        // 1/3, c), componentwise ((a * t) + (b * t)) + (c * t); the normal is
+        // normalised over all four components
+        const float third = 1.0f / 3.0f;
+        CV m;
+#pragma unroll
+        for (int c = 0; c < 6; c++) m.k[c] = ((v[0].k[c] * third) + (v[1].k[c] * third)) + (v[2].k[c] * third);
+        const float pw = ((1.0f * third) + (1.0f * third)) + (1.0f * third);
+        const float nw = ((0.0f * third) + (0.0f * third)) + (0.0f * third);
+        const float p[4] = {m.k[3], m.k[4], m.k[5], pw};
+        float n[4] = {m.k[0], m.k[1], m.k[2], nw};
+        const float len = sqrtf(((n[0] * n[0] + n[1] * n[1]) + n[2] * n[2]) + n[3] * n[3]);
+        float e[4];
+#pragma unroll
+        for (int c = 0; c < 4; c++) {
+            n[c] = n[c] / len;
+            e[c] = p[c] + n[c] * L;
+        }
+        CV a = m, b = m;
+        pv_times(P.G, p, a.p);
+        pv_times(P.G, e, b.p);
+        store_screen(P, 2 * t, a);
+        store_screen(P, 2 * t + 1, b);
+    }
+}
+
+// ===========================================================================
 // Host-side launchers
 // ===========================================================================
 
@@ -1247,6 +1313,13 @@ void launch_gs_emit(int mode, int prim, const GSParams& P, hipStream_t s) {
     GSE(SR_GS_CLIP_REFERENCE, SR_PRIM_TRIANGLES) GSE(SR_GS_CLIP_REFERENCE, SR_PRIM_LINES)
     GSE(SR_GS_CLIP_REFERENCE, SR_PRIM_POINTS)
 #undef GSE
+}
+
+void launch_gs_normals(int mode, const GSParams& P, hipStream_t s) {
+    if (!P.n_in) return;
+    const dim3 g(blocks_for(P.n_in, 256)), b(256);
+    if (mode == SR_GS_VERTEX_NORMALS) hipLaunchKernelGGL((k_gs_normals<SR_GS_VERTEX_NORMALS>), g, b, 0, s, P);
+    else hipLaunchKernelGGL((k_gs_normals<SR_GS_FACE_NORMALS>), g, b, 0, s, P);
 }
 
 void launch_tri_setup(const SetupParams& P, hipStream_t s) {

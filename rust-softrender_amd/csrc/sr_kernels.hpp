@@ -25,6 +25,7 @@ struct GSParams {
     const uint32_t* offsets;
     VertexBuf out;       // generated vertices, screen space (finish fused)
     sr_viewport viewport;
+    float G[17];         // normal visualisation: projection * view (row-major), length
 };
 
 struct SetupParams {
@@ -134,6 +135,9 @@ struct StripParams {
 void launch_vertex(const VSParams& P, hipStream_t s);
 void launch_gs_count(int mode, int prim, const GSParams& P, hipStream_t s);
 void launch_gs_emit(int mode, int prim, const GSParams& P, hipStream_t s);
+// normal visualisation on a triangle list: lines (2 vertices each) into P.out,
+// 3 per triangle (SR_GS_VERTEX_NORMALS) or 1 (SR_GS_FACE_NORMALS)
+void launch_gs_normals(int mode, const GSParams& P, hipStream_t s);
 void launch_tri_setup(const SetupParams& P, hipStream_t s);
 void launch_tri_emit(const EmitParams& P, hipStream_t s);
 void launch_vertex_batch(const BatchTables& T, const VertexBuf& out, hipStream_t s);

@@ -261,8 +261,11 @@ int sr_rt::validate_draw(sr_context ctx, sr_framebuffer fb, sr_mesh mesh, const 
     if (fb->ctx != ctx || mesh->ctx != ctx) return fail(ctx, SR_ERR_INVALID_ARGUMENT, "draw: handle from another context");
     if (d->vertex_program != SR_VP_MVP_NORMAL_UV && d->vertex_program != SR_VP_MVP_NORMAL_WORLD)
         return fail(ctx, SR_ERR_INVALID_ARGUMENT, "draw: unknown vertex program");
-    if (d->geometry_program < SR_GS_NONE || d->geometry_program > SR_GS_CLIP_NEAR)
+    if (d->geometry_program < SR_GS_NONE || d->geometry_program > SR_GS_FACE_NORMALS)
         return fail(ctx, SR_ERR_INVALID_ARGUMENT, "draw: unknown geometry program");
+    if ((d->geometry_program == SR_GS_VERTEX_NORMALS || d->geometry_program == SR_GS_FACE_NORMALS) &&
+        d->vertex_program != SR_VP_MVP_NORMAL_WORLD)
+        return fail(ctx, SR_ERR_INVALID_ARGUMENT, "draw: normal visualisation needs world-space varyings");
     if (d->fragment_program < SR_FP_FLAT || d->fragment_program > SR_FP_UV_COLOR)
         return fail(ctx, SR_ERR_INVALID_ARGUMENT, "draw: unknown fragment program");
     if (d->fragment_program == SR_FP_TEX_LAMBERT && !tex)
@@ -998,7 +1001,11 @@ int sr_draw(sr_context ctx, sr_framebuffer fb, sr_mesh mesh, const sr_draw_desc*
     vs.nx = mesh->soa[3]; vs.ny = mesh->soa[4]; vs.nz = mesh->soa[5];
     vs.u = mesh->soa[6]; vs.v = mesh->soa[7];
     vs.n = nv; vs.program = d->vertex_program; vs.K = pl.K;
-    vs.normalize = d->geometry_program == SR_GS_NONE;
+    // normal visualisation re-emits lines and points unchanged (shaders.rs:59,88):
+    // for them it is run_to_fragment
+    const bool normals = d->geometry_program == SR_GS_VERTEX_NORMALS || d->geometry_program == SR_GS_FACE_NORMALS;
+    const int gs = normals && mesh->prim != SR_PRIM_TRIANGLES ? SR_GS_NONE : d->geometry_program;
+    vs.normalize = gs == SR_GS_NONE;
     vs.viewport = d->viewport;
     memcpy(vs.U, d->vs_uniforms, sizeof(vs.U));
     vs.out = vb;
@@ -1007,7 +1014,26 @@ int sr_draw(sr_context ctx, sr_framebuffer fb, sr_mesh mesh, const sr_draw_desc*
 
     const uint32_t* idx = mesh->idx;
     uint32_t nwork = nprim;
-    if (d->geometry_program != SR_GS_NONE) {
+    int out_prim = mesh->prim;
+    if (normals && gs != SR_GS_NONE) {
+        // ---- geometry stage: lines at fixed offsets, no count pass
+        const uint32_t lines_per = d->geometry_program == SR_GS_VERTEX_NORMALS ? 3u : 1u;
+        const uint64_t nl = (uint64_t)nprim * lines_per;
+        if (nl > 0x7fffffffull) return fail(ctx, SR_ERR_INVALID_ARGUMENT, "draw: too many normal lines");
+        if ((rc = ensure(ctx, ctx->gverts, sizeof(float) * (size_t)(4 + pl.K) * 2 * nl))) return rc;
+        GSParams gp;
+        memset(&gp, 0, sizeof(gp));
+        gp.in = vb; gp.idx = idx; gp.n_in = nprim; gp.K = pl.K;
+        gp.viewport = d->viewport;
+        memcpy(gp.G, d->gs_uniforms, sizeof(gp.G));
+        gp.out = VertexBuf{P<float>(ctx->gverts), (uint32_t)(2 * nl)};
+        launch_gs_normals(d->geometry_program, gp, s);
+        vb = gp.out;
+        idx = nullptr;
+        nwork = (uint32_t)nl;
+        out_prim = SR_PRIM_LINES;
+        ctx->last.generated_lines = nwork;
+    } else if (gs != SR_GS_NONE) {
         // ---- geometry stage: count, scan, emit (fused with finish)
         if ((rc = ensure(ctx, ctx->counts, sizeof(uint32_t) * (nprim + 1))) ||
             (rc = ensure(ctx, ctx->offsets, sizeof(uint32_t) * (nprim + 1))))
@@ -1036,9 +1062,9 @@ int sr_draw(sr_context ctx, sr_framebuffer fb, sr_mesh mesh, const sr_draw_desc*
 
     if (nwork == 0) {
         rc = SR_OK;
-    } else if (mesh->prim == SR_PRIM_TRIANGLES) {
+    } else if (out_prim == SR_PRIM_TRIANGLES) {
         rc = draw_tris(ctx, fb, pl, d, tex, vb, idx, nwork);
-    } else if (mesh->prim == SR_PRIM_LINES) {
+    } else if (out_prim == SR_PRIM_LINES) {
         rc = draw_lines_impl(ctx, fb, pl, d, tex, vb, idx, nwork);
     } else {
         rc = draw_points_impl(ctx, fb, pl, d, tex, vb, idx, nwork);

@@ -95,7 +95,15 @@ struct VertexProgram {
     }
 };
 
-enum class GeometryProgram { None = SR_GS_NONE, ClipReference = SR_GS_CLIP_REFERENCE, ClipNear = SR_GS_CLIP_NEAR };
+enum class GeometryProgram { None = SR_GS_NONE, ClipReference = SR_GS_CLIP_REFERENCE, ClipNear = SR_GS_CLIP_NEAR,
+                             VertexNormals = SR_GS_VERTEX_NORMALS, FaceNormals = SR_GS_FACE_NORMALS };
+
+// Uniforms of the normal visualisation programs (full_example/src/shaders.rs:
+// `projection * view` and NORMAL_LENGTH)
+struct NormalLines {
+    std::array<float, 16> projection_view{};  // row-major
+    float length = 0.05f;                     // shaders.rs:33
+};
 
 struct FragmentProgram {
     int32_t id = SR_FP_FLAT;
@@ -242,7 +250,14 @@ class GeometryShader {
         if (gs != GeometryProgram::None) desc_.geometry_program = (int32_t)gs;
         return std::move(*this);
     }
+    GeometryShader run(GeometryProgram gs, const NormalLines& u) && {
+        std::memcpy(desc_.gs_uniforms, u.projection_view.data(), 16 * sizeof(float));
+        desc_.gs_uniforms[16] = u.length;
+        return std::move(*this).run(gs);
+    }
     GeometryShader clip_primitives() && { return std::move(*this).run(GeometryProgram::ClipReference); }
+    // duplicate (geometry.rs:43-58): a second chain over the same vertices
+    GeometryShader duplicate() const& { return GeometryShader(pipe_, mesh_, desc_); }
     FragmentShader finish(const Viewport& vp) && {
         desc_.viewport = sr_viewport{vp.x, vp.y, vp.width, vp.height, vp.near_z, vp.far_z};
         return FragmentShader(pipe_, std::move(mesh_), desc_);

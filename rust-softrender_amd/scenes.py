@@ -225,6 +225,62 @@ def sphere_scene(width: int = 512, height: int = 512):
 
 
 # ---------------------------------------------------------------------------
+# The full example's frame (full_example/src/main.rs:66-92): every mesh is
+# drawn lit, then its normals are drawn as green lines over it by a geometry
+# program fed by the same vertex program.
+# ---------------------------------------------------------------------------
+GS_VERTEX_NORMALS, GS_FACE_NORMALS = 3, 4
+NORMAL_LENGTH = 0.05  # full_example/src/shaders.rs:33
+
+
+def rotation_y(deg: float) -> np.ndarray:
+    c, s = np.cos(np.radians(deg)), np.sin(np.radians(deg))
+    return np.array([[c, 0, s, 0], [0, 1, 0, 0], [-s, 0, c, 0], [0, 0, 0, 1]], np.float64)
+
+
+def normals_scene(pos, nrm, idx, width: int = 320, height: int = 320,
+                  program: int = GS_FACE_NORMALS, aa: int = 0,
+                  length: float = NORMAL_LENGTH, yaw: float = 30.0):
+    """Lit mesh plus its vertex or face normals, as the full example draws a
+    model; `pos`/`nrm` are (n, 3) arrays and `idx` a triangle list."""
+    model = rotation_y(yaw)
+    model[:3, 3] = (0.0, 0.1, 0.0)
+    view = look_at((0, 0, 3), (0, 0, 0), (0, 1, 0))
+    proj = perspective_zero_to_one(60.0, width / height, 0.1, 100.0)
+    pv = proj @ view
+    mvp = pv @ model
+    mit = np.linalg.inv(model[:3, :3]).T
+    pos = np.asarray(pos, np.float32)
+    nrm = np.asarray(nrm, np.float32)
+    idx = np.asarray(idx, np.uint32)
+    tri = pos[idx].reshape(-1, 3, 3)
+    centre = int(np.argmax(tri[:, :, 2].mean(1)))
+    cw_front = screen_winding(mvp, width, height, tri[centre])
+    fs = np.zeros(32, np.float32)
+    fs[0:3] = _unit((1, 1, 1))
+    fs[3:6] = (0, 0, 3)
+    fs[6:9] = (0.25, 0.25, 0.25)
+    fs[9], fs[10], fs[11], fs[12] = 0.1, 0.8, 0.5, 1.0
+    green = np.zeros(32, np.float32)
+    green[0:4] = (0.0, 1.0, 0.0, 1.0)  # fragment_shader_green, shaders.rs:97-99
+    vsu = vs_uniforms(mvp, mit, model[:3])
+    vp = (0.0, 0.0, float(width), float(height), 0.0, 1.0)
+    mesh = dict(primitive=3, px=pos[:, 0].copy(), py=pos[:, 1].copy(), pz=pos[:, 2].copy(),
+                nx=nrm[:, 0].copy(), ny=nrm[:, 1].copy(), nz=nrm[:, 2].copy(), u=None, v=None, indices=idx)
+    # the example's Color is f32 RGBA (full_example/src/color.rs): f32 alpha
+    lit = dict(mesh, desc=dict(vertex_program=1, fragment_program=1, blend=1, alpha_mode=1,
+                               cull=2 if cw_front else 1, vs_uniforms=vsu, fs_uniforms=fs, viewport=vp))
+    gsu = np.zeros(20, np.float32)
+    gsu[:16] = pv.reshape(16)
+    gsu[16] = length
+    lines = dict(mesh, desc=dict(vertex_program=1, geometry_program=program, fragment_program=0,
+                                 blend=1, aa_lines=aa, alpha_mode=1, vs_uniforms=vsu, fs_uniforms=green,
+                                 gs_uniforms=gsu, viewport=vp))
+    return dict(name=f"normals_{program}_{width}x{height}", width=width, height=height,
+                clear=0xFF000000, texture=None, draws=[lit, lines])
+
+
+# ---------------------------------------------------------------------------
 # Clip-space triangle soup (the 1080p / 1M-triangle frame)
 # ---------------------------------------------------------------------------
 def _unit3(x, y, z):

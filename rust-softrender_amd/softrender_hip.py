@@ -52,6 +52,7 @@ class DrawDesc(C.Structure):
         ("stencil_ref", C.c_uint32), ("alpha_mode", C.c_int32), ("tex_filter", C.c_int32),
         ("tex_edge", C.c_int32), ("tex_border", C.c_uint32), ("viewport", Viewport),
         ("vs_uniforms", C.c_float * 48), ("fs_uniforms", C.c_float * 32),
+        ("gs_uniforms", C.c_float * 20),
         ("exec_tile", C.c_int32), ("exec_flags", C.c_uint32), ("reserved", C.c_int32 * 6),
     ]
 
@@ -134,6 +135,12 @@ def make_desc(**kw) -> DrawDesc:
     fsu = kw.pop("fs_uniforms", None)
     if fsu is not None:
         d.fs_uniforms[:] = [float(v) for v in np.asarray(fsu, np.float32).reshape(32)]
+    gsu = kw.pop("gs_uniforms", None)
+    if gsu is not None:
+        g = np.zeros(20, np.float32)
+        flat = np.asarray(gsu, np.float32).ravel()
+        g[:flat.size] = flat
+        d.gs_uniforms[:] = [float(v) for v in g]
     for k, v in kw.items():
         if not hasattr(d, k):
             raise KeyError(k)
