@@ -375,7 +375,7 @@ __device__ __forceinline__ void raster_tile(const RasterParams& P, uint32_t tile
                 RecBlock blk = load_block(P.tris, id);
                 uint32_t idn = load_ids(P.items, b0 + 64, sb, se, lane);
                 for (uint32_t c0 = b0; c0 < se; c0 += 64) {
-                    const RecBlock nxt = load_block(P.tris, idn);
+                    RecBlock nxt = load_block(P.tris, idn);
                     const uint32_t idc = idn;
                     idn = load_ids(P.items, c0 + 128, sb, se, lane);
                     const uint32_t lo = max(c0, sb), hi = min(c0 + 64, se);
@@ -394,36 +394,48 @@ __device__ __forceinline__ void raster_tile(const RasterParams& P, uint32_t tile
                     bool full = false;
 #pragma unroll
                     for (int j = 0; j < PPT; j++) {
-                        if (!(inT[j] && px[j] >= cx0 && px[j] <= cx1 && py[j] >= cy0 &&
-                              py[j] <= cy1))
-                            continue;
+                        // straight-line per pixel: every test is a mask, so
+                        // the wave runs one exec-mask region per pixel
+                        // instead of a nested branch per test
+                        bool ok = inT[j] & (px[j] >= cx0) & (px[j] <= cx1) & (py[j] >= cy0) & (py[j] <= cy1);
                         if constexpr (SMODE == 2) {  // stencil before coverage, triangle.rs:90-99
-                            if (!stencil_test(P.s_test, st[j], P.s_ref)) continue;
-                            st[j] = stencil_op(P.s_op, P.s_wrap, st[j], P.s_ref);
+                            const bool sp = ok && stencil_test(P.s_test, st[j], P.s_ref);
+                            st[j] = sp ? stencil_op(P.s_op, P.s_wrap, st[j], P.s_ref) : st[j];
+                            ok = sp;
                         } else if constexpr (SMODE == 1) {
-                            if (px[j] >= X0 && px[j] <= X1 && py[j] >= Y0 && py[j] <= Y1) cnt[j]++;
+                            cnt[j] += (ok & (px[j] >= X0) & (px[j] <= X1) & (py[j] >= Y0) & (py[j] <= Y1)) ? 1u : 0u;
                         }
                         const float dx = fx[j] - rc.x3, dy = fy[j] - rc.y3;
                         const float un = (rc.e23y * dx) + (rc.e23x * dy);  // triangle.rs:108
                         const float vn = (rc.e31y * dx) + (rc.e31x * dy);  // triangle.rs:109
                         const float u = quot(un, rc.R), v = quot(vn, rc.R);
                         const float w = (1.0f - u) - v;                    // triangle.rs:110
-                        if (u < 0.0f || v < 0.0f || w < 0.0f) continue;   // :113
                         const float z = ((rc.z1 * u) + (rc.z2 * v)) + (rc.z3 * w);  // :115-117
-                        if (!(z < 0.0f && z >= dep[j])) continue;                   // :120-126
-                        dep[j] = z;
-                        const int slot = (lane * PPT + j) * Q;
-                        if constexpr (BLEND == SR_BLEND_OVER) {
-                            wq[slot + qn[j]] = QRec{cur, u, v};
-                            qn[j]++;
-                            full |= qn[j] == (uint32_t)Q;
-                        } else {
-                            wq[slot] = QRec{cur, u, v};  // `()` blend: the last fragment wins
-                            qn[j] = 1;
+                        // :113 inside, :120-126 depth
+                        const bool pass = ok & !(u < 0.0f) & !(v < 0.0f) & !(w < 0.0f) & (z < 0.0f) & (z >= dep[j]);
+                        dep[j] = pass ? z : dep[j];
+                        if (pass) {
+                            const int slot = (lane * PPT + j) * Q;
+                            if constexpr (BLEND == SR_BLEND_OVER) {
+                                wq[slot + qn[j]] = QRec{cur, u, v};
+                                qn[j]++;
+                                full |= qn[j] == (uint32_t)Q;
+                            } else {
+                                wq[slot] = QRec{cur, u, v};  // `()` blend: the last fragment wins
+                                qn[j] = 1;
+                            }
                         }
                     }
                     if constexpr (BLEND == SR_BLEND_OVER) {
-                        if (__ballot(full)) flush(PD);
+                        if (__ballot(full)) {
+                            // the record blocks are gathered again after the
+                            // flush rather than kept live across the shading
+                            // code: that keeps the kernel ~26 VGPRs smaller
+                            // (mid-walk flushes are rare)
+                            flush(PD);
+                            blk = load_block(P.tris, id);
+                            nxt = load_block(P.tris, idc);
+                        }
                     }
                     }
                     blk = nxt;

@@ -290,8 +290,10 @@ int sr_rt::validate_draw(sr_context ctx, sr_framebuffer fb, sr_mesh mesh, const 
 // one pass of the pipeline.  Every stage runs once over the whole batch:
 // vertices and triangles are numbered across scenes, execution tiles get
 // global ids so one stable sort bins all scenes, and the raster is one
-// persistent launch per (tile size, program, blend) class pulling tiles
-// from per-XCD queues.  Sizes that depend on the geometry (bin pairs) use a
+// persistent launch per (tile size, program, blend) class whose
+// work-groups each take a run of tiles of near-equal cost (cost = a fixed
+// term + the tile's binned items, cut on the device from a prefix sum).
+// Sizes that depend on the geometry (bin pairs) use a
 // capacity checked on the device: on overflow nothing is drawn, the
 // capacity grows and the batch runs again, so the sequence has no host
 // round trip; it can be captured once into a hipGraph and replayed while
