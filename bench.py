@@ -65,6 +65,9 @@ def parse():
     ap.add_argument("--no-extras", action="store_true",
                     help="skip the batched (256 scenes) and serving (Poisson) measurements")
     ap.add_argument("--serve-seconds", type=float, default=2.0, help="seconds per offered rate")
+    ap.add_argument("--only", choices=["clip", "batch"], default="",
+                    help="run one side workload alone and print its JSON (for profiling): the 4K clip "
+                         "frame or the 256-scene batch")
     ap.add_argument("--copy-calib", type=int, default=0, metavar="MIB",
                     help="only run device-to-device copies of MIB MiB (calibrates PMC byte counters)")
     return ap.parse_args()
@@ -113,6 +116,13 @@ def main():
         sys.exit(2)
     if a.copy_calib:
         copy_calibration(a.copy_calib)
+        return
+    if a.only == "clip":
+        print(json.dumps(clip_frame(a, steps=a.steps)), flush=True)
+        return
+    if a.only == "batch":
+        import batchbench
+        print(json.dumps(batchbench.run(256, reps=a.steps)), flush=True)
         return
     import numpy as np
 

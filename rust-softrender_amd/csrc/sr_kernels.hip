@@ -831,66 +831,88 @@ __device__ __forceinline__ void strip_status(int lo, int hi, uint32_t t, uint32_
     first_after = hi < 0 ? 0u : min((uint32_t)hi / t + 1u, n);
 }
 
+// Corner counters live in LDS per work-group when the four arrays fit
+// (STRIP_LDS_CORNERS entries each): most triangles add to the same few
+// corner cells (e.g. the grid origin), which as global atomics serialise.
+constexpr uint32_t STRIP_LDS_CORNERS = 2048;
+
 __global__ __launch_bounds__(256) void k_strips_accum(StripParams P) {
-    const uint32_t t = blockIdx.x * 256u + threadIdx.x;
-    if (t >= P.n) return;
-    const TriRec& r = P.recs[t];
-    const int X0 = r.X0, X1 = r.X1, Y0 = r.Y0, Y1 = r.Y1;
-    if (X0 > X1) return;  // culled
-    uint32_t nR, jL, nB, iA;  // columns right-of-box-status prefix etc.
-    strip_status(X0, X1, P.tw, P.scols, P.W, nR, jL);
-    strip_status(Y0, Y1, P.th, P.srows, P.H, nB, iA);
-    if (jL < nR) jL = nR;
-    if (iA < nB) iA = nB;
+    extern __shared__ uint32_t sc[];  // 4 x NC corner counters (lds mode)
     const uint32_t HS = P.H + 1, WS = P.W + 1, CS = P.scols + 1;
-    const uint32_t xmax = P.W - 1, ymax = P.H - 1;
-    // rows over the box: (L,O) and (R,O) strips on tile columns
-    for (uint32_t i = nB; i < iA; i++) {
-        const int ty0 = (int)(i * P.th), ty1 = (int)min(i * P.th + P.th, ymax);
-        const int ylo = max(Y0, ty0), yhi = min(Y1, ty1);
-        if (ylo > yhi) continue;
-        if (jL < P.scols) {  // columns j >= jL see the box on their left
-            atomicAdd(&P.colL[jL * HS + ylo], 1u);
-            atomicAdd(&P.colL[jL * HS + yhi + 1], 0xFFFFFFFFu);
+    const uint32_t NC = (P.srows + 1) * CS;
+    const bool lds = NC <= STRIP_LDS_CORNERS;
+    uint32_t* const corner[4] = {P.cLL, P.cLR, P.cRL, P.cRR};
+    if (lds) {
+        for (uint32_t i = threadIdx.x; i < 4 * NC; i += 256) sc[i] = 0;
+        __syncthreads();
+    }
+    auto cadd = [&](int a, uint32_t idx, uint32_t v) {
+        if (lds) atomicAdd(&sc[a * NC + idx], v);
+        else atomicAdd(&corner[a][idx], v);
+    };
+    const uint32_t t = blockIdx.x * 256u + threadIdx.x;
+    const TriRec* r = t < P.n ? &P.recs[t] : nullptr;
+    const int X0 = r ? r->X0 : 1, X1 = r ? r->X1 : 0, Y0 = r ? r->Y0 : 1, Y1 = r ? r->Y1 : 0;
+    if (X0 <= X1) {  // not culled
+        uint32_t nR, jL, nB, iA;  // columns right-of-box-status prefix etc.
+        strip_status(X0, X1, P.tw, P.scols, P.W, nR, jL);
+        strip_status(Y0, Y1, P.th, P.srows, P.H, nB, iA);
+        if (jL < nR) jL = nR;
+        if (iA < nB) iA = nB;
+        const uint32_t xmax = P.W - 1, ymax = P.H - 1;
+        // rows over the box: (L,O) and (R,O) strips on tile columns
+        for (uint32_t i = nB; i < iA; i++) {
+            const int ty0 = (int)(i * P.th), ty1 = (int)min(i * P.th + P.th, ymax);
+            const int ylo = max(Y0, ty0), yhi = min(Y1, ty1);
+            if (ylo > yhi) continue;
+            if (jL < P.scols) {  // columns j >= jL see the box on their left
+                atomicAdd(&P.colL[jL * HS + ylo], 1u);
+                atomicAdd(&P.colL[jL * HS + yhi + 1], 0xFFFFFFFFu);
+            }
+            if (nR > 0) {  // columns j < nR see it on their right
+                atomicAdd(&P.colR[0 * HS + ylo], 1u);
+                atomicAdd(&P.colR[0 * HS + yhi + 1], 0xFFFFFFFFu);
+                atomicAdd(&P.colR[nR * HS + ylo], 0xFFFFFFFFu);
+                atomicAdd(&P.colR[nR * HS + yhi + 1], 1u);
+            }
         }
-        if (nR > 0) {  // columns j < nR see it on their right
-            atomicAdd(&P.colR[0 * HS + ylo], 1u);
-            atomicAdd(&P.colR[0 * HS + yhi + 1], 0xFFFFFFFFu);
-            atomicAdd(&P.colR[nR * HS + ylo], 0xFFFFFFFFu);
-            atomicAdd(&P.colR[nR * HS + yhi + 1], 1u);
+        // columns over the box: (O,A) and (O,B) strips on tile rows
+        for (uint32_t j = nR; j < jL; j++) {
+            const int tx0 = (int)(j * P.tw), tx1 = (int)min(j * P.tw + P.tw, xmax);
+            const int xlo = max(X0, tx0), xhi = min(X1, tx1);
+            if (xlo > xhi) continue;
+            if (iA < P.srows) {
+                atomicAdd(&P.rowT[iA * WS + xlo], 1u);
+                atomicAdd(&P.rowT[iA * WS + xhi + 1], 0xFFFFFFFFu);
+            }
+            if (nB > 0) {
+                atomicAdd(&P.rowB[0 * WS + xlo], 1u);
+                atomicAdd(&P.rowB[0 * WS + xhi + 1], 0xFFFFFFFFu);
+                atomicAdd(&P.rowB[nB * WS + xlo], 0xFFFFFFFFu);
+                atomicAdd(&P.rowB[nB * WS + xhi + 1], 1u);
+            }
+        }
+        // corners: 2D difference arrays over the tile grid [row][col]
+        if (jL < P.scols && iA < P.srows) cadd(0, iA * CS + jL, 1u);  // left & above
+        if (jL < P.scols && nB > 0) {                                  // left & below
+            cadd(1, 0 * CS + jL, 1u);
+            cadd(1, nB * CS + jL, 0xFFFFFFFFu);
+        }
+        if (nR > 0 && iA < P.srows) {  // right & above
+            cadd(2, iA * CS + 0, 1u);
+            cadd(2, iA * CS + nR, 0xFFFFFFFFu);
+        }
+        if (nR > 0 && nB > 0) {  // right & below
+            cadd(3, 0, 1u);
+            cadd(3, nB * CS, 0xFFFFFFFFu);
+            cadd(3, nR, 0xFFFFFFFFu);
+            cadd(3, nB * CS + nR, 1u);
         }
     }
-    // columns over the box: (O,A) and (O,B) strips on tile rows
-    for (uint32_t j = nR; j < jL; j++) {
-        const int tx0 = (int)(j * P.tw), tx1 = (int)min(j * P.tw + P.tw, xmax);
-        const int xlo = max(X0, tx0), xhi = min(X1, tx1);
-        if (xlo > xhi) continue;
-        if (iA < P.srows) {
-            atomicAdd(&P.rowT[iA * WS + xlo], 1u);
-            atomicAdd(&P.rowT[iA * WS + xhi + 1], 0xFFFFFFFFu);
-        }
-        if (nB > 0) {
-            atomicAdd(&P.rowB[0 * WS + xlo], 1u);
-            atomicAdd(&P.rowB[0 * WS + xhi + 1], 0xFFFFFFFFu);
-            atomicAdd(&P.rowB[nB * WS + xlo], 0xFFFFFFFFu);
-            atomicAdd(&P.rowB[nB * WS + xhi + 1], 1u);
-        }
-    }
-    // corners: 2D difference arrays over the tile grid [row][col]
-    if (jL < P.scols && iA < P.srows) atomicAdd(&P.cLL[iA * CS + jL], 1u);      // left & above
-    if (jL < P.scols && nB > 0) {                                               // left & below
-        atomicAdd(&P.cLR[0 * CS + jL], 1u);
-        atomicAdd(&P.cLR[nB * CS + jL], 0xFFFFFFFFu);
-    }
-    if (nR > 0 && iA < P.srows) {                                               // right & above
-        atomicAdd(&P.cRL[iA * CS + 0], 1u);
-        atomicAdd(&P.cRL[iA * CS + nR], 0xFFFFFFFFu);
-    }
-    if (nR > 0 && nB > 0) {                                                     // right & below
-        atomicAdd(&P.cRR[0], 1u);
-        atomicAdd(&P.cRR[nB * CS], 0xFFFFFFFFu);
-        atomicAdd(&P.cRR[nR], 0xFFFFFFFFu);
-        atomicAdd(&P.cRR[nB * CS + nR], 1u);
+    if (lds) {  // one global add per non-zero cell and work-group
+        __syncthreads();
+        for (uint32_t i = threadIdx.x; i < 4 * NC; i += 256)
+            if (sc[i]) atomicAdd(&corner[i / NC][i % NC], sc[i]);
     }
 }
 
@@ -1423,7 +1445,8 @@ void launch_raster_tris(const RasterBatch& P, uint32_t ntiles, hipStream_t s) {
 
 void launch_strips(const StripParams& P, hipStream_t s) {
     const uint32_t HS = P.H + 1, WS = P.W + 1, CS = P.scols + 1, RS = P.srows + 1;
-    if (P.n) hipLaunchKernelGGL(k_strips_accum, dim3(blocks_for(P.n, 256)), dim3(256), 0, s, P);
+    const size_t corner_lds = RS * CS <= STRIP_LDS_CORNERS ? 4u * RS * CS * sizeof(uint32_t) : 0u;
+    if (P.n) hipLaunchKernelGGL(k_strips_accum, dim3(blocks_for(P.n, 256)), dim3(256), corner_lds, s, P);
     // colL/colR: prefix along pixel rows (per tile column), then across tile columns
     hipLaunchKernelGGL(k_prefix_rows, dim3(CS), dim3(256), 0, s, P.colL, CS, HS, HS, 1u);
     hipLaunchKernelGGL(k_prefix_rows, dim3(HS), dim3(256), 0, s, P.colL, HS, CS, 1u, HS);

@@ -78,11 +78,43 @@ __device__ __forceinline__ void ordered_rounds(bool pending, int lp, uint32_t ra
     }
 }
 
+// Exclusive prefix sum over the work-group's 256 threads (in thread order).
+__device__ __forceinline__ uint32_t lp_scan(uint32_t v, uint32_t& total, uint32_t* lds4) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    uint32_t x = v;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t y = __shfl_up(x, o, 64);
+        if (lane >= o) x += y;
+    }
+    if (lane == 63) lds4[wave] = x;
+    __syncthreads();
+    uint32_t off = 0;
+    total = 0;
+#pragma unroll
+    for (int w = 0; w < 4; w++) {
+        const uint32_t q = lds4[w];
+        if (w < wave) off += q;
+        total += q;
+    }
+    __syncthreads();
+    return x - v + off;
+}
+
+// Lines: per screen tile (ascending), the tile's line items are staged in
+// LDS (256 at a time, list = line order) with what their plots need, and
+// the concatenated plot sequence of the staged items is processed 256 plots
+// per pass: plot g belongs to the first staged item whose plot-count prefix
+// exceeds g.  Plots of one pass run in parallel; same-pixel plots are
+// ordered by their position in the sequence (ordered_rounds), which is the
+// reference's line-then-plot order.
 template <int FP, int BLEND>
 __global__ __launch_bounds__(256) void k_raster_lines(RasterParams P) {
     constexpr int KV = fp_varyings<FP>();
     __shared__ uint32_t s_col[256], s_st[256], claim[256], lds4[4];
     __shared__ float s_dep[256], slut[256];
+    __shared__ float i_cx1[256], i_cy1[256], i_dlen[256], i_zs[256], i_ze[256];
+    __shared__ uint32_t i_v0[256], i_v1[256], i_off[256], i_end[256];  // i_end: inclusive plot prefix
     const int t = threadIdx.x;
     const uint32_t tile = xcd_remap(blockIdx.x, gridDim.x);
     const uint32_t etx = tile % P.ntx, ety = tile / P.ntx;
@@ -109,47 +141,79 @@ __global__ __launch_bounds__(256) void k_raster_lines(RasterParams P) {
         }
         best = block_min_256(best, lds4);
         if (best == 0xFFFFFFFFu) break;
-        for (uint32_t i = beg; i < end; i++) {
-            const LineItem it = P.line_items[P.items[i]];
-            if (it.tile != best) continue;
-            const uint32_t v0 = P.line_v ? P.line_v[2 * it.line] : 2 * it.line;
-            const uint32_t v1 = P.line_v ? P.line_v[2 * it.line + 1] : 2 * it.line + 1;
-            const float zs = P.verts.comp(2)[v0], ze = P.verts.comp(2)[v1];
-            for (uint32_t b = 0; b < it.plot_cnt; b += 256) {
-                const uint32_t k = b + t;
+        for (uint32_t ib = beg; ib < end; ib += 256) {
+            // stage this screen tile's items among the next 256 of the list
+            const uint32_t i = ib + (uint32_t)t;
+            LineItem it{};
+            bool mine = false;
+            if (i < end) {
+                it = P.line_items[P.items[i]];
+                mine = it.tile == best && it.plot_cnt > 0;
+            }
+            uint32_t nstaged;
+            const uint32_t slot = lp_scan(mine ? 1u : 0u, nstaged, lds4);
+            if (nstaged == 0) continue;  // uniform
+            uint32_t total;
+            const uint32_t pstart = lp_scan(mine ? it.plot_cnt : 0u, total, lds4);
+            if (mine) {
+                const uint32_t v0 = P.line_v ? P.line_v[2 * it.line] : 2 * it.line;
+                const uint32_t v1 = P.line_v ? P.line_v[2 * it.line + 1] : 2 * it.line + 1;
+                i_cx1[slot] = it.cx1;
+                i_cy1[slot] = it.cy1;
+                i_dlen[slot] = it.dlen;
+                i_zs[slot] = P.verts.comp(2)[v0];
+                i_ze[slot] = P.verts.comp(2)[v1];
+                i_v0[slot] = v0;
+                i_v1[slot] = v1;
+                i_off[slot] = it.plot_off;
+                i_end[slot] = pstart + it.plot_cnt;
+            }
+            __syncthreads();
+            for (uint32_t c = 0; c < total; c += 256) {
+                const uint32_t g = c + (uint32_t)t;
                 bool pending = false;
                 int lp = 0;
+                uint32_t si = 0;
                 Plot pl{};
-                if (k < it.plot_cnt) {
-                    pl = P.plots[it.plot_off + k];
+                if (g < total) {
+                    uint32_t lo = 0, hi = nstaged - 1;  // first staged item with i_end > g
+                    while (lo < hi) {
+                        const uint32_t m = (lo + hi) >> 1;
+                        if (i_end[m] > g) hi = m;
+                        else lo = m + 1;
+                    }
+                    si = lo;
+                    const uint32_t first_g = si ? i_end[si - 1] : 0u;
+                    pl = P.plots[i_off[si] + (g - first_g)];
                     const int px = (int)(pl.index % P.W) - ex0, py = (int)(pl.index / P.W) - ey0;
                     if (px >= 0 && px < LP_ETS && py >= 0 && py < LP_ETS) {
                         pending = true;
                         lp = py * LP_ETS + px;
                     }
                 }
-                ordered_rounds(pending, lp, k, claim, [&]() {
+                ordered_rounds(pending, lp, (uint32_t)t, claim, [&]() {
                     // the rasterize_fragment closure, line.rs:54-109
                     const uint32_t sv = s_st[lp];
                     if (!stencil_test(P.s_test, sv, P.s_ref)) return;
                     s_st[lp] = stencil_op(P.s_op, P.s_wrap, sv, P.s_ref);
                     const float xf = (float)pl.x + 0.5f, yf = (float)pl.y + 0.5f;
-                    const float tt = lp_hypot(it.cx1 - xf, it.cy1 - yf) / it.dlen;  // :75
+                    const float tt = lp_hypot(i_cx1[si] - xf, i_cy1[si] - yf) / i_dlen[si];  // :75
                     const float omt = 1.0f - tt;
-                    const float z = (omt * zs) + (tt * ze);                         // :77
+                    const float z = (omt * i_zs[si]) + (tt * i_ze[si]);                     // :77
                     if (!(z < 0.0f && z >= s_dep[lp])) return;
                     float kv[KV > 0 ? KV : 1];
 #pragma unroll
-                    for (int c = 0; c < KV; c++) {
-                        const float* vc = P.verts.comp(4 + c);
-                        kv[c] = (omt * vc[v0]) + (tt * vc[v1]);
+                    for (int q = 0; q < KV; q++) {
+                        const float* vc = P.verts.comp(4 + q);
+                        kv[q] = (omt * vc[i_v0[si]]) + (tt * vc[i_v1[si]]);
                     }
-                    uint32_t c = fragment<FP>(P, kv, lut);
-                    c = mul_alpha(P.alpha_mode, c, pl.alpha);                     // :100
-                    s_col[lp] = blend<BLEND>(c, s_col[lp], lut);
+                    uint32_t col = fragment<FP>(P, kv, lut);
+                    col = mul_alpha(P.alpha_mode, col, pl.alpha);                         // :100
+                    s_col[lp] = blend<BLEND>(col, s_col[lp], lut);
                     s_dep[lp] = z;
                 });
             }
+            __syncthreads();  // the staging arrays are rewritten next
         }
         prev = best;
         first = false;
