@@ -39,7 +39,7 @@ static_assert(sizeof(TriRec) == 80, "TriRec layout");
 
 // Per (line, screen tile) work item: the segment clipped to one screen
 // tile by liang_barsky_iterative (geometry/line.rs:6-54), as rasterize_line
-// receives it (line.rs:51-52).  64 bytes.
+// receives it (line.rs:51-52).  80 bytes.
 struct alignas(16) LineItem {
     uint32_t line;        // primitive index
     uint32_t tile;        // screen tile index (row-major, the reference's tile order)
@@ -49,9 +49,13 @@ struct alignas(16) LineItem {
     float cx2, cy2;       // clipped end
     float dlen;           // hypot of the clipped segment, line.rs:52
     int32_t bx0, by0, bx1, by1;  // execution tiles touched by its plots (bx0 > bx1: none)
+    // the execution tiles that hold a plot, as bits (row-major, mcols per
+    // row) of a window starting at tile (mbx, mby); mvalid = 0: use the box
+    uint16_t mbx, mby, mcols, mvalid;
     uint32_t pad;
+    uint64_t mlo, mhi;
 };
-static_assert(sizeof(LineItem) == 64, "LineItem layout");
+static_assert(sizeof(LineItem) == 80, "LineItem layout");
 
 // One plot of a line item: framebuffer index (x + y*W, line.rs:56-58), the
 // plot coordinates (for the pixel centre in line.rs:72-73) and the

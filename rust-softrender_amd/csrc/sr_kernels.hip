@@ -1066,7 +1066,9 @@ __global__ __launch_bounds__(256) void k_line_tiles(LineSetupParams P, int pass)
                     it.cx1 = c[0]; it.cy1 = c[1]; it.cx2 = c[2]; it.cy2 = c[3];
                     it.dlen = hypot_libm(c[0] - c[2], c[1] - c[3]);  // line.rs:52
                     it.bx0 = 1; it.bx1 = 0; it.by0 = 1; it.by1 = 0;
+                    it.mbx = it.mby = it.mcols = it.mvalid = 0;
                     it.pad = 0;
+                    it.mlo = it.mhi = 0;
                     P.items[o + n] = it;
                 }
                 n++;
@@ -1166,6 +1168,14 @@ __global__ __launch_bounds__(256) void k_line_walk(WalkParams P, int pass) {
     uint32_t n = 0;
     uint32_t o = pass ? P.plot_offsets[it_id] : 0u;
     int bx0 = INT32_MAX, by0 = INT32_MAX, bx1 = -1, by1 = -1;
+    // window of execution tiles the plots can reach: the item's screen tile
+    // grown by one pixel on each side (Wu plots y+1 next to y)
+    const uint32_t Tx0 = (it.tile % P.scols) * P.tw, Ty0 = (it.tile / P.scols) * P.th;
+    const int mbx = (int)((Tx0 > 0 ? Tx0 - 1 : 0) / P.ets), mby = (int)((Ty0 > 0 ? Ty0 - 1 : 0) / P.ets);
+    const int mcols = (int)(min(Tx0 + P.tw + 1, P.W - 1) / P.ets) - mbx + 1;
+    const int mrows = (int)(min(Ty0 + P.th + 1, P.H - 1) / P.ets) - mby + 1;
+    bool mvalid = mcols > 0 && mrows > 0 && mcols * mrows <= 128;
+    uint64_t mlo = 0, mhi = 0;
     auto plot = [&](int64_t x, int64_t y, double alpha) {
         if (!(x >= 0 && y >= 0)) return;
         const uint64_t index = (uint64_t)(uint32_t)x + (uint64_t)(uint32_t)y * P.W;
@@ -1182,6 +1192,14 @@ __global__ __launch_bounds__(256) void k_line_walk(WalkParams P, int pass) {
             const int ex = (int)((uint32_t)(index % P.W) / P.ets), ey = (int)((uint32_t)(index / P.W) / P.ets);
             bx0 = min(bx0, ex); bx1 = max(bx1, ex);
             by0 = min(by0, ey); by1 = max(by1, ey);
+            const int rx = ex - mbx, ry = ey - mby;
+            if (rx < 0 || rx >= mcols || ry < 0 || ry >= mrows) {
+                mvalid = false;  // outside the window: bin the whole box
+            } else {
+                const int b = ry * mcols + rx;
+                if (b < 64) mlo |= 1ull << b;
+                else mhi |= 1ull << (b - 64);
+            }
         }
         n++;
     };
@@ -1196,9 +1214,14 @@ __global__ __launch_bounds__(256) void k_line_walk(WalkParams P, int pass) {
         it.plot_cnt = n;
         if (n) {
             it.bx0 = bx0; it.bx1 = bx1; it.by0 = by0; it.by1 = by1;
-            P.bin_counts[it_id] = (uint32_t)((bx1 - bx0 + 1) * (by1 - by0 + 1));
+            it.mbx = (uint16_t)mbx; it.mby = (uint16_t)mby; it.mcols = (uint16_t)mcols;
+            it.mvalid = mvalid ? 1 : 0;
+            it.mlo = mlo; it.mhi = mhi;
+            P.bin_counts[it_id] = mvalid ? (uint32_t)(__popcll(mlo) + __popcll(mhi))
+                                         : (uint32_t)((bx1 - bx0 + 1) * (by1 - by0 + 1));
         } else {
             it.bx0 = 1; it.bx1 = 0; it.by0 = 1; it.by1 = 0;
+            it.mvalid = 0;
             P.bin_counts[it_id] = 0;
         }
     } else {
@@ -1211,6 +1234,20 @@ __global__ __launch_bounds__(256) void k_item_emit(ItemEmitParams P) {
     if (t >= P.n || P.counts[t] == 0) return;
     const LineItem& it = P.items[t];
     uint32_t o = P.offsets[t];
+    if (it.mvalid) {  // only the tiles holding a plot
+        for (int h = 0; h < 2; h++) {
+            uint64_t m = h ? it.mhi : it.mlo;
+            while (m) {
+                const int b = h * 64 + __ffsll((long long)m) - 1;
+                m &= m - 1;
+                const uint32_t ex = it.mbx + (uint32_t)(b % it.mcols), ey = it.mby + (uint32_t)(b / it.mcols);
+                P.keys[o] = ey * P.ntx + ex;
+                P.vals[o] = t;
+                o++;
+            }
+        }
+        return;
+    }
     for (int ey = it.by0; ey <= it.by1; ey++)
         for (int ex = it.bx0; ex <= it.bx1; ex++) {
             P.keys[o] = (uint32_t)ey * P.ntx + (uint32_t)ex;

@@ -89,6 +89,7 @@ struct WalkParams {
     uint32_t W, H;
     int aa;
     uint32_t ets, ntx, nty;
+    uint32_t tw, th, scols;  // screen tiles
     uint32_t* plot_counts;   // pass 0: plots per item
     uint32_t* bin_counts;    // pass 0: execution tiles per item
     const uint32_t* plot_offsets;  // pass 1

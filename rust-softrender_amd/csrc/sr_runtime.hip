@@ -1157,6 +1157,7 @@ int draw_lines_impl(sr_context ctx, sr_framebuffer fb, const DrawPlan& pl, const
     memset(&wp, 0, sizeof(wp));
     wp.items = P<LineItem>(ctx->litems); wp.n_items = nitems; wp.W = pl.W; wp.H = pl.H;
     wp.aa = d->aa_lines ? 1 : 0; wp.ets = LP_ETS; wp.ntx = ntx; wp.nty = nty;
+    wp.tw = pl.tw; wp.th = pl.th; wp.scols = pl.scols;
     wp.plot_counts = P<uint32_t>(ctx->lcounts); wp.bin_counts = P<uint32_t>(ctx->pcounts);
     wp.plot_offsets = P<uint32_t>(ctx->loffsets); wp.err = P<uint32_t>(ctx->misc);
     launch_line_walk(wp, 0, s);
