@@ -64,7 +64,7 @@ struct alignas(8) Plot {
     double alpha;
     uint32_t index;
     int32_t x, y;
-    uint32_t pad;
+    uint32_t row;  // index / W: the pixel's row (its column is index - row * W)
 };
 static_assert(sizeof(Plot) == 24, "Plot layout");
 

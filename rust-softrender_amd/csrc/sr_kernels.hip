@@ -1164,7 +1164,6 @@ __global__ __launch_bounds__(256) void k_line_walk(WalkParams P, int pass) {
     const uint32_t it_id = blockIdx.x * 256u + threadIdx.x;
     if (it_id >= P.n_items) return;
     LineItem& it = P.items[it_id];
-    const uint64_t area = (uint64_t)P.W * P.H;
     uint32_t n = 0;
     uint32_t o = pass ? P.plot_offsets[it_id] : 0u;
     int bx0 = INT32_MAX, by0 = INT32_MAX, bx1 = -1, by1 = -1;
@@ -1176,20 +1175,29 @@ __global__ __launch_bounds__(256) void k_line_walk(WalkParams P, int pass) {
     const int mrows = (int)(min(Ty0 + P.th + 1, P.H - 1) / P.ets) - mby + 1;
     bool mvalid = mcols > 0 && mrows > 0 && mcols * mrows <= 128;
     uint64_t mlo = 0, mhi = 0;
+    const uint32_t esh = (uint32_t)__builtin_ctz(P.ets);  // execution tiles are a power of two
     auto plot = [&](int64_t x, int64_t y, double alpha) {
         if (!(x >= 0 && y >= 0)) return;
-        const uint64_t index = (uint64_t)(uint32_t)x + (uint64_t)(uint32_t)y * P.W;
-        if (x > 0xFFFFFFFFll || y > 0xFFFFFFFFll || index >= area) return;
+        if (x > 0xFFFFFFFFll || y > 0xFFFFFFFFll) return;
+        // index = x + y*W (line.rs:56) as pixel (col, row): x < W except
+        // where a plot runs past the right edge and wraps to a later row
+        uint32_t col = (uint32_t)x, row = (uint32_t)y;
+        if (col >= P.W) {
+            row += col / P.W;
+            col %= P.W;
+            if (row < (uint32_t)y) return;  // wrapped past 2^32 rows: beyond the framebuffer
+        }
+        if (row >= P.H) return;  // index >= W*H
         if (pass) {
             Plot pl;
             pl.alpha = alpha;
-            pl.index = (uint32_t)index;
+            pl.index = col + row * P.W;
             pl.x = (int32_t)x;
             pl.y = (int32_t)y;
-            pl.pad = 0;
+            pl.row = row;
             P.plots[o + n] = pl;
         } else {
-            const int ex = (int)((uint32_t)(index % P.W) / P.ets), ey = (int)((uint32_t)(index / P.W) / P.ets);
+            const int ex = (int)(col >> esh), ey = (int)(row >> esh);
             bx0 = min(bx0, ex); bx1 = max(bx1, ex);
             by0 = min(by0, ey); by1 = max(by1, ey);
             const int rx = ex - mbx, ry = ey - mby;

@@ -185,7 +185,7 @@ __global__ __launch_bounds__(256) void k_raster_lines(RasterParams P) {
                     si = lo;
                     const uint32_t first_g = si ? i_end[si - 1] : 0u;
                     pl = P.plots[i_off[si] + (g - first_g)];
-                    const int px = (int)(pl.index % P.W) - ex0, py = (int)(pl.index / P.W) - ey0;
+                    const int py = (int)pl.row - ey0, px = (int)(pl.index - pl.row * P.W) - ex0;
                     if (px >= 0 && px < LP_ETS && py >= 0 && py < LP_ETS) {
                         pending = true;
                         lp = py * LP_ETS + px;
