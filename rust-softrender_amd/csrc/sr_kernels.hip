@@ -1070,6 +1070,10 @@ __global__ __launch_bounds__(256) void k_line_tiles(LineSetupParams P, int pass)
                     it.pad = 0;
                     it.mlo = it.mhi = 0;
                     P.items[o + n] = it;
+                    if (P.walk_len) {
+                        const float len = fmaxf(fabsf(c[2] - c[0]), fabsf(c[3] - c[1]));
+                        P.walk_len[o + n] = len < 255.0f ? (uint32_t)len : 255u;
+                    }
                 }
                 n++;
             }
@@ -1161,8 +1165,11 @@ __device__ void wu_walk(double x0, double y0, double x1, double y1, F&& plot) {
 // (line.rs:55) and x + y*W lies inside the framebuffer (past its end the
 // reference writes out of bounds; such plots are dropped).
 __global__ __launch_bounds__(256) void k_line_walk(WalkParams P, int pass) {
-    const uint32_t it_id = blockIdx.x * 256u + threadIdx.x;
-    if (it_id >= P.n_items) return;
+    const uint32_t t = blockIdx.x * 256u + threadIdx.x;
+    if (t >= P.n_items) return;
+    // a wave's lanes walk in lock step until the longest walk ends, so the
+    // items run grouped by length when an order is given
+    const uint32_t it_id = P.order ? P.order[t] : t;
     LineItem& it = P.items[it_id];
     uint32_t n = 0;
     uint32_t o = pass ? P.plot_offsets[it_id] : 0u;

@@ -81,6 +81,7 @@ struct LineSetupParams {
     const uint32_t* offsets;
     LineItem* items;
     uint32_t* err;
+    uint32_t* walk_len;    // pass 1 (optional): per item, its walk length in pixels (<= 255)
 };
 
 struct WalkParams {
@@ -90,6 +91,7 @@ struct WalkParams {
     int aa;
     uint32_t ets, ntx, nty;
     uint32_t tw, th, scols;  // screen tiles
+    const uint32_t* order;   // optional: thread t walks item order[t] (items of similar length share a wave)
     uint32_t* plot_counts;   // pass 0: plots per item
     uint32_t* bin_counts;    // pass 0: execution tiles per item
     const uint32_t* plot_offsets;  // pass 1

@@ -1152,9 +1152,20 @@ int draw_lines_impl(sr_context ctx, sr_framebuffer fb, const DrawPlan& pl, const
         (rc = ensure(ctx, ctx->poffsets, sizeof(uint32_t) * (nitems + 1))))
         return rc;
     lp.items = P<LineItem>(ctx->litems);
+    // items ordered by walk length (one 8-bit radix pass over (length, item))
+    if ((rc = ensure_pairs(ctx, nitems)) ||
+        (rc = ensure(ctx, ctx->hist, sizeof(uint32_t) * (radix_hist_words(nitems) + 256))) ||
+        (rc = ensure(ctx, ctx->scan_tmp, sizeof(uint32_t) * (scan_tmp_words(radix_hist_words(nitems) + 256) + 16))))
+        return rc;
+    lp.walk_len = P<uint32_t>(ctx->keys);
     launch_line_tiles(lp, 1, s);
+    launch_iota(P<uint32_t>(ctx->vals), nitems, s);
+    const int swapped = radix_sort(P<uint32_t>(ctx->keys), P<uint32_t>(ctx->vals), P<uint32_t>(ctx->keys2),
+                                   P<uint32_t>(ctx->vals2), nitems, 8, P<uint32_t>(ctx->hist),
+                                   P<uint32_t>(ctx->scan_tmp), P<uint32_t>(ctx->misc) + 8, s);
     WalkParams wp;
     memset(&wp, 0, sizeof(wp));
+    wp.order = swapped ? P<uint32_t>(ctx->vals2) : P<uint32_t>(ctx->vals);
     wp.items = P<LineItem>(ctx->litems); wp.n_items = nitems; wp.W = pl.W; wp.H = pl.H;
     wp.aa = d->aa_lines ? 1 : 0; wp.ets = LP_ETS; wp.ntx = ntx; wp.nty = nty;
     wp.tw = pl.tw; wp.th = pl.th; wp.scols = pl.scols;
