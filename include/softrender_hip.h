@@ -258,6 +258,17 @@ int sr_draw(sr_context ctx, sr_framebuffer fb, sr_mesh mesh, const sr_draw_desc*
             sr_texture tex);
 
 /* ---- batching: many independent draws in one persistent launch ---------- */
+/* Several draw chains over one mesh that share its vertex stage
+ * (GeometryShader::duplicate, src/pipeline/stages/geometry.rs:43-58; the
+ * full example draws a model lit and then its normals from one vertex
+ * pass, full_example/src/main.rs:72-91).  The vertex program runs once, in
+ * clip space; every chain then runs its own geometry program (or finish),
+ * raster and fragment stages, in order.  Every desc must name the same
+ * vertex program and vs_uniforms.  Equal, chain by chain, to sr_draw calls
+ * in order.  sr_ctx_last_stats then describes the last chain. */
+int sr_draw_chains(sr_context ctx, sr_framebuffer fb, sr_mesh mesh, const sr_draw_desc* descs, uint32_t n,
+                   sr_texture tex);
+
 typedef struct sr_batch_item {
     sr_framebuffer fb;
     sr_mesh mesh;

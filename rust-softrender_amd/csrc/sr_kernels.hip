@@ -97,6 +97,19 @@ __device__ __forceinline__ void vertex_one(const float* U, int program, int K, i
     for (int k = 0; k < K; k++) out.comp(4 + k)[o] = var[k];
 }
 
+// GeometryShader::finish on clip-space vertices a draw chain shares with
+// others (geometry.rs:60-129): the same normalisation the vertex kernel
+// fuses for run_to_fragment, varyings copied.
+__global__ __launch_bounds__(256) void k_finish(VertexBuf in, VertexBuf out, uint32_t n, int K, sr_viewport vp) {
+    const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+    if (i >= n) return;
+    float sp[4];
+    normalize_pos(in.comp(0)[i], in.comp(1)[i], in.comp(2)[i], in.comp(3)[i], vp, sp);
+#pragma unroll
+    for (int c = 0; c < 4; c++) out.comp(c)[i] = sp[c];
+    for (int c = 0; c < K; c++) out.comp(4 + c)[i] = in.comp(4 + c)[i];
+}
+
 // One vertex program invocation per thread.  Inputs are coalesced SoA
 // streams; outputs are SoA (x, y, z, w, varyings).
 __global__ __launch_bounds__(256) void k_vertex(VSParams P) {
@@ -1364,6 +1377,11 @@ __global__ __launch_bounds__(256) void k_gs_normals(GSParams P) {
 // ===========================================================================
 // Host-side launchers
 // ===========================================================================
+
+void launch_finish(const VertexBuf& in, const VertexBuf& out, uint32_t n, int K, const sr_viewport& vp,
+                   hipStream_t s) {
+    if (n) hipLaunchKernelGGL(k_finish, dim3(blocks_for(n, 256)), dim3(256), 0, s, in, out, n, K, vp);
+}
 
 void launch_vertex(const VSParams& P, hipStream_t s) {
     if (P.n) hipLaunchKernelGGL(k_vertex, dim3(blocks_for(P.n, 256)), dim3(256), 0, s, P);

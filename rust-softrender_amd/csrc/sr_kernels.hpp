@@ -136,6 +136,9 @@ struct StripParams {
 };
 
 void launch_vertex(const VSParams& P, hipStream_t s);
+// clip space -> screen space (finish) for n vertices with K varyings
+void launch_finish(const VertexBuf& in, const VertexBuf& out, uint32_t n, int K, const sr_viewport& vp,
+                   hipStream_t s);
 void launch_gs_count(int mode, int prim, const GSParams& P, hipStream_t s);
 void launch_gs_emit(int mode, int prim, const GSParams& P, hipStream_t s);
 // normal visualisation on a triangle list: lines (2 vertices each) into P.out,

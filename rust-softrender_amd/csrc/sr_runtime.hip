@@ -954,9 +954,11 @@ int sr_fb_size(sr_framebuffer fb, uint32_t* w, uint32_t* h) {
     return SR_OK;
 }
 
-int sr_draw(sr_context ctx, sr_framebuffer fb, sr_mesh mesh, const sr_draw_desc* d, sr_texture tex) {
-    if (int rc = validate_draw(ctx, fb, mesh, d, tex)) return rc;
-    hipSetDevice(ctx->device);
+// One draw chain.  `clip` (optional) holds the mesh's vertices already run
+// through the chain's vertex program, in clip space: the vertex stage is
+// skipped (a chain that shares them with others, GeometryShader::duplicate).
+static int draw_one(sr_context ctx, sr_framebuffer fb, sr_mesh mesh, const sr_draw_desc* d, sr_texture tex,
+                    const VertexBuf* clip) {
     memset(&ctx->last, 0, sizeof(ctx->last));
     ctx->n_stage = 0;
 
@@ -995,8 +997,10 @@ int sr_draw(sr_context ctx, sr_framebuffer fb, sr_mesh mesh, const sr_draw_desc*
 
     // ---- vertex stage
     const uint32_t nv = mesh->nv;
-    if ((rc = ensure(ctx, ctx->verts, sizeof(float) * (size_t)(4 + pl.K) * std::max<uint32_t>(nv, 1)))) return rc;
-    VertexBuf vb{P<float>(ctx->verts), std::max<uint32_t>(nv, 1)};
+    if (!clip &&
+        (rc = ensure(ctx, ctx->verts, sizeof(float) * (size_t)(4 + pl.K) * std::max<uint32_t>(nv, 1))))
+        return rc;
+    VertexBuf vb = clip ? *clip : VertexBuf{P<float>(ctx->verts), std::max<uint32_t>(nv, 1)};
     VSParams vs;
     memset(&vs, 0, sizeof(vs));
     vs.px = mesh->soa[0]; vs.py = mesh->soa[1]; vs.pz = mesh->soa[2];
@@ -1011,7 +1015,16 @@ int sr_draw(sr_context ctx, sr_framebuffer fb, sr_mesh mesh, const sr_draw_desc*
     vs.viewport = d->viewport;
     memcpy(vs.U, d->vs_uniforms, sizeof(vs.U));
     vs.out = vb;
-    launch_vertex(vs, s);
+    if (!clip) {
+        launch_vertex(vs, s);
+    } else if (gs == SR_GS_NONE) {
+        // finish the shared clip-space vertices into this chain's own buffer
+        if ((rc = ensure(ctx, ctx->gverts, sizeof(float) * (size_t)(4 + pl.K) * std::max<uint32_t>(nv, 1))))
+            return rc;
+        const VertexBuf sv{P<float>(ctx->gverts), std::max<uint32_t>(nv, 1)};
+        launch_finish(*clip, sv, nv, pl.K, d->viewport, s);
+        vb = sv;
+    }
     mark(ctx, pl.timing, ST_VERTEX);
 
     const uint32_t* idx = mesh->idx;
@@ -1082,6 +1095,46 @@ int sr_draw(sr_context ctx, sr_framebuffer fb, sr_mesh mesh, const sr_draw_desc*
         }
         ctx->n_stage = ST_COUNT;
     }
+    return SR_OK;
+}
+
+int sr_draw(sr_context ctx, sr_framebuffer fb, sr_mesh mesh, const sr_draw_desc* d, sr_texture tex) {
+    if (int rc = validate_draw(ctx, fb, mesh, d, tex)) return rc;
+    hipSetDevice(ctx->device);
+    return draw_one(ctx, fb, mesh, d, tex, nullptr);
+}
+
+int sr_draw_chains(sr_context ctx, sr_framebuffer fb, sr_mesh mesh, const sr_draw_desc* descs, uint32_t n,
+                   sr_texture tex) {
+    if (!ctx || (!descs && n)) return SR_ERR_INVALID_ARGUMENT;
+    if (n == 0) return SR_OK;
+    for (uint32_t i = 0; i < n; i++) {
+        if (int rc = validate_draw(ctx, fb, mesh, &descs[i], tex)) return rc;
+        if (descs[i].vertex_program != descs[0].vertex_program ||
+            memcmp(descs[i].vs_uniforms, descs[0].vs_uniforms, sizeof(descs[0].vs_uniforms)) != 0)
+            return fail(ctx, SR_ERR_INVALID_ARGUMENT, "draw_chains: every chain must share the vertex program");
+    }
+    hipSetDevice(ctx->device);
+    if (n == 1) return draw_one(ctx, fb, mesh, &descs[0], tex, nullptr);
+    // the vertex stage once, in clip space (VertexShader::run, vertex.rs:87-120)
+    const int K = varyings_of(descs[0].vertex_program);
+    const uint32_t nv = mesh->nv;
+    int rc;
+    if ((rc = ensure(ctx, ctx->verts, sizeof(float) * (size_t)(4 + K) * std::max<uint32_t>(nv, 1)))) return rc;
+    const VertexBuf clip{P<float>(ctx->verts), std::max<uint32_t>(nv, 1)};
+    VSParams vs;
+    memset(&vs, 0, sizeof(vs));
+    vs.px = mesh->soa[0]; vs.py = mesh->soa[1]; vs.pz = mesh->soa[2];
+    vs.nx = mesh->soa[3]; vs.ny = mesh->soa[4]; vs.nz = mesh->soa[5];
+    vs.u = mesh->soa[6]; vs.v = mesh->soa[7];
+    vs.n = nv; vs.program = descs[0].vertex_program; vs.K = K;
+    vs.normalize = 0;
+    vs.viewport = descs[0].viewport;
+    memcpy(vs.U, descs[0].vs_uniforms, sizeof(vs.U));
+    vs.out = clip;
+    launch_vertex(vs, ctx->stream);
+    for (uint32_t i = 0; i < n; i++)
+        if ((rc = draw_one(ctx, fb, mesh, &descs[i], tex, &clip))) return rc;
     return SR_OK;
 }
 

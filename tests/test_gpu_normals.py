@@ -83,3 +83,51 @@ def test_normals_need_world_varyings(ctx):
         ctx.draw(fb, mesh, sr.make_desc(**dict(d["desc"], vertex_program=0)))
     mesh.free()
     fb.free()
+
+
+def test_chains_share_one_vertex_pass(ctx):
+    """sr_draw_chains (duplicate): the lit chain and the normals chain from one
+    vertex pass equal the two draws made one by one, and the oracle."""
+    s = suzanne_normals(FN, 1)
+    ref, _ = render_ref(s)
+    apart = render_gpu(ctx, s)
+    fb = ctx.framebuffer(s["width"], s["height"])
+    fb.clear(s["clear"])
+    mesh = ctx.mesh(s["draws"][0])
+    ctx.draw_chains(fb, mesh, [sr.make_desc(**d["desc"]) for d in s["draws"]])
+    shared = fb.read()
+    for a, b in zip(shared, apart):
+        assert (a.view(np.uint32) == b.view(np.uint32)).all()
+    assert_parity(shared, ref)
+    mesh.free()
+    fb.free()
+
+
+@pytest.mark.parametrize("gs", [0, 1, 2, VN])
+def test_chains_with_each_geometry_program(ctx, gs):
+    pos, nrm, idx = scenes.uv_sphere(12, 24)
+    s = scenes.normals_scene(pos, nrm, idx, 200, 150, program=VN, aa=0)
+    lit, lines = s["draws"]
+    second = dict(lines, desc=dict(lines["desc"], geometry_program=gs)) if gs == VN else \
+        dict(lit, desc=dict(lit["desc"], geometry_program=gs, blend=1, cull=0))
+    s["draws"] = [lit, second]
+    ref, _ = render_ref(s)
+    fb = ctx.framebuffer(200, 150)
+    fb.clear(s["clear"])
+    mesh = ctx.mesh(lit)
+    ctx.draw_chains(fb, mesh, [sr.make_desc(**d["desc"]) for d in s["draws"]])
+    assert_parity(fb.read(), ref)
+    mesh.free()
+    fb.free()
+
+
+def test_chains_must_share_the_vertex_program(ctx):
+    s = suzanne_normals(FN, 0, 32, 32)
+    fb = ctx.framebuffer(32, 32)
+    mesh = ctx.mesh(s["draws"][0])
+    d0 = sr.make_desc(**s["draws"][0]["desc"])
+    d1 = sr.make_desc(**dict(s["draws"][1]["desc"], vs_uniforms=np.zeros(48, np.float32)))
+    with pytest.raises(sr.SoftrenderError):
+        ctx.draw_chains(fb, mesh, [d0, d1])
+    mesh.free()
+    fb.free()
