@@ -132,15 +132,69 @@ __device__ __forceinline__ uint32_t scene_of(const uint32_t* end, uint32_t n, ui
     return a;
 }
 
+// Loads through the constant address space: with a wave-uniform index they
+// become scalar loads into SGPRs (a flat load counts as divergent).
+__device__ __forceinline__ uint32_t ld_const(const uint32_t* p, uint32_t i) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return ((const __attribute__((address_space(4))) uint32_t*)p)[i];
+#else
+    return p[i];
+#endif
+}
+
+// scene_of for a wave-uniform element index, by scalar loads
+__device__ __forceinline__ uint32_t scene_of_uniform(const uint32_t* end, uint32_t n, uint32_t i) {
+    uint32_t a = 0, b = n;
+    while (a < b) {
+        const uint32_t m = (a + b) >> 1;
+        if (ld_const(end, m) <= i) a = m + 1;
+        else b = m;
+    }
+    return a;
+}
+
+// A scene's geometry record as scalar loads (s wave-uniform).
+__device__ __forceinline__ SceneGeom load_scene(const SceneGeom* base, uint32_t s) {
+    constexpr int N = sizeof(SceneGeom) / 4;
+    const uint32_t* src = (const uint32_t*)(base + s);
+    uint32_t w[N];
+#pragma unroll
+    for (int k = 0; k < N; k++) w[k] = ld_const(src, (uint32_t)k);
+    SceneGeom r;
+    __builtin_memcpy(&r, w, sizeof(r));
+    return r;
+}
+
+// Calls f(scene) for the scene owning element i of [0, n): when the wave's
+// first and last elements share a scene (the common case: scenes hold
+// thousands of elements) the scene is found and read once per wave by
+// scalar loads, otherwise per lane.
+template <class F>
+__device__ __forceinline__ void with_scene(const BatchTables& T, const uint32_t* end, uint32_t n, uint32_t i,
+                                           F&& f) {
+    const uint32_t w0 = __builtin_amdgcn_readfirstlane(blockIdx.x * 256u + (threadIdx.x & ~63u));
+    if (w0 >= n) return;
+    const uint32_t w1 = min(w0 + 63u, n - 1u);
+    const uint32_t s0 = scene_of_uniform(end, T.nscenes, w0);
+    if (scene_of_uniform(end, T.nscenes, w1) == s0) {
+        if (i >= n) return;
+        const SceneGeom S = load_scene(T.scenes, s0);
+        f(S);
+    } else {
+        if (i >= n) return;
+        f(T.scenes[scene_of(end, T.nscenes, i)]);
+    }
+}
+
 __global__ __launch_bounds__(256) void k_vertex_batch(BatchTables T, VertexBuf out) {
     const uint32_t i = blockIdx.x * 256u + threadIdx.x;
-    if (i >= T.nv) return;
-    const SceneGeom& S = T.scenes[scene_of(T.vend, T.nscenes, i)];
-    const uint32_t l = i - S.vbase;
-    const float pin[4] = {S.px[l], S.py[l], S.pz[l], 1.0f};
-    const float nin[3] = {S.nx ? S.nx[l] : 0.0f, S.ny ? S.ny[l] : 0.0f, S.nz ? S.nz[l] : 0.0f};
-    vertex_one(S.U, S.program, S.K, 1, S.viewport, pin, nin, S.u ? S.u[l] : 0.0f,
-               S.v ? S.v[l] : 0.0f, out, i);
+    with_scene(T, T.vend, T.nv, i, [&](const SceneGeom& S) {
+        const uint32_t l = i - S.vbase;
+        const float pin[4] = {S.px[l], S.py[l], S.pz[l], 1.0f};
+        const float nin[3] = {S.nx ? S.nx[l] : 0.0f, S.ny ? S.ny[l] : 0.0f, S.nz ? S.nz[l] : 0.0f};
+        vertex_one(S.U, S.program, S.K, 1, S.viewport, pin, nin, S.u ? S.u[l] : 0.0f,
+                   S.v ? S.v[l] : 0.0f, out, i);
+    });
 }
 
 // ===========================================================================
@@ -527,14 +581,14 @@ __global__ __launch_bounds__(256) void k_tri_setup(SetupParams P) {
 __global__ __launch_bounds__(256) void k_tri_setup_batch(BatchTables T, VertexBuf v, TriRec* recs,
                                                         uint32_t* counts, uint32_t* err) {
     const uint32_t t = blockIdx.x * 256u + threadIdx.x;
-    if (t >= T.nt) return;
-    const SceneGeom& S = T.scenes[scene_of(T.tend, T.nscenes, t)];
-    const uint32_t l = t - S.tbase;
-    TriRec r;
-    counts[t] = tri_setup_one(v, S.vbase + prim_vertex(S.idx, 3 * l), S.vbase + prim_vertex(S.idx, 3 * l + 1),
-                              S.vbase + prim_vertex(S.idx, 3 * l + 2), S.cull, S.W, S.H, S.ets, r, err,
-                              S.empty_cull);
-    recs[t] = r;
+    with_scene(T, T.tend, T.nt, t, [&](const SceneGeom& S) {
+        const uint32_t l = t - S.tbase;
+        TriRec r;
+        counts[t] = tri_setup_one(v, S.vbase + prim_vertex(S.idx, 3 * l), S.vbase + prim_vertex(S.idx, 3 * l + 1),
+                                  S.vbase + prim_vertex(S.idx, 3 * l + 2), S.cull, S.W, S.H, S.ets, r, err,
+                                  S.empty_cull);
+        recs[t] = r;
+    });
 }
 
 // (execution tile, triangle) pairs of a batch; keys are global tile ids.
@@ -552,8 +606,8 @@ __global__ __launch_bounds__(256) void k_tri_emit_batch(BatchTables T, const Tri
         atomicOr(overflow, 1u);
         return;
     }
-    const SceneGeom& S = T.scenes[scene_of(T.tend, T.nscenes, t)];
     const TriRec r = recs[t];
+    const SceneGeom& S = T.scenes[scene_of(T.tend, T.nscenes, t)];
     TileSpan sp;
     if (!tile_span(r, S.W, S.H, S.ets, sp)) return;
     emit_pairs(r, sp, S.ets, S.ntx, S.empty_cull, S.tile_base, t, keys + o, vals + o);
