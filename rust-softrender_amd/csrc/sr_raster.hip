@@ -505,9 +505,11 @@ __global__ __launch_bounds__(256) SR_RASTER_OCCUPANCY void k_raster_tris_batch(R
     if (B.overflow && *B.overflow) return;
     for (int i = threadIdx.x; i < 256; i += SH::NT) sh.lut[i] = (float)i / 255.0f;
     __syncthreads();
-    // Persistent: work-group b takes run p of near-equal cost (a contiguous
-    // run of tiles); runs are numbered so that each XCD holds a contiguous
-    // stretch of tiles (neighbouring tiles share records in its L2).
+    // Work-group b takes run p of near-equal estimated cost (a contiguous
+    // run of tiles; many more runs than resident work-groups, so the
+    // hardware's hand-out balances what the estimate misses); runs are
+    // numbered so that each XCD holds a contiguous stretch of tiles
+    // (neighbouring tiles share records in its L2).
     const uint32_t p = xcd_remap(blockIdx.x, gridDim.x);
     const uint32_t t0 = load_item(B.wg_start, p), t1 = load_item(B.wg_start, p + 1);
     uint32_t group = 0, group_end = 0;
