@@ -42,15 +42,20 @@ DEFAULT_UNITS = {"bbox_tests": 86093801, "covered": 33448710, "shaded": 6973823}
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_raster_tris_soup1080p_1M.md")
 
 
-def profiled_traffic(exec_tile):
+def profiled_value(key):
     try:
         with open(TRAFFIC_FILE) as f:
             for line in f:
-                if line.startswith(f"traffic_bytes_per_launch_ets{exec_tile}:"):
-                    return int(line.split(":", 1)[1].strip())
+                if line.startswith(f"{key}:"):
+                    return line.split(":", 1)[1].strip()
     except OSError:
         pass
     return None
+
+
+def profiled_traffic(exec_tile):
+    v = profiled_value(f"traffic_bytes_per_launch_ets{exec_tile}")
+    return int(v) if v is not None else None
 
 
 def parse():
@@ -222,6 +227,12 @@ def main():
                            "unit": "TFLOP/s", "frac": round(ach / FP32_PEAK_TFLOPS, 4),
                            "traffic": profiled_traffic(ets) if default_workload else None,
                            "kernel": "k_raster_tris", "avg_launch_ms": round(stages["raster"], 4)}
+        # the instruction-level bound: VALU issue share from the SQ counters of
+        # the committed profile (the flop fraction above counts only the
+        # reference's arithmetic)
+        busy = profiled_value(f"valu_issue_fraction_ets{ets}") if default_workload else None
+        if busy is not None:
+            out["roofline"]["valu_issue_frac_profiled"] = float(busy)
     mesh.free()
     tex.free()
     fb.free()
