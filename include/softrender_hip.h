@@ -197,8 +197,10 @@ int sr_ctx_stage_times(sr_context ctx, float* ms_out, int n_max, int* n_out);
 int sr_ctx_last_stats(sr_context ctx, sr_draw_stats* out);
 
 /* ---- mesh (mesh.rs:12-64) ---------------------------------------------- */
-/* indices == NULL means non-indexed (index i = i); n_indices must be a
- * multiple of the primitive's vertex count (assert_eq!, pipeline/mod.rs:148). */
+/* indices == NULL means non-indexed (index i = i); a non-NULL list with
+ * n_indices == 0 is an empty Mesh::indices and draws nothing.  n_indices
+ * must be a multiple of the primitive's vertex count (assert_eq!,
+ * pipeline/mod.rs:148). */
 int sr_mesh_upload(sr_context ctx, const sr_vertex_soa* vertices, uint32_t n_vertices,
                    const uint32_t* indices, uint32_t n_indices, int32_t primitive,
                    sr_mesh* out);

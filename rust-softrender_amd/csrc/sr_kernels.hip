@@ -853,13 +853,22 @@ __global__ __launch_bounds__(256) void k_partition(const uint32_t* prefix, uint3
 // Grid-stride over the sorted keys (the grid is bounded; the live count
 // may be far below the capacity the launch was sized for).
 __global__ __launch_bounds__(256) void k_ranges(const uint32_t* keys, uint32_t n, uint32_t key_base,
-                                               uint32_t* start, uint32_t* end, const uint32_t* n_dev) {
+                                               uint32_t nkeys, uint32_t* start, uint32_t* end,
+                                               const uint32_t* n_dev) {
     if (n_dev) n = min(n, *n_dev);
     for (uint32_t i = blockIdx.x * 256u + threadIdx.x; i < n; i += gridDim.x * 256u) {
         const uint32_t k = keys[i];
+        if (k - key_base >= nkeys) continue;  // never for complete pairs; guards the tables
         if (i == 0 || keys[i - 1] != k) start[k - key_base] = i;
         if (i == n - 1 || keys[i + 1] != k) end[k - key_base] = i + 1;
     }
+}
+
+// The pair count the sort and the ranges may use: the emitted total, or 0
+// when the capacity overflowed (the slots past the first triangle that did
+// not fit were never written and hold stale data).
+__global__ void k_live_pairs(uint32_t* misc, int overflow_word, int total_word, int live_word) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) misc[live_word] = misc[overflow_word] ? 0u : misc[total_word];
 }
 
 // ===========================================================================
@@ -1533,10 +1542,13 @@ void launch_iota(uint32_t* p, uint32_t n, hipStream_t s) {
     if (n) hipLaunchKernelGGL(k_iota_u32, dim3(blocks_for(n, 256)), dim3(256), 0, s, p, n);
 }
 
-void launch_ranges(const uint32_t* keys, uint32_t n, uint32_t key_base, uint32_t* start,
+void launch_ranges(const uint32_t* keys, uint32_t n, uint32_t key_base, uint32_t nkeys, uint32_t* start,
                    uint32_t* end, hipStream_t s, const uint32_t* n_dev) {
     if (n) hipLaunchKernelGGL(k_ranges, dim3(std::min<uint32_t>(blocks_for(n, 256), 8192u)), dim3(256), 0, s,
-                              keys, n, key_base, start, end, n_dev);
+                              keys, n, key_base, nkeys, start, end, n_dev);
+}
+void launch_live_pairs(uint32_t* misc, int overflow_word, int total_word, int live_word, hipStream_t s) {
+    hipLaunchKernelGGL(k_live_pairs, dim3(1), dim3(64), 0, s, misc, overflow_word, total_word, live_word);
 }
 
 void launch_tile_cost(const uint32_t* start, const uint32_t* end, uint32_t n, uint32_t fixed, uint32_t* cost,

@@ -165,8 +165,12 @@ int radix_sort(uint32_t* keys, uint32_t* vals, uint32_t* keys_tmp, uint32_t* val
                const uint32_t* n_dev = nullptr);
 void launch_fill(uint32_t* p, uint32_t v, uint32_t n, hipStream_t s);
 void launch_iota(uint32_t* p, uint32_t n, hipStream_t s);  // p[i] = i
-void launch_ranges(const uint32_t* keys, uint32_t n, uint32_t key_base, uint32_t* start,
+// start/end of each key's run in sorted keys; keys outside [key_base,
+// key_base + nkeys) are skipped
+void launch_ranges(const uint32_t* keys, uint32_t n, uint32_t key_base, uint32_t nkeys, uint32_t* start,
                    uint32_t* end, hipStream_t s, const uint32_t* n_dev = nullptr);
+// misc[live_word] = misc[overflow_word] ? 0 : misc[total_word]
+void launch_live_pairs(uint32_t* misc, int overflow_word, int total_word, int live_word, hipStream_t s);
 // persistent-raster partition: cost[t] = fixed + items of tile t for
 // t < n, cost[n] = 0 (scan it into a prefix of n + 1 entries), then G runs
 // of near-equal cost over tiles [tile0, tile0 + n): wg_start[0..G]

@@ -95,7 +95,7 @@ int bin_pairs(sr_context ctx, uint32_t npairs, uint32_t ntiles, const uint32_t**
     *sorted_vals = swapped ? P<uint32_t>(ctx->vals2) : P<uint32_t>(ctx->vals);
     launch_fill(P<uint32_t>(ctx->tstart), 0u, ntiles, s);
     launch_fill(P<uint32_t>(ctx->tend), 0u, ntiles, s);
-    launch_ranges(skeys, npairs, 0u, P<uint32_t>(ctx->tstart), P<uint32_t>(ctx->tend), s);
+    launch_ranges(skeys, npairs, 0u, ntiles, P<uint32_t>(ctx->tstart), P<uint32_t>(ctx->tend), s);
     return SR_OK;
 }
 
@@ -210,7 +210,7 @@ int draw_tris(sr_context ctx, sr_framebuffer fb, const DrawPlan& pl, const sr_dr
         mark(ctx, pl.timing, ST_SORT);
         launch_fill(P<uint32_t>(ctx->tstart), 0u, pl.ntiles, s);
         launch_fill(P<uint32_t>(ctx->tend), 0u, pl.ntiles, s);
-        launch_ranges(skeys, npairs, 0u, P<uint32_t>(ctx->tstart), P<uint32_t>(ctx->tend), s);
+        launch_ranges(skeys, npairs, 0u, pl.ntiles, P<uint32_t>(ctx->tstart), P<uint32_t>(ctx->tend), s);
         mark(ctx, pl.timing, ST_RANGES);
         R.tile_start = P<uint32_t>(ctx->tstart);
         R.tile_end = P<uint32_t>(ctx->tend);
@@ -411,7 +411,7 @@ int batch_plan(sr_context ctx, const sr_batch_item* items, uint32_t n, BatchPlan
             const sr_draw_desc* d = &pl.descs[k];
             const sr_mesh m = pl.items[k].mesh;
             const bool drawable = W > 1 && H > 1;  // W or H == 1: no tiles (fragment.rs:196,201)
-            const uint32_t ntri = drawable ? (m->ni ? m->ni : m->nv) / 3 : 0;
+            const uint32_t ntri = drawable ? mesh_count(m) / 3 : 0;
             SceneGeom& g = pl.geom[k];
             memset(&g, 0, sizeof(g));
             g.px = m->soa[0]; g.py = m->soa[1]; g.pz = m->soa[2]; g.nx = m->soa[3]; g.ny = m->soa[4];
@@ -503,6 +503,7 @@ int batch_enqueue(sr_context ctx, BatchWork& W, const BatchPlan& pl, bool use_gr
     uint32_t* d_group_tile_end = d_tend + ns;
     uint32_t* d_group_first = d_group_tile_end + ng;
     uint32_t* misc = P<uint32_t>(W.misc);  // [0] overflow [1] pairs [2] err [3..4] sort [5] tile cost total
+                                            // [6] pairs to sort (0 on overflow)
     const VertexBuf vb{P<float>(W.verts), nvb};
     const int swapped_passes = (bits_for((uint32_t)std::max<uint64_t>(NTILES, 2)) + 7) / 8;
     const uint32_t* sorted_vals = (swapped_passes & 1) ? P<uint32_t>(W.vals2) : P<uint32_t>(W.vals);
@@ -577,13 +578,16 @@ int batch_enqueue(sr_context ctx, BatchWork& W, const BatchPlan& pl, bool use_gr
         launch_scan(P<uint32_t>(W.counts), P<uint32_t>(W.offsets), (uint32_t)NT, P<uint32_t>(W.scan), misc + 1, s);
         launch_tri_emit_batch(T, P<TriRec>(W.recs), P<uint32_t>(W.counts), P<uint32_t>(W.offsets), cap,
                               P<uint32_t>(W.keys), P<uint32_t>(W.vals), misc + 0, s);
+        // on overflow nothing is sorted or binned (the raster skips the batch
+        // and it runs again with a larger capacity)
+        launch_live_pairs(misc, 0, 1, 6, s);
         const int sw = radix_sort(P<uint32_t>(W.keys), P<uint32_t>(W.vals), P<uint32_t>(W.keys2),
                                   P<uint32_t>(W.vals2), cap, bits_for((uint32_t)std::max<uint64_t>(NTILES, 2)),
-                                  P<uint32_t>(W.hist), P<uint32_t>(W.scan), misc + 3, s, misc + 1);
+                                  P<uint32_t>(W.hist), P<uint32_t>(W.scan), misc + 3, s, misc + 6);
         const uint32_t* skeys = sw ? P<uint32_t>(W.keys2) : P<uint32_t>(W.keys);
         launch_fill(P<uint32_t>(W.tstart), 0u, (uint32_t)NTILES, s);
         launch_fill(P<uint32_t>(W.tend), 0u, (uint32_t)NTILES, s);
-        launch_ranges(skeys, cap, 0u, P<uint32_t>(W.tstart), P<uint32_t>(W.tend), s, misc + 1);
+        launch_ranges(skeys, cap, 0u, (uint32_t)NTILES, P<uint32_t>(W.tstart), P<uint32_t>(W.tend), s, misc + 6);
         // cost-balanced runs of tiles for the persistent raster
         launch_tile_cost(P<uint32_t>(W.tstart), P<uint32_t>(W.tend), (uint32_t)NTILES, TILE_FIXED_COST,
                          P<uint32_t>(W.cost), s);
@@ -818,7 +822,7 @@ int sr_mesh_upload(sr_context ctx, const sr_vertex_soa* v, uint32_t nv, const ui
     hipSetDevice(ctx->device);
     sr_mesh m = new (std::nothrow) sr_mesh_t();
     if (!m) return SR_ERR_OUT_OF_MEMORY;
-    m->ctx = ctx; m->nv = nv; m->ni = indices ? ni : 0; m->prim = primitive;
+    m->ctx = ctx; m->nv = nv; m->ni = indices ? ni : 0; m->indexed = indices != nullptr; m->prim = primitive;
     const float* src[8] = {v->px, v->py, v->pz, v->nx, v->ny, v->nz, v->u, v->v};
     for (int c = 0; c < 8; c++) {
         if (!src[c] || nv == 0) continue;
@@ -986,7 +990,7 @@ static int draw_one(sr_context ctx, sr_framebuffer fb, sr_mesh mesh, const sr_dr
     pl.exhaustive = (d->exec_flags & SR_EXEC_EXHAUSTIVE) != 0 || (value_test && d->stencil_op != SR_STENCIL_KEEP);
 
     const uint32_t per = (uint32_t)mesh->prim;
-    const uint32_t nidx = mesh->ni ? mesh->ni : mesh->nv;
+    const uint32_t nidx = mesh_count(mesh);
     const uint32_t nprim = nidx / per;
     ctx->last.triangles_in = mesh->prim == SR_PRIM_TRIANGLES ? nprim : 0;
     ctx->last.lines_in = mesh->prim == SR_PRIM_LINES ? nprim : 0;

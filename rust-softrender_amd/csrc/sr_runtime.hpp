@@ -52,8 +52,13 @@ struct sr_mesh_t {
     float* soa[8] = {};  // px py pz nx ny nz u v (null if absent)
     uint32_t* idx = nullptr;
     uint32_t nv = 0, ni = 0;
+    bool indexed = false;  // an index list was given (it may be empty: nothing is drawn)
     int prim = 0;
 };
+
+// Vertices the mesh's primitives are assembled from: the index list, or
+// every vertex in order when there is none.
+inline uint32_t mesh_count(const sr_mesh_t* m) { return m->indexed ? m->ni : m->nv; }
 
 struct sr_texture_t {
     sr_context ctx;

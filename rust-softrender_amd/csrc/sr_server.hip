@@ -97,7 +97,7 @@ namespace {
 
 size_t fb_bytes(uint32_t w, uint32_t h) { return (size_t)w * h * 9; }
 
-uint64_t mesh_tris(sr_mesh m) { return (m->ni ? m->ni : m->nv) / 3; }
+uint64_t mesh_tris(sr_mesh m) { return mesh_count(m) / 3; }
 
 size_t request_cost(const sr_server_t* S, const sr_request& r) {
     const uint64_t t = mesh_tris(r.mesh);
