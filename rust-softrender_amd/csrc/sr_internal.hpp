@@ -48,10 +48,15 @@ struct alignas(16) LineItem {
     float cx1, cy1;       // clipped start
     float cx2, cy2;       // clipped end
     float dlen;           // hypot of the clipped segment, line.rs:52
-    int32_t bx0, by0, bx1, by1;  // execution tiles touched by its plots (bx0 > bx1: none)
+    int16_t bx0, by0, bx1, by1;  // execution tiles touched by its plots (bx0 > bx1: none)
     // the execution tiles that hold a plot, as bits (row-major, mcols per
     // row) of a window starting at tile (mbx, mby); mvalid = 0: use the box
     uint16_t mbx, mby, mcols, mvalid;
+    // ranged = 1 (Wu walks that dropped or wrapped no plot): plots 0..3 are
+    // the endpoints and plot 4 + 2i + {0, 1} lies on major coordinate
+    // major0 + i (the pixel column, or row when steep)
+    int32_t major0;
+    uint16_t ranged, steep;
     uint32_t pad;
     uint64_t mlo, mhi;
 };

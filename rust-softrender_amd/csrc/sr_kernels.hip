@@ -1143,6 +1143,8 @@ __global__ __launch_bounds__(256) void k_line_tiles(LineSetupParams P, int pass)
                     it.dlen = hypot_libm(c[0] - c[2], c[1] - c[3]);  // line.rs:52
                     it.bx0 = 1; it.bx1 = 0; it.by0 = 1; it.by1 = 0;
                     it.mbx = it.mby = it.mcols = it.mvalid = 0;
+                    it.major0 = 0;
+                    it.ranged = it.steep = 0;
                     it.pad = 0;
                     it.mlo = it.mhi = 0;
                     P.items[o + n] = it;
@@ -1183,10 +1185,17 @@ __device__ void bresenham_walk(int64_t x0, int64_t y0, int64_t x1, int64_t y1, F
     }
 }
 
-// draw_line_xiaolin_wu, line.rs:159-238 (f64, exactly as written)
+// draw_line_xiaolin_wu, line.rs:159-238 (f64, exactly as written).
+// Reports whether the walk is steep and the major coordinate of the first
+// interior plot pair.
+struct WuShape {
+    bool steep;
+    double major0;
+};
 template <class F>
-__device__ void wu_walk(double x0, double y0, double x1, double y1, F&& plot) {
+__device__ void wu_walk(double x0, double y0, double x1, double y1, F&& plot, WuShape& shape) {
     const bool steep = fabs(y1 - y0) > fabs(x1 - x0);
+    shape.steep = steep;
     double t;
     if (steep) { t = x0; x0 = y0; y0 = t; t = x1; x1 = y1; y1 = t; }
     if (x0 > x1) { t = x0; x0 = x1; x1 = t; t = y0; y0 = y1; y1 = t; }
@@ -1217,6 +1226,7 @@ __device__ void wu_walk(double x0, double y0, double x1, double y1, F&& plot) {
         pf(xpxl2, ypxl2 + 1.0, fract64(yend) * xgap);
     }
     double x = xpxl1 + 1.0;
+    shape.major0 = x;
     if (steep) {
         while (x <= (xpxl2 - 1.0)) {
             const double y = trunc(intery);
@@ -1259,13 +1269,17 @@ __global__ __launch_bounds__(256) void k_line_walk(WalkParams P, int pass) {
     bool mvalid = mcols > 0 && mrows > 0 && mcols * mrows <= 128;
     uint64_t mlo = 0, mhi = 0;
     const uint32_t esh = (uint32_t)__builtin_ctz(P.ets);  // execution tiles are a power of two
+    uint32_t calls = 0;
+    bool wrapped = false;
     auto plot = [&](int64_t x, int64_t y, double alpha) {
+        calls++;
         if (!(x >= 0 && y >= 0)) return;
         if (x > 0xFFFFFFFFll || y > 0xFFFFFFFFll) return;
         // index = x + y*W (line.rs:56) as pixel (col, row): x < W except
         // where a plot runs past the right edge and wraps to a later row
         uint32_t col = (uint32_t)x, row = (uint32_t)y;
         if (col >= P.W) {
+            wrapped = true;
             row += col / P.W;
             col %= P.W;
             if (row < (uint32_t)y) return;  // wrapped past 2^32 rows: beyond the framebuffer
@@ -1294,8 +1308,9 @@ __global__ __launch_bounds__(256) void k_line_walk(WalkParams P, int pass) {
         }
         n++;
     };
+    WuShape shape{false, 0.0};
     if (P.aa) {
-        wu_walk((double)it.cx1, (double)it.cy1, (double)it.cx2, (double)it.cy2, plot);
+        wu_walk((double)it.cx1, (double)it.cy1, (double)it.cx2, (double)it.cy2, plot, shape);
     } else {
         // cast::<f32, i64>(..).unwrap(): truncation (the clipped points are finite)
         bresenham_walk((int64_t)it.cx1, (int64_t)it.cy1, (int64_t)it.cx2, (int64_t)it.cy2, plot);
@@ -1304,15 +1319,23 @@ __global__ __launch_bounds__(256) void k_line_walk(WalkParams P, int pass) {
         P.plot_counts[it_id] = n;
         it.plot_cnt = n;
         if (n) {
-            it.bx0 = bx0; it.bx1 = bx1; it.by0 = by0; it.by1 = by1;
+            it.bx0 = (int16_t)bx0; it.bx1 = (int16_t)bx1; it.by0 = (int16_t)by0; it.by1 = (int16_t)by1;
             it.mbx = (uint16_t)mbx; it.mby = (uint16_t)mby; it.mcols = (uint16_t)mcols;
             it.mvalid = mvalid ? 1 : 0;
             it.mlo = mlo; it.mhi = mhi;
+            // every plot kept, none wrapped: the interior plots of a major
+            // coordinate sit at a known position
+            const bool ranged = P.aa && !wrapped && n == calls && n >= 4 && shape.major0 >= 0.0 &&
+                                shape.major0 < 2147483647.0;
+            it.ranged = ranged ? 1 : 0;
+            it.steep = shape.steep ? 1 : 0;
+            it.major0 = ranged ? (int32_t)shape.major0 : 0;
             P.bin_counts[it_id] = mvalid ? (uint32_t)(__popcll(mlo) + __popcll(mhi))
                                          : (uint32_t)((bx1 - bx0 + 1) * (by1 - by0 + 1));
         } else {
             it.bx0 = 1; it.bx1 = 0; it.by0 = 1; it.by1 = 0;
             it.mvalid = 0;
+            it.ranged = 0;
             P.bin_counts[it_id] = 0;
         }
     } else {

@@ -115,6 +115,7 @@ __global__ __launch_bounds__(256) void k_raster_lines(RasterParams P) {
     __shared__ float s_dep[256], slut[256];
     __shared__ float i_cx1[256], i_cy1[256], i_dlen[256], i_zs[256], i_ze[256];
     __shared__ uint32_t i_v0[256], i_v1[256], i_off[256], i_end[256];  // i_end: inclusive plot prefix
+    __shared__ uint32_t i_skip[256];  // interior plots before this tile's (ranged items)
     const int t = threadIdx.x;
     const uint32_t tile = xcd_remap(blockIdx.x, gridDim.x);
     const uint32_t etx = tile % P.ntx, ety = tile / P.ntx;
@@ -153,8 +154,20 @@ __global__ __launch_bounds__(256) void k_raster_lines(RasterParams P) {
             uint32_t nstaged;
             const uint32_t slot = lp_scan(mine ? 1u : 0u, nstaged, lds4);
             if (nstaged == 0) continue;  // uniform
+            // plots to scan: all, or for a ranged Wu item the 4 endpoint
+            // plots and the interior pairs whose major coordinate lies in
+            // this tile's span (the others fall outside the tile)
+            uint32_t scan = mine ? it.plot_cnt : 0u, skip = 0;
+            if (mine && it.ranged) {
+                const int64_t M0 = it.steep ? ey0 : ex0, M1 = M0 + LP_ETS - 1;
+                const int64_t nint = (int64_t)(it.plot_cnt - 4) / 2;
+                const int64_t lo = max<int64_t>(M0 - it.major0, 0);
+                const int64_t hi = max<int64_t>(min<int64_t>(M1 - it.major0 + 1, nint), lo);
+                scan = 4u + 2u * (uint32_t)(hi - lo);
+                skip = 2u * (uint32_t)lo;
+            }
             uint32_t total;
-            const uint32_t pstart = lp_scan(mine ? it.plot_cnt : 0u, total, lds4);
+            const uint32_t pstart = lp_scan(scan, total, lds4);
             if (mine) {
                 const uint32_t v0 = P.line_v ? P.line_v[2 * it.line] : 2 * it.line;
                 const uint32_t v1 = P.line_v ? P.line_v[2 * it.line + 1] : 2 * it.line + 1;
@@ -166,7 +179,8 @@ __global__ __launch_bounds__(256) void k_raster_lines(RasterParams P) {
                 i_v0[slot] = v0;
                 i_v1[slot] = v1;
                 i_off[slot] = it.plot_off;
-                i_end[slot] = pstart + it.plot_cnt;
+                i_skip[slot] = skip;
+                i_end[slot] = pstart + scan;
             }
             __syncthreads();
             for (uint32_t c = 0; c < total; c += 256) {
@@ -183,8 +197,8 @@ __global__ __launch_bounds__(256) void k_raster_lines(RasterParams P) {
                         else lo = m + 1;
                     }
                     si = lo;
-                    const uint32_t first_g = si ? i_end[si - 1] : 0u;
-                    pl = P.plots[i_off[si] + (g - first_g)];
+                    const uint32_t j = g - (si ? i_end[si - 1] : 0u);
+                    pl = P.plots[i_off[si] + (j < 4u ? j : j + i_skip[si])];
                     const int py = (int)pl.row - ey0, px = (int)(pl.index - pl.row * P.W) - ex0;
                     if (px >= 0 && px < LP_ETS && py >= 0 && py < LP_ETS) {
                         pending = true;
