@@ -316,8 +316,12 @@ __global__ __launch_bounds__(256) void k_gs_count(GSParams P) {
 __device__ __forceinline__ void store_screen(const GSParams& P, uint32_t slot, const CV& v) {
     float s[4];
     normalize_pos(v.p[0], v.p[1], v.p[2], v.p[3], P.viewport, s);
+#pragma unroll
     for (int c = 0; c < 4; c++) P.out.comp(c)[slot] = s[c];
-    for (int c = 0; c < P.K; c++) P.out.comp(4 + c)[slot] = v.k[c];
+    // static indices into v.k (a runtime index would put v in scratch memory)
+#pragma unroll
+    for (int c = 0; c < 6; c++)
+        if (c < P.K) P.out.comp(4 + c)[slot] = v.k[c];
 }
 
 template <int MODE, int PRIM>
@@ -328,17 +332,47 @@ __global__ __launch_bounds__(256) void k_gs_emit(GSParams P) {
     if constexpr (PRIM == SR_PRIM_TRIANGLES) {
         if (P.counts[t] == 0) return;
         CV v[3];
+#pragma unroll
         for (int j = 0; j < 3; j++) load_cv(P.in, prim_vertex(P.idx, 3 * t + j), P.K, v[j]);
+        if constexpr (MODE == SR_GS_CLIP_NEAR) {
+            // One plane: the polygon the loop below would build, written out
+            // per inside pattern (a runtime-indexed polygon array would live
+            // in scratch memory).  Edges (a,b), (b,c), (c,a) each push their
+            // intersection if they cross, then their end vertex if inside.
+            const CV &a = v[0], &b = v[1], &c = v[2];
+            const int m = (inside(4, a) ? 1 : 0) | (inside(4, b) ? 2 : 0) | (inside(4, c) ? 4 : 0);
+            CV i0, i1, i2;
+            if ((m & 1) != ((m >> 1) & 1)) intersect(4, a, b, i0);
+            if (((m >> 1) & 1) != ((m >> 2) & 1)) intersect(4, b, c, i1);
+            if (((m >> 2) & 1) != (m & 1)) intersect(4, c, a, i2);
+            auto tri = [&](uint32_t k, const CV& x, const CV& y, const CV& z) __attribute__((always_inline)) {
+                store_screen(P, 3 * (o + k) + 0, x);
+                store_screen(P, 3 * (o + k) + 1, y);
+                store_screen(P, 3 * (o + k) + 2, z);
+            };
+            // four vertices q0..q3: emit_triangle(q3, q0, q1), (q3, q1, q2)
+            auto quad = [&](const CV& q0, const CV& q1, const CV& q2, const CV& q3) __attribute__((always_inline)) {
+                tri(0, q3, q0, q1);
+                tri(1, q3, q1, q2);
+            };
+            switch (m) {
+                case 7: tri(0, b, c, a); break;     // [b, c, a]
+                case 1: tri(0, i0, i2, a); break;   // a inside: [I(a,b), I(c,a), a]
+                case 2: tri(0, i0, b, i1); break;   // b inside: [I(a,b), b, I(b,c)]
+                case 4: tri(0, i1, c, i2); break;   // c inside: [I(b,c), c, I(c,a)]
+                case 3: quad(b, i1, i2, a); break;  // a, b inside: [b, I(b,c), I(c,a), a]
+                case 5: quad(i0, i1, c, a); break;  // a, c inside: [I(a,b), I(b,c), c, a]
+                case 6: quad(i0, b, c, i2); break;  // b, c inside: [I(a,b), b, c, I(c,a)]
+                default: break;                     // none inside
+            }
+            return;
+        }
         CV poly[36];
         int n = 0;
         for (int e = 0; e < 3; e++) {
             const CV& s = v[e];
             const CV& p = v[(e + 1) % 3];
-            if constexpr (MODE == SR_GS_CLIP_NEAR) {
-                const bool si = inside(4, s), pi = inside(4, p);
-                if (si != pi) intersect(4, s, p, poly[n++]);
-                if (pi) poly[n++] = p;
-            } else {
+            {
                 for (int pl = 0; pl < 6; pl++) {
                     const bool si = inside(pl, s), pi = inside(pl, p);
                     if (si != pi) intersect(pl, s, p, poly[n++]);
