@@ -315,8 +315,8 @@ int sr_ctx_set_batch_capacity(sr_context ctx, uint32_t pairs);
  * request on its own.  submit / poll / release / stats are thread-safe. */
 typedef struct sr_server_config {
     uint64_t hbm_budget;       /* bytes of framebuffers + bin workspaces in flight; 0: 16 GiB */
-    uint32_t max_inflight;     /* batches on the device at once, one stream each; 0: 2 */
-    uint32_t max_batch_frames; /* requests per batch; 0: 256 */
+    uint32_t max_inflight;     /* batches on the device at once, one stream each; 0: 4 */
+    uint32_t max_batch_frames; /* requests per batch; 0: 32 */
     uint64_t max_batch_tris;   /* triangles per batch (a larger request goes alone); 0: 32M */
     float pairs_per_tri;       /* bin pairs per triangle used to size workspaces; 0: 4 */
     int32_t reserved[7];

@@ -183,7 +183,7 @@ def run(rates=(200, 500, 1000, 2000), duration=10.0, n_meshes=64, capacity_n=512
     t = time.time()
     world = World(ctx, n_meshes)
     build_s = time.time() - t
-    cfg = dict(hbm_budget=64 << 30, max_inflight=2, max_batch_frames=256, max_batch_tris=48 << 20,
+    cfg = dict(hbm_budget=64 << 30, max_inflight=4, max_batch_frames=32, max_batch_tris=48 << 20,
                pairs_per_tri=(tune or {}).get("pairs_per_tri", 4.0))
     cfg.update(server_cfg or {})
     res = dict(meshes=n_meshes, mesh_tris_total=int(sum(world.tris)),
@@ -221,12 +221,14 @@ def main():
     ap.add_argument("--meshes", type=int, default=64)
     ap.add_argument("--capacity-n", type=int, default=512)
     ap.add_argument("--no-plain", action="store_true")
-    ap.add_argument("--inflight", type=int, default=2)
+    ap.add_argument("--inflight", type=int, default=4)
+    ap.add_argument("--max-batch-frames", type=int, default=32)
     ap.add_argument("--tune", default="", help="autotuner JSON to apply")
     a = ap.parse_args()
     tune = json.load(open(a.tune)) if a.tune else None
     rates = [float(x) for x in a.rates.split(",") if x]
-    out = run(rates, a.duration, a.meshes, a.capacity_n, dict(max_inflight=a.inflight),
+    out = run(rates, a.duration, a.meshes, a.capacity_n,
+              dict(max_inflight=a.inflight, max_batch_frames=a.max_batch_frames),
               plain=not a.no_plain, tune=tune)
     print(json.dumps(out), flush=True)
 
