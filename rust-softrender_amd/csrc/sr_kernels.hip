@@ -1126,6 +1126,25 @@ __device__ __forceinline__ float hypot_libm(float a, float b) {
     return (float)sqrt(x * x + y * y);
 }
 
+// Plot calls a line item's walk makes (every plot it may keep): Wu makes
+// 4 endpoint plots and 2 per interior major coordinate xpxl1+1 ..
+// xpxl2-1 (line.rs:159-238, the same rounding as wu_walk); Bresenham makes
+// at most |dx| + |dy| + 1 (each step moves x, y or both).
+__device__ __forceinline__ uint32_t walk_calls_bound(const float c[4], int aa) {
+    if (aa) {
+        double x0 = (double)c[0], y0 = (double)c[1], x1 = (double)c[2], y1 = (double)c[3], t;
+        if (fabs(y1 - y0) > fabs(x1 - x0)) {
+            t = x0; x0 = y0; y0 = t;
+            t = x1; x1 = y1; y1 = t;
+        }
+        if (x0 > x1) { t = x0; x0 = x1; x1 = t; }
+        const double inner = round(x1) - round(x0) - 1.0;
+        return 4u + 2u * (inner > 0.0 ? (uint32_t)inner : 0u);
+    }
+    const int64_t dx = (int64_t)c[2] - (int64_t)c[0], dy = (int64_t)c[3] - (int64_t)c[1];
+    return (uint32_t)((dx < 0 ? -dx : dx) + (dy < 0 ? -dy : dy) + 1);
+}
+
 // Per line: the screen tiles whose closed rectangle the segment reaches
 // (liang_barsky accepts), in the reference's row-major tile order.
 __global__ __launch_bounds__(256) void k_line_tiles(LineSetupParams P, int pass) {
@@ -1182,10 +1201,9 @@ __global__ __launch_bounds__(256) void k_line_tiles(LineSetupParams P, int pass)
                     it.pad = 0;
                     it.mlo = it.mhi = 0;
                     P.items[o + n] = it;
-                    if (P.walk_len) {
-                        const float len = fmaxf(fabsf(c[2] - c[0]), fabsf(c[3] - c[1]));
-                        P.walk_len[o + n] = len < 255.0f ? (uint32_t)len : 255u;
-                    }
+                    const uint32_t ub = walk_calls_bound(c, P.aa);
+                    P.plot_ub[o + n] = ub;
+                    if (P.walk_len) P.walk_len[o + n] = min(ub / 2u, 255u);
                 }
                 n++;
             }
@@ -1284,7 +1302,7 @@ __device__ void wu_walk(double x0, double y0, double x1, double y1, F&& plot, Wu
 // they touch, pass 1 writes them.  A plot is kept when x >= 0 and y >= 0
 // (line.rs:55) and x + y*W lies inside the framebuffer (past its end the
 // reference writes out of bounds; such plots are dropped).
-__global__ __launch_bounds__(256) void k_line_walk(WalkParams P, int pass) {
+__global__ __launch_bounds__(256) void k_line_walk(WalkParams P) {
     const uint32_t t = blockIdx.x * 256u + threadIdx.x;
     if (t >= P.n_items) return;
     // a wave's lanes walk in lock step until the longest walk ends, so the
@@ -1292,7 +1310,7 @@ __global__ __launch_bounds__(256) void k_line_walk(WalkParams P, int pass) {
     const uint32_t it_id = P.order ? P.order[t] : t;
     LineItem& it = P.items[it_id];
     uint32_t n = 0;
-    uint32_t o = pass ? P.plot_offsets[it_id] : 0u;
+    const uint32_t o = P.plot_offsets[it_id], room = P.plot_ub[it_id];
     int bx0 = INT32_MAX, by0 = INT32_MAX, bx1 = -1, by1 = -1;
     // window of execution tiles the plots can reach: the item's screen tile
     // grown by one pixel on each side (Wu plots y+1 next to y)
@@ -1319,7 +1337,11 @@ __global__ __launch_bounds__(256) void k_line_walk(WalkParams P, int pass) {
             if (row < (uint32_t)y) return;  // wrapped past 2^32 rows: beyond the framebuffer
         }
         if (row >= P.H) return;  // index >= W*H
-        if (pass) {
+        if (n >= room) {  // never: the bound counts every call
+            atomicOr(&P.err[0], 4u);
+            return;
+        }
+        {
             Plot pl;
             pl.alpha = alpha;
             pl.index = col + row * P.W;
@@ -1327,7 +1349,8 @@ __global__ __launch_bounds__(256) void k_line_walk(WalkParams P, int pass) {
             pl.y = (int32_t)y;
             pl.row = row;
             P.plots[o + n] = pl;
-        } else {
+        }
+        {
             const int ex = (int)(col >> esh), ey = (int)(row >> esh);
             bx0 = min(bx0, ex); bx1 = max(bx1, ex);
             by0 = min(by0, ey); by1 = max(by1, ey);
@@ -1349,8 +1372,8 @@ __global__ __launch_bounds__(256) void k_line_walk(WalkParams P, int pass) {
         // cast::<f32, i64>(..).unwrap(): truncation (the clipped points are finite)
         bresenham_walk((int64_t)it.cx1, (int64_t)it.cy1, (int64_t)it.cx2, (int64_t)it.cy2, plot);
     }
-    if (pass == 0) {
-        P.plot_counts[it_id] = n;
+    {
+        it.plot_off = o;
         it.plot_cnt = n;
         if (n) {
             it.bx0 = (int16_t)bx0; it.bx1 = (int16_t)bx1; it.by0 = (int16_t)by0; it.by1 = (int16_t)by1;
@@ -1372,8 +1395,6 @@ __global__ __launch_bounds__(256) void k_line_walk(WalkParams P, int pass) {
             it.ranged = 0;
             P.bin_counts[it_id] = 0;
         }
-    } else {
-        it.plot_off = o;
     }
 }
 
@@ -1660,8 +1681,8 @@ void launch_strips(const StripParams& P, hipStream_t s) {
 void launch_line_tiles(const LineSetupParams& P, int pass, hipStream_t s) {
     if (P.n_lines) hipLaunchKernelGGL(k_line_tiles, dim3(blocks_for(P.n_lines, 256)), dim3(256), 0, s, P, pass);
 }
-void launch_line_walk(const WalkParams& P, int pass, hipStream_t s) {
-    if (P.n_items) hipLaunchKernelGGL(k_line_walk, dim3(blocks_for(P.n_items, 256)), dim3(256), 0, s, P, pass);
+void launch_line_walk(const WalkParams& P, hipStream_t s) {
+    if (P.n_items) hipLaunchKernelGGL(k_line_walk, dim3(blocks_for(P.n_items, 256)), dim3(256), 0, s, P);
 }
 void launch_item_emit(const ItemEmitParams& P, hipStream_t s) {
     if (P.n) hipLaunchKernelGGL(k_item_emit, dim3(blocks_for(P.n, 256)), dim3(256), 0, s, P);

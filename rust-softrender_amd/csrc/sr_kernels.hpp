@@ -81,7 +81,9 @@ struct LineSetupParams {
     const uint32_t* offsets;
     LineItem* items;
     uint32_t* err;
-    uint32_t* walk_len;    // pass 1 (optional): per item, its walk length in pixels (<= 255)
+    int aa;                // Wu (1) or Bresenham (0) walks
+    uint32_t* walk_len;    // pass 1 (optional): per item, a sort key for its walk length (<= 255)
+    uint32_t* plot_ub;     // pass 1: per item, an upper bound of its plots (the walk's plot calls)
 };
 
 struct WalkParams {
@@ -92,10 +94,10 @@ struct WalkParams {
     uint32_t ets, ntx, nty;
     uint32_t tw, th, scols;  // screen tiles
     const uint32_t* order;   // optional: thread t walks item order[t] (items of similar length share a wave)
-    uint32_t* plot_counts;   // pass 0: plots per item
-    uint32_t* bin_counts;    // pass 0: execution tiles per item
-    const uint32_t* plot_offsets;  // pass 1
-    Plot* plots;             // pass 1
+    const uint32_t* plot_ub;       // per item: room for its plots (an upper bound)
+    const uint32_t* plot_offsets;  // per item: first plot slot (exclusive scan of plot_ub)
+    uint32_t* bin_counts;    // out: execution tiles per item
+    Plot* plots;             // out
     uint32_t* err;
 };
 
@@ -190,7 +192,7 @@ void launch_raster_tris_64(const RasterBatch& P, uint32_t grid, hipStream_t s);
 
 // lines / points / stencil strips
 void launch_line_tiles(const LineSetupParams& P, int pass, hipStream_t s);
-void launch_line_walk(const WalkParams& P, int pass, hipStream_t s);
+void launch_line_walk(const WalkParams& P, hipStream_t s);
 void launch_item_emit(const ItemEmitParams& P, hipStream_t s);
 void launch_point_setup(const PointSetupParams& P, hipStream_t s);
 void launch_raster_lines(const RasterParams& P, uint32_t ntiles, hipStream_t s);

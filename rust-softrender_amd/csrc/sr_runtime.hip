@@ -1220,7 +1220,14 @@ int draw_lines_impl(sr_context ctx, sr_framebuffer fb, const DrawPlan& pl, const
         (rc = ensure(ctx, ctx->scan_tmp, sizeof(uint32_t) * (scan_tmp_words(radix_hist_words(nitems) + 256) + 16))))
         return rc;
     lp.walk_len = P<uint32_t>(ctx->keys);
+    lp.plot_ub = P<uint32_t>(ctx->lcounts);
+    lp.aa = d->aa_lines ? 1 : 0;
     launch_line_tiles(lp, 1, s);
+    // room for each item's plots (an upper bound: the walk's plot calls)
+    if ((rc = scan_counts(ctx, lp.plot_ub, P<uint32_t>(ctx->loffsets), nitems, 2))) return rc;
+    if ((rc = read_words(ctx, 3))) return rc;
+    const uint32_t nplots = ctx->h_misc[2];
+    if ((rc = ensure(ctx, ctx->plots, sizeof(Plot) * std::max<uint32_t>(nplots, 1)))) return rc;
     launch_iota(P<uint32_t>(ctx->vals), nitems, s);
     const int swapped = radix_sort(P<uint32_t>(ctx->keys), P<uint32_t>(ctx->vals), P<uint32_t>(ctx->keys2),
                                    P<uint32_t>(ctx->vals2), nitems, 8, P<uint32_t>(ctx->hist),
@@ -1231,19 +1238,17 @@ int draw_lines_impl(sr_context ctx, sr_framebuffer fb, const DrawPlan& pl, const
     wp.items = P<LineItem>(ctx->litems); wp.n_items = nitems; wp.W = pl.W; wp.H = pl.H;
     wp.aa = d->aa_lines ? 1 : 0; wp.ets = LP_ETS; wp.ntx = ntx; wp.nty = nty;
     wp.tw = pl.tw; wp.th = pl.th; wp.scols = pl.scols;
-    wp.plot_counts = P<uint32_t>(ctx->lcounts); wp.bin_counts = P<uint32_t>(ctx->pcounts);
+    wp.plot_ub = P<uint32_t>(ctx->lcounts); wp.bin_counts = P<uint32_t>(ctx->pcounts);
     wp.plot_offsets = P<uint32_t>(ctx->loffsets); wp.err = P<uint32_t>(ctx->misc);
-    launch_line_walk(wp, 0, s);
-    if ((rc = scan_counts(ctx, wp.plot_counts, P<uint32_t>(ctx->loffsets), nitems, 2))) return rc;
+    wp.plots = P<Plot>(ctx->plots);
+    launch_line_walk(wp, s);
     if ((rc = scan_counts(ctx, wp.bin_counts, P<uint32_t>(ctx->poffsets), nitems, 3))) return rc;
     mark(ctx, pl.timing, ST_SCAN);
     if ((rc = read_words(ctx, 4))) return rc;
-    const uint32_t nplots = ctx->h_misc[2], npairs = ctx->h_misc[3];
+    if (ctx->h_misc[0] & 4u) return fail(ctx, SR_ERR_DEVICE, "line walk: plot bound exceeded");
+    const uint32_t npairs = ctx->h_misc[3];
     ctx->last.bin_pairs = npairs;
     if (npairs == 0) return SR_OK;
-    if ((rc = ensure(ctx, ctx->plots, sizeof(Plot) * std::max<uint32_t>(nplots, 1)))) return rc;
-    wp.plots = P<Plot>(ctx->plots);
-    launch_line_walk(wp, 1, s);
     if ((rc = ensure_pairs(ctx, npairs))) return rc;
     ItemEmitParams ip;
     ip.items = P<LineItem>(ctx->litems); ip.counts = P<uint32_t>(ctx->pcounts);
