@@ -204,8 +204,8 @@ struct RasterShared {
     static constexpr int PPT = (S * S) / 64;          // pixels per lane
     static constexpr int WPIX = 64 * PPT;             // pixels per wave
     static constexpr int Q = BLEND == SR_BLEND_OVER ? queue_depth<ETS>() : 1;
+    QRec qbuf[NW][WPIX * Q];  // first: queue stores take their offset in the instruction
     float lut[256];
-    QRec qbuf[NW][WPIX * Q];
     uint16_t flat[NW][WPIX * Q];
 };
 
@@ -310,6 +310,19 @@ __device__ __forceinline__ void raster_tile(const RasterParams& P, uint32_t tile
         fy[j] = (float)py[j] + 0.5f;
     }
     QRec* wq = sh.qbuf[wave];
+    // the queue's LDS byte address: slot addresses in the pixel loop are
+    // formed with 32-bit arithmetic (a 64-bit multiply-add otherwise)
+    typedef __attribute__((address_space(3))) QRec LQRec;
+    const uint32_t wq_lds = (uint32_t)(uintptr_t)(LQRec*)wq;
+    uint32_t qlane[PPT];  // each pixel's first queue slot
+#pragma unroll
+    for (int j = 0; j < PPT; j++) qlane[j] = wq_lds + (uint32_t)((lane * PPT + j) * Q) * (uint32_t)sizeof(QRec);
+    auto qput = [&](int j, uint32_t k, uint32_t tri, float u, float v) {  // k < Q: a 24-bit multiply-add
+        LQRec* const q = (LQRec*)(uintptr_t)(qlane[j] + __umul24(k, (uint32_t)sizeof(QRec)));
+        q->tri = tri;
+        q->u = u;
+        q->v = v;
+    };
     uint16_t* wflat = sh.flat[wave];
 
     // Shade every queued fragment of this wave and blend them in order.
@@ -415,16 +428,17 @@ __device__ __forceinline__ void raster_tile(const RasterParams& P, uint32_t tile
                         const bool pass = ok & !(u < 0.0f) & !(v < 0.0f) & !(w < 0.0f) & (z < 0.0f) & (z >= dep[j]);
                         dep[j] = pass ? z : dep[j];
                         if (pass) {
-                            const int slot = (lane * PPT + j) * Q;
                             if constexpr (BLEND == SR_BLEND_OVER) {
-                                wq[slot + qn[j]] = QRec{cur, u, v};
+                                qput(j, qn[j], cur, u, v);
                                 qn[j]++;
-                                full |= qn[j] == (uint32_t)Q;
                             } else {
-                                wq[slot] = QRec{cur, u, v};  // `()` blend: the last fragment wins
+                                qput(j, 0, cur, u, v);  // `()` blend: the last fragment wins
                                 qn[j] = 1;
                             }
                         }
+                        // tested on every lane, outside the branch: the
+                        // ballot below then takes the compare's mask as is
+                        if constexpr (BLEND == SR_BLEND_OVER) full |= qn[j] == (uint32_t)Q;
                     }
                     if constexpr (BLEND == SR_BLEND_OVER) {
                         if (__ballot(full)) {
