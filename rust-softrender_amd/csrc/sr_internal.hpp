@@ -37,6 +37,16 @@ struct alignas(16) TriRec {
 };
 static_assert(sizeof(TriRec) == 80, "TriRec layout");
 
+// Per-triangle binning record: what pair emission and the stencil strips
+// read (the box and the coverable-tile mask), apart from the full record
+// for boxes of more than 32 tiles.  Kept apart from TriRec so those passes
+// stream 20 bytes per triangle instead of 80.
+struct TriBin {
+    int32_t X0, X1, Y0, Y1;  // as in TriRec
+    uint32_t tmask;          // as in TriRec
+};
+static_assert(sizeof(TriBin) == 20, "TriBin layout");
+
 // Per (line, screen tile) work item: the segment clipped to one screen
 // tile by liang_barsky_iterative (geometry/line.rs:6-54), as rasterize_line
 // receives it (line.rs:51-52).  80 bytes.

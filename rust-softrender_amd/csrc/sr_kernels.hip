@@ -438,7 +438,8 @@ struct TileSpan {
     uint32_t ex0, ex1, ey0, ey1;
 };
 
-__device__ __forceinline__ bool tile_span(const TriRec& r, uint32_t Wi, uint32_t Hi, uint32_t ets, TileSpan& s) {
+template <class Box>  // TriRec or TriBin
+__device__ __forceinline__ bool tile_span(const Box& r, uint32_t Wi, uint32_t Hi, uint32_t ets, TileSpan& s) {
     s.px0 = r.X0 > 0 ? r.X0 : 0;
     s.px1 = r.X1 < (int32_t)Wi - 1 ? r.X1 : (int32_t)Wi - 1;
     s.py0 = r.Y0 > 0 ? r.Y0 : 0;
@@ -576,14 +577,15 @@ __device__ __forceinline__ uint32_t tri_setup_one(const VertexBuf& v, uint32_t i
 
 // Writes the (tile, item) pairs of one triangle: every tile of its box, or
 // with culling the tiles setup found coverable (its tile mask, or the test
-// again for boxes of more than 32 tiles), in row-major order.
-__device__ __forceinline__ void emit_pairs(const TriRec& r, const TileSpan& sp, uint32_t ets, uint32_t ntx,
-                                           int empty_cull, uint32_t key_base, uint32_t val, uint32_t* keys,
-                                           uint32_t* vals) {
+// again, on the full record `rec`, for boxes of more than 32 tiles), in
+// row-major order.
+__device__ __forceinline__ void emit_pairs(const TriBin& b, const TriRec* rec, const TileSpan& sp, uint32_t ets,
+                                           uint32_t ntx, int empty_cull, uint32_t key_base, uint32_t val,
+                                           uint32_t* keys, uint32_t* vals) {
     const uint32_t w = sp.ex1 - sp.ex0 + 1, h = sp.ey1 - sp.ey0 + 1;
     uint32_t o = 0;
     if (empty_cull && w * h <= 32) {
-        for (uint32_t m = r.tmask; m; m &= m - 1) {
+        for (uint32_t m = b.tmask; m; m &= m - 1) {
             const uint32_t i = __builtin_ctz(m);
             keys[o] = key_base + (sp.ey0 + i / w) * ntx + sp.ex0 + i % w;
             vals[o] = val;
@@ -592,7 +594,10 @@ __device__ __forceinline__ void emit_pairs(const TriRec& r, const TileSpan& sp, 
         return;
     }
     CoverPlanes cp;
-    if (empty_cull) cp = cover_planes(r, sp);
+    if (empty_cull) {
+        const TriRec r = *rec;
+        cp = cover_planes(r, sp);
+    }
     for (uint32_t ey = sp.ey0; ey <= sp.ey1; ey++)
         for (uint32_t ex = sp.ex0; ex <= sp.ex1; ex++) {
             if (empty_cull && !span_tile_covered(cp, sp, ets, ex, ey)) continue;
@@ -610,10 +615,11 @@ __global__ __launch_bounds__(256) void k_tri_setup(SetupParams P) {
                                 prim_vertex(P.idx, 3 * t + 2), P.cull, P.W, P.H, P.ets, r, P.err,
                                 P.empty_cull);
     P.recs[t] = r;
+    P.bins[t] = TriBin{r.X0, r.X1, r.Y0, r.Y1, r.tmask};
 }
 
 __global__ __launch_bounds__(256) void k_tri_setup_batch(BatchTables T, VertexBuf v, TriRec* recs,
-                                                        uint32_t* counts, uint32_t* err) {
+                                                        TriBin* bins, uint32_t* counts, uint32_t* err) {
     const uint32_t t = blockIdx.x * 256u + threadIdx.x;
     with_scene(T, T.tend, T.nt, t, [&](const SceneGeom& S) {
         const uint32_t l = t - S.tbase;
@@ -622,12 +628,13 @@ __global__ __launch_bounds__(256) void k_tri_setup_batch(BatchTables T, VertexBu
                                   S.vbase + prim_vertex(S.idx, 3 * l + 2), S.cull, S.W, S.H, S.ets, r, err,
                                   S.empty_cull);
         recs[t] = r;
+        bins[t] = TriBin{r.X0, r.X1, r.Y0, r.Y1, r.tmask};
     });
 }
 
 // (execution tile, triangle) pairs of a batch; keys are global tile ids.
 // Pairs past the capacity are not written and raise the overflow flag.
-__global__ __launch_bounds__(256) void k_tri_emit_batch(BatchTables T, const TriRec* recs,
+__global__ __launch_bounds__(256) void k_tri_emit_batch(BatchTables T, const TriRec* recs, const TriBin* bins,
                                                        const uint32_t* counts, const uint32_t* offsets,
                                                        uint32_t cap, uint32_t* keys, uint32_t* vals,
                                                        uint32_t* overflow) {
@@ -640,22 +647,23 @@ __global__ __launch_bounds__(256) void k_tri_emit_batch(BatchTables T, const Tri
         atomicOr(overflow, 1u);
         return;
     }
-    const TriRec r = recs[t];
+    const TriBin b = bins[t];
     const SceneGeom& S = T.scenes[scene_of(T.tend, T.nscenes, t)];
     TileSpan sp;
-    if (!tile_span(r, S.W, S.H, S.ets, sp)) return;
-    emit_pairs(r, sp, S.ets, S.ntx, S.empty_cull, S.tile_base, t, keys + o, vals + o);
+    if (!tile_span(b, S.W, S.H, S.ets, sp)) return;
+    emit_pairs(b, recs + t, sp, S.ets, S.ntx, S.empty_cull, S.tile_base, t, keys + o, vals + o);
 }
 
 __global__ __launch_bounds__(256) void k_tri_emit(EmitParams P) {
     const uint32_t t = blockIdx.x * 256u + threadIdx.x;
     if (t >= P.n) return;
     if (P.counts[t] == 0) return;
-    const TriRec r = P.recs[t];
+    const TriBin b = P.bins[t];
     TileSpan sp;
-    if (!tile_span(r, P.W, P.H, P.ets, sp)) return;
+    if (!tile_span(b, P.W, P.H, P.ets, sp)) return;
     const uint32_t o = P.offsets[t];
-    emit_pairs(r, sp, P.ets, P.ntx, P.empty_cull, P.key_base, P.val_base + t, P.keys + o, P.vals + o);
+    emit_pairs(b, P.recs + t, sp, P.ets, P.ntx, P.empty_cull, P.key_base, P.val_base + t, P.keys + o,
+               P.vals + o);
 }
 
 // ===========================================================================
@@ -961,7 +969,7 @@ __global__ __launch_bounds__(256) void k_strips_accum(StripParams P) {
         else atomicAdd(&corner[a][idx], v);
     };
     const uint32_t t = blockIdx.x * 256u + threadIdx.x;
-    const TriRec* r = t < P.n ? &P.recs[t] : nullptr;
+    const TriBin* r = t < P.n ? &P.bins[t] : nullptr;
     const int X0 = r ? r->X0 : 1, X1 = r ? r->X1 : 0, Y0 = r ? r->Y0 : 1, Y1 = r ? r->Y1 : 0;
     if (X0 <= X1) {  // not culled
         uint32_t nR, jL, nB, iA;  // columns right-of-box-status prefix etc.
@@ -1561,16 +1569,16 @@ void launch_tri_setup(const SetupParams& P, hipStream_t s) {
 void launch_vertex_batch(const BatchTables& T, const VertexBuf& out, hipStream_t s) {
     if (T.nv) hipLaunchKernelGGL(k_vertex_batch, dim3(blocks_for(T.nv, 256)), dim3(256), 0, s, T, out);
 }
-void launch_tri_setup_batch(const BatchTables& T, const VertexBuf& v, TriRec* recs, uint32_t* counts,
-                            uint32_t* err, hipStream_t s) {
+void launch_tri_setup_batch(const BatchTables& T, const VertexBuf& v, TriRec* recs, TriBin* bins,
+                            uint32_t* counts, uint32_t* err, hipStream_t s) {
     if (T.nt) hipLaunchKernelGGL(k_tri_setup_batch, dim3(blocks_for(T.nt, 256)), dim3(256), 0, s, T, v,
-                                 recs, counts, err);
+                                 recs, bins, counts, err);
 }
-void launch_tri_emit_batch(const BatchTables& T, const TriRec* recs, const uint32_t* counts,
+void launch_tri_emit_batch(const BatchTables& T, const TriRec* recs, const TriBin* bins, const uint32_t* counts,
                            const uint32_t* offsets, uint32_t cap, uint32_t* keys, uint32_t* vals,
                            uint32_t* overflow, hipStream_t s) {
     if (T.nt) hipLaunchKernelGGL(k_tri_emit_batch, dim3(blocks_for(T.nt, 256)), dim3(256), 0, s, T, recs,
-                                 counts, offsets, cap, keys, vals, overflow);
+                                 bins, counts, offsets, cap, keys, vals, overflow);
 }
 void launch_tri_emit(const EmitParams& P, hipStream_t s) {
     if (P.n) hipLaunchKernelGGL(k_tri_emit, dim3(blocks_for(P.n, 256)), dim3(256), 0, s, P);

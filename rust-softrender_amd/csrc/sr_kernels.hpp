@@ -35,6 +35,7 @@ struct SetupParams {
     int cull;
     uint32_t W, H, ets;
     TriRec* recs;
+    TriBin* bins;
     uint32_t* counts;
     uint32_t* err;
     int empty_cull;  // bin only execution tiles the triangle may cover (no stencil effects)
@@ -42,6 +43,7 @@ struct SetupParams {
 
 struct EmitParams {
     const TriRec* recs;
+    const TriBin* bins;
     const uint32_t* counts;
     const uint32_t* offsets;
     uint32_t n, W, H, ets, ntx;
@@ -123,7 +125,7 @@ struct PointSetupParams {
 };
 
 struct StripParams {
-    const TriRec* recs;
+    const TriBin* bins;
     uint32_t n;
     uint32_t W, H, tw, th, scols, srows;
     uint32_t* colL;  // [scols+1][H+1]  rows of column tx0_j, difference along y
@@ -149,9 +151,9 @@ void launch_gs_normals(int mode, const GSParams& P, hipStream_t s);
 void launch_tri_setup(const SetupParams& P, hipStream_t s);
 void launch_tri_emit(const EmitParams& P, hipStream_t s);
 void launch_vertex_batch(const BatchTables& T, const VertexBuf& out, hipStream_t s);
-void launch_tri_setup_batch(const BatchTables& T, const VertexBuf& v, TriRec* recs, uint32_t* counts,
-                            uint32_t* err, hipStream_t s);
-void launch_tri_emit_batch(const BatchTables& T, const TriRec* recs, const uint32_t* counts,
+void launch_tri_setup_batch(const BatchTables& T, const VertexBuf& v, TriRec* recs, TriBin* bins,
+                            uint32_t* counts, uint32_t* err, hipStream_t s);
+void launch_tri_emit_batch(const BatchTables& T, const TriRec* recs, const TriBin* bins, const uint32_t* counts,
                            const uint32_t* offsets, uint32_t cap, uint32_t* keys, uint32_t* vals,
                            uint32_t* overflow, hipStream_t s);
 uint32_t scan_tmp_words(uint32_t n);

@@ -139,12 +139,15 @@ int draw_tris(sr_context ctx, sr_framebuffer fb, const DrawPlan& pl, const sr_dr
     int rc;
     hipStream_t s = ctx->stream;
     ctx->last.triangles_setup = ntris;
-    if ((rc = ensure(ctx, ctx->recs, sizeof(TriRec) * std::max<uint32_t>(ntris, 1)))) return rc;
+    // one buffer: the triangle records, then their binning records
+    const uint32_t nrec = std::max<uint32_t>(ntris, 1);
+    if ((rc = ensure(ctx, ctx->recs, (sizeof(TriRec) + sizeof(TriBin)) * nrec))) return rc;
+    TriBin* const bins = (TriBin*)(P<TriRec>(ctx->recs) + nrec);
     if ((rc = ensure(ctx, ctx->counts, sizeof(uint32_t) * (ntris + 1)))) return rc;
     if ((rc = ensure(ctx, ctx->offsets, sizeof(uint32_t) * (ntris + 1)))) return rc;
     SetupParams sp;
     sp.v = vb; sp.idx = idx; sp.n = ntris; sp.cull = d->cull; sp.W = pl.W; sp.H = pl.H;
-    sp.ets = pl.ets; sp.recs = P<TriRec>(ctx->recs); sp.counts = P<uint32_t>(ctx->counts);
+    sp.ets = pl.ets; sp.recs = P<TriRec>(ctx->recs); sp.bins = bins; sp.counts = P<uint32_t>(ctx->counts);
     sp.err = P<uint32_t>(ctx->misc);
     // with no stencil effects, tiles whose pixels the triangle cannot cover
     // are left out of the bins
@@ -194,7 +197,7 @@ int draw_tris(sr_context ctx, sr_framebuffer fb, const DrawPlan& pl, const sr_dr
             (rc = ensure(ctx, ctx->tend, sizeof(uint32_t) * pl.ntiles)))
             return rc;
         EmitParams ep;
-        ep.recs = P<TriRec>(ctx->recs); ep.counts = P<uint32_t>(ctx->counts);
+        ep.recs = P<TriRec>(ctx->recs); ep.bins = bins; ep.counts = P<uint32_t>(ctx->counts);
         ep.offsets = P<uint32_t>(ctx->offsets); ep.n = ntris; ep.W = pl.W; ep.H = pl.H;
         ep.ets = pl.ets; ep.ntx = pl.ntx; ep.key_base = 0; ep.val_base = 0;
         ep.keys = P<uint32_t>(ctx->keys); ep.vals = P<uint32_t>(ctx->vals);
@@ -223,7 +226,7 @@ int draw_tris(sr_context ctx, sr_framebuffer fb, const DrawPlan& pl, const sr_dr
         if ((rc = ensure(ctx, ctx->strip_aux, sizeof(uint32_t) * aux))) return rc;
         StripParams stp;
         uint32_t* a = P<uint32_t>(ctx->strip_aux);
-        stp.recs = P<TriRec>(ctx->recs); stp.n = ntris; stp.W = pl.W; stp.H = pl.H;
+        stp.bins = bins; stp.n = ntris; stp.W = pl.W; stp.H = pl.H;
         stp.tw = pl.tw; stp.th = pl.th; stp.scols = pl.scols; stp.srows = pl.srows;
         stp.colL = a; a += (size_t)(pl.scols + 1) * (pl.H + 1);
         stp.colR = a; a += (size_t)(pl.scols + 1) * (pl.H + 1);
@@ -472,7 +475,7 @@ int batch_enqueue(sr_context ctx, BatchWork& W, const BatchPlan& pl, bool use_gr
     const uint32_t nvb = (uint32_t)std::max<uint64_t>(NV, 1);
     const size_t before = W.device_bytes();
     if ((rc = ensure(ctx, W.verts, sizeof(float) * 10ull * nvb, s)) ||
-        (rc = ensure(ctx, W.recs, sizeof(TriRec) * std::max<uint64_t>(NT, 1), s)) ||
+        (rc = ensure(ctx, W.recs, (sizeof(TriRec) + sizeof(TriBin)) * std::max<uint64_t>(NT, 1), s)) ||
         (rc = ensure(ctx, W.counts, sizeof(uint32_t) * (NT + 1), s)) ||
         (rc = ensure(ctx, W.offsets, sizeof(uint32_t) * (NT + 1), s)) ||
         (rc = ensure(ctx, W.keys, sizeof(uint32_t) * (size_t)cap, s)) ||
@@ -574,9 +577,11 @@ int batch_enqueue(sr_context ctx, BatchWork& W, const BatchPlan& pl, bool use_gr
         BatchTables T{d_geom, d_vend, d_tend, ns, (uint32_t)NV, (uint32_t)NT};
         launch_fill(misc, 0u, 8u, s);
         launch_vertex_batch(T, vb, s);
-        launch_tri_setup_batch(T, vb, P<TriRec>(W.recs), P<uint32_t>(W.counts), misc + 2, s);
+        // the triangle records, then their binning records, in one buffer
+        TriBin* const bins = (TriBin*)(P<TriRec>(W.recs) + std::max<uint64_t>(NT, 1));
+        launch_tri_setup_batch(T, vb, P<TriRec>(W.recs), bins, P<uint32_t>(W.counts), misc + 2, s);
         launch_scan(P<uint32_t>(W.counts), P<uint32_t>(W.offsets), (uint32_t)NT, P<uint32_t>(W.scan), misc + 1, s);
-        launch_tri_emit_batch(T, P<TriRec>(W.recs), P<uint32_t>(W.counts), P<uint32_t>(W.offsets), cap,
+        launch_tri_emit_batch(T, P<TriRec>(W.recs), bins, P<uint32_t>(W.counts), P<uint32_t>(W.offsets), cap,
                               P<uint32_t>(W.keys), P<uint32_t>(W.vals), misc + 0, s);
         // on overflow nothing is sorted or binned (the raster skips the batch
         // and it runs again with a larger capacity)
