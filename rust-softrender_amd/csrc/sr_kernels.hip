@@ -751,29 +751,111 @@ __global__ __launch_bounds__(256) void k_scan_down(const uint32_t* in, uint32_t*
 // ===========================================================================
 // Stable LSD radix sort of (key, value) by key, 8-bit digits.
 // ===========================================================================
+// Sort segments (segmented mode): segment g holds elements [lo[g], hi[g]),
+// its blocks are first[g] .. first[g+1] - 1, its digits are taken from
+// key - kbase[g], and its histograms occupy hist[256 * first[g] ..) digit-
+// major, so one exclusive scan over all of them yields, for every segment,
+// offsets that start at lo[g] (the segments tile [0, live) in order).
+struct SortSegs {
+    const uint32_t* first;  // [nseg + 1]; first[nseg] = blocks of all segments
+    const uint32_t* lo;
+    const uint32_t* hi;
+    const uint32_t* kbase;
+    uint32_t nseg;          // 0: unsegmented
+};
+
+// The elements, histogram column and key base of one block.
+struct BlockSpan {
+    uint32_t lo, hi;       // elements [lo, hi)
+    uint32_t hbase, hstride;  // its count of digit d is hist[d * hstride + hbase]
+    uint32_t kbase;
+    bool live;
+};
+
+__device__ __forceinline__ BlockSpan block_span(uint32_t b, uint32_t n, uint32_t nblocks, const uint32_t* n_dev,
+                                                const SortSegs& S) {
+    BlockSpan r;
+    if (S.nseg == 0) {
+        if (n_dev) {
+            n = min(n, *n_dev);
+            nblocks = blocks_for(n, SCAN_TILE);
+        }
+        r.live = b < nblocks;
+        r.lo = b * SCAN_TILE;
+        r.hi = min(r.lo + SCAN_TILE, n);
+        r.hbase = b;
+        r.hstride = nblocks;
+        r.kbase = 0;
+        return r;
+    }
+    r.live = b < S.first[S.nseg];
+    if (!r.live) return r;
+    uint32_t a = 0, z = S.nseg;  // first[a] <= b < first[z]: the last segment starting at or before b
+    while (z - a > 1) {
+        const uint32_t m = (a + z) >> 1;
+        if (S.first[m] <= b) a = m;
+        else z = m;
+    }
+    const uint32_t i = b - S.first[a];
+    r.lo = S.lo[a] + i * SCAN_TILE;
+    r.hi = min(r.lo + SCAN_TILE, S.hi[a]);
+    r.hbase = 256u * S.first[a] + i;
+    r.hstride = S.first[a + 1] - S.first[a];
+    r.kbase = S.kbase[a];
+    return r;
+}
+
+// The segment table of a batch's groups (one work-group): group g's pairs
+// are those of its triangles [t0, t1), [offsets[t0], offsets[t1]) clamped
+// to the live count; hist_count = 256 * all blocks, the scan's length.
+__global__ __launch_bounds__(256) void k_seg_table(BatchGroups G, const uint32_t* offsets, const uint32_t* live,
+                                                  uint32_t* first, uint32_t* lo, uint32_t* hi, uint32_t* kbase,
+                                                  uint32_t* hist_count) {
+    __shared__ uint32_t lds4[4];
+    const uint32_t L = *live;
+    auto pair_at = [&](uint32_t t) { return t < G.ntris ? min(offsets[t], L) : L; };
+    uint32_t run = 0;
+    for (uint32_t c = 0; c < G.ngroups; c += 256) {
+        const uint32_t g = c + threadIdx.x;
+        uint32_t nb = 0;
+        if (g < G.ngroups) {
+            const uint32_t d0 = G.group_first[g], d1 = G.group_first[g + 1];
+            const uint32_t t0 = d0 ? G.tend[d0 - 1] : 0u, t1 = d1 ? G.tend[d1 - 1] : 0u;
+            lo[g] = pair_at(t0);
+            hi[g] = pair_at(t1);
+            kbase[g] = g ? G.group_tile_end[g - 1] : 0u;
+            nb = blocks_for(hi[g] - lo[g], SCAN_TILE);
+        }
+        uint32_t tot;
+        const uint32_t ex = block_scan_256(nb, tot, lds4);
+        if (g < G.ngroups) first[g] = run + ex;
+        run += tot;
+    }
+    if (threadIdx.x == 0) {
+        first[G.ngroups] = run;
+        *hist_count = 256u * run;
+    }
+}
+
 // Digit histograms, digit-major: hist[d * nb + block].  With a device-side
 // element count the layout uses the live block count nb = ceil(count /
 // SCAN_TILE) (block 0 stores 256 * nb in hist_count for the scan) and
-// blocks past it exit.
+// blocks past it exit.  Segmented: the layout of SortSegs.
 __global__ __launch_bounds__(256) void k_radix_hist(const uint32_t* keys, uint32_t n, int shift,
                                                    uint32_t* hist, uint32_t nblocks,
-                                                   const uint32_t* n_dev, uint32_t* hist_count) {
+                                                   const uint32_t* n_dev, uint32_t* hist_count, SortSegs S) {
     __shared__ uint32_t h[256];
-    if (n_dev) {
-        n = min(n, *n_dev);
-        nblocks = blocks_for(n, SCAN_TILE);
-        if (blockIdx.x == 0 && threadIdx.x == 0) *hist_count = 256u * nblocks;
-    }
-    if (blockIdx.x >= nblocks) return;
+    const BlockSpan B = block_span(blockIdx.x, n, nblocks, n_dev, S);
+    if (S.nseg == 0 && n_dev && blockIdx.x == 0 && threadIdx.x == 0) *hist_count = 256u * B.hstride;
+    if (!B.live) return;
     h[threadIdx.x] = 0;
     __syncthreads();
-    const uint32_t base = blockIdx.x * SCAN_TILE;
     for (int r = 0; r < 16; r++) {
-        const uint32_t e = base + r * 256u + threadIdx.x;
-        if (e < n) atomicAdd(&h[(keys[e] >> shift) & 255u], 1u);
+        const uint32_t e = B.lo + r * 256u + threadIdx.x;
+        if (e < B.hi) atomicAdd(&h[((keys[e] - B.kbase) >> shift) & 255u], 1u);
     }
     __syncthreads();
-    hist[threadIdx.x * nblocks + blockIdx.x] = h[threadIdx.x];
+    hist[threadIdx.x * B.hstride + B.hbase] = h[threadIdx.x];
 }
 
 // Elements are taken in 16 rounds of 256 in index order.  Within a round a
@@ -788,20 +870,18 @@ __global__ __launch_bounds__(256) void k_radix_scatter(const uint32_t* keys_in,
                                                       const uint32_t* vals_in, uint32_t n,
                                                       int shift, const uint32_t* offsets,
                                                       uint32_t nblocks, uint32_t* keys_out,
-                                                      uint32_t* vals_out, const uint32_t* n_dev) {
+                                                      uint32_t* vals_out, const uint32_t* n_dev, SortSegs S) {
     __shared__ uint32_t lkeys[SCAN_TILE], lvals[SCAN_TILE];
     __shared__ uint32_t run[256], lstart[256], gbase[256];
     __shared__ uint32_t wcnt[4][256];
     __shared__ uint32_t lds4[4];
-    if (n_dev) {
-        n = min(n, *n_dev);
-        nblocks = blocks_for(n, SCAN_TILE);
-    }
-    if (blockIdx.x >= nblocks) return;
+    const BlockSpan B = block_span(blockIdx.x, n, nblocks, n_dev, S);
+    if (!B.live) return;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const uint32_t base = blockIdx.x * SCAN_TILE;
-    const uint32_t cnt_blk = min(n - base, (uint32_t)SCAN_TILE);
-    gbase[tid] = offsets[tid * nblocks + blockIdx.x];
+    const uint32_t base = B.lo;
+    n = B.hi;  // elements [base, n)
+    const uint32_t cnt_blk = n - base;
+    gbase[tid] = offsets[tid * B.hstride + B.hbase];
     run[tid] = 0;
     __syncthreads();
     uint32_t K[16], V[16];
@@ -810,7 +890,7 @@ __global__ __launch_bounds__(256) void k_radix_scatter(const uint32_t* keys_in,
         const uint32_t e = base + r * 256u + tid;
         K[r] = e < n ? keys_in[e] : 0u;
         V[r] = e < n ? vals_in[e] : 0u;
-        if (e < n) atomicAdd(&run[(K[r] >> shift) & 255u], 1u);
+        if (e < n) atomicAdd(&run[((K[r] - B.kbase) >> shift) & 255u], 1u);
     }
     __syncthreads();
     uint32_t tot;
@@ -824,7 +904,7 @@ __global__ __launch_bounds__(256) void k_radix_scatter(const uint32_t* keys_in,
         const uint32_t e = base + r * 256u + tid;
         const bool valid = e < n;
         const uint32_t k = K[r];
-        const uint32_t d = (k >> shift) & 255u;
+        const uint32_t d = ((k - B.kbase) >> shift) & 255u;
         uint64_t mask = __ballot(valid);
 #pragma unroll
         for (int b = 0; b < 8; b++) {
@@ -852,7 +932,7 @@ __global__ __launch_bounds__(256) void k_radix_scatter(const uint32_t* keys_in,
     }
     for (uint32_t i = tid; i < cnt_blk; i += 256u) {
         const uint32_t k = lkeys[i];
-        const uint32_t d = (k >> shift) & 255u;
+        const uint32_t d = ((k - B.kbase) >> shift) & 255u;
         const uint32_t pos = gbase[d] + (i - lstart[d]);
         keys_out[pos] = k;
         vals_out[pos] = lvals[i];
@@ -1607,17 +1687,47 @@ int radix_sort(uint32_t* keys, uint32_t* vals, uint32_t* keys_tmp, uint32_t* val
     const uint32_t nb = blocks_for(n, SCAN_TILE);
     const int passes = (key_bits + 7) / 8;
     uint32_t *ki = keys, *vi = vals, *ko = keys_tmp, *vo = vals_tmp;
+    const SortSegs none{nullptr, nullptr, nullptr, nullptr, 0u};
     for (int p = 0; p < passes; p++) {
         const int shift = 8 * p;
-        hipLaunchKernelGGL(k_radix_hist, dim3(nb), dim3(256), 0, s, ki, n, shift, hist, nb, n_dev, total + 1);
+        hipLaunchKernelGGL(k_radix_hist, dim3(nb), dim3(256), 0, s, ki, n, shift, hist, nb, n_dev, total + 1, none);
         launch_scan(hist, hist, 256u * nb, scan_tmp, total, s, n_dev ? total + 1 : nullptr);
         hipLaunchKernelGGL(k_radix_scatter, dim3(nb), dim3(256), 0, s, ki, vi, n, shift,
-                           (const uint32_t*)hist, nb, ko, vo, n_dev);
+                           (const uint32_t*)hist, nb, ko, vo, n_dev, none);
         uint32_t* t;
         t = ki; ki = ko; ko = t;
         t = vi; vi = vo; vo = t;
     }
     return passes & 1;  // 1: result is in the tmp buffers
+}
+
+int radix_sort_groups(uint32_t* keys, uint32_t* vals, uint32_t* keys_tmp, uint32_t* vals_tmp, uint32_t cap,
+                      int local_key_bits, const BatchGroups& G, const uint32_t* tri_offsets,
+                      const uint32_t* live, uint32_t* segs, uint32_t* hist, uint32_t* scan_tmp,
+                      uint32_t* total, hipStream_t s) {
+    if (cap <= 1 || G.ngroups == 0) return 0;
+    // at most one partial block per group beyond the full ones
+    const uint32_t nb = blocks_for(cap, SCAN_TILE) + G.ngroups;
+    const int passes = (local_key_bits + 7) / 8;
+    uint32_t* first = segs;
+    uint32_t* lo = first + G.ngroups + 1;
+    uint32_t* hi = lo + G.ngroups;
+    uint32_t* kb = hi + G.ngroups;
+    uint32_t* hist_count = kb + G.ngroups;
+    hipLaunchKernelGGL(k_seg_table, dim3(1), dim3(256), 0, s, G, tri_offsets, live, first, lo, hi, kb, hist_count);
+    const SortSegs S{first, lo, hi, kb, G.ngroups};
+    uint32_t *ki = keys, *vi = vals, *ko = keys_tmp, *vo = vals_tmp;
+    for (int p = 0; p < passes; p++) {
+        const int shift = 8 * p;
+        hipLaunchKernelGGL(k_radix_hist, dim3(nb), dim3(256), 0, s, ki, cap, shift, hist, nb, live, hist_count, S);
+        launch_scan(hist, hist, 256u * nb, scan_tmp, total, s, hist_count);
+        hipLaunchKernelGGL(k_radix_scatter, dim3(nb), dim3(256), 0, s, ki, vi, cap, shift,
+                           (const uint32_t*)hist, nb, ko, vo, live, S);
+        uint32_t* t;
+        t = ki; ki = ko; ko = t;
+        t = vi; vi = vo; vo = t;
+    }
+    return passes & 1;
 }
 
 void launch_fill(uint32_t* p, uint32_t v, uint32_t n, hipStream_t s) {

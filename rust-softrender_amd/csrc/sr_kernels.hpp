@@ -167,6 +167,25 @@ uint32_t radix_hist_words(uint32_t n);
 int radix_sort(uint32_t* keys, uint32_t* vals, uint32_t* keys_tmp, uint32_t* vals_tmp, uint32_t n,
                int key_bits, uint32_t* hist, uint32_t* scan_tmp, uint32_t* total, hipStream_t s,
                const uint32_t* n_dev = nullptr);
+
+// Segmented sort of a batch's (execution tile, triangle) pairs: the pairs of
+// each framebuffer group are already contiguous and in group order (they
+// are emitted triangle by triangle), so each group's run is sorted apart by
+// its local tile (key - the group's first tile).  That takes the passes the
+// largest group's tile count needs instead of those of the batch's tile
+// count.  `segs` holds 4 * ngroups + 2 words of scratch; the histograms
+// need radix_hist_words(cap) + 256 * (ngroups + 1) words.  Sorts the first
+// *live of `cap` pairs; returns 1 if the result is in the *_tmp buffers.
+struct BatchGroups {
+    const uint32_t* tend;            // per draw: inclusive prefix of triangles
+    const uint32_t* group_first;     // per group: first draw (+1 entry)
+    const uint32_t* group_tile_end;  // per group: end of its execution tiles
+    uint32_t ngroups, ntris;
+};
+int radix_sort_groups(uint32_t* keys, uint32_t* vals, uint32_t* keys_tmp, uint32_t* vals_tmp, uint32_t cap,
+                      int local_key_bits, const BatchGroups& G, const uint32_t* tri_offsets,
+                      const uint32_t* live, uint32_t* segs, uint32_t* hist, uint32_t* scan_tmp,
+                      uint32_t* total, hipStream_t s);
 void launch_fill(uint32_t* p, uint32_t v, uint32_t n, hipStream_t s);
 void launch_iota(uint32_t* p, uint32_t n, hipStream_t s);  // p[i] = i
 // start/end of each key's run in sorted keys; keys outside [key_base,

@@ -354,10 +354,12 @@ size_t BatchWork::device_bytes() const {
 }
 
 size_t batch_bytes_estimate(uint64_t tris, uint64_t verts, double pairs_per_tri) {
-    // screen vertices (10 words), triangle records, counts + offsets, four
-    // pair arrays of the sort, radix histograms; 25% growth headroom
+    // screen vertices (10 words), triangle and binning records, counts +
+    // offsets, four pair arrays of the sort, radix histograms; 25% growth
+    // headroom
     const double pairs = (double)tris * pairs_per_tri + 65536.0;
-    const double b = 40.0 * (double)verts + (double)sizeof(TriRec) * (double)tris + 8.0 * (double)tris +
+    const double b = 40.0 * (double)verts + (double)(sizeof(TriRec) + sizeof(TriBin)) * (double)tris +
+                     8.0 * (double)tris +
                      16.0 * pairs + 4.0 * (double)radix_hist_words((uint32_t)std::min(pairs, 2e9));
     return (size_t)(b * 1.25);
 }
@@ -473,6 +475,15 @@ int batch_enqueue(sr_context ctx, BatchWork& W, const BatchPlan& pl, bool use_gr
     if (W.cap == 0) W.cap = (uint32_t)std::min<uint64_t>(4 * NT + (1u << 16), 0x7FFFFFFFull);
     const uint32_t cap = W.cap;
     const uint32_t nvb = (uint32_t)std::max<uint64_t>(NV, 1);
+    const uint32_t ng = (uint32_t)pl.group_tile_end.size();
+    // bins are sorted group by group on local tile ids: the passes follow
+    // the largest group's tile count
+    uint32_t max_group_tiles = 2;
+    for (uint32_t g = 0; g < ng; g++)
+        max_group_tiles = std::max(max_group_tiles, pl.group_tile_end[g] - (g ? pl.group_tile_end[g - 1] : 0u));
+    const int local_bits = bits_for(max_group_tiles);
+    const uint64_t hist_words = radix_hist_words(cap) + 256ull * ng;
+    const uint64_t seg_words = 4ull * ng + 2;
     const size_t before = W.device_bytes();
     if ((rc = ensure(ctx, W.verts, sizeof(float) * 10ull * nvb, s)) ||
         (rc = ensure(ctx, W.recs, (sizeof(TriRec) + sizeof(TriBin)) * std::max<uint64_t>(NT, 1), s)) ||
@@ -484,14 +495,13 @@ int batch_enqueue(sr_context ctx, BatchWork& W, const BatchPlan& pl, bool use_gr
         (rc = ensure(ctx, W.vals2, sizeof(uint32_t) * (size_t)cap, s)) ||
         (rc = ensure(ctx, W.tstart, sizeof(uint32_t) * std::max<uint64_t>(NTILES, 1), s)) ||
         (rc = ensure(ctx, W.tend, sizeof(uint32_t) * std::max<uint64_t>(NTILES, 1), s)) ||
-        (rc = ensure(ctx, W.hist, sizeof(uint32_t) * (radix_hist_words(cap) + 256), s)) ||
-        (rc = ensure(ctx, W.scan, sizeof(uint32_t) * (scan_tmp_words(std::max<uint64_t>(
-                                      std::max<uint64_t>(radix_hist_words(cap) + 256, NT + 1), NTILES + 1)) + 16), s)) ||
+        (rc = ensure(ctx, W.hist, sizeof(uint32_t) * (hist_words + seg_words + 256), s)) ||
+        (rc = ensure(ctx, W.scan, sizeof(uint32_t) * (scan_tmp_words((uint32_t)std::max<uint64_t>(
+                                      std::max<uint64_t>(hist_words, NT + 1), NTILES + 1)) + 16), s)) ||
         (rc = ensure(ctx, W.misc, sizeof(uint32_t) * 8, s)) ||
         (rc = ensure(ctx, W.cost, sizeof(uint32_t) * (NTILES + 1), s)) ||
         (rc = ensure(ctx, W.parts, sizeof(uint32_t) * (raster_runs_total(pl) + pl.classes.size()), s)))
         return rc;
-    const uint32_t ng = (uint32_t)pl.group_tile_end.size();
     const size_t tbytes = sizeof(SceneGeom) * ns + sizeof(RasterParams) * ns +
                           sizeof(uint32_t) * (2 * ns + 2 * ng + 1) + 64;
     if ((rc = ensure(ctx, W.tables, tbytes, s))) return rc;
@@ -508,7 +518,7 @@ int batch_enqueue(sr_context ctx, BatchWork& W, const BatchPlan& pl, bool use_gr
     uint32_t* misc = P<uint32_t>(W.misc);  // [0] overflow [1] pairs [2] err [3..4] sort [5] tile cost total
                                             // [6] pairs to sort (0 on overflow)
     const VertexBuf vb{P<float>(W.verts), nvb};
-    const int swapped_passes = (bits_for((uint32_t)std::max<uint64_t>(NTILES, 2)) + 7) / 8;
+    const int swapped_passes = (local_bits + 7) / 8;
     const uint32_t* sorted_vals = (swapped_passes & 1) ? P<uint32_t>(W.vals2) : P<uint32_t>(W.vals);
     std::vector<RasterParams> rp(ns);
     for (uint32_t k = 0; k < ns; k++) {
@@ -586,9 +596,11 @@ int batch_enqueue(sr_context ctx, BatchWork& W, const BatchPlan& pl, bool use_gr
         // on overflow nothing is sorted or binned (the raster skips the batch
         // and it runs again with a larger capacity)
         launch_live_pairs(misc, 0, 1, 6, s);
-        const int sw = radix_sort(P<uint32_t>(W.keys), P<uint32_t>(W.vals), P<uint32_t>(W.keys2),
-                                  P<uint32_t>(W.vals2), cap, bits_for((uint32_t)std::max<uint64_t>(NTILES, 2)),
-                                  P<uint32_t>(W.hist), P<uint32_t>(W.scan), misc + 3, s, misc + 6);
+        const BatchGroups BG{d_tend, d_group_first, d_group_tile_end, ng, (uint32_t)NT};
+        const int sw = radix_sort_groups(P<uint32_t>(W.keys), P<uint32_t>(W.vals), P<uint32_t>(W.keys2),
+                                         P<uint32_t>(W.vals2), cap, local_bits, BG, P<uint32_t>(W.offsets),
+                                         misc + 6, P<uint32_t>(W.hist) + hist_words, P<uint32_t>(W.hist),
+                                         P<uint32_t>(W.scan), misc + 3, s);
         const uint32_t* skeys = sw ? P<uint32_t>(W.keys2) : P<uint32_t>(W.keys);
         launch_fill(P<uint32_t>(W.tstart), 0u, (uint32_t)NTILES, s);
         launch_fill(P<uint32_t>(W.tend), 0u, (uint32_t)NTILES, s);
