@@ -974,15 +974,35 @@ __global__ __launch_bounds__(256) void k_partition(const uint32_t* prefix, uint3
 
 // Grid-stride over the sorted keys (the grid is bounded; the live count
 // may be far below the capacity the launch was sized for).
+// Each thread takes four consecutive keys (one 16-byte load when `keys` is
+// 16-byte aligned) and their two neighbours.
 __global__ __launch_bounds__(256) void k_ranges(const uint32_t* keys, uint32_t n, uint32_t key_base,
                                                uint32_t nkeys, uint32_t* start, uint32_t* end,
-                                               const uint32_t* n_dev) {
+                                               const uint32_t* n_dev, int aligned) {
     if (n_dev) n = min(n, *n_dev);
-    for (uint32_t i = blockIdx.x * 256u + threadIdx.x; i < n; i += gridDim.x * 256u) {
-        const uint32_t k = keys[i];
-        if (k - key_base >= nkeys) continue;  // never for complete pairs; guards the tables
-        if (i == 0 || keys[i - 1] != k) start[k - key_base] = i;
-        if (i == n - 1 || keys[i + 1] != k) end[k - key_base] = i + 1;
+    const uint32_t nq = (n + 3u) / 4u;
+    for (uint32_t q = blockIdx.x * 256u + threadIdx.x; q < nq; q += gridDim.x * 256u) {
+        const uint32_t i0 = 4u * q;
+        uint32_t k[4];
+        if (aligned && i0 + 3u < n) {
+            const uint4 v = *(const uint4*)(keys + i0);
+            k[0] = v.x; k[1] = v.y; k[2] = v.z; k[3] = v.w;
+        } else {
+#pragma unroll
+            for (int j = 0; j < 4; j++) k[j] = i0 + j < n ? keys[i0 + j] : 0u;
+        }
+        const uint32_t prev = i0 ? keys[i0 - 1] : 0u;
+        const uint32_t next = i0 + 4u < n ? keys[i0 + 4u] : 0u;
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const uint32_t i = i0 + j;
+            if (i >= n) break;
+            const uint32_t kk = k[j];
+            if (kk - key_base >= nkeys) continue;  // never for complete pairs; guards the tables
+            const uint32_t before = j ? k[j - 1] : prev, after = j < 3 ? k[j + 1] : next;
+            if (i == 0 || before != kk) start[kk - key_base] = i;
+            if (i == n - 1 || after != kk) end[kk - key_base] = i + 1;
+        }
     }
 }
 
@@ -1740,8 +1760,9 @@ void launch_iota(uint32_t* p, uint32_t n, hipStream_t s) {
 
 void launch_ranges(const uint32_t* keys, uint32_t n, uint32_t key_base, uint32_t nkeys, uint32_t* start,
                    uint32_t* end, hipStream_t s, const uint32_t* n_dev) {
-    if (n) hipLaunchKernelGGL(k_ranges, dim3(std::min<uint32_t>(blocks_for(n, 256), 8192u)), dim3(256), 0, s,
-                              keys, n, key_base, nkeys, start, end, n_dev);
+    const int aligned = ((uintptr_t)keys & 15u) == 0;
+    if (n) hipLaunchKernelGGL(k_ranges, dim3(std::min<uint32_t>(blocks_for(n, 1024), 8192u)), dim3(256), 0, s,
+                              keys, n, key_base, nkeys, start, end, n_dev, aligned);
 }
 void launch_live_pairs(uint32_t* misc, int overflow_word, int total_word, int live_word, hipStream_t s) {
     hipLaunchKernelGGL(k_live_pairs, dim3(1), dim3(64), 0, s, misc, overflow_word, total_word, live_word);
