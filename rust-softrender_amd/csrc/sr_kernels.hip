@@ -1052,86 +1052,138 @@ __device__ __forceinline__ void strip_status(int lo, int hi, uint32_t t, uint32_
 // Corner counters live in LDS per work-group when the four arrays fit
 // (STRIP_LDS_CORNERS entries each): most triangles add to the same few
 // corner cells (e.g. the grid origin), which as global atomics serialise.
+// Likewise colR's column 0 and rowB's row 0 (every triangle right of or
+// below some tile adds to them): with STRIP_HOT_GROUPS work-groups looping
+// over the triangles, each accumulates them in LDS and writes its partial
+// row out; k_strips_hot_reduce sums the partials (no global atomics).
 constexpr uint32_t STRIP_LDS_CORNERS = 2048;
+constexpr uint32_t STRIP_LDS_WORDS = 16384;  // 64 KiB
+constexpr uint32_t STRIP_HOT_GROUPS = 512;
+
+__device__ __host__ __forceinline__ uint32_t strip_corner_words(const StripParams& P) {
+    const uint32_t NC = (P.srows + 1) * (P.scols + 1);
+    return NC <= STRIP_LDS_CORNERS ? 4u * NC : 0u;
+}
+
+uint32_t strip_hot_groups(const StripParams& P) {
+    const uint64_t words = (uint64_t)strip_corner_words(P) + P.H + 1 + P.W + 1;
+    if (P.n == 0 || words > STRIP_LDS_WORDS) return 0;
+    return std::min<uint32_t>(blocks_for(P.n, 256), STRIP_HOT_GROUPS);
+}
 
 __global__ __launch_bounds__(256) void k_strips_accum(StripParams P) {
-    extern __shared__ uint32_t sc[];  // 4 x NC corner counters (lds mode)
+    // LDS: 4 x NC corner counters (when they fit), then colR's column 0
+    // (HS words) and rowB's row 0 (WS words) when hot_part is set
+    extern __shared__ uint32_t sc[];
     const uint32_t HS = P.H + 1, WS = P.W + 1, CS = P.scols + 1;
     const uint32_t NC = (P.srows + 1) * CS;
-    const bool lds = NC <= STRIP_LDS_CORNERS;
+    const uint32_t ncw = strip_corner_words(P);
+    const bool lds = ncw != 0;
+    const bool hot = P.hot_part != nullptr;
+    uint32_t* const hR = sc + ncw;  // colR[0][y]
+    uint32_t* const hB = hR + HS;   // rowB[0][x]
     uint32_t* const corner[4] = {P.cLL, P.cLR, P.cRL, P.cRR};
-    if (lds) {
-        for (uint32_t i = threadIdx.x; i < 4 * NC; i += 256) sc[i] = 0;
-        __syncthreads();
-    }
+    const uint32_t nlds = ncw + (hot ? HS + WS : 0u);
+    for (uint32_t i = threadIdx.x; i < nlds; i += 256) sc[i] = 0;
+    __syncthreads();
     auto cadd = [&](int a, uint32_t idx, uint32_t v) {
         if (lds) atomicAdd(&sc[a * NC + idx], v);
         else atomicAdd(&corner[a][idx], v);
     };
-    const uint32_t t = blockIdx.x * 256u + threadIdx.x;
-    const TriBin* r = t < P.n ? &P.bins[t] : nullptr;
-    const int X0 = r ? r->X0 : 1, X1 = r ? r->X1 : 0, Y0 = r ? r->Y0 : 1, Y1 = r ? r->Y1 : 0;
-    if (X0 <= X1) {  // not culled
-        uint32_t nR, jL, nB, iA;  // columns right-of-box-status prefix etc.
-        strip_status(X0, X1, P.tw, P.scols, P.W, nR, jL);
-        strip_status(Y0, Y1, P.th, P.srows, P.H, nB, iA);
-        if (jL < nR) jL = nR;
-        if (iA < nB) iA = nB;
-        const uint32_t xmax = P.W - 1, ymax = P.H - 1;
-        // rows over the box: (L,O) and (R,O) strips on tile columns
-        for (uint32_t i = nB; i < iA; i++) {
-            const int ty0 = (int)(i * P.th), ty1 = (int)min(i * P.th + P.th, ymax);
-            const int ylo = max(Y0, ty0), yhi = min(Y1, ty1);
-            if (ylo > yhi) continue;
-            if (jL < P.scols) {  // columns j >= jL see the box on their left
-                atomicAdd(&P.colL[jL * HS + ylo], 1u);
-                atomicAdd(&P.colL[jL * HS + yhi + 1], 0xFFFFFFFFu);
+    auto radd = [&](uint32_t y, uint32_t v) {  // colR[0 * HS + y]
+        if (hot) atomicAdd(&hR[y], v);
+        else atomicAdd(&P.colR[y], v);
+    };
+    auto badd = [&](uint32_t x, uint32_t v) {  // rowB[0 * WS + x]
+        if (hot) atomicAdd(&hB[x], v);
+        else atomicAdd(&P.rowB[x], v);
+    };
+    // hot mode: a grid-stride loop (the grid is at most STRIP_HOT_GROUPS)
+    const uint32_t stride = hot ? gridDim.x * 256u : 0xFFFFFFFFu;
+    for (uint32_t t = blockIdx.x * 256u + threadIdx.x;; t += stride) {
+        const bool in = t < P.n;
+        if (!hot && !in) break;
+        if (hot && __syncthreads_or(in) == 0) break;
+        const TriBin* r = in ? &P.bins[t] : nullptr;
+        const int X0 = r ? r->X0 : 1, X1 = r ? r->X1 : 0, Y0 = r ? r->Y0 : 1, Y1 = r ? r->Y1 : 0;
+        if (X0 <= X1) {  // not culled
+            uint32_t nR, jL, nB, iA;  // columns right-of-box-status prefix etc.
+            strip_status(X0, X1, P.tw, P.scols, P.W, nR, jL);
+            strip_status(Y0, Y1, P.th, P.srows, P.H, nB, iA);
+            if (jL < nR) jL = nR;
+            if (iA < nB) iA = nB;
+            const uint32_t xmax = P.W - 1, ymax = P.H - 1;
+            // rows over the box: (L,O) and (R,O) strips on tile columns
+            for (uint32_t i = nB; i < iA; i++) {
+                const int ty0 = (int)(i * P.th), ty1 = (int)min(i * P.th + P.th, ymax);
+                const int ylo = max(Y0, ty0), yhi = min(Y1, ty1);
+                if (ylo > yhi) continue;
+                if (jL < P.scols) {  // columns j >= jL see the box on their left
+                    atomicAdd(&P.colL[jL * HS + ylo], 1u);
+                    atomicAdd(&P.colL[jL * HS + yhi + 1], 0xFFFFFFFFu);
+                }
+                if (nR > 0) {  // columns j < nR see it on their right
+                    radd(ylo, 1u);
+                    radd(yhi + 1, 0xFFFFFFFFu);
+                    atomicAdd(&P.colR[nR * HS + ylo], 0xFFFFFFFFu);
+                    atomicAdd(&P.colR[nR * HS + yhi + 1], 1u);
+                }
             }
-            if (nR > 0) {  // columns j < nR see it on their right
-                atomicAdd(&P.colR[0 * HS + ylo], 1u);
-                atomicAdd(&P.colR[0 * HS + yhi + 1], 0xFFFFFFFFu);
-                atomicAdd(&P.colR[nR * HS + ylo], 0xFFFFFFFFu);
-                atomicAdd(&P.colR[nR * HS + yhi + 1], 1u);
+            // columns over the box: (O,A) and (O,B) strips on tile rows
+            for (uint32_t j = nR; j < jL; j++) {
+                const int tx0 = (int)(j * P.tw), tx1 = (int)min(j * P.tw + P.tw, xmax);
+                const int xlo = max(X0, tx0), xhi = min(X1, tx1);
+                if (xlo > xhi) continue;
+                if (iA < P.srows) {
+                    atomicAdd(&P.rowT[iA * WS + xlo], 1u);
+                    atomicAdd(&P.rowT[iA * WS + xhi + 1], 0xFFFFFFFFu);
+                }
+                if (nB > 0) {
+                    badd(xlo, 1u);
+                    badd(xhi + 1, 0xFFFFFFFFu);
+                    atomicAdd(&P.rowB[nB * WS + xlo], 0xFFFFFFFFu);
+                    atomicAdd(&P.rowB[nB * WS + xhi + 1], 1u);
+                }
+            }
+            // corners: 2D difference arrays over the tile grid [row][col]
+            if (jL < P.scols && iA < P.srows) cadd(0, iA * CS + jL, 1u);  // left & above
+            if (jL < P.scols && nB > 0) {                                  // left & below
+                cadd(1, 0 * CS + jL, 1u);
+                cadd(1, nB * CS + jL, 0xFFFFFFFFu);
+            }
+            if (nR > 0 && iA < P.srows) {  // right & above
+                cadd(2, iA * CS + 0, 1u);
+                cadd(2, iA * CS + nR, 0xFFFFFFFFu);
+            }
+            if (nR > 0 && nB > 0) {  // right & below
+                cadd(3, 0, 1u);
+                cadd(3, nB * CS, 0xFFFFFFFFu);
+                cadd(3, nR, 0xFFFFFFFFu);
+                cadd(3, nB * CS + nR, 1u);
             }
         }
-        // columns over the box: (O,A) and (O,B) strips on tile rows
-        for (uint32_t j = nR; j < jL; j++) {
-            const int tx0 = (int)(j * P.tw), tx1 = (int)min(j * P.tw + P.tw, xmax);
-            const int xlo = max(X0, tx0), xhi = min(X1, tx1);
-            if (xlo > xhi) continue;
-            if (iA < P.srows) {
-                atomicAdd(&P.rowT[iA * WS + xlo], 1u);
-                atomicAdd(&P.rowT[iA * WS + xhi + 1], 0xFFFFFFFFu);
-            }
-            if (nB > 0) {
-                atomicAdd(&P.rowB[0 * WS + xlo], 1u);
-                atomicAdd(&P.rowB[0 * WS + xhi + 1], 0xFFFFFFFFu);
-                atomicAdd(&P.rowB[nB * WS + xlo], 0xFFFFFFFFu);
-                atomicAdd(&P.rowB[nB * WS + xhi + 1], 1u);
-            }
-        }
-        // corners: 2D difference arrays over the tile grid [row][col]
-        if (jL < P.scols && iA < P.srows) cadd(0, iA * CS + jL, 1u);  // left & above
-        if (jL < P.scols && nB > 0) {                                  // left & below
-            cadd(1, 0 * CS + jL, 1u);
-            cadd(1, nB * CS + jL, 0xFFFFFFFFu);
-        }
-        if (nR > 0 && iA < P.srows) {  // right & above
-            cadd(2, iA * CS + 0, 1u);
-            cadd(2, iA * CS + nR, 0xFFFFFFFFu);
-        }
-        if (nR > 0 && nB > 0) {  // right & below
-            cadd(3, 0, 1u);
-            cadd(3, nB * CS, 0xFFFFFFFFu);
-            cadd(3, nR, 0xFFFFFFFFu);
-            cadd(3, nB * CS + nR, 1u);
-        }
+        if (!hot) break;
     }
-    if (lds) {  // one global add per non-zero cell and work-group
-        __syncthreads();
+    __syncthreads();
+    if (lds)  // one global add per non-zero cell and work-group
         for (uint32_t i = threadIdx.x; i < 4 * NC; i += 256)
             if (sc[i]) atomicAdd(&corner[i / NC][i % NC], sc[i]);
+    if (hot) {  // this work-group's partial rows
+        uint32_t* const part = P.hot_part + (size_t)blockIdx.x * (HS + WS);
+        for (uint32_t i = threadIdx.x; i < HS + WS; i += 256) part[i] = hR[i];
     }
+}
+
+// colR's column 0 and rowB's row 0 = the sums of the work-groups' partials
+// (no other term adds to them).
+__global__ __launch_bounds__(256) void k_strips_hot_reduce(StripParams P, uint32_t groups) {
+    const uint32_t HS = P.H + 1, WS = P.W + 1;
+    const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+    if (i >= HS + WS) return;
+    uint32_t s = 0;
+    for (uint32_t g = 0; g < groups; g++) s += P.hot_part[(size_t)g * (HS + WS) + i];
+    if (i < HS) P.colR[i] = s;
+    else P.rowB[i - HS] = s;
 }
 
 // In-place inclusive prefix over `len` elements of each of `rows` rows
@@ -1798,8 +1850,12 @@ void launch_raster_tris(const RasterBatch& P, uint32_t ntiles, hipStream_t s) {
 
 void launch_strips(const StripParams& P, hipStream_t s) {
     const uint32_t HS = P.H + 1, WS = P.W + 1, CS = P.scols + 1, RS = P.srows + 1;
-    const size_t corner_lds = RS * CS <= STRIP_LDS_CORNERS ? 4u * RS * CS * sizeof(uint32_t) : 0u;
-    if (P.n) hipLaunchKernelGGL(k_strips_accum, dim3(blocks_for(P.n, 256)), dim3(256), corner_lds, s, P);
+    const uint32_t G = P.hot_part ? strip_hot_groups(P) : 0u;
+    StripParams Q = P;
+    if (G == 0) Q.hot_part = nullptr;
+    const size_t lds_bytes = sizeof(uint32_t) * (strip_corner_words(Q) + (G ? HS + WS : 0u));
+    if (P.n) hipLaunchKernelGGL(k_strips_accum, dim3(G ? G : blocks_for(P.n, 256)), dim3(256), lds_bytes, s, Q);
+    if (G) hipLaunchKernelGGL(k_strips_hot_reduce, dim3(blocks_for(HS + WS, 256)), dim3(256), 0, s, Q, G);
     // colL/colR: prefix along pixel rows (per tile column), then across tile columns
     hipLaunchKernelGGL(k_prefix_rows, dim3(CS), dim3(256), 0, s, P.colL, CS, HS, HS, 1u);
     hipLaunchKernelGGL(k_prefix_rows, dim3(HS), dim3(256), 0, s, P.colL, HS, CS, 1u, HS);

@@ -137,6 +137,10 @@ struct StripParams {
     uint32_t* cRL;
     uint32_t* cRR;
     uint32_t* out;   // per pixel
+    // per work-group partials of colR's column 0 and rowB's row 0 (the
+    // cells nearly every triangle adds to): strip_hot_groups(P) rows of
+    // H + 1 + W + 1 words, or null (then they take global atomics)
+    uint32_t* hot_part;
 };
 
 void launch_vertex(const VSParams& P, hipStream_t s);
@@ -219,5 +223,9 @@ void launch_point_setup(const PointSetupParams& P, hipStream_t s);
 void launch_raster_lines(const RasterParams& P, uint32_t ntiles, hipStream_t s);
 void launch_raster_points(const RasterParams& P, uint32_t ntiles, hipStream_t s);
 void launch_strips(const StripParams& P, hipStream_t s);
+// Work-groups of the strip pass when colR's column 0 and rowB's row 0 are
+// accumulated in LDS (P.hot_part then needs that many rows of H + W + 2
+// words), or 0 when they do not fit.
+uint32_t strip_hot_groups(const StripParams& P);
 
 }  // namespace sr

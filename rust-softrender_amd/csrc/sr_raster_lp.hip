@@ -78,6 +78,59 @@ __device__ __forceinline__ void ordered_rounds(bool pending, int lp, uint32_t ra
     }
 }
 
+// A work-group OR with one barrier: three LDS flags (zero at the start of
+// the kernel, before a barrier) used in turn; each use clears the next
+// flag before its barrier, when every read of that flag (two uses back)
+// has completed.  All threads call it the same number of times.
+struct BlockOr {
+    uint32_t* f;     // 3 LDS words
+    uint32_t k = 0;  // uniform
+    __device__ bool operator()(bool p) {
+        const uint32_t nk = k == 2u ? 0u : k + 1u;
+        if (p) f[k] = 1u;
+        if (threadIdx.x == 0) f[nk] = 0u;
+        __syncthreads();
+        const bool r = f[k] != 0u;
+        k = nk;
+        return r;
+    }
+};
+
+// The same for ranks that are the thread index (lines), with two barriers
+// per round and no clearing: a claim is (round << 8) | (255 - thread), so
+// every round's claims exceed all earlier ones and atomicMax leaves the
+// lowest pending thread of each pixel.  `round` (uniform) carries across
+// calls; claim[] starts at zero and restarts at zero before the 24-bit
+// round number would wrap.
+template <class F>
+__device__ __forceinline__ void ordered_rounds_t(bool pending, int lp, uint32_t& round, uint32_t* claim,
+                                                 BlockOr& any, F&& apply) {
+    const uint32_t me = 255u - (uint32_t)threadIdx.x;
+    auto bid = [&]() -> uint32_t {
+        if (round == 0xFFFFFFu) {
+            __syncthreads();
+            claim[threadIdx.x] = 0u;
+            __syncthreads();
+            round = 0u;
+        }
+        round++;
+        const uint32_t v = (round << 8) | me;
+        if (pending) atomicMax(&claim[lp], v);
+        return v;
+    };
+    uint32_t v = bid();
+    if (!any(pending)) return;  // its barrier also makes the claims visible
+    for (;;) {
+        if (pending && claim[lp] == v) {
+            apply();
+            pending = false;
+        }
+        if (!any(pending)) return;  // claims read, pixels written
+        v = bid();
+        __syncthreads();
+    }
+}
+
 // Exclusive prefix sum over the work-group's 256 threads (in thread order).
 __device__ __forceinline__ uint32_t lp_scan(uint32_t v, uint32_t& total, uint32_t* lds4) {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -116,7 +169,10 @@ __global__ __launch_bounds__(256) void k_raster_lines(RasterParams P) {
     __shared__ float i_cx1[256], i_cy1[256], i_dlen[256], i_zs[256], i_ze[256];
     __shared__ uint32_t i_v0[256], i_v1[256], i_off[256], i_end[256];  // i_end: inclusive plot prefix
     __shared__ uint32_t i_skip[256];  // interior plots before this tile's (ranged items)
+    __shared__ uint32_t orf[3];
     const int t = threadIdx.x;
+    uint32_t round = 0;  // ordered_rounds_t's claim rounds
+    BlockOr any{orf};
     const uint32_t tile = xcd_remap(blockIdx.x, gridDim.x);
     const uint32_t etx = tile % P.ntx, ety = tile / P.ntx;
     const uint32_t beg = P.tile_start[tile], end = P.tile_end[tile];
@@ -126,7 +182,8 @@ __global__ __launch_bounds__(256) void k_raster_lines(RasterParams P) {
     bool live;
     load_tile(P, etx, ety, S, gx, gy, live);
     slut[t] = (float)t / 255.0f;
-    claim[t] = 0xFFFFFFFFu;
+    claim[t] = 0u;
+    if (t < 3) orf[t] = 0u;
     __syncthreads();
     const Lut lut{slut};
     const int ex0 = (int)(etx * LP_ETS), ey0 = (int)(ety * LP_ETS);
@@ -205,7 +262,7 @@ __global__ __launch_bounds__(256) void k_raster_lines(RasterParams P) {
                         lp = py * LP_ETS + px;
                     }
                 }
-                ordered_rounds(pending, lp, (uint32_t)t, claim, [&]() {
+                ordered_rounds_t(pending, lp, round, claim, any, [&]() {
                     // the rasterize_fragment closure, line.rs:54-109
                     const uint32_t sv = s_st[lp];
                     if (!stencil_test(P.s_test, sv, P.s_ref)) return;

@@ -223,11 +223,16 @@ int draw_tris(sr_context ctx, sr_framebuffer fb, const DrawPlan& pl, const sr_dr
         if ((rc = ensure(ctx, ctx->strips, sizeof(uint32_t) * (size_t)pl.W * pl.H))) return rc;
         const size_t aux = (size_t)(pl.scols + 1) * (pl.H + 1) * 2 + (size_t)(pl.srows + 1) * (pl.W + 1) * 2 +
                            (size_t)(pl.srows + 1) * (pl.scols + 1) * 4;
-        if ((rc = ensure(ctx, ctx->strip_aux, sizeof(uint32_t) * aux))) return rc;
         StripParams stp;
-        uint32_t* a = P<uint32_t>(ctx->strip_aux);
         stp.bins = bins; stp.n = ntris; stp.W = pl.W; stp.H = pl.H;
         stp.tw = pl.tw; stp.th = pl.th; stp.scols = pl.scols; stp.srows = pl.srows;
+        // after the difference arrays: the work-groups' partial hot rows
+        // (written whole by each work-group, so not cleared)
+        const uint32_t hot_groups = strip_hot_groups(stp);
+        const size_t hot_words = (size_t)hot_groups * (pl.H + 1 + pl.W + 1);
+        if ((rc = ensure(ctx, ctx->strip_aux, sizeof(uint32_t) * (aux + hot_words)))) return rc;
+        uint32_t* a = P<uint32_t>(ctx->strip_aux);
+        stp.hot_part = hot_groups ? a + aux : nullptr;
         stp.colL = a; a += (size_t)(pl.scols + 1) * (pl.H + 1);
         stp.colR = a; a += (size_t)(pl.scols + 1) * (pl.H + 1);
         stp.rowT = a; a += (size_t)(pl.srows + 1) * (pl.W + 1);

@@ -53,6 +53,23 @@ def test_stencil_always(ctx, op, wrap):
     assert_parity(render_gpu(ctx, s, ov), ref)
 
 
+def test_stencil_strips_many_triangles(ctx):
+    # more triangles than one pass of the strip kernel's work-groups takes
+    # (512 x 256): each work-group accumulates several rounds in LDS
+    s = scenes.soup_scene(150_000, 320, 200, seed=21)
+    ov = dict(stencil_test=0, stencil_op=4, stencil_wrap=1, stencil_ref=0, tile_w=64, tile_h=48)
+    ref, _ = render_ref(s, ov)
+    assert_parity(render_gpu(ctx, s, ov), ref)
+
+
+def test_stencil_strips_wide_framebuffer(ctx):
+    # W + H too large for the LDS copies of the strip rows: global atomics
+    s = scenes.soup_scene(1500, 16500, 6, seed=22)
+    ov = dict(stencil_test=0, stencil_op=4, stencil_wrap=1, stencil_ref=0, tile_w=64, tile_h=4)
+    ref, _ = render_ref(s, ov)
+    assert_parity(render_gpu(ctx, s, ov), ref)
+
+
 @pytest.mark.parametrize("test,op", [(2, 5), (6, 3), (7, 4), (4, 0)])
 def test_stencil_value_tests(ctx, test, op):
     s = scenes.soup_scene(800, 160, 96, seed=14)
