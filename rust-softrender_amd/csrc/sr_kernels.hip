@@ -858,53 +858,58 @@ __global__ __launch_bounds__(256) void k_radix_hist(const uint32_t* keys, uint32
     hist[threadIdx.x * B.hstride + B.hbase] = h[threadIdx.x];
 }
 
-// Elements are taken in 16 rounds of 256 in index order.  Within a round a
-// lane's rank among equal digits comes from 8 ballots (a wave-wide match on
-// the digit), the waves before it from per-wave digit counts in LDS; the
-// element goes to its digit's run in an LDS copy of the block, which is
-// therefore the block stably sorted by digit.  The copy is then written
-// out with consecutive threads on consecutive addresses of each digit's
-// run (the digit's global offset for this block comes from the scanned
+// Each wave ranks a contiguous quarter of the block (1024 elements, 16
+// rounds of 64 in index order) against its own digit counters in LDS: a
+// lane's rank among equal digits in a round comes from 8 ballots (a
+// wave-wide match on the digit), the earlier rounds from the wave's
+// counters, which only the wave touches (its LDS operations run in order,
+// so the rounds need no barrier).  Then the waves' counts give each
+// digit's start in the block and each wave's offset within it, and every
+// element goes to its place in an LDS copy of the block, which is
+// therefore the block stably sorted by digit.  The copy is written out
+// with consecutive threads on consecutive addresses of each digit's run
+// (the digit's global offset for this block comes from the scanned
 // histograms), so the writes coalesce instead of scattering 4-byte words.
+// Orders LDS accesses of different lanes of one wave (no work-group barrier).
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 __global__ __launch_bounds__(256) void k_radix_scatter(const uint32_t* keys_in,
                                                       const uint32_t* vals_in, uint32_t n,
                                                       int shift, const uint32_t* offsets,
                                                       uint32_t nblocks, uint32_t* keys_out,
                                                       uint32_t* vals_out, const uint32_t* n_dev, SortSegs S) {
     __shared__ uint32_t lkeys[SCAN_TILE], lvals[SCAN_TILE];
-    __shared__ uint32_t run[256], lstart[256], gbase[256];
-    __shared__ uint32_t wcnt[4][256];
+    __shared__ uint32_t lstart[256], gbase[256];
+    __shared__ uint32_t wcnt[4][256];  // per wave: digit counts, then the wave's offset in each digit
     __shared__ uint32_t lds4[4];
     const BlockSpan B = block_span(blockIdx.x, n, nblocks, n_dev, S);
     if (!B.live) return;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const uint32_t base = B.lo;
-    n = B.hi;  // elements [base, n)
-    const uint32_t cnt_blk = n - base;
+    const uint32_t cnt_blk = B.hi - B.lo;
+    const uint32_t wbase = B.lo + (uint32_t)wave * (SCAN_TILE / 4);
     gbase[tid] = offsets[tid * B.hstride + B.hbase];
-    run[tid] = 0;
-    __syncthreads();
-    uint32_t K[16], V[16];
+    uint32_t* const mine = wcnt[wave];
 #pragma unroll
-    for (int r = 0; r < 16; r++) {
-        const uint32_t e = base + r * 256u + tid;
-        K[r] = e < n ? keys_in[e] : 0u;
-        V[r] = e < n ? vals_in[e] : 0u;
-        if (e < n) atomicAdd(&run[((K[r] - B.kbase) >> shift) & 255u], 1u);
-    }
-    __syncthreads();
-    uint32_t tot;
-    const uint32_t st = block_scan_256(run[tid], tot, lds4);
-    lstart[tid] = st;
-    __syncthreads();
-    run[tid] = st;
+    for (int q = 0; q < 4; q++) mine[lane + 64 * q] = 0;
+    wave_lds_sync();
     const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    uint32_t K[16], V[16], R[16];
 #pragma unroll
     for (int r = 0; r < 16; r++) {
-        const uint32_t e = base + r * 256u + tid;
-        const bool valid = e < n;
-        const uint32_t k = K[r];
-        const uint32_t d = ((k - B.kbase) >> shift) & 255u;
+        const uint32_t e = wbase + r * 64u + lane;
+        const bool valid = e < B.hi;
+        K[r] = valid ? keys_in[e] : 0u;
+        V[r] = valid ? vals_in[e] : 0u;
+    }
+#pragma unroll
+    for (int r = 0; r < 16; r++) {
+        const uint32_t e = wbase + r * 64u + lane;
+        const bool valid = e < B.hi;
+        const uint32_t d = ((K[r] - B.kbase) >> shift) & 255u;
         uint64_t mask = __ballot(valid);
 #pragma unroll
         for (int b = 0; b < 8; b++) {
@@ -913,23 +918,38 @@ __global__ __launch_bounds__(256) void k_radix_scatter(const uint32_t* keys_in,
             mask &= bit ? bal : ~bal;
         }
         const uint32_t rank = __popcll(mask & lt);
-        const uint32_t cnt = __popcll(mask);
+        // the counter is read by every lane of the digit before its first
+        // lane (in a later LDS instruction of this wave) advances it
+        const uint32_t prior = valid ? mine[d] : 0u;
+        R[r] = prior + rank;
+        wave_lds_sync();
+        if (valid && rank == 0) mine[d] = prior + (uint32_t)__popcll(mask);
+        wave_lds_sync();
+    }
+    __syncthreads();
+    // thread tid = digit: block total, the waves' exclusive offsets, the
+    // digit's start in the block
+    uint32_t tot = 0;
 #pragma unroll
-        for (int w = 0; w < 4; w++) wcnt[w][tid] = 0;
-        __syncthreads();
-        if (valid && rank == 0) wcnt[wave][d] = cnt;
-        __syncthreads();
-        if (valid) {
-            uint32_t pre = 0;
-            for (int w = 0; w < wave; w++) pre += wcnt[w][d];
-            const uint32_t pos = run[d] + pre + rank;
-            lkeys[pos] = k;
+    for (int w = 0; w < 4; w++) {
+        const uint32_t c = wcnt[w][tid];
+        wcnt[w][tid] = tot;
+        tot += c;
+    }
+    uint32_t all;
+    lstart[tid] = block_scan_256(tot, all, lds4);
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < 16; r++) {
+        const uint32_t e = wbase + r * 64u + lane;
+        if (e < B.hi) {
+            const uint32_t d = ((K[r] - B.kbase) >> shift) & 255u;
+            const uint32_t pos = lstart[d] + mine[d] + R[r];
+            lkeys[pos] = K[r];
             lvals[pos] = V[r];
         }
-        __syncthreads();
-        run[tid] += wcnt[0][tid] + wcnt[1][tid] + wcnt[2][tid] + wcnt[3][tid];
-        __syncthreads();
     }
+    __syncthreads();
     for (uint32_t i = tid; i < cnt_blk; i += 256u) {
         const uint32_t k = lkeys[i];
         const uint32_t d = ((k - B.kbase) >> shift) & 255u;
