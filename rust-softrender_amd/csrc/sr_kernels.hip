@@ -607,19 +607,50 @@ __device__ __forceinline__ void emit_pairs(const TriBin& b, const TriRec* rec, c
         }
 }
 
+// Setup records of one work-group, staged in LDS so that they leave with
+// consecutive threads on consecutive 16-byte (records) and 4-byte (box
+// records) addresses rather than one 80-byte record per thread.
+struct SetupStage {
+    TriRec rec[256];
+    TriBin bin[256];
+};
+
+__device__ __forceinline__ void stage_setup(SetupStage& st, const TriRec& r) {
+    st.rec[threadIdx.x] = r;
+    st.bin[threadIdx.x] = TriBin{r.X0, r.X1, r.Y0, r.Y1, r.tmask};
+}
+
+// Every thread of the work-group calls it (after staging its record, if any).
+__device__ __forceinline__ void store_setup(const SetupStage& st, TriRec* recs, TriBin* bins, uint32_t n) {
+    __syncthreads();
+    const uint32_t b0 = blockIdx.x * 256u;
+    if (b0 >= n) return;
+    const uint32_t m = min(256u, n - b0);
+    constexpr uint32_t RW = sizeof(TriRec) / 16, BW = sizeof(TriBin) / 4;
+    const uint4* rs = (const uint4*)st.rec;
+    uint4* rd = (uint4*)(recs + b0);
+    for (uint32_t i = threadIdx.x; i < m * RW; i += 256u) rd[i] = rs[i];
+    const uint32_t* bs = (const uint32_t*)st.bin;
+    uint32_t* bd = (uint32_t*)(bins + b0);
+    for (uint32_t i = threadIdx.x; i < m * BW; i += 256u) bd[i] = bs[i];
+}
+
 __global__ __launch_bounds__(256) void k_tri_setup(SetupParams P) {
+    __shared__ SetupStage st;
     const uint32_t t = blockIdx.x * 256u + threadIdx.x;
-    if (t >= P.n) return;
-    TriRec r;
-    P.counts[t] = tri_setup_one(P.v, prim_vertex(P.idx, 3 * t), prim_vertex(P.idx, 3 * t + 1),
-                                prim_vertex(P.idx, 3 * t + 2), P.cull, P.W, P.H, P.ets, r, P.err,
-                                P.empty_cull);
-    P.recs[t] = r;
-    P.bins[t] = TriBin{r.X0, r.X1, r.Y0, r.Y1, r.tmask};
+    if (t < P.n) {
+        TriRec r;
+        P.counts[t] = tri_setup_one(P.v, prim_vertex(P.idx, 3 * t), prim_vertex(P.idx, 3 * t + 1),
+                                    prim_vertex(P.idx, 3 * t + 2), P.cull, P.W, P.H, P.ets, r, P.err,
+                                    P.empty_cull);
+        stage_setup(st, r);
+    }
+    store_setup(st, P.recs, P.bins, P.n);
 }
 
 __global__ __launch_bounds__(256) void k_tri_setup_batch(BatchTables T, VertexBuf v, TriRec* recs,
                                                         TriBin* bins, uint32_t* counts, uint32_t* err) {
+    __shared__ SetupStage st;
     const uint32_t t = blockIdx.x * 256u + threadIdx.x;
     with_scene(T, T.tend, T.nt, t, [&](const SceneGeom& S) {
         const uint32_t l = t - S.tbase;
@@ -627,9 +658,9 @@ __global__ __launch_bounds__(256) void k_tri_setup_batch(BatchTables T, VertexBu
         counts[t] = tri_setup_one(v, S.vbase + prim_vertex(S.idx, 3 * l), S.vbase + prim_vertex(S.idx, 3 * l + 1),
                                   S.vbase + prim_vertex(S.idx, 3 * l + 2), S.cull, S.W, S.H, S.ets, r, err,
                                   S.empty_cull);
-        recs[t] = r;
-        bins[t] = TriBin{r.X0, r.X1, r.Y0, r.Y1, r.tmask};
+        stage_setup(st, r);
     });
+    store_setup(st, recs, bins, T.nt);
 }
 
 // (execution tile, triangle) pairs of a batch; keys are global tile ids.
