@@ -762,20 +762,46 @@ __global__ __launch_bounds__(256) void k_scan_top(uint32_t* sums, uint32_t n, ui
     if (threadIdx.x == 0 && total_out) *total_out = run;
 }
 
+// Each thread scans 16 consecutive elements in registers (16-byte loads and
+// stores when the arrays are 16-byte aligned), so the block needs one
+// block scan of the threads' totals.  In place is fine: a thread writes
+// only the elements it read.
 __global__ __launch_bounds__(256) void k_scan_down(const uint32_t* in, uint32_t* out, uint32_t n,
                                                   const uint32_t* block_off, const uint32_t* count) {
     __shared__ uint32_t lds4[4];
     n = live_count(n, count);
     if (blockIdx.x >= blocks_for(n, SCAN_TILE)) return;
-    const uint32_t base = blockIdx.x * SCAN_TILE;
-    uint32_t run = block_off[blockIdx.x];
-    for (int r = 0; r < 16; r++) {
-        const uint32_t e = base + r * 256u + threadIdx.x;
-        const uint32_t v = e < n ? in[e] : 0u;
-        uint32_t tot;
-        const uint32_t ex = block_scan_256(v, tot, lds4);
-        if (e < n) out[e] = run + ex;
-        run += tot;
+    const uint32_t e0 = blockIdx.x * SCAN_TILE + threadIdx.x * 16u;
+    const bool vec = ((((uintptr_t)in) | ((uintptr_t)out)) & 15u) == 0 && e0 + 15u < n;
+    uint32_t v[16];
+    if (vec) {
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            const uint4 x = ((const uint4*)(in + e0))[q];
+            v[4 * q] = x.x; v[4 * q + 1] = x.y; v[4 * q + 2] = x.z; v[4 * q + 3] = x.w;
+        }
+    } else {
+#pragma unroll
+        for (int j = 0; j < 16; j++) v[j] = e0 + j < n ? in[e0 + j] : 0u;
+    }
+    uint32_t local = 0;
+#pragma unroll
+    for (int j = 0; j < 16; j++) {
+        const uint32_t x = v[j];
+        v[j] = local;
+        local += x;
+    }
+    uint32_t tot;
+    const uint32_t off = block_off[blockIdx.x] + block_scan_256(local, tot, lds4);
+    if (vec) {
+#pragma unroll
+        for (int q = 0; q < 4; q++)
+            ((uint4*)(out + e0))[q] = make_uint4(off + v[4 * q], off + v[4 * q + 1], off + v[4 * q + 2],
+                                                 off + v[4 * q + 3]);
+    } else {
+#pragma unroll
+        for (int j = 0; j < 16; j++)
+            if (e0 + j < n) out[e0 + j] = off + v[j];
     }
 }
 
