@@ -665,36 +665,148 @@ __global__ __launch_bounds__(256) void k_tri_setup_batch(BatchTables T, VertexBu
 
 // (execution tile, triangle) pairs of a batch; keys are global tile ids.
 // Pairs past the capacity are not written and raise the overflow flag.
+// Orders LDS accesses of different lanes of one wave (no work-group barrier).
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Cooperative emission.  A lane's triangle is "direct" when its pairs
+// follow from its box alone: every tile of the box (no culling), or the
+// set bits of its tile mask (culling, boxes of at most 32 tiles).  The
+// wave writes the direct triangles' pairs together, 64 consecutive pairs
+// per step (each lane finds the pair's triangle by binary search over the
+// wave's count prefix), so the writes coalesce and no lane waits on a
+// neighbour's larger count; the rest (culled boxes of more than 32 tiles)
+// run emit_pairs on their own lane.
+struct EmitWave {
+    uint32_t pre[64], cnt[64], o[64], mask[64], ex0[64], ey0[64], w[64], ntx[64], kbase[64], val[64];
+};
+
+// Index of the j-th set bit (j < popcount(m)).
+__device__ __forceinline__ uint32_t select_bit(uint32_t m, uint32_t j) {
+    uint32_t pos = 0;
+#pragma unroll
+    for (uint32_t step = 16; step >= 1; step >>= 1) {
+        const uint32_t c = (uint32_t)__popc(m & ((1u << step) - 1u));
+        if (j >= c) {
+            j -= c;
+            m >>= step;
+            pos += step;
+        }
+    }
+    return pos;
+}
+
+__device__ __forceinline__ uint32_t wave_prefix_excl(uint32_t v, uint32_t& total) {
+    const int lane = threadIdx.x & 63;
+    uint32_t x = v;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t y = __shfl_up(x, o, 64);
+        if (lane >= o) x += y;
+    }
+    total = __shfl(x, 63, 64);
+    return x - v;
+}
+
+// All lanes of the wave call it.  mask = 0xFFFFFFFF with all = true: every
+// tile of the box in row-major order.
+__device__ __forceinline__ void emit_wave(EmitWave& e, bool direct, uint32_t c, uint32_t o, uint32_t mask,
+                                          bool all, const TileSpan& sp, uint32_t ntx, uint32_t kbase,
+                                          uint32_t val, uint32_t* keys, uint32_t* vals) {
+    const int lane = threadIdx.x & 63;
+    const uint32_t cd = direct ? c : 0u;
+    uint32_t total;
+    const uint32_t ex = wave_prefix_excl(cd, total);
+    e.pre[lane] = ex + cd;
+    e.cnt[lane] = cd;
+    e.o[lane] = o;
+    e.mask[lane] = all ? 0u : mask;  // 0: every tile
+    e.ex0[lane] = sp.ex0;
+    e.ey0[lane] = sp.ey0;
+    e.w[lane] = sp.ex1 - sp.ex0 + 1;
+    e.ntx[lane] = ntx;
+    e.kbase[lane] = kbase;
+    e.val[lane] = val;
+    wave_lds_sync();
+    for (uint32_t k = lane; k < total; k += 64) {
+        uint32_t lo = 0, hi = 63;  // the first lane whose inclusive prefix exceeds k
+        while (lo < hi) {
+            const uint32_t m = (lo + hi) >> 1;
+            if (e.pre[m] > k) hi = m;
+            else lo = m + 1;
+        }
+        const uint32_t j = k - (e.pre[lo] - e.cnt[lo]);
+        const uint32_t mk = e.mask[lo];
+        const uint32_t i = mk ? select_bit(mk, j) : j;
+        const uint32_t w = e.w[lo];
+        const uint32_t at = e.o[lo] + j;
+        keys[at] = e.kbase[lo] + (e.ey0[lo] + i / w) * e.ntx[lo] + e.ex0[lo] + i % w;
+        vals[at] = e.val[lo];
+    }
+    wave_lds_sync();
+}
+
 __global__ __launch_bounds__(256) void k_tri_emit_batch(BatchTables T, const TriRec* recs, const TriBin* bins,
                                                        const uint32_t* counts, const uint32_t* offsets,
                                                        uint32_t cap, uint32_t* keys, uint32_t* vals,
                                                        uint32_t* overflow) {
+    __shared__ EmitWave ew[4];
     const uint32_t t = blockIdx.x * 256u + threadIdx.x;
-    if (t >= T.nt) return;
-    const uint32_t c = counts[t];
-    if (c == 0) return;
-    uint32_t o = offsets[t];
-    if ((uint64_t)o + c > cap) {
+    if (__builtin_amdgcn_readfirstlane(blockIdx.x * 256u + (threadIdx.x & ~63u)) >= T.nt) return;  // whole wave
+    uint32_t c = t < T.nt ? counts[t] : 0u;
+    const uint32_t o = c ? offsets[t] : 0u;
+    if (c && (uint64_t)o + c > cap) {
         atomicOr(overflow, 1u);
-        return;
+        c = 0;
     }
-    const TriBin b = bins[t];
-    const SceneGeom& S = T.scenes[scene_of(T.tend, T.nscenes, t)];
-    TileSpan sp;
-    if (!tile_span(b, S.W, S.H, S.ets, sp)) return;
-    emit_pairs(b, recs + t, sp, S.ets, S.ntx, S.empty_cull, S.tile_base, t, keys + o, vals + o);
+    TriBin b{1, 0, 1, 0, 0u};
+    TileSpan sp{0, 0, 0, 0, 0u, 0u, 0u, 0u};
+    bool direct = false, all = false;
+    uint32_t ntx = 0, kbase = 0;
+    int ets = 8, empty_cull = 0;
+    if (c) {
+        b = bins[t];
+        const SceneGeom& S = T.scenes[scene_of(T.tend, T.nscenes, t)];
+        ntx = S.ntx; kbase = S.tile_base; ets = (int)S.ets; empty_cull = S.empty_cull;
+        if (tile_span(b, S.W, S.H, S.ets, sp)) {
+            const uint32_t nt = (sp.ex1 - sp.ex0 + 1) * (sp.ey1 - sp.ey0 + 1);
+            all = !empty_cull;
+            direct = all || nt <= 32;
+        } else {
+            c = 0;
+        }
+    }
+    emit_wave(ew[threadIdx.x >> 6], direct, c, o, b.tmask, all, sp, ntx, kbase, t, keys, vals);
+    if (c && !direct)
+        emit_pairs(b, recs + t, sp, (uint32_t)ets, ntx, empty_cull, kbase, t, keys + o, vals + o);
 }
 
 __global__ __launch_bounds__(256) void k_tri_emit(EmitParams P) {
+    __shared__ EmitWave ew[4];
     const uint32_t t = blockIdx.x * 256u + threadIdx.x;
-    if (t >= P.n) return;
-    if (P.counts[t] == 0) return;
-    const TriBin b = P.bins[t];
-    TileSpan sp;
-    if (!tile_span(b, P.W, P.H, P.ets, sp)) return;
-    const uint32_t o = P.offsets[t];
-    emit_pairs(b, P.recs + t, sp, P.ets, P.ntx, P.empty_cull, P.key_base, P.val_base + t, P.keys + o,
-               P.vals + o);
+    if (__builtin_amdgcn_readfirstlane(blockIdx.x * 256u + (threadIdx.x & ~63u)) >= P.n) return;  // whole wave
+    uint32_t c = t < P.n ? P.counts[t] : 0u;
+    TriBin b{1, 0, 1, 0, 0u};
+    TileSpan sp{0, 0, 0, 0, 0u, 0u, 0u, 0u};
+    bool direct = false;
+    if (c) {
+        b = P.bins[t];
+        if (tile_span(b, P.W, P.H, P.ets, sp)) {
+            const uint32_t nt = (sp.ex1 - sp.ex0 + 1) * (sp.ey1 - sp.ey0 + 1);
+            direct = !P.empty_cull || nt <= 32;
+        } else {
+            c = 0;
+        }
+    }
+    const uint32_t o = c ? P.offsets[t] : 0u;
+    emit_wave(ew[threadIdx.x >> 6], direct, c, o, b.tmask, !P.empty_cull, sp, P.ntx, P.key_base, P.val_base + t,
+              P.keys, P.vals);
+    if (c && !direct)
+        emit_pairs(b, P.recs + t, sp, P.ets, P.ntx, P.empty_cull, P.key_base, P.val_base + t, P.keys + o,
+                   P.vals + o);
 }
 
 // ===========================================================================
@@ -927,13 +1039,6 @@ __global__ __launch_bounds__(256) void k_radix_hist(const uint32_t* keys, uint32
 // with consecutive threads on consecutive addresses of each digit's run
 // (the digit's global offset for this block comes from the scanned
 // histograms), so the writes coalesce instead of scattering 4-byte words.
-// Orders LDS accesses of different lanes of one wave (no work-group barrier).
-__device__ __forceinline__ void wave_lds_sync() {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
-
 __global__ __launch_bounds__(256) void k_radix_scatter(const uint32_t* keys_in,
                                                       const uint32_t* vals_in, uint32_t n,
                                                       int shift, const uint32_t* offsets,
