@@ -1545,7 +1545,9 @@ __global__ __launch_bounds__(256) void k_line_tiles(LineSetupParams P, int pass)
                     P.items[o + n] = it;
                     const uint32_t ub = walk_calls_bound(c, P.aa);
                     P.plot_ub[o + n] = ub;
-                    if (P.walk_len) P.walk_len[o + n] = min(ub / 2u, 255u);
+                    // longest walks first (ascending key): the long waves
+                    // start early and the short ones fill the tail
+                    if (P.walk_len) P.walk_len[o + n] = 255u - min(ub / 2u, 255u);
                 }
                 n++;
             }

@@ -84,7 +84,7 @@ struct LineSetupParams {
     LineItem* items;
     uint32_t* err;
     int aa;                // Wu (1) or Bresenham (0) walks
-    uint32_t* walk_len;    // pass 1 (optional): per item, a sort key for its walk length (<= 255)
+    uint32_t* walk_len;    // pass 1 (optional): per item, a sort key, 255 - its walk length (capped)
     uint32_t* plot_ub;     // pass 1: per item, an upper bound of its plots (the walk's plot calls)
 };
 

@@ -1236,7 +1236,7 @@ int draw_lines_impl(sr_context ctx, sr_framebuffer fb, const DrawPlan& pl, const
         (rc = ensure(ctx, ctx->poffsets, sizeof(uint32_t) * (nitems + 1))))
         return rc;
     lp.items = P<LineItem>(ctx->litems);
-    // items ordered by walk length (one 8-bit radix pass over (length, item))
+    // items ordered by walk length, longest first (one 8-bit radix pass)
     if ((rc = ensure_pairs(ctx, nitems)) ||
         (rc = ensure(ctx, ctx->hist, sizeof(uint32_t) * (radix_hist_words(nitems) + 256))) ||
         (rc = ensure(ctx, ctx->scan_tmp, sizeof(uint32_t) * (scan_tmp_words(radix_hist_words(nitems) + 256) + 16))))
