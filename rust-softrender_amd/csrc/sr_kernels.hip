@@ -1237,7 +1237,8 @@ __device__ __forceinline__ void strip_status(int lo, int hi, uint32_t t, uint32_
 // Likewise colR's column 0 and rowB's row 0 (every triangle right of or
 // below some tile adds to them): with STRIP_HOT_GROUPS work-groups looping
 // over the triangles, each accumulates them in LDS and writes its partial
-// row out; k_strips_hot_reduce sums the partials (no global atomics).
+// row out; k_strips_hot_reduce sums the partials, 32 rows per thread and
+// one global add per cell and 32 rows.
 constexpr uint32_t STRIP_LDS_CORNERS = 2048;
 constexpr uint32_t STRIP_LDS_WORDS = 16384;  // 64 KiB
 constexpr uint32_t STRIP_HOT_GROUPS = 512;
@@ -1356,16 +1357,18 @@ __global__ __launch_bounds__(256) void k_strips_accum(StripParams P) {
     }
 }
 
-// colR's column 0 and rowB's row 0 = the sums of the work-groups' partials
-// (no other term adds to them).
+// colR's column 0 and rowB's row 0 (zero before, and no other term adds to
+// them) += the work-groups' partials: blockIdx.y takes partial rows
+// [32 y, 32 y + 32).
+constexpr uint32_t STRIP_REDUCE_ROWS = 32;
 __global__ __launch_bounds__(256) void k_strips_hot_reduce(StripParams P, uint32_t groups) {
     const uint32_t HS = P.H + 1, WS = P.W + 1;
     const uint32_t i = blockIdx.x * 256u + threadIdx.x;
     if (i >= HS + WS) return;
+    const uint32_t g0 = blockIdx.y * STRIP_REDUCE_ROWS, g1 = min(g0 + STRIP_REDUCE_ROWS, groups);
     uint32_t s = 0;
-    for (uint32_t g = 0; g < groups; g++) s += P.hot_part[(size_t)g * (HS + WS) + i];
-    if (i < HS) P.colR[i] = s;
-    else P.rowB[i - HS] = s;
+    for (uint32_t g = g0; g < g1; g++) s += P.hot_part[(size_t)g * (HS + WS) + i];
+    if (s) atomicAdd(i < HS ? &P.colR[i] : &P.rowB[i - HS], s);
 }
 
 // In-place inclusive prefix over `len` elements of each of `rows` rows
@@ -2039,7 +2042,9 @@ void launch_strips(const StripParams& P, hipStream_t s) {
     if (G == 0) Q.hot_part = nullptr;
     const size_t lds_bytes = sizeof(uint32_t) * (strip_corner_words(Q) + (G ? HS + WS : 0u));
     if (P.n) hipLaunchKernelGGL(k_strips_accum, dim3(G ? G : blocks_for(P.n, 256)), dim3(256), lds_bytes, s, Q);
-    if (G) hipLaunchKernelGGL(k_strips_hot_reduce, dim3(blocks_for(HS + WS, 256)), dim3(256), 0, s, Q, G);
+    if (G)
+        hipLaunchKernelGGL(k_strips_hot_reduce, dim3(blocks_for(HS + WS, 256), blocks_for(G, STRIP_REDUCE_ROWS)),
+                           dim3(256), 0, s, Q, G);
     // colL/colR: prefix along pixel rows (per tile column), then across tile columns
     hipLaunchKernelGGL(k_prefix_rows, dim3(CS), dim3(256), 0, s, P.colL, CS, HS, HS, 1u);
     hipLaunchKernelGGL(k_prefix_rows, dim3(HS), dim3(256), 0, s, P.colL, HS, CS, 1u, HS);
