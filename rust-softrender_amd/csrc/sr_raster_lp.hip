@@ -293,20 +293,20 @@ __global__ __launch_bounds__(256) void k_raster_lines(RasterParams P) {
     store_tile(P, S, gx, gy, live);
 }
 
+// Points are sparse (a few per tile), so a work-group reads and writes the
+// pixels its points land on in place rather than staging the whole tile:
+// it alone touches its tile's pixels, and the claim rounds' barriers order
+// same-pixel points.
 template <int FP, int BLEND>
 __global__ __launch_bounds__(256) void k_raster_points(RasterParams P) {
     constexpr int KV = fp_varyings<FP>();
-    __shared__ uint32_t s_col[256], s_st[256], claim[256];
-    __shared__ float s_dep[256], slut[256];
+    __shared__ uint32_t claim[256];
+    __shared__ float slut[256];
     const int t = threadIdx.x;
     const uint32_t tile = xcd_remap(blockIdx.x, gridDim.x);
     const uint32_t etx = tile % P.ntx, ety = tile / P.ntx;
     const uint32_t beg = P.tile_start[tile], end = P.tile_end[tile];
     if (beg >= end) return;
-    const PixState S{s_col, s_dep, s_st};
-    int gx, gy;
-    bool live;
-    load_tile(P, etx, ety, S, gx, gy, live);
     slut[t] = (float)t / 255.0f;
     claim[t] = 0xFFFFFFFFu;
     __syncthreads();
@@ -320,30 +320,31 @@ __global__ __launch_bounds__(256) void k_raster_points(RasterParams P) {
         int lp = 0;
         uint32_t v = 0;
         float x = 0.f, y = 0.f;
+        size_t idx = 0;
         if (k < end) {
             const uint32_t pt = P.items[k];
             v = P.point_v ? P.point_v[pt] : pt;
             x = P.verts.comp(0)[v];
             y = P.verts.comp(1)[v];
-            lp = ((int)(uint32_t)y - ey0) * LP_ETS + ((int)(uint32_t)x - ex0);  // point.rs:47
-            pending = true;
+            const uint32_t px = (uint32_t)x, py = (uint32_t)y;                  // point.rs:47
+            lp = ((int)py - ey0) * LP_ETS + ((int)px - ex0);
+            pending = px < P.W && py < P.H;  // binning keeps only such points
+            idx = (size_t)py * P.W + px;
         }
         ordered_rounds(pending, lp, k, claim, [&]() {
-            const uint32_t sv = s_st[lp];
+            const uint32_t sv = P.stencil[idx];
             if (!stencil_test(P.s_test, sv, P.s_ref)) return;  // point.rs:293
-            s_st[lp] = stencil_op(P.s_op, P.s_wrap, sv, P.s_ref);
+            P.stencil[idx] = (uint8_t)stencil_op(P.s_op, P.s_wrap, sv, P.s_ref);
             const float z = P.verts.comp(2)[v];
-            if (!(z < 0.0f && z >= s_dep[lp])) return;           // point.rs:300-306
+            if (!(z < 0.0f && z >= P.depth[idx])) return;        // point.rs:300-306
             float kv[KV > 0 ? KV : 1];
 #pragma unroll
             for (int c = 0; c < KV; c++) kv[c] = P.verts.comp(4 + c)[v];
             const uint32_t c = fragment<FP>(P, kv, lut);
-            s_col[lp] = blend<BLEND>(c, s_col[lp], lut);
-            s_dep[lp] = z;
+            P.color[idx] = blend<BLEND>(c, P.color[idx], lut);
+            P.depth[idx] = z;
         });
     }
-    __syncthreads();
-    store_tile(P, S, gx, gy, live);
 }
 
 #define SR_LP_DISPATCH(KERNEL)                                                                   \
