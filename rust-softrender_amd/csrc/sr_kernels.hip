@@ -341,29 +341,42 @@ __global__ __launch_bounds__(256) void k_gs_emit(GSParams P) {
             // intersection if they cross, then their end vertex if inside.
             const CV &a = v[0], &b = v[1], &c = v[2];
             const int m = (inside(4, a) ? 1 : 0) | (inside(4, b) ? 2 : 0) | (inside(4, c) ? 4 : 0);
+            if (m == 0) return;  // none inside (its count is 0 too)
+            // all three intersections (unused ones are never stored), so that
+            // every lane runs the same code below whatever its pattern
             CV i0, i1, i2;
-            if ((m & 1) != ((m >> 1) & 1)) intersect(4, a, b, i0);
-            if (((m >> 1) & 1) != ((m >> 2) & 1)) intersect(4, b, c, i1);
-            if (((m >> 2) & 1) != (m & 1)) intersect(4, c, a, i2);
-            auto tri = [&](uint32_t k, const CV& x, const CV& y, const CV& z) __attribute__((always_inline)) {
-                store_screen(P, 3 * (o + k) + 0, x);
-                store_screen(P, 3 * (o + k) + 1, y);
-                store_screen(P, 3 * (o + k) + 2, z);
-            };
-            // four vertices q0..q3: emit_triangle(q3, q0, q1), (q3, q1, q2)
-            auto quad = [&](const CV& q0, const CV& q1, const CV& q2, const CV& q3) __attribute__((always_inline)) {
-                tri(0, q3, q0, q1);
-                tri(1, q3, q1, q2);
-            };
+            intersect(4, a, b, i0);
+            intersect(4, b, c, i1);
+            intersect(4, c, a, i2);
+            // The polygon's fan as vertices q0..q3: emit_triangle(q3, q0, q1),
+            // then for four vertices (q3, q1, q2).
+            //   7: [b, c, a]                       as (q3, q0, q1) = (b, c, a)
+            //   1: a inside: [I(a,b), I(c,a), a]    (q3, q0, q1) = (i0, i2, a)
+            //   2: b inside: [I(a,b), b, I(b,c)]    (q3, q0, q1) = (i0, b, i1)
+            //   4: c inside: [I(b,c), c, I(c,a)]    (q3, q0, q1) = (i1, c, i2)
+            //   3: a, b inside: [b, I(b,c), I(c,a), a]
+            //   5: a, c inside: [I(a,b), I(b,c), c, a]
+            //   6: b, c inside: [I(a,b), b, c, I(c,a)]
+            // the cases only copy vertices (cheap under divergence); the
+            // normalisation and stores after them run once for every lane
+            CV q0, q1, q2, q3;
             switch (m) {
-                case 7: tri(0, b, c, a); break;     // [b, c, a]
-                case 1: tri(0, i0, i2, a); break;   // a inside: [I(a,b), I(c,a), a]
-                case 2: tri(0, i0, b, i1); break;   // b inside: [I(a,b), b, I(b,c)]
-                case 4: tri(0, i1, c, i2); break;   // c inside: [I(b,c), c, I(c,a)]
-                case 3: quad(b, i1, i2, a); break;  // a, b inside: [b, I(b,c), I(c,a), a]
-                case 5: quad(i0, i1, c, a); break;  // a, c inside: [I(a,b), I(b,c), c, a]
-                case 6: quad(i0, b, c, i2); break;  // b, c inside: [I(a,b), b, c, I(c,a)]
-                default: break;                     // none inside
+                case 7: q3 = b; q0 = c; q1 = a; q2 = a; break;
+                case 1: q3 = i0; q0 = i2; q1 = a; q2 = a; break;
+                case 2: q3 = i0; q0 = b; q1 = i1; q2 = i1; break;
+                case 4: q3 = i1; q0 = c; q1 = i2; q2 = i2; break;
+                case 3: q0 = b; q1 = i1; q2 = i2; q3 = a; break;
+                case 5: q0 = i0; q1 = i1; q2 = c; q3 = a; break;
+                default: q0 = i0; q1 = b; q2 = c; q3 = i2; break;  // 6
+            }
+            const bool quad = m == 3 || m == 5 || m == 6;
+            store_screen(P, 3 * o + 0, q3);
+            store_screen(P, 3 * o + 1, q0);
+            store_screen(P, 3 * o + 2, q1);
+            if (quad) {
+                store_screen(P, 3 * (o + 1) + 0, q3);
+                store_screen(P, 3 * (o + 1) + 1, q1);
+                store_screen(P, 3 * (o + 1) + 2, q2);
             }
             return;
         }
