@@ -859,11 +859,18 @@ __global__ __launch_bounds__(256) void k_scan_reduce(const uint32_t* in, uint32_
     __shared__ uint32_t lds4[4];
     n = live_count(n, count);
     if (blockIdx.x >= blocks_for(n, SCAN_TILE)) return;
-    const uint32_t base = blockIdx.x * SCAN_TILE;
+    // 16 consecutive elements per thread (16-byte loads when aligned)
+    const uint32_t e0 = blockIdx.x * SCAN_TILE + threadIdx.x * 16u;
     uint32_t acc = 0;
-    for (int r = 0; r < 16; r++) {
-        const uint32_t e = base + r * 256u + threadIdx.x;
-        acc += e < n ? in[e] : 0u;
+    if ((((uintptr_t)in) & 15u) == 0 && e0 + 15u < n) {
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            const uint4 x = ((const uint4*)(in + e0))[q];
+            acc += (x.x + x.y) + (x.z + x.w);
+        }
+    } else {
+#pragma unroll
+        for (int j = 0; j < 16; j++) acc += e0 + j < n ? in[e0 + j] : 0u;
     }
     uint32_t tot;
     block_scan_256(acc, tot, lds4);
@@ -1032,10 +1039,22 @@ __global__ __launch_bounds__(256) void k_radix_hist(const uint32_t* keys, uint32
     if (!B.live) return;
     h[threadIdx.x] = 0;
     __syncthreads();
-    for (int r = 0; r < 16; r++) {
-        const uint32_t e = B.lo + r * 256u + threadIdx.x;
-        if (e < B.hi) atomicAdd(&h[((keys[e] - B.kbase) >> shift) & 255u], 1u);
+    // 16 consecutive keys per thread (16-byte loads when aligned)
+    const uint32_t e0 = B.lo + threadIdx.x * 16u;
+    uint32_t k[16];
+    if ((((uintptr_t)(keys + e0)) & 15u) == 0 && e0 + 15u < B.hi) {
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            const uint4 x = ((const uint4*)(keys + e0))[q];
+            k[4 * q] = x.x; k[4 * q + 1] = x.y; k[4 * q + 2] = x.z; k[4 * q + 3] = x.w;
+        }
+    } else {
+#pragma unroll
+        for (int j = 0; j < 16; j++) k[j] = e0 + j < B.hi ? keys[e0 + j] : 0u;
     }
+#pragma unroll
+    for (int j = 0; j < 16; j++)
+        if (e0 + j < B.hi) atomicAdd(&h[((k[j] - B.kbase) >> shift) & 255u], 1u);
     __syncthreads();
     hist[threadIdx.x * B.hstride + B.hbase] = h[threadIdx.x];
 }
