@@ -778,13 +778,16 @@ __global__ __launch_bounds__(256) void k_tri_emit_batch(BatchTables T, const Tri
     TriBin b{1, 0, 1, 0, 0u};
     TileSpan sp{0, 0, 0, 0, 0u, 0u, 0u, 0u};
     bool direct = false, all = false;
-    uint32_t ntx = 0, kbase = 0;
+    uint32_t ntx = 0, kbase = 0, W = 0, H = 0;
     int ets = 8, empty_cull = 0;
+    // the scene's fields, through the scalar cache when the wave's
+    // triangles share one scene (nearly always)
+    with_scene(T, T.tend, T.nt, t, [&](const SceneGeom& S) {
+        ntx = S.ntx; kbase = S.tile_base; ets = (int)S.ets; empty_cull = S.empty_cull; W = S.W; H = S.H;
+    });
     if (c) {
         b = bins[t];
-        const SceneGeom& S = T.scenes[scene_of(T.tend, T.nscenes, t)];
-        ntx = S.ntx; kbase = S.tile_base; ets = (int)S.ets; empty_cull = S.empty_cull;
-        if (tile_span(b, S.W, S.H, S.ets, sp)) {
+        if (tile_span(b, W, H, (uint32_t)ets, sp)) {
             const uint32_t nt = (sp.ex1 - sp.ex0 + 1) * (sp.ey1 - sp.ey0 + 1);
             all = !empty_cull;
             direct = all || nt <= 32;
