@@ -1042,22 +1042,26 @@ __global__ __launch_bounds__(256) void k_radix_hist(const uint32_t* keys, uint32
     if (!B.live) return;
     h[threadIdx.x] = 0;
     __syncthreads();
-    // 16 consecutive keys per thread (16-byte loads when aligned)
+    // a full block at a 16-byte aligned start: 16 consecutive keys per
+    // thread in 16-byte loads; otherwise (partial blocks, segments at any
+    // offset) 16 rounds of 256 consecutive keys, which keep the loads
+    // coalesced (the counts do not depend on the order)
     const uint32_t e0 = B.lo + threadIdx.x * 16u;
-    uint32_t k[16];
-    if ((((uintptr_t)(keys + e0)) & 15u) == 0 && e0 + 15u < B.hi) {
+    if ((((uintptr_t)(keys + B.lo)) & 15u) == 0 && B.hi - B.lo == SCAN_TILE) {
+        uint32_t k[16];
 #pragma unroll
         for (int q = 0; q < 4; q++) {
             const uint4 x = ((const uint4*)(keys + e0))[q];
             k[4 * q] = x.x; k[4 * q + 1] = x.y; k[4 * q + 2] = x.z; k[4 * q + 3] = x.w;
         }
+#pragma unroll
+        for (int j = 0; j < 16; j++) atomicAdd(&h[((k[j] - B.kbase) >> shift) & 255u], 1u);
     } else {
-#pragma unroll
-        for (int j = 0; j < 16; j++) k[j] = e0 + j < B.hi ? keys[e0 + j] : 0u;
+        for (int r = 0; r < 16; r++) {
+            const uint32_t e = B.lo + r * 256u + threadIdx.x;
+            if (e < B.hi) atomicAdd(&h[((keys[e] - B.kbase) >> shift) & 255u], 1u);
+        }
     }
-#pragma unroll
-    for (int j = 0; j < 16; j++)
-        if (e0 + j < B.hi) atomicAdd(&h[((k[j] - B.kbase) >> shift) & 255u], 1u);
     __syncthreads();
     hist[threadIdx.x * B.hstride + B.hbase] = h[threadIdx.x];
 }
