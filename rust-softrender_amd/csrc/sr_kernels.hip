@@ -805,6 +805,8 @@ __global__ __launch_bounds__(256) void k_tri_emit(EmitParams P) {
     const uint32_t t = blockIdx.x * 256u + threadIdx.x;
     if (__builtin_amdgcn_readfirstlane(blockIdx.x * 256u + (threadIdx.x & ~63u)) >= P.n) return;  // whole wave
     uint32_t c = t < P.n ? P.counts[t] : 0u;
+    const uint32_t o = c ? P.offsets[t] : 0u;
+    if (c && (uint64_t)o + c > P.cap) c = 0;  // the host emits again into a larger buffer
     TriBin b{1, 0, 1, 0, 0u};
     TileSpan sp{0, 0, 0, 0, 0u, 0u, 0u, 0u};
     bool direct = false;
@@ -817,7 +819,6 @@ __global__ __launch_bounds__(256) void k_tri_emit(EmitParams P) {
             c = 0;
         }
     }
-    const uint32_t o = c ? P.offsets[t] : 0u;
     emit_wave(ew[threadIdx.x >> 6], direct, c, o, b.tmask, !P.empty_cull, sp, P.ntx, P.key_base, P.val_base + t,
               P.keys, P.vals);
     if (c && !direct)

@@ -46,6 +46,7 @@ struct EmitParams {
     const TriBin* bins;
     const uint32_t* counts;
     const uint32_t* offsets;
+    uint32_t cap;  // pairs keys/vals hold: a triangle whose pairs would pass it writes none
     uint32_t n, W, H, ets, ntx;
     uint32_t key_base, val_base;
     uint32_t* keys;

@@ -180,14 +180,35 @@ int draw_tris(sr_context ctx, sr_framebuffer fb, const DrawPlan& pl, const sr_dr
         if ((rc = scan_counts(ctx, P<uint32_t>(ctx->counts), P<uint32_t>(ctx->offsets), ntris)))
             return rc;
         mark(ctx, pl.timing, ST_SCAN);
-        if ((rc = read_words(ctx, 2))) return rc;
+        // The error flag and the pair count travel to the host while the
+        // pairs are emitted into the pair buffers as they are; a draw with
+        // more pairs than they hold (its first, or a larger one) emits
+        // again after they grow.
+        HIP_OK(ctx, hipMemcpyAsync(ctx->h_misc, ctx->misc.p, 2 * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+        HIP_OK(ctx, hipEventRecord(ctx->ev_read, s));
+        EmitParams ep;
+        ep.recs = P<TriRec>(ctx->recs); ep.bins = bins; ep.counts = P<uint32_t>(ctx->counts);
+        ep.offsets = P<uint32_t>(ctx->offsets); ep.n = ntris; ep.W = pl.W; ep.H = pl.H;
+        ep.ets = pl.ets; ep.ntx = pl.ntx; ep.key_base = 0; ep.val_base = 0;
+        ep.empty_cull = sp.empty_cull;
+        auto emit = [&]() {
+            ep.keys = P<uint32_t>(ctx->keys);
+            ep.vals = P<uint32_t>(ctx->vals);
+            ep.cap = (uint32_t)std::min<size_t>(std::min(ctx->keys.bytes, ctx->vals.bytes) / sizeof(uint32_t),
+                                                0xFFFFFFFFu);
+            if (ep.keys && ep.vals) launch_tri_emit(ep, s);
+        };
+        emit();
+        HIP_OK(ctx, hipEventSynchronize(ctx->ev_read));
         if (ctx->h_misc[0]) return fail(ctx, SR_ERR_INVALID_COORDINATE, "NaN screen coordinate");
         const uint32_t npairs = ctx->h_misc[1];
         ctx->last.bin_pairs = npairs;
         const size_t pb = sizeof(uint32_t) * std::max<uint32_t>(npairs, 1);
-        if ((rc = ensure(ctx, ctx->keys, pb)) || (rc = ensure(ctx, ctx->vals, pb)) ||
-            (rc = ensure(ctx, ctx->keys2, pb)) || (rc = ensure(ctx, ctx->vals2, pb)))
-            return rc;
+        if (npairs > ep.cap || !ep.keys || !ep.vals) {
+            if ((rc = ensure(ctx, ctx->keys, pb)) || (rc = ensure(ctx, ctx->vals, pb))) return rc;
+            emit();
+        }
+        if ((rc = ensure(ctx, ctx->keys2, pb)) || (rc = ensure(ctx, ctx->vals2, pb))) return rc;
         if ((rc = ensure(ctx, ctx->hist, sizeof(uint32_t) * (radix_hist_words(npairs) + 256))))
             return rc;
         if ((rc = ensure(ctx, ctx->scan_tmp,
@@ -196,13 +217,6 @@ int draw_tris(sr_context ctx, sr_framebuffer fb, const DrawPlan& pl, const sr_dr
         if ((rc = ensure(ctx, ctx->tstart, sizeof(uint32_t) * pl.ntiles)) ||
             (rc = ensure(ctx, ctx->tend, sizeof(uint32_t) * pl.ntiles)))
             return rc;
-        EmitParams ep;
-        ep.recs = P<TriRec>(ctx->recs); ep.bins = bins; ep.counts = P<uint32_t>(ctx->counts);
-        ep.offsets = P<uint32_t>(ctx->offsets); ep.n = ntris; ep.W = pl.W; ep.H = pl.H;
-        ep.ets = pl.ets; ep.ntx = pl.ntx; ep.key_base = 0; ep.val_base = 0;
-        ep.keys = P<uint32_t>(ctx->keys); ep.vals = P<uint32_t>(ctx->vals);
-        ep.empty_cull = sp.empty_cull;
-        launch_tri_emit(ep, s);
         mark(ctx, pl.timing, ST_EMIT);
         const int swapped = radix_sort(P<uint32_t>(ctx->keys), P<uint32_t>(ctx->vals),
                                        P<uint32_t>(ctx->keys2), P<uint32_t>(ctx->vals2), npairs,
@@ -777,6 +791,12 @@ int sr_ctx_create(int device, sr_context* out) {
         return SR_ERR_DEVICE;
     }
     for (auto& e : ctx->ev) hipEventCreate(&e);
+    if (hipEventCreateWithFlags(&ctx->ev_read, hipEventDisableTiming) != hipSuccess) {
+        hipHostFree(ctx->h_misc);
+        hipStreamDestroy(ctx->stream);
+        delete ctx;
+        return SR_ERR_DEVICE;
+    }
     if (ensure(ctx, ctx->misc, 64 * sizeof(uint32_t)) != SR_OK) {
         delete ctx;
         return SR_ERR_OUT_OF_MEMORY;
@@ -797,6 +817,7 @@ int sr_ctx_destroy(sr_context ctx) {
         if (b->p) hipFree(b->p);
     for (auto& e : ctx->ev)
         if (e) hipEventDestroy(e);
+    if (ctx->ev_read) hipEventDestroy(ctx->ev_read);
     destroy_batch(ctx);
     if (ctx->h_misc) hipHostFree(ctx->h_misc);
     hipStreamDestroy(ctx->stream);

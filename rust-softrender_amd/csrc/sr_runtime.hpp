@@ -37,6 +37,7 @@ struct sr_context_t {
         hist, misc, strips, strip_aux, litems, plots, lcounts, loffsets, pcounts, poffsets;
     uint32_t* h_misc = nullptr;  // pinned readback words
     hipEvent_t ev[sr_rt::ST_COUNT + 1] = {};
+    hipEvent_t ev_read = nullptr;  // a readback into h_misc has landed
     float stage_ms[sr_rt::ST_COUNT] = {};
     int n_stage = 0;
     sr_draw_stats last = {};
