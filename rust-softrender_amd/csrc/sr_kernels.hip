@@ -902,11 +902,32 @@ __global__ __launch_bounds__(256) void k_scan_top(uint32_t* sums, uint32_t n, ui
 // stores when the arrays are 16-byte aligned), so the block needs one
 // block scan of the threads' totals.  In place is fine: a thread writes
 // only the elements it read.
+// block_off: the scanned block sums (k_scan_top), or with `sums` the raw
+// block sums, which each block then adds up below itself (at most
+// SCAN_FUSED_BLOCKS of them; the last block also writes the grand total):
+// one launch fewer for small scans.
+constexpr uint32_t SCAN_FUSED_BLOCKS = 1024;
 __global__ __launch_bounds__(256) void k_scan_down(const uint32_t* in, uint32_t* out, uint32_t n,
-                                                  const uint32_t* block_off, const uint32_t* count) {
+                                                  const uint32_t* block_off, const uint32_t* count,
+                                                  const uint32_t* sums, uint32_t* total_out) {
     __shared__ uint32_t lds4[4];
     n = live_count(n, count);
-    if (blockIdx.x >= blocks_for(n, SCAN_TILE)) return;
+    const uint32_t nb = blocks_for(n, SCAN_TILE);
+    if (blockIdx.x >= nb) {
+        if (nb == 0 && blockIdx.x == 0 && threadIdx.x == 0 && sums && total_out) *total_out = 0u;
+        return;
+    }
+    uint32_t base_off;
+    if (sums) {
+        uint32_t acc = 0;
+        for (uint32_t i = threadIdx.x; i < blockIdx.x; i += 256u) acc += sums[i];
+        uint32_t below;
+        block_scan_256(acc, below, lds4);
+        base_off = below;
+        if (blockIdx.x == nb - 1u && threadIdx.x == 0 && total_out) *total_out = below + sums[blockIdx.x];
+    } else {
+        base_off = block_off[blockIdx.x];
+    }
     const uint32_t e0 = blockIdx.x * SCAN_TILE + threadIdx.x * 16u;
     const bool vec = ((((uintptr_t)in) | ((uintptr_t)out)) & 15u) == 0 && e0 + 15u < n;
     uint32_t v[16];
@@ -928,7 +949,7 @@ __global__ __launch_bounds__(256) void k_scan_down(const uint32_t* in, uint32_t*
         local += x;
     }
     uint32_t tot;
-    const uint32_t off = block_off[blockIdx.x] + block_scan_256(local, tot, lds4);
+    const uint32_t off = base_off + block_scan_256(local, tot, lds4);
     if (vec) {
 #pragma unroll
         for (int q = 0; q < 4; q++)
@@ -1973,8 +1994,14 @@ void launch_scan(const uint32_t* in, uint32_t* out, uint32_t n, uint32_t* tmp, u
         return;
     }
     hipLaunchKernelGGL(k_scan_reduce, dim3(nb), dim3(256), 0, s, in, n, tmp, count);
+    if (nb <= SCAN_FUSED_BLOCKS) {
+        hipLaunchKernelGGL(k_scan_down, dim3(nb), dim3(256), 0, s, in, out, n, (const uint32_t*)nullptr, count,
+                           (const uint32_t*)tmp, total);
+        return;
+    }
     hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(256), 0, s, tmp, n, total, count);
-    hipLaunchKernelGGL(k_scan_down, dim3(nb), dim3(256), 0, s, in, out, n, (const uint32_t*)tmp, count);
+    hipLaunchKernelGGL(k_scan_down, dim3(nb), dim3(256), 0, s, in, out, n, (const uint32_t*)tmp, count,
+                       (const uint32_t*)nullptr, (uint32_t*)nullptr);
 }
 
 uint32_t radix_hist_words(uint32_t n) { return 256u * blocks_for(n, SCAN_TILE); }
