@@ -48,49 +48,58 @@ _lib = None
 
 
 def lib():
+    """The oracle library built in this folder (make -C oracle)."""
     global _lib
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             raise RuntimeError(f"{LIB_PATH} not built (make -C oracle)")
-        L = C.CDLL(LIB_PATH)
-        L.or_draw.argtypes = [C.POINTER(OrFb), C.POINTER(OrMesh), C.POINTER(DrawDesc),
-                              C.POINTER(OrTexture), C.c_int, C.POINTER(OrStats)]
-        L.or_draw.restype = C.c_int
-        L.or_fb_clear.argtypes = [C.POINTER(OrFb), C.c_uint32, C.c_float, C.c_uint8]
-        fp = C.POINTER(C.c_float)
-        L.or_normalize.argtypes = [fp, C.POINTER(DrawDesc._fields_[16][1]), fp]
-        L.or_clip_triangle_reference.argtypes = [fp, fp, fp, C.c_int, fp, C.c_int]
-        L.or_clip_triangle_reference.restype = C.c_int
-        L.or_clip_triangle_near.argtypes = [fp, fp, fp, C.c_int, fp, C.c_int]
-        L.or_clip_triangle_near.restype = C.c_int
-        L.or_clip_line_reference.argtypes = [fp, fp, C.c_int, fp]
-        L.or_clip_line_reference.restype = C.c_int
-        L.or_tiles.argtypes = [C.c_uint32] * 4 + [C.POINTER(C.c_uint32), C.c_int]
-        L.or_tiles.restype = C.c_int
-        L.or_liang_barsky.argtypes = [C.c_float] * 8 + [fp]
-        L.or_liang_barsky.restype = C.c_int
-        i64p, f64p = C.POINTER(C.c_int64), C.POINTER(C.c_double)
-        L.or_bresenham.argtypes = [C.c_int64] * 4 + [i64p, f64p, C.c_int]
-        L.or_bresenham.restype = C.c_int
-        L.or_wu.argtypes = [C.c_double] * 4 + [i64p, f64p, C.c_int]
-        L.or_wu.restype = C.c_int
-        L.or_stencil_test.argtypes = [C.c_int, C.c_uint32, C.c_uint32]
-        L.or_stencil_test.restype = C.c_int
-        L.or_stencil_op.argtypes = [C.c_int, C.c_int, C.c_uint32, C.c_uint32]
-        L.or_stencil_op.restype = C.c_uint32
-        L.or_sample.argtypes = [C.POINTER(OrTexture), C.c_float, C.c_float, C.c_int, C.c_int,
-                                C.c_uint32, fp]
-        L.or_blend.argtypes = [C.c_int, C.c_uint32, C.c_uint32]
-        L.or_blend.restype = C.c_uint32
-        L.or_mul_alpha.argtypes = [C.c_int, C.c_uint32, C.c_double]
-        L.or_mul_alpha.restype = C.c_uint32
-        L.or_hypotf.argtypes = [C.c_float, C.c_float]
-        L.or_hypotf.restype = C.c_float
-        L.or_div_f32.argtypes = [C.c_float, C.c_float]
-        L.or_div_f32.restype = C.c_float
-        L.or_gs_normal_lines.argtypes = [C.c_int] + [fp] * 6 + [C.c_int, fp, fp, fp, C.POINTER(C.c_int)]
-        _lib = L
+        _lib = load(LIB_PATH)
     return _lib
+
+
+def load(path: str):
+    """Loads an oracle build from `path` and declares its entry points (the
+    pinning tests also load variants with planted mistakes this way)."""
+    L = C.CDLL(path)
+    L.or_draw.argtypes = [C.POINTER(OrFb), C.POINTER(OrMesh), C.POINTER(DrawDesc),
+                          C.POINTER(OrTexture), C.c_int, C.POINTER(OrStats)]
+    L.or_draw.restype = C.c_int
+    L.or_fb_clear.argtypes = [C.POINTER(OrFb), C.c_uint32, C.c_float, C.c_uint8]
+    fp = C.POINTER(C.c_float)
+    L.or_normalize.argtypes = [fp, C.POINTER(DrawDesc._fields_[16][1]), fp]
+    L.or_clip_triangle_reference.argtypes = [fp, fp, fp, C.c_int, fp, C.c_int]
+    L.or_clip_triangle_reference.restype = C.c_int
+    L.or_clip_triangle_near.argtypes = [fp, fp, fp, C.c_int, fp, C.c_int]
+    L.or_clip_triangle_near.restype = C.c_int
+    L.or_clip_line_reference.argtypes = [fp, fp, C.c_int, fp]
+    L.or_clip_line_reference.restype = C.c_int
+    L.or_tiles.argtypes = [C.c_uint32] * 4 + [C.POINTER(C.c_uint32), C.c_int]
+    L.or_tiles.restype = C.c_int
+    L.or_liang_barsky.argtypes = [C.c_float] * 8 + [fp]
+    L.or_liang_barsky.restype = C.c_int
+    i64p, f64p = C.POINTER(C.c_int64), C.POINTER(C.c_double)
+    L.or_bresenham.argtypes = [C.c_int64] * 4 + [i64p, f64p, C.c_int]
+    L.or_bresenham.restype = C.c_int
+    L.or_wu.argtypes = [C.c_double] * 4 + [i64p, f64p, C.c_int]
+    L.or_wu.restype = C.c_int
+    L.or_stencil_test.argtypes = [C.c_int, C.c_uint32, C.c_uint32]
+    L.or_stencil_test.restype = C.c_int
+    L.or_stencil_op.argtypes = [C.c_int, C.c_int, C.c_uint32, C.c_uint32]
+    L.or_stencil_op.restype = C.c_uint32
+    L.or_sample.argtypes = [C.POINTER(OrTexture), C.c_float, C.c_float, C.c_int, C.c_int,
+                            C.c_uint32, fp]
+    L.or_blend.argtypes = [C.c_int, C.c_uint32, C.c_uint32]
+    L.or_blend.restype = C.c_uint32
+    L.or_mul_alpha.argtypes = [C.c_int, C.c_uint32, C.c_double]
+    L.or_mul_alpha.restype = C.c_uint32
+    L.or_hypotf.argtypes = [C.c_float, C.c_float]
+    L.or_hypotf.restype = C.c_float
+    L.or_div_f32.argtypes = [C.c_float, C.c_float]
+    L.or_div_f32.restype = C.c_float
+    L.or_gs_normal_lines.argtypes = [C.c_int] + [fp] * 6 + [C.c_int, fp, fp, fp, C.POINTER(C.c_int)]
+    L.or_vertex.argtypes = [C.POINTER(DrawDesc), C.POINTER(OrMesh), C.c_uint32, fp, fp]
+    L.or_vertex.restype = C.c_int
+    return L
 
 
 class Framebuffer:
@@ -122,8 +131,7 @@ def _texture(tex):
     return OrTexture(w, h, rgba.ctypes.data_as(C.POINTER(C.c_uint32))), rgba
 
 
-def draw(fb: Framebuffer, mesh: dict, desc: DrawDesc, tex=None, nthreads: int = 1):
-    """Runs one draw chain; returns (status, stats dict)."""
+def _or_mesh(mesh: dict):
     soa, keep = vertex_soa(mesh)
     idx = mesh.get("indices")
     n = len(keep["px"])
@@ -132,11 +140,28 @@ def draw(fb: Framebuffer, mesh: dict, desc: DrawDesc, tex=None, nthreads: int = 
         om = OrMesh(soa, n, idx.ctypes.data_as(C.POINTER(C.c_uint32)), len(idx), mesh["primitive"])
     else:
         om = OrMesh(soa, n, None, 0, mesh["primitive"])
+    return om, (keep, idx)
+
+
+def vertex(mesh: dict, desc: DrawDesc, i: int, L=None):
+    """The vertex program on vertex i: (clip position (4,), varyings (K,))."""
+    om, keep = _or_mesh(mesh)
+    clip = np.zeros(4, np.float32)
+    k = np.zeros(8, np.float32)
+    fp = C.POINTER(C.c_float)
+    K = (L or lib()).or_vertex(C.byref(desc), C.byref(om), i, clip.ctypes.data_as(fp), k.ctypes.data_as(fp))
+    return clip, k[:K].copy()
+
+
+def draw(fb: Framebuffer, mesh: dict, desc: DrawDesc, tex=None, nthreads: int = 1, L=None):
+    """Runs one draw chain; returns (status, stats dict).  `L`: an oracle
+    library from load() (default: lib())."""
+    om, keep = _or_mesh(mesh)
     ot, keep_t = _texture(tex)
     st = OrStats()
     fbs = fb.struct()
-    rc = lib().or_draw(C.byref(fbs), C.byref(om), C.byref(desc),
-                       C.byref(ot) if ot is not None else None, nthreads, C.byref(st))
+    rc = (L or lib()).or_draw(C.byref(fbs), C.byref(om), C.byref(desc),
+                              C.byref(ot) if ot is not None else None, nthreads, C.byref(st))
     return rc, {k: getattr(st, k) for k in STAT_FIELDS}
 
 

@@ -100,6 +100,16 @@ static void vs_run(const sr_draw_desc* d, const or_mesh* m, uint32_t i, ovtx* o)
     }
 }
 
+int or_vertex(const sr_draw_desc* d, const or_mesh* m, uint32_t i, float clip[4], float* k) {
+    ovtx o;
+    memset(&o, 0, sizeof(o));
+    vs_run(d, m, i, &o);
+    memcpy(clip, o.p, sizeof(o.p));
+    const int K = vp_varyings(d->vertex_program);
+    memcpy(k, o.k, sizeof(float) * K);
+    return K;
+}
+
 /* ------------------------------------------------------------------------ */
 /* ClipVertex::normalize (geometry/clipvertex.rs:89-126).                    */
 /* ------------------------------------------------------------------------ */

@@ -68,6 +68,10 @@ int or_draw(or_fb* fb, const or_mesh* mesh, const sr_draw_desc* desc,
 void or_fb_clear(or_fb* fb, uint32_t rgba, float depth, uint8_t stencil);
 
 /* ---- single-function entry points for known-answer tests ---- */
+/* VertexShader::run with the draw's vertex program on vertex i of the mesh
+ * (clip space, before the viewport transform); returns K, the number of
+ * varyings written to k. */
+int or_vertex(const sr_draw_desc* d, const or_mesh* m, uint32_t i, float clip[4], float* k);
 /* ClipVertex::normalize on a bare position. out = screen x,y,z,w */
 void or_normalize(const float clip[4], const sr_viewport* vp, float out[4]);
 /* clip_primitives on one triangle with K varyings per vertex (vertex layout:
