@@ -50,6 +50,7 @@ extern "C" {
 #define SR_ERR_NOT_READY (-7)          /* serving: ticket still in flight */
 #define SR_ERR_NO_DEVICE (-8)
 #define SR_ERR_CANCELLED (-9)          /* serving: the server was destroyed first */
+#define SR_ERR_WORKSPACE (-10)         /* sr_draw_static: the draw needs more than the static workspace */
 
 /* ---- enums (values are part of the ABI) -------------------------------- */
 /* primitive.rs:7-167: zero-sized type tags selecting index assembly */
@@ -258,6 +259,27 @@ int sr_mesh_load_obj(sr_context ctx, const char* path, uint32_t model, sr_mesh* 
  * where the workspace has to grow. */
 int sr_draw(sr_context ctx, sr_framebuffer fb, sr_mesh mesh, const sr_draw_desc* desc,
             sr_texture tex);
+
+/* Size of the static device workspace of sr_draw_static, per GPU. */
+#define SR_STATIC_WORKSPACE_MIB 64
+/* sr_draw without any device allocation: no counterpart in the reference
+ * (whose draws allocate host vectors freely); it is for callers that cannot
+ * allocate device memory during a draw (code under stream or graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  Same arguments,
+ * semantics and result as sr_draw, but every workspace buffer (screen
+ * vertices, triangle records, bin pairs, sort buffers, stencil strip
+ * counts, line plots) is carved from one fixed region of
+ * SR_STATIC_WORKSPACE_MIB MiB that the library holds on each GPU from its
+ * first context there until the process ends; the caller's framebuffer,
+ * mesh and texture are its only other memory.  A draw that needs more
+ * returns SR_ERR_WORKSPACE (sr_last_error names the limit) with the
+ * framebuffer unchanged; at 1920x1080 that is roughly 200k triangles.
+ * Every call on one GPU shares that GPU's region: calls from different
+ * contexts on the same GPU that overlap in time race (calls on one context
+ * run in order on its stream).  The context's own workspace is neither
+ * used nor changed, so sr_draw keeps working as before. */
+int sr_draw_static(sr_context ctx, sr_framebuffer fb, sr_mesh mesh, const sr_draw_desc* desc,
+                   sr_texture tex);
 
 /* ---- batching: many independent draws in one persistent launch ---------- */
 /* Several draw chains over one mesh that share its vertex stage

@@ -8,10 +8,44 @@
 // executed as: vertex kernel -> [clip kernels] -> setup -> scan -> pair
 // emission -> stable radix sort by execution tile -> tile ranges ->
 // [stencil strip counts] -> raster (one work-group per execution tile).
+#include <mutex>
+
 #include "sr_runtime.hpp"
 
 using namespace sr;
 using namespace sr_rt;
+
+// ---------------------------------------------------------------------------
+// Static device workspace of sr_draw_static: one fixed region per GPU, held
+// from the library's first context on that GPU until the process ends.  In
+// host code `scratch` names a host-side stand-in, not device memory: the
+// device address of each GPU's copy is looked up once per device at context
+// creation and only that address is ever handed to kernels or copies.
+// ---------------------------------------------------------------------------
+namespace {
+constexpr size_t kStaticWorkspaceBytes = (size_t)SR_STATIC_WORKSPACE_MIB << 20;
+constexpr int kMaxDevices = 64;
+__device__ __align__(256) uint8_t scratch[kStaticWorkspaceBytes];
+std::mutex g_scratch_mu;
+uint8_t* g_scratch_dev[kMaxDevices] = {};  // device address of each GPU's `scratch`
+
+// With `device` current.  SR_OK once the device's address is known.
+int lookup_scratch(int device) {
+    if (device < 0 || device >= kMaxDevices) return SR_ERR_UNSUPPORTED;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (g_scratch_dev[device]) return SR_OK;
+    void* p = nullptr;
+    if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return SR_ERR_DEVICE;
+    g_scratch_dev[device] = (uint8_t*)p;
+    return SR_OK;
+}
+
+uint8_t* scratch_of(int device) {
+    if (device < 0 || device >= kMaxDevices) return nullptr;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    return g_scratch_dev[device];
+}
+}  // namespace
 
 namespace sr_rt {
 
@@ -22,6 +56,19 @@ int fail(sr_context ctx, int code, const std::string& msg) {
 
 int ensure(sr_context ctx, DevBuf& b, size_t bytes, hipStream_t s) {
     if (bytes <= b.bytes) return SR_OK;
+    if (ctx->arena) {
+        // fixed region: a new 256-byte aligned slice past everything carved
+        // so far (nothing is freed; a grown buffer simply takes a new slice)
+        const size_t off = (ctx->arena_used + 255) & ~(size_t)255;
+        if (bytes > ctx->arena_size || off > ctx->arena_size - bytes)
+            return fail(ctx, SR_ERR_WORKSPACE,
+                        "draw needs more than the static workspace of " +
+                            std::to_string(ctx->arena_size >> 20) + " MiB");
+        b.p = ctx->arena + off;
+        b.bytes = bytes;
+        ctx->arena_used = off + bytes;
+        return SR_OK;
+    }
     if (b.p) {
         HIP_OK(ctx, hipStreamSynchronize(s));
         HIP_OK(ctx, hipFree(b.p));
@@ -801,6 +848,9 @@ int sr_ctx_create(int device, sr_context* out) {
         delete ctx;
         return SR_ERR_OUT_OF_MEMORY;
     }
+    // the static workspace's address on this device (sr_draw_static reports
+    // a failed lookup; every other entry point works without it)
+    lookup_scratch(device);
     *out = ctx;
     return SR_OK;
 }
@@ -1154,6 +1204,36 @@ int sr_draw(sr_context ctx, sr_framebuffer fb, sr_mesh mesh, const sr_draw_desc*
     if (int rc = validate_draw(ctx, fb, mesh, d, tex)) return rc;
     hipSetDevice(ctx->device);
     return draw_one(ctx, fb, mesh, d, tex, nullptr);
+}
+
+int sr_draw_static(sr_context ctx, sr_framebuffer fb, sr_mesh mesh, const sr_draw_desc* d, sr_texture tex) {
+    if (int rc = validate_draw(ctx, fb, mesh, d, tex)) return rc;
+    hipSetDevice(ctx->device);
+    uint8_t* const base = scratch_of(ctx->device);
+    if (!base) return fail(ctx, SR_ERR_DEVICE, "draw_static: no static workspace on this device");
+    // The draw runs exactly as sr_draw does, with every workspace buffer
+    // carved from the static region (ensure() allocates nothing while
+    // ctx->arena is set); the context's own buffers are set aside meanwhile
+    // and restored untouched.  misc (the error and count words) was
+    // allocated with the context.
+    DevBuf* const bufs[] = {&ctx->verts, &ctx->gverts, &ctx->recs, &ctx->counts, &ctx->offsets, &ctx->keys,
+                            &ctx->vals, &ctx->keys2, &ctx->vals2, &ctx->tstart, &ctx->tend, &ctx->scan_tmp,
+                            &ctx->hist, &ctx->strips, &ctx->strip_aux, &ctx->litems, &ctx->plots,
+                            &ctx->lcounts, &ctx->loffsets, &ctx->pcounts, &ctx->poffsets};
+    constexpr size_t NB = sizeof(bufs) / sizeof(bufs[0]);
+    DevBuf saved[NB];
+    for (size_t i = 0; i < NB; i++) {
+        saved[i] = *bufs[i];
+        *bufs[i] = DevBuf{};
+    }
+    ctx->arena = base;
+    ctx->arena_size = kStaticWorkspaceBytes;
+    ctx->arena_used = 0;
+    const int rc = draw_one(ctx, fb, mesh, d, tex, nullptr);
+    ctx->arena = nullptr;
+    ctx->arena_size = ctx->arena_used = 0;
+    for (size_t i = 0; i < NB; i++) *bufs[i] = saved[i];
+    return rc;
 }
 
 int sr_draw_chains(sr_context ctx, sr_framebuffer fb, sr_mesh mesh, const sr_draw_desc* descs, uint32_t n,

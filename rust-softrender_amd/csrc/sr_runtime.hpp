@@ -46,6 +46,10 @@ struct sr_context_t {
     std::vector<std::pair<uint64_t, int>> tile_policy{{~0ull, 8}};
     uint32_t batch_cap_hint = 0;             // initial bin-pair capacity of new batch workspaces
     sr_rt::BatchWork* batch = nullptr;       // workspace of sr_batch_draw (on ctx->stream)
+    // While sr_draw_static runs: the draw's buffers are carved in order
+    // from this fixed device region instead of being allocated.
+    uint8_t* arena = nullptr;
+    size_t arena_size = 0, arena_used = 0;
 };
 
 struct sr_mesh_t {

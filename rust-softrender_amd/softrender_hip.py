@@ -20,14 +20,17 @@ ERRORS = {
     -1: "SR_ERR_INVALID_ARGUMENT", -2: "SR_ERR_OUT_OF_MEMORY", -3: "SR_ERR_DEVICE",
     -4: "SR_ERR_INVALID_COORDINATE", -5: "SR_ERR_UNSUPPORTED", -6: "SR_ERR_BUDGET",
     -7: "SR_ERR_NOT_READY", -8: "SR_ERR_NO_DEVICE", -9: "SR_ERR_CANCELLED",
+    -10: "SR_ERR_WORKSPACE",
 }
+SR_ERR_WORKSPACE = -10
+STATIC_WORKSPACE_BYTES = 64 << 20  # SR_STATIC_WORKSPACE_MIB
 
 EXPORTED = [
     "sr_ctx_create", "sr_ctx_destroy", "sr_last_error", "sr_finish", "sr_ctx_stream",
     "sr_ctx_stage_times", "sr_ctx_last_stats", "sr_mesh_upload", "sr_mesh_free",
     "sr_texture_upload", "sr_texture_free", "sr_texture_from_fb", "sr_obj_parse", "sr_mesh_load_obj",
     "sr_fb_create", "sr_fb_free", "sr_fb_clear",
-    "sr_fb_read", "sr_fb_write", "sr_fb_size", "sr_draw", "sr_draw_chains", "sr_batch_draw",
+    "sr_fb_read", "sr_fb_write", "sr_fb_size", "sr_draw", "sr_draw_static", "sr_draw_chains", "sr_batch_draw",
     "sr_ctx_set_exec_tile_policy", "sr_ctx_set_batch_capacity",
     "sr_server_create", "sr_server_destroy", "sr_server_submit", "sr_server_poll",
     "sr_server_release", "sr_server_get_stats", "sr_server_last_error", "sr_server_clock",
@@ -199,6 +202,7 @@ def lib():
                                   C.POINTER(C.c_uint8)]
         L.sr_fb_size.argtypes = [vp, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]
         L.sr_draw.argtypes = [vp, vp, vp, C.POINTER(DrawDesc), vp]
+        L.sr_draw_static.argtypes = [vp, vp, vp, C.POINTER(DrawDesc), vp]
         L.sr_draw_chains.argtypes = [vp, vp, vp, C.POINTER(DrawDesc), C.c_uint32, vp]
         L.sr_batch_draw.argtypes = [vp, C.POINTER(BatchItem), C.c_uint32]
         L.sr_ctx_set_exec_tile_policy.argtypes = [vp, C.c_uint32, C.POINTER(C.c_uint64),
@@ -289,6 +293,11 @@ class Context:
 
     def draw(self, fb: "Framebuffer", mesh: "Mesh", desc: DrawDesc, tex: "Texture" = None):
         self._check(lib().sr_draw(self.h, fb.h, mesh.h, C.byref(desc), tex.h if tex else None))
+
+    def draw_static(self, fb: "Framebuffer", mesh: "Mesh", desc: DrawDesc, tex: "Texture" = None):
+        """sr_draw_static: the same draw with its workspace in the library's
+        fixed per-GPU region (no device allocation)."""
+        self._check(lib().sr_draw_static(self.h, fb.h, mesh.h, C.byref(desc), tex.h if tex else None))
 
     def draw_chains(self, fb: "Framebuffer", mesh: "Mesh", descs, tex: "Texture" = None):
         """Several chains over one mesh sharing one vertex pass (duplicate())."""
