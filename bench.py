@@ -24,8 +24,15 @@ METRIC = "frames/s and Mtris/s at 1080p/1M tris; % HBM roofline; p50/p99 serve l
 HBM_PEAK_GBS = 8000.0
 FP32_PEAK_TFLOPS = 157.3
 
-# Algorithmic work per unit (IEEE f32 operations in the reference's
-# expressions; a divide or sqrt counts as one operation).
+# Algorithmic work of the raster per unit, as the survey of the reference
+# fixes it: a pixel test inside a triangle's tile-clamped box (triangle.rs:
+# 104-126: edge numerators, two exact divides, w, inside and depth tests) is
+# 35 VALU operations; a shaded fragment (interpolation, shading, bilinear
+# sample, blend) is 100.
+OPS_PER_BBOX_TEST = 35
+OPS_PER_SHADED = 100
+# The same work counted operation by operation in the reference's
+# expressions (a divide or sqrt as one operation), reported beside it.
 FLOPS_PER_TEST = 14      # triangle.rs:104-113: x,y (2), edge numerators (8), 2 divides, w (2)
 FLOPS_PER_COVERED = 5    # z = ((z1*u)+(z2*v))+(z3*w)
 FLOPS_PER_SHADED = 145   # 5 varyings x 5 (interpolation), bilinear sample + tex x Lambert, over-blend
@@ -37,25 +44,10 @@ FLOPS_PER_SHADED = 145   # 5 varyings x 5 (interpolation), bilinear sample + tex
 # thread pool) recounts the two order-independent ones and checks them.
 DEFAULT_UNITS = {"bbox_tests": 86093801, "covered": 33448710, "shaded": 6973823}
 
-# HBM bytes per k_raster_tris launch from the rocprofv3 FETCH_SIZE /
-# WRITE_SIZE summary of this workload (scaled by the copy-kernel factor).
-TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_raster_tris_soup1080p_1M.md")
-
-
-def profiled_value(key):
-    try:
-        with open(TRAFFIC_FILE) as f:
-            for line in f:
-                if line.startswith(f"{key}:"):
-                    return line.split(":", 1)[1].strip()
-    except OSError:
-        pass
-    return None
-
-
-def profiled_traffic(exec_tile):
-    v = profiled_value(f"traffic_bytes_per_launch_ets{exec_tile}")
-    return int(v) if v is not None else None
+MI355X_SIMDS = 256 * 4   # 256 CUs x 4 SIMDs; a wave64 VALU instruction issues over 4 cycles
+RASTER_KERNEL = "k_raster_tris<"
+COPY_KERNEL = "copyBuffer"
+CALIB_MIB = 256
 
 
 def parse():
@@ -75,6 +67,10 @@ def parse():
                          "frame or the 256-scene batch")
     ap.add_argument("--copy-calib", type=int, default=0, metavar="MIB",
                     help="only run device-to-device copies of MIB MiB (calibrates PMC byte counters)")
+    ap.add_argument("--no-counters", action="store_true",
+                    help="skip the rocprofv3 counter passes (raster HBM traffic and VALU issue)")
+    ap.add_argument("--counter-probe", action="store_true",
+                    help="(run under rocprofv3 by the counter passes) copy calibration, then 3 frames")
     return ap.parse_args()
 
 
@@ -104,6 +100,141 @@ def copy_calibration(mib):
     print(json.dumps({"copy_calibration_bytes": n, "copies": 5}), flush=True)
 
 
+def counter_probe(a):
+    """What the counter passes profile: a device-to-device copy of CALIB_MIB
+    MiB (the byte counters' calibration), then three frames of the default
+    workload."""
+    copy_calibration(CALIB_MIB)
+    import scenes
+    import softrender_hip as sr
+
+    scene = scenes.soup_scene(a.tris, 1920, 1080)
+    ctx = sr.Context(0)
+    fb = ctx.framebuffer(1920, 1080)
+    mesh = ctx.mesh(scene)
+    tex = ctx.texture(*scene["texture"])
+    kw = dict(scene["desc"])
+    if a.exec_tile:
+        kw["exec_tile"] = a.exec_tile
+    desc = sr.make_desc(**kw)
+    for _ in range(3):
+        fb.clear(scene["clear"])
+        ctx.draw(fb, mesh, desc, tex)
+    ctx.finish()
+    mesh.free()
+    tex.free()
+    fb.free()
+    ctx.close()
+    print(json.dumps({"counter_probe_frames": 3}), flush=True)
+
+
+def pmc_pass(a, counters, timeout=180):
+    """One `rocprofv3 --pmc <counters>` pass over `bench.py --counter-probe`
+    in a child process.  Returns {kernel: {counter: mean value per
+    dispatch}} for the raster and the calibration copy, or raises."""
+    import csv
+    import glob
+    import shutil
+    import signal
+    import subprocess
+    import tempfile
+
+    prof = shutil.which("rocprofv3")
+    if not prof:
+        raise RuntimeError("rocprofv3 not found")
+    out = tempfile.mkdtemp(prefix="sr_pmc_", dir="/tmp")
+    cmd = [prof, "--pmc", *counters, "--output-format", "csv", "-d", out, "-o", "pmc", "--",
+           sys.executable, "-u", os.path.abspath(__file__), "--counter-probe", "--tris", str(a.tris)]
+    if a.exec_tile:
+        cmd += ["--exec-tile", str(a.exec_tile)]
+    p = subprocess.Popen(cmd, cwd="/tmp", env=dict(os.environ, TMPDIR="/tmp"), stdout=subprocess.PIPE,
+                         stderr=subprocess.STDOUT, text=True, start_new_session=True)
+    try:
+        log, _ = p.communicate(timeout=timeout)
+    except subprocess.TimeoutExpired:
+        os.killpg(p.pid, signal.SIGKILL)
+        p.communicate()
+        raise RuntimeError(f"rocprofv3 --pmc {' '.join(counters)} timed out")
+    sums = {}
+    for fn in glob.glob(os.path.join(out, "**", "*counter_collection.csv"), recursive=True):
+        with open(fn) as f:
+            for row in csv.DictReader(f):
+                name = row.get("Kernel_Name", "")
+                key = RASTER_KERNEL if RASTER_KERNEL in name else (COPY_KERNEL if COPY_KERNEL in name else None)
+                if key is None:
+                    continue
+                c = sums.setdefault(key, {}).setdefault(row["Counter_Name"], {})
+                d = row.get("Dispatch_Id") or row.get("Correlation_Id") or str(len(c))
+                c[d] = c.get(d, 0.0) + float(row["Counter_Value"])
+    shutil.rmtree(out, ignore_errors=True)
+    res = {k: {cn: sum(v.values()) / len(v) for cn, v in cs.items()} for k, cs in sums.items()}
+    if RASTER_KERNEL not in res:
+        raise RuntimeError(f"rocprofv3 --pmc {' '.join(counters)}: no raster dispatch in its output "
+                           f"(exit {p.returncode}): {log[-400:]}")
+    return res
+
+
+def counter_passes(a):
+    """HBM traffic and VALU issue of the raster kernel, measured in this run:
+    FETCH_SIZE and WRITE_SIZE (KiB per dispatch), each scaled by what the
+    same counter reads for a device-to-device copy of known size, and the
+    SQ instruction counts over the kernel's active cycles."""
+    calib = CALIB_MIB << 20
+    t0 = time.time()
+    f = pmc_pass(a, ["FETCH_SIZE"])
+    print(f"bench.py: counter pass FETCH_SIZE done ({time.time() - t0:.0f} s)", file=sys.stderr, flush=True)
+    w = pmc_pass(a, ["WRITE_SIZE"])
+    print(f"bench.py: counter pass WRITE_SIZE done ({time.time() - t0:.0f} s)", file=sys.stderr, flush=True)
+    q = pmc_pass(a, ["SQ_INSTS_VALU", "SQ_INSTS_SALU", "SQ_WAVES", "GRBM_GUI_ACTIVE"])
+    print(f"bench.py: counter pass SQ done ({time.time() - t0:.0f} s)", file=sys.stderr, flush=True)
+    f_scale = calib / (f[COPY_KERNEL]["FETCH_SIZE"] * 1024.0)
+    w_scale = calib / (w[COPY_KERNEL]["WRITE_SIZE"] * 1024.0)
+    rd = f[RASTER_KERNEL]["FETCH_SIZE"] * 1024.0 * f_scale
+    wr = w[RASTER_KERNEL]["WRITE_SIZE"] * 1024.0 * w_scale
+    sq = q[RASTER_KERNEL]
+    cycles = sq["GRBM_GUI_ACTIVE"] / 8.0  # summed over the 8 XCDs
+    return {"traffic": int(rd + wr), "traffic_read": int(rd), "traffic_write": int(wr),
+            "fetch_scale": round(f_scale, 4), "write_scale": round(w_scale, 4),
+            "valu_insts": int(sq["SQ_INSTS_VALU"]), "salu_insts": int(sq["SQ_INSTS_SALU"]),
+            "valu_issue_frac": round(sq["SQ_INSTS_VALU"] * 4.0 / (MI355X_SIMDS * cycles), 4),
+            "source": f"rocprofv3 --pmc passes over bench.py --counter-probe in this run; bytes scaled "
+                      f"by a {CALIB_MIB} MiB device-to-device copy"}
+
+
+T_START = time.time()
+
+
+def note(msg):
+    """A progress line on stderr (the JSON line alone goes to stdout)."""
+    print(f"bench.py: {msg} ({time.time() - T_START:.0f} s)", file=sys.stderr, flush=True)
+
+
+def cpu_request_latency(nthreads, n_requests=6):
+    """Per-request latency of the serving mix on the CPU oracle (one frame at
+    a time, reference scheduling on `nthreads` threads): the first
+    `n_requests` requests of the seeded mix (64-mesh cache, {512^2, 1080p}),
+    mean and max milliseconds."""
+    import oracle
+    import scenes
+
+    specs = scenes.serve_mesh_specs(64)
+    reqs = scenes.serve_requests(n_requests)
+    ms = []
+    tex = (1024, 1024, scenes.checkerboard(1024, 32))
+    for r in reqs:
+        n, sd = specs[r["mesh"]]
+        mesh = scenes.serve_mesh(n, sd)
+        fb = oracle.Framebuffer(r["width"], r["height"], 0xFF000000)
+        import softrender_hip as sr
+        d = sr.make_desc(**r["desc"])
+        t0 = time.perf_counter()
+        oracle.draw(fb, mesh, d, tex, nthreads)
+        ms.append(1e3 * (time.perf_counter() - t0))
+    return {"threads": nthreads, "requests": n_requests, "mean_ms": round(float(sum(ms) / len(ms)), 2),
+            "max_ms": round(max(ms), 2),
+            "mesh_tris": [specs[r["mesh"]][0] for r in reqs]}
+
+
 def cpu_cores():
     try:
         n = len(os.sched_getaffinity(0))
@@ -129,6 +260,17 @@ def main():
         import batchbench
         print(json.dumps(batchbench.run(256, reps=a.steps)), flush=True)
         return
+    if a.counter_probe:
+        counter_probe(a)
+        return
+    # The counter passes run first, in child processes, before this process
+    # touches the GPU.
+    counters, counter_err = None, None
+    if not a.no_counters:
+        try:
+            counters = counter_passes(a)
+        except Exception as e:  # reported in the JSON line, never silently
+            counter_err = str(e)[:300]
     import numpy as np
 
     import scenes
@@ -164,6 +306,7 @@ def main():
             stages[k] = stages.get(k, 0.0) + v / a.steps
     ctx.finish()
     dt = time.perf_counter() - t0
+    note(f"{a.steps} timed frames done")
     ms = 1000.0 * dt / a.steps
     fps = a.steps / dt
     stats = ctx.last_stats()
@@ -208,31 +351,53 @@ def main():
         t0 = time.perf_counter()
         rc, st = oracle.draw(ofb, scene, desc, scene["texture"], n)
         cdt = time.perf_counter() - t0
+        note(f"CPU baseline frame done ({n} threads)")
         out["cpu_baseline"] = {
             "value": round(1.0 / cdt, 4), "unit": "frames/s", "cores": n, "kind": "port",
             "sample": f"1 full frame of the same workload, {n} threads, tile queue "
                       f"(every 128x128 tile visits all {ntris} triangles); {cdt:.2f} s"}
+        # the same frame on one thread (the sequential semantics), for scale
+        ofb1 = oracle.Framebuffer(W, H, scene["clear"])
+        t0 = time.perf_counter()
+        oracle.draw(ofb1, scene, desc, scene["texture"], 1)
+        c1 = time.perf_counter() - t0
+        note("CPU baseline frame done (1 thread)")
+        out["cpu_baseline"]["one_thread"] = {"value": round(1.0 / c1, 4), "unit": "frames/s", "cores": 1,
+                                             "seconds": round(c1, 2)}
+        out["cpu_baseline"]["serve_request_ms"] = cpu_request_latency(n)
+        note("CPU serving requests done")
         counted = {"bbox_tests": st["bbox_tests"] - st["strip_tests"], "covered": st["covered"]}
         if units is None:
             units = dict(counted, shaded=st["shaded"])  # seam races make this one approximate
         elif any(units[k] != v for k, v in counted.items()):
             raise SystemExit(f"bench.py: raster units changed: {counted} vs {units}")
     if units and stages.get("raster"):
-        flops = (FLOPS_PER_TEST * units["bbox_tests"] + FLOPS_PER_COVERED * units["covered"]
-                 + FLOPS_PER_SHADED * units["shaded"])
-        ach = flops / (stages["raster"] * 1e-3) / 1e12
-        ets = desc.exec_tile or 8
+        t_launch = stages["raster"] * 1e-3  # s, HIP events on the context stream, averaged over the timed steps
+        ops = OPS_PER_BBOX_TEST * units["bbox_tests"] + OPS_PER_SHADED * units["shaded"]
+        ach = ops / t_launch / 1e12
+        fine = (FLOPS_PER_TEST * units["bbox_tests"] + FLOPS_PER_COVERED * units["covered"]
+                + FLOPS_PER_SHADED * units["shaded"]) / t_launch / 1e12
         out["raster_units"] = units
         out["roofline"] = {"bound": "valu", "achieved": round(ach, 3), "peak": FP32_PEAK_TFLOPS,
                            "unit": "TFLOP/s", "frac": round(ach / FP32_PEAK_TFLOPS, 4),
-                           "traffic": profiled_traffic(ets) if default_workload else None,
-                           "kernel": "k_raster_tris", "avg_launch_ms": round(stages["raster"], 4)}
-        # the instruction-level bound: VALU issue share from the SQ counters of
-        # the committed profile (the flop fraction above counts only the
-        # reference's arithmetic)
-        busy = profiled_value(f"valu_issue_fraction_ets{ets}") if default_workload else None
-        if busy is not None:
-            out["roofline"]["valu_issue_frac_profiled"] = float(busy)
+                           "traffic": counters["traffic"] if counters else None,
+                           "kernel": "k_raster_tris", "avg_launch_ms": round(stages["raster"], 4),
+                           "ops_per_launch": ops,
+                           "ops_count": f"{OPS_PER_BBOX_TEST} per box pixel test + {OPS_PER_SHADED} per shaded fragment",
+                           "achieved_expression_count": round(fine, 3),
+                           "expression_count": f"{FLOPS_PER_TEST} per test + {FLOPS_PER_COVERED} per covered pixel + "
+                                               f"{FLOPS_PER_SHADED} per shaded fragment"}
+        if counters:
+            r = out["roofline"]
+            r["traffic_read"], r["traffic_write"] = counters["traffic_read"], counters["traffic_write"]
+            r["traffic_source"] = counters["source"]
+            r["hbm_gbs"] = round(counters["traffic"] / t_launch / 1e9, 1)
+            r["valu_issue_frac"] = counters["valu_issue_frac"]
+            if pairs:
+                r["valu_insts_per_pair"] = round(counters["valu_insts"] / pairs, 1)
+                r["salu_insts_per_pair"] = round(counters["salu_insts"] / pairs, 1)
+        elif counter_err:
+            out["roofline"]["traffic_error"] = counter_err
     mesh.free()
     tex.free()
     fb.free()
@@ -250,8 +415,10 @@ def extras(a):
     import serve
 
     out = {"clip_frame": clip_frame(a)}
+    note("clip frame done")
     t = time.time()
     b = batchbench.run(256, reps=3)
+    note("batch done")
     batch = {k: (round(v, 4) if isinstance(v, float) else v) for k, v in b.items()}
     batch["wall_s"] = round(time.time() - t, 1)
     t = time.time()
@@ -260,7 +427,9 @@ def extras(a):
     srv = {"meshes": sv["meshes"], "mean_mesh_tris": round(sv["mean_mesh_tris"]),
            "capacity_served_fps": round(sv["capacity_served"]["fps"], 1),
            "capacity_plain_fps": round(sv["capacity_plain"]["fps"], 1),
-           "capacity_ratio": round(sv["capacity_ratio"], 2), "rates": {}}
+           "capacity_ratio": round(sv["capacity_ratio"], 2),
+           "copy_ceiling_fps": {k: round(v, 1) for k, v in sv["copy_ceiling_fps"].items()},
+           "capacity_of_copy_ceiling": round(sv["capacity_of_copy_ceiling"], 3), "rates": {}}
     for r, row in sv["rates"].items():
         srv["rates"][r] = {
             "served": {k: round(v, 2) if isinstance(v, float) else v

@@ -83,10 +83,41 @@ extern "C" {
 #define SR_FP_BLINN_PHONG 1 /* needs SR_VP_MVP_NORMAL_WORLD */
 #define SR_FP_TEX_LAMBERT 2 /* needs SR_VP_MVP_NORMAL_UV and a texture */
 #define SR_FP_UV_COLOR 3    /* r=u g=v b=fs[0] a=fs[1]; needs SR_VP_MVP_NORMAL_UV */
+/* Several colour outputs at once (a texture buffer's colour tuple,
+ * texturebuffer.rs:84-86,133-136), for framebuffers made with
+ * sr_fb_create_ex; needs SR_VP_MVP_NORMAL_UV and a texture.  With n the
+ * normalised interpolated normal, diff = max(n . L, 0) (L = fs[0..2]),
+ * lit = fs[3] + fs[4] * diff and t the texture sample at (u, v):
+ *   plane 0: (t.r * lit, t.g * lit, t.b * lit, t.a * fs[5])  (as TEX_LAMBERT)
+ *   plane 1: (n.x * 0.5 + 0.5, n.y * 0.5 + 0.5, n.z * 0.5 + 0.5, 1)
+ *   plane 2: (u, v, diff, 1)
+ *   plane 3: (lit, lit, lit, 1)
+ * A framebuffer with fewer planes keeps the first ones; every other
+ * program writes its one colour to every plane. */
+#define SR_FP_GBUFFER 4
 
-/* Blend (color/blend.rs:9-74): `()` replaces, OVER is src-alpha over. */
+/* Blend (color/blend.rs:9-74): `()` replaces, OVER is src-alpha over,
+ * OVER_NORMALIZED the full example's blend (full_example/src/color.rs:5-17):
+ * per colour channel (x*a + y*b*(1-a)) / (a + b*(1-a)), alpha a + b*(1-a),
+ * with x, a the source and y, b the destination colour and alpha.  On RGBA8
+ * planes every blend works on unorm floats and quantises; on RGBAF32 planes
+ * it works on the floats as they are (no clamp, no quantisation). */
 #define SR_BLEND_REPLACE 0
 #define SR_BLEND_OVER 1
+#define SR_BLEND_OVER_NORMALIZED 2
+
+/* Framebuffer formats (sr_fb_create_ex).  Colour planes: Vector4<u8> and
+ * RGBAf32Color = Vector4<f32> (color/predefined.rs:17,68-90); depth
+ * (framebuffer/attachments/depth.rs:10-41): f32 or f64, far = the type's
+ * minimum, compared as d >= dt after Depth::from_scalar (exact for f64);
+ * stencil: u8 or `()` (stencil.rs:65-86: no test, no op). */
+#define SR_COLOR_RGBA8 0
+#define SR_COLOR_RGBAF32 1
+#define SR_DEPTH_F32 0
+#define SR_DEPTH_F64 1
+#define SR_STENCIL_U8 0
+#define SR_STENCIL_NONE 1
+#define SR_MAX_COLOR_PLANES 4
 
 /* Face culling (geometry/winding.rs:11-43; triangle.rs:54-61) */
 #define SR_CULL_NONE 0
@@ -166,15 +197,24 @@ typedef struct sr_draw_desc {
     float fs_uniforms[32];
     /* gs_uniforms: geometry program uniforms (normal visualisation). */
     float gs_uniforms[20];
-    int32_t exec_tile;   /* execution tile edge in pixels: 0 = auto, else 8/16/32/64 */
+    int32_t exec_tile;   /* execution tile edge in pixels: 0 = auto, else 8/16/32 */
     uint32_t exec_flags; /* SR_EXEC_* */
-    int32_t reserved[6];
+    /* waves (64 lanes each) sharing one execution tile: 0 = auto, else 1, 2
+     * or 4 where the tile size has it (8: 1; 16: 1, 2, 4; 32: 2, 4),
+     * the tile size's default (4 from 16 up) otherwise */
+    int32_t exec_waves;
+    int32_t reserved[5];
 } sr_draw_desc;
 
 typedef struct sr_draw_stats {
     uint64_t triangles_in, triangles_setup, lines_in, points_in;
     uint64_t bin_pairs;  /* (item, execution tile) pairs */
     uint64_t generated_tris, generated_lines, generated_points;
+    /* 1 when some triangle could pass the coverage test outside its own
+     * box (a sliver, through f32 rounding): it was binned to every
+     * execution tile such a pixel may lie in and tested in its box clamped
+     * into each screen tile, as triangle.rs:66-99 does */
+    uint64_t strip_slivers;
 } sr_draw_stats;
 
 /* ---- opaque handles ----------------------------------------------------- */
@@ -233,6 +273,38 @@ int sr_fb_read(sr_framebuffer fb, uint32_t* rgba, float* depth, uint8_t* stencil
 int sr_fb_write(sr_framebuffer fb, const uint32_t* rgba, const float* depth,
                 const uint8_t* stencil);
 int sr_fb_size(sr_framebuffer fb, uint32_t* width, uint32_t* height);
+
+/* Framebuffers of other formats, and texture buffers: `n_color` colour
+ * planes stored apart (declare_texture_buffer!, texturebuffer.rs:72-198:
+ * one Vec per colour field, so each can be sampled without a copy), each
+ * RGBA8 (4 bytes a pixel, r | g<<8 | b<<16 | a<<24) or RGBAF32 (16 bytes:
+ * r, g, b, a floats), plus depth and stencil.  sr_fb_create's framebuffer
+ * is {1, {RGBA8}, F32, U8}.  The functions above that take one RGBA8 colour
+ * (sr_fb_clear, sr_fb_read, sr_fb_write, sr_texture_from_fb) apply to
+ * plane 0 when it is RGBA8 and the depth is f32.  Triangle draws (with or
+ * without a geometry program) render into any format; line and point draws
+ * need sr_fb_create's format (SR_ERR_UNSUPPORTED otherwise). */
+typedef struct sr_fb_format {
+    uint32_t n_color;                      /* 1..SR_MAX_COLOR_PLANES */
+    int32_t color[SR_MAX_COLOR_PLANES];    /* SR_COLOR_* of each plane */
+    int32_t depth;                         /* SR_DEPTH_* */
+    int32_t stencil;                       /* SR_STENCIL_* */
+} sr_fb_format;
+int sr_fb_create_ex(sr_context ctx, uint32_t width, uint32_t height, const sr_fb_format* format,
+                    sr_framebuffer* out);
+int sr_fb_get_format(sr_framebuffer fb, sr_fb_format* out);
+/* Framebuffer::clear (renderbuffer/mod.rs:125-133, texturebuffer.rs:
+ * 183-198): every plane to its colour, depth to far, stencil to 0.
+ * `colors` holds each plane's colour in its format (4 or 16 bytes), packed
+ * in plane order. */
+int sr_fb_clear_ex(sr_framebuffer fb, const void* colors);
+/* Copy planes out / in: planes[i] (NULL to skip) is w*h pixels of plane i's
+ * format; depth is w*h floats or doubles; any pointer may be NULL. */
+int sr_fb_read_ex(sr_framebuffer fb, void* const* planes, void* depth, uint8_t* stencil);
+int sr_fb_write_ex(sr_framebuffer fb, const void* const* planes, const void* depth, const uint8_t* stencil);
+/* A texture reading colour plane `plane` of the framebuffer (RGBA8 or
+ * RGBAF32 texels, no copy): TextureBufferRef, texturebuffer.rs:12-49. */
+int sr_texture_from_fb_plane(sr_framebuffer fb, uint32_t plane, sr_texture* out);
 
 /* ---- ingestion: Wavefront OBJ -------------------------------------------
  * The reference's examples load meshes with tobj 0.1.3's load_obj
@@ -310,15 +382,39 @@ typedef struct sr_batch_item {
  * hipGraph and replayed while the batch is unchanged) when every item on
  * their framebuffer qualifies, shares execution tile, program and blend,
  * and only the first of them clears; other framebuffers' items are drawn
- * one by one.  No item may sample a framebuffer the batch draws into.
- * Synchronous. */
+ * one by one (before the pass).  No item may sample a framebuffer the batch
+ * draws into.  A NaN screen coordinate in any item of the one-pass group
+ * returns SR_ERR_INVALID_COORDINATE (as sr_draw does) with none of that
+ * group's framebuffers changed, its clears included.  Synchronous. */
 int sr_batch_draw(sr_context ctx, const sr_batch_item* items, uint32_t n);
+
+/* sr_batch_draw without waiting (the multi-mesh submission src/lib.rs:41-51
+ * asks for, decoupled from the host): items that cannot join the one-pass
+ * group are drawn before it returns, the one-pass group is queued on one
+ * of the context's batch streams (after the work already queued on the
+ * context's stream) and `ticket` names it.  Until sr_batch_wait returns for
+ * that ticket the batch's framebuffers, meshes and textures belong to it:
+ * no other call may read or write them.  Up to four batches are in flight;
+ * a fifth submit first completes the oldest.  Items and descriptors are
+ * copied. */
+int sr_batch_submit(sr_context ctx, const sr_batch_item* items, uint32_t n, uint64_t* ticket);
+/* SR_OK when the ticket's batch has finished on the device (sr_batch_wait
+ * will not block on it), SR_ERR_NOT_READY while it runs. */
+int sr_batch_query(sr_context ctx, uint64_t ticket);
+/* Blocks until the ticket's batch is complete (reruns after a bin
+ * overflow included) and returns its status;
+ * afterwards the ticket is unknown. */
+int sr_batch_wait(sr_context ctx, uint64_t ticket);
 
 /* Execution tile used when sr_draw_desc.exec_tile == 0: entry i applies to
  * framebuffers of at most max_pixels[i] pixels (first match wins).  Set by
  * the autotuner (rust-softrender_amd/autotune.py). */
 int sr_ctx_set_exec_tile_policy(sr_context ctx, uint32_t n, const uint64_t* max_pixels,
                                 const int32_t* exec_tile);
+/* The same with the waves per tile of each entry (exec_waves NULL: each
+ * tile size's default), used when sr_draw_desc.exec_tile == 0. */
+int sr_ctx_set_exec_policy(sr_context ctx, uint32_t n, const uint64_t* max_pixels, const int32_t* exec_tile,
+                           const int32_t* exec_waves);
 /* Initial (execution tile, triangle) pair capacity of the batch path; it
  * grows by itself when a batch overflows it. */
 int sr_ctx_set_batch_capacity(sr_context ctx, uint32_t pairs);

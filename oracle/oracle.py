@@ -22,7 +22,9 @@ F32_MIN = float(np.finfo(np.float32).min)
 
 class OrFb(C.Structure):
     _fields_ = [("w", C.c_uint32), ("h", C.c_uint32), ("color", C.POINTER(C.c_uint32)),
-                ("depth", C.POINTER(C.c_float)), ("stencil", C.POINTER(C.c_uint8))]
+                ("depth", C.POINTER(C.c_float)), ("stencil", C.POINTER(C.c_uint8)),
+                ("n_color", C.c_uint32), ("color_fmt", C.c_int32 * 4), ("planes", C.c_void_p * 4),
+                ("depth_fmt", C.c_int32), ("depth64", C.POINTER(C.c_double)), ("stencil_fmt", C.c_int32)]
 
 
 class OrMesh(C.Structure):
@@ -31,7 +33,8 @@ class OrMesh(C.Structure):
 
 
 class OrTexture(C.Structure):
-    _fields_ = [("w", C.c_uint32), ("h", C.c_uint32), ("rgba", C.POINTER(C.c_uint32))]
+    _fields_ = [("w", C.c_uint32), ("h", C.c_uint32), ("rgba", C.POINTER(C.c_uint32)),
+                ("rgbaf", C.POINTER(C.c_float))]
 
 
 STAT_FIELDS = ["tiles", "tri_calls", "tri_culled", "bbox_tests", "strip_tests", "covered",
@@ -123,12 +126,59 @@ class Framebuffer:
         return f
 
 
+class FramebufferEx:
+    """Host framebuffer of another format (sr_fb_create_ex): `planes` lists
+    each colour plane's format (0 RGBA8, 1 RGBAF32); cleared to `colors`
+    (per plane: RGBA8 uint32 or 4 floats), depth far, stencil 0."""
+
+    def __init__(self, w, h, planes, depth=0, stencil=0, colors=None):
+        self.w, self.h, self.fmts, self.depth_fmt, self.stencil_fmt = w, h, list(planes), depth, stencil
+        n = w * h
+        colors = colors or [0] * len(planes)
+        self.planes = []
+        for f, c in zip(self.fmts, colors):
+            if f == 1:
+                self.planes.append(np.tile(np.broadcast_to(np.asarray(c, np.float32), (4,)).reshape(1, 4), (n, 1)))
+            else:
+                self.planes.append(np.full(n, c, np.uint32))
+        self.depth = np.full(n, F32_MIN, np.float32)
+        self.depth64 = np.full(n, -np.finfo(np.float64).max, np.float64) if depth == 1 else None
+        self.stencil = np.zeros(n, np.uint8)
+
+    def struct(self):
+        fb = OrFb()
+        fb.w, fb.h = self.w, self.h
+        fb.depth = self.depth.ctypes.data_as(C.POINTER(C.c_float))
+        fb.stencil = self.stencil.ctypes.data_as(C.POINTER(C.c_uint8))
+        fb.n_color = len(self.fmts)
+        for i, (f, p) in enumerate(zip(self.fmts, self.planes)):
+            fb.color_fmt[i] = f
+            fb.planes[i] = p.ctypes.data
+        fb.depth_fmt = self.depth_fmt
+        if self.depth64 is not None:
+            fb.depth64 = self.depth64.ctypes.data_as(C.POINTER(C.c_double))
+        fb.stencil_fmt = self.stencil_fmt
+        return fb
+
+
+def plane_texture(fb: FramebufferEx, plane: int):
+    """A texture reading one plane of an oracle framebuffer (for draw's `tex`)."""
+    return ("plane", fb.w, fb.h, fb.planes[plane], fb.fmts[plane])
+
+
 def _texture(tex):
     if tex is None:
         return None, None
+    if tex[0] == "plane":
+        _, w, h, data, fmt = tex
+        if fmt == 1:
+            a = np.ascontiguousarray(data, np.float32)
+            return OrTexture(w, h, None, a.ctypes.data_as(C.POINTER(C.c_float))), a
+        data = np.ascontiguousarray(data, np.uint32)
+        return OrTexture(w, h, data.ctypes.data_as(C.POINTER(C.c_uint32)), None), data
     w, h, rgba = tex
     rgba = np.ascontiguousarray(rgba, np.uint32)
-    return OrTexture(w, h, rgba.ctypes.data_as(C.POINTER(C.c_uint32))), rgba
+    return OrTexture(w, h, rgba.ctypes.data_as(C.POINTER(C.c_uint32)), None), rgba
 
 
 def _or_mesh(mesh: dict):

@@ -354,21 +354,27 @@ static inline uint32_t pack4(uint32_t r, uint32_t g, uint32_t b, uint32_t a) {
     return r | (g << 8) | (b << 16) | (a << 24);
 }
 
+/* One texel as RGBA floats: RGBA8 texels as unorm (c / 255), RGBAF32
+ * texels (a texture buffer plane, texturebuffer.rs:12-49) as stored; the
+ * border colour is RGBA8 either way. */
 static void texel(const or_texture* t, int ix, int iy, int edge, uint32_t border, float o[4]) {
     int W = (int)t->w, H = (int)t->h;
-    uint32_t c;
+    int use_border = 0;
     if (edge == SR_EDGE_WRAP) {
         ix = ((ix % W) + W) % W;
         iy = ((iy % H) + H) % H;
-        c = t->rgba[(size_t)iy * W + ix];
     } else if (edge == SR_EDGE_BORDER) {
-        if (ix < 0 || iy < 0 || ix >= W || iy >= H) c = border;
-        else c = t->rgba[(size_t)iy * W + ix];
+        use_border = ix < 0 || iy < 0 || ix >= W || iy >= H;
     } else {
         ix = ix < 0 ? 0 : (ix > W - 1 ? W - 1 : ix);
         iy = iy < 0 ? 0 : (iy > H - 1 ? H - 1 : iy);
-        c = t->rgba[(size_t)iy * W + ix];
     }
+    if (!use_border && !t->rgba) {
+        const float* f = t->rgbaf + 4 * ((size_t)iy * W + ix);
+        for (int c = 0; c < 4; c++) o[c] = f[c];
+        return;
+    }
+    uint32_t c = use_border ? border : t->rgba[(size_t)iy * W + ix];
     o[0] = unorm8(c);
     o[1] = unorm8(c >> 8);
     o[2] = unorm8(c >> 16);
@@ -404,8 +410,11 @@ void or_sample(const or_texture* t, float u, float v, int filter, int edge, uint
     }
 }
 
-/* Fragment programs; k = interpolated varyings. */
-static uint32_t fs_run(const sr_draw_desc* d, const float* k, const or_texture* tex) {
+/* Fragment programs; k = interpolated varyings.  Each program's colours
+ * before quantisation: out[i] is output i (RGBA floats); returns how many
+ * outputs the program has (SR_FP_GBUFFER: 4, the others 1).  An RGBA8
+ * plane stores quant8 of each channel, an RGBAF32 plane the floats. */
+static int fs_run_f(const sr_draw_desc* d, const float* k, const or_texture* tex, float out[4][4]) {
     const float* U = d->fs_uniforms;
     switch (d->fragment_program) {
         case SR_FP_BLINN_PHONG: {
@@ -423,10 +432,12 @@ static uint32_t fs_run(const sr_draw_desc* d, const float* k, const or_texture* 
             s = s * s; s = s * s; s = s * s; s = s * s; s = s * s; /* ^32 */
             float lit = U[9] + U[10] * diff;
             float spec = U[11] * s;
-            return pack4(quant8(U[6] * lit + spec), quant8(U[7] * lit + spec),
-                         quant8(U[8] * lit + spec), quant8(U[12]));
+            out[0][0] = U[6] * lit + spec; out[0][1] = U[7] * lit + spec;
+            out[0][2] = U[8] * lit + spec; out[0][3] = U[12];
+            return 1;
         }
-        case SR_FP_TEX_LAMBERT: {
+        case SR_FP_TEX_LAMBERT:
+        case SR_FP_GBUFFER: {
             /* U: L[0..3) ambient[3] kd[4] alpha[5] */
             float n[3] = {k[0], k[1], k[2]};
             normalize3(n);
@@ -435,19 +446,61 @@ static uint32_t fs_run(const sr_draw_desc* d, const float* k, const or_texture* 
             float lit = U[3] + U[4] * diff;
             float t[4];
             or_sample(tex, k[3], k[4], d->tex_filter, d->tex_edge, d->tex_border, t);
-            return pack4(quant8(t[0] * lit), quant8(t[1] * lit), quant8(t[2] * lit),
-                         quant8(t[3] * U[5]));
+            out[0][0] = t[0] * lit; out[0][1] = t[1] * lit; out[0][2] = t[2] * lit; out[0][3] = t[3] * U[5];
+            if (d->fragment_program == SR_FP_TEX_LAMBERT) return 1;
+            out[1][0] = n[0] * 0.5f + 0.5f; out[1][1] = n[1] * 0.5f + 0.5f;
+            out[1][2] = n[2] * 0.5f + 0.5f; out[1][3] = 1.0f;
+            out[2][0] = k[3]; out[2][1] = k[4]; out[2][2] = diff; out[2][3] = 1.0f;
+            out[3][0] = lit; out[3][1] = lit; out[3][2] = lit; out[3][3] = 1.0f;
+            return 4;
         }
         case SR_FP_UV_COLOR:
-            return pack4(quant8(k[3]), quant8(k[4]), quant8(U[0]), quant8(U[1]));
+            out[0][0] = k[3]; out[0][1] = k[4]; out[0][2] = U[0]; out[0][3] = U[1];
+            return 1;
         default:
-            return pack4(quant8(U[0]), quant8(U[1]), quant8(U[2]), quant8(U[3]));
+            out[0][0] = U[0]; out[0][1] = U[1]; out[0][2] = U[2]; out[0][3] = U[3];
+            return 1;
     }
 }
 
-/* Blend::blend(src, dst) (color/blend.rs:9-31): `()` returns src; OVER is
- * the build-defined src-alpha-over in f32, quantised back to RGBA8. */
+static inline uint32_t pack_q(const float c[4]) {
+    return pack4(quant8(c[0]), quant8(c[1]), quant8(c[2]), quant8(c[3]));
+}
+
+static uint32_t fs_run(const sr_draw_desc* d, const float* k, const or_texture* tex) {
+    float out[4][4];
+    fs_run_f(d, k, tex, out);
+    return pack_q(out[0]);
+}
+
+/* Blend::blend(src, dst) (color/blend.rs:9-31) on RGBA floats: `()`
+ * returns src; OVER is the build-defined src-alpha over; OVER_NORMALIZED
+ * is the full example's over (full_example/src/color.rs:5-17), Rust's
+ * left-to-right evaluation: ((x * a) + ((y * b) * a1)) / (a + (b * a1)). */
+void or_blend_f32(int blend, const float s[4], const float d[4], float o[4]) {
+    if (blend == SR_BLEND_OVER) {
+        float a = s[3], ia = 1.0f - a;
+        for (int c = 0; c < 3; c++) o[c] = (s[c] * a) + (d[c] * ia);
+        o[3] = a + (d[3] * ia);
+    } else if (blend == SR_BLEND_OVER_NORMALIZED) {
+        float a = s[3], b = d[3], a1 = 1.0f - a;
+        for (int c = 0; c < 3; c++) o[c] = ((s[c] * a) + ((d[c] * b) * a1)) / (a + (b * a1));
+        o[3] = a + (b * (1.0f - a));
+    } else {
+        for (int c = 0; c < 4; c++) o[c] = s[c];
+    }
+}
+
+/* The blends on RGBA8: OVER as below, OVER_NORMALIZED on unorm floats,
+ * quantised back. */
 uint32_t or_blend(int blend, uint32_t s, uint32_t dst) {
+    if (blend == SR_BLEND_OVER_NORMALIZED) {
+        float fs[4] = {unorm8(s), unorm8(s >> 8), unorm8(s >> 16), unorm8(s >> 24)};
+        float fd[4] = {unorm8(dst), unorm8(dst >> 8), unorm8(dst >> 16), unorm8(dst >> 24)};
+        float o[4];
+        or_blend_f32(blend, fs, fd, o);
+        return pack_q(o);
+    }
     if (blend != SR_BLEND_OVER) return s;
     float a = unorm8(s >> 24), ia = 1.0f - a;
     uint32_t r = quant8((unorm8(s) * a) + (unorm8(dst) * ia));
@@ -538,8 +591,40 @@ typedef struct {
 /* Fragment tail shared by all three rasterisers: depth (triangle.rs:117-147,
  * line.rs:79-106, point.rs:300-322).  Depth attachment is f32, so
  * Depth::from_scalar is the identity (depth.rs:26-41). */
+/* shade_write on a framebuffer of another format (or_fb.n_color > 0):
+ * Depth::from_scalar is exact for f64, the program's outputs go to the
+ * planes (outputs past the program's count repeat its last), each blended
+ * in its own format. */
+static void shade_write_ex(rstate* rs, size_t index, float z, const float* k, uint64_t* shaded) {
+    or_fb* fb = rs->fb;
+    if (!(z < 0.0f)) return;
+    const double dt = fb->depth_fmt == SR_DEPTH_F64 ? fb->depth64[index] : (double)fb->depth[index];
+    if (!((double)z >= dt)) return;
+    float out[4][4];
+    const int n = fs_run_f(rs->d, k, rs->tex, out);
+    (*shaded)++;
+    for (uint32_t p = 0; p < fb->n_color; p++) {
+        const float* c = out[(int)p < n ? (int)p : n - 1];
+        if (fb->color_fmt[p] == SR_COLOR_RGBAF32) {
+            float* dst = (float*)fb->planes[p] + 4 * index;
+            float o[4];
+            or_blend_f32(rs->d->blend, c, dst, o);
+            for (int q = 0; q < 4; q++) dst[q] = o[q];
+        } else {
+            uint32_t* dst = (uint32_t*)fb->planes[p] + index;
+            *dst = or_blend(rs->d->blend, pack_q(c), *dst);
+        }
+    }
+    if (fb->depth_fmt == SR_DEPTH_F64) fb->depth64[index] = (double)z;
+    else fb->depth[index] = z;
+}
+
 static inline void shade_write(rstate* rs, size_t index, float z, const float* k, int is_line,
                                double alpha, uint64_t* shaded) {
+    if (rs->fb->n_color) {
+        shade_write_ex(rs, index, z, k, shaded);
+        return;
+    }
     if (z < 0.0f) {
         float dt = rs->fb->depth[index];
         if (z >= dt) {
@@ -597,9 +682,11 @@ static void raster_tri(rstate* rs, const otile* tile, const ovtx* a, const ovtx*
             float pcx = (float)px, pcy = (float)py;
             int strip = !(pcx + 1.0f > fmnx && pcx <= fmxx && pcy + 1.0f > fmny && pcy <= fmxy);
             if (strip) st->strip_tests++;
-            uint32_t sv = fb->stencil[index];
-            if (!or_stencil_test(d->stencil_test, sv, sref)) continue; /* :94 */
-            fb->stencil[index] = (uint8_t)or_stencil_op(d->stencil_op, d->stencil_wrap, sv, sref);
+            if (!(fb->n_color && fb->stencil_fmt == SR_STENCIL_NONE)) { /* `()`: no test, no op */
+                uint32_t sv = fb->stencil[index];
+                if (!or_stencil_test(d->stencil_test, sv, sref)) continue; /* :94 */
+                fb->stencil[index] = (uint8_t)or_stencil_op(d->stencil_op, d->stencil_wrap, sv, sref);
+            }
             float x = (float)px + 0.5f, y = (float)py + 0.5f; /* :104-105 */
             float u = ((y2 - y3) * (x - x3) + (x3 - x2) * (y - y3)) / det; /* :108 */
             float v = ((y3 - y1) * (x - x3) + (x1 - x3) * (y - y3)) / det; /* :109 */
@@ -1046,8 +1133,13 @@ int or_draw(or_fb* fb, const or_mesh* mesh, const sr_draw_desc* d, const or_text
     if (mesh->indices)
         for (size_t i = 0; i < nidx; i++)
             if (mesh->indices[i] >= mesh->n_vertices) return SR_ERR_INVALID_ARGUMENT;
-    if (d->fragment_program == SR_FP_TEX_LAMBERT && (!tex || !tex->rgba || !tex->w || !tex->h))
+    if ((d->fragment_program == SR_FP_TEX_LAMBERT || d->fragment_program == SR_FP_GBUFFER) &&
+        (!tex || !(tex->rgba || tex->rgbaf) || !tex->w || !tex->h))
         return SR_ERR_INVALID_ARGUMENT;
+    /* other framebuffer formats take triangle draws only */
+    if (fb->n_color && (mesh->primitive != SR_PRIM_TRIANGLES || d->geometry_program == SR_GS_VERTEX_NORMALS ||
+                        d->geometry_program == SR_GS_FACE_NORMALS))
+        return SR_ERR_UNSUPPORTED;
     if (nthreads < 1) nthreads = 1;
     or_stats local;
     memset(&local, 0, sizeof(local));

@@ -26,6 +26,16 @@ typedef struct or_fb {
     uint32_t* color;  /* RGBA8: r | g<<8 | b<<16 | a<<24 */
     float* depth;
     uint8_t* stencil;
+    /* Other formats (sr_fb_create_ex); n_color == 0 is the format above.
+     * Plane i holds w*h pixels of color_fmt[i] (RGBA8 u32 or RGBAF32 as 4
+     * floats); depth is `depth` (f32) or `depth64` (f64); with
+     * SR_STENCIL_NONE there is no stencil plane, test or op. */
+    uint32_t n_color;
+    int32_t color_fmt[SR_MAX_COLOR_PLANES];
+    void* planes[SR_MAX_COLOR_PLANES];
+    int32_t depth_fmt;
+    double* depth64;
+    int32_t stencil_fmt;
 } or_fb;
 
 typedef struct or_mesh {
@@ -38,7 +48,8 @@ typedef struct or_mesh {
 
 typedef struct or_texture {
     uint32_t w, h;
-    const uint32_t* rgba;
+    const uint32_t* rgba;  /* RGBA8 texels, or */
+    const float* rgbaf;    /* RGBAF32 texels (4 floats each) when rgba is NULL */
 } or_texture;
 
 typedef struct or_stats {
@@ -103,6 +114,8 @@ uint32_t or_stencil_op(int op, int wrap, uint32_t value, uint32_t ref);
 void or_sample(const or_texture* tex, float u, float v, int filter, int edge, uint32_t border,
                float out[4]);
 uint32_t or_blend(int blend, uint32_t src, uint32_t dst);
+/* The same blends on RGBAF32 colours (no clamp, no quantisation). */
+void or_blend_f32(int blend, const float src[4], const float dst[4], float out[4]);
 uint32_t or_mul_alpha(int alpha_mode, uint32_t color, double alpha);
 float or_hypotf(float x, float y);
 float or_div_f32(float a, float b);

@@ -31,8 +31,8 @@ COMMON = [
 ]
 
 UNITS = [("sr_kernels.hip", []), ("sr_runtime.hip", []), ("sr_server.hip", []), ("sr_obj.cpp", []),
-         ("sr_raster_lp.hip", [])] + [
-    ("sr_raster.hip", [f"-DSR_ETS={e}"]) for e in (8, 16, 32, 64)]
+         ("sr_raster_lp.hip", []), ("sr_raster_fmt.hip", [])] + [
+    ("sr_raster.hip", [f"-DSR_ETS={e}"]) for e in (8, 16, 32)]
 
 
 def _obj(src, defs):

@@ -89,20 +89,27 @@ struct Lut {
     __device__ __forceinline__ float operator()(uint32_t c) const { return t[c & 0xFFu]; }
 };
 
+// One texel as RGBA floats: RGBA8 texels as unorm (c / 255), RGBAF32
+// texels (a texture buffer plane, texturebuffer.rs:12-49) as stored; the
+// border colour is RGBA8 either way.
 __device__ __forceinline__ void texel(const RasterParams& P, int ix, int iy, Lut lut, float o[4]) {
     const int W = (int)P.texW, H = (int)P.texH;
-    uint32_t c;
+    bool border = false;
     if (P.tex_edge == SR_EDGE_WRAP) {
         ix = ((ix % W) + W) % W;
         iy = ((iy % H) + H) % H;
-        c = P.tex[(size_t)iy * W + ix];
     } else if (P.tex_edge == SR_EDGE_BORDER) {
-        c = (ix < 0 || iy < 0 || ix >= W || iy >= H) ? P.tex_border : P.tex[(size_t)iy * W + ix];
+        border = ix < 0 || iy < 0 || ix >= W || iy >= H;
     } else {
         ix = ix < 0 ? 0 : (ix > W - 1 ? W - 1 : ix);
         iy = iy < 0 ? 0 : (iy > H - 1 ? H - 1 : iy);
-        c = P.tex[(size_t)iy * W + ix];
     }
+    if (P.tex_fmt == SR_COLOR_RGBAF32 && !border) {
+        const float4 f = ((const float4*)P.tex)[(size_t)iy * W + ix];
+        o[0] = f.x; o[1] = f.y; o[2] = f.z; o[3] = f.w;
+        return;
+    }
+    const uint32_t c = border ? P.tex_border : P.tex[(size_t)iy * W + ix];
     o[0] = lut(c);
     o[1] = lut(c >> 8);
     o[2] = lut(c >> 16);
@@ -139,9 +146,11 @@ __device__ __forceinline__ void sample(const RasterParams& P, float u, float v, 
 }
 
 // Fragment programs (the closure passed to FragmentShader::run,
-// fragment.rs:168-169).  k = interpolated varyings.
+// fragment.rs:168-169).  k = interpolated varyings; o[i] = output i as
+// RGBA floats before quantisation.  Returns the program's output count
+// (SR_FP_GBUFFER: 4, the others 1).
 template <int FP>
-__device__ __forceinline__ uint32_t fragment(const RasterParams& P, const float* k, Lut lut) {
+__device__ __forceinline__ int fragment_rgba(const RasterParams& P, const float* k, Lut lut, float o[][4]) {
     const float* U = P.fsu;
     if constexpr (FP == SR_FP_BLINN_PHONG) {
         float nx = k[0], ny = k[1], nz = k[2];
@@ -157,9 +166,10 @@ __device__ __forceinline__ uint32_t fragment(const RasterParams& P, const float*
         s = s * s; s = s * s; s = s * s; s = s * s; s = s * s;
         float lit = U[9] + U[10] * diff;
         float spec = U[11] * s;
-        return pack4(quant8(U[6] * lit + spec), quant8(U[7] * lit + spec),
-                     quant8(U[8] * lit + spec), quant8(U[12]));
-    } else if constexpr (FP == SR_FP_TEX_LAMBERT) {
+        o[0][0] = U[6] * lit + spec; o[0][1] = U[7] * lit + spec;
+        o[0][2] = U[8] * lit + spec; o[0][3] = U[12];
+        return 1;
+    } else if constexpr (FP == SR_FP_TEX_LAMBERT || FP == SR_FP_GBUFFER) {
         float nx = k[0], ny = k[1], nz = k[2];
         normalize3(nx, ny, nz);
         float ndl = dot3(nx, ny, nz, U[0], U[1], U[2]);
@@ -167,23 +177,74 @@ __device__ __forceinline__ uint32_t fragment(const RasterParams& P, const float*
         float lit = U[3] + U[4] * diff;
         float t[4];
         sample(P, k[3], k[4], lut, t);
-        return pack4(quant8(t[0] * lit), quant8(t[1] * lit), quant8(t[2] * lit), quant8(t[3] * U[5]));
+        o[0][0] = t[0] * lit; o[0][1] = t[1] * lit; o[0][2] = t[2] * lit; o[0][3] = t[3] * U[5];
+        if constexpr (FP == SR_FP_TEX_LAMBERT) {
+            return 1;
+        } else {
+            o[1][0] = nx * 0.5f + 0.5f; o[1][1] = ny * 0.5f + 0.5f; o[1][2] = nz * 0.5f + 0.5f; o[1][3] = 1.0f;
+            o[2][0] = k[3]; o[2][1] = k[4]; o[2][2] = diff; o[2][3] = 1.0f;
+            o[3][0] = lit; o[3][1] = lit; o[3][2] = lit; o[3][3] = 1.0f;
+            return 4;
+        }
     } else if constexpr (FP == SR_FP_UV_COLOR) {
-        return pack4(quant8(k[3]), quant8(k[4]), quant8(U[0]), quant8(U[1]));
+        o[0][0] = k[3]; o[0][1] = k[4]; o[0][2] = U[0]; o[0][3] = U[1];
+        return 1;
     } else {
-        return pack4(quant8(U[0]), quant8(U[1]), quant8(U[2]), quant8(U[3]));
+        o[0][0] = U[0]; o[0][1] = U[1]; o[0][2] = U[2]; o[0][3] = U[3];
+        return 1;
     }
+}
+
+__device__ __forceinline__ uint32_t pack_q(const float c[4]) {
+    return pack4(quant8(c[0]), quant8(c[1]), quant8(c[2]), quant8(c[3]));
+}
+
+// The program's (first) colour on an RGBA8 plane.
+template <int FP>
+__device__ __forceinline__ uint32_t fragment(const RasterParams& P, const float* k, Lut lut) {
+    float o[FP == SR_FP_GBUFFER ? 4 : 1][4];
+    fragment_rgba<FP>(P, k, lut, o);
+    return pack_q(o[0]);
 }
 
 template <int FP>
 constexpr int fp_varyings() {
-    return FP == SR_FP_BLINN_PHONG ? 6 : (FP == SR_FP_TEX_LAMBERT || FP == SR_FP_UV_COLOR) ? 5 : 0;
+    return FP == SR_FP_BLINN_PHONG ? 6
+                                   : (FP == SR_FP_TEX_LAMBERT || FP == SR_FP_UV_COLOR || FP == SR_FP_GBUFFER) ? 5 : 0;
+}
+
+// Blend::blend(src, dst) on RGBA floats (RGBAF32 planes): `()` returns
+// src, OVER is src-alpha over, OVER_NORMALIZED the full example's over
+// (full_example/src/color.rs:5-17) in Rust's left-to-right order:
+// ((x * a) + ((y * b) * a1)) / (a + (b * a1)).
+template <int BLEND>
+__device__ __forceinline__ void blend_f32(const float s[4], const float d[4], float o[4]) {
+    if constexpr (BLEND == SR_BLEND_OVER) {
+        const float a = s[3], ia = 1.0f - a;
+#pragma unroll
+        for (int c = 0; c < 3; c++) o[c] = (s[c] * a) + (d[c] * ia);
+        o[3] = a + (d[3] * ia);
+    } else if constexpr (BLEND == SR_BLEND_OVER_NORMALIZED) {
+        const float a = s[3], b = d[3], a1 = 1.0f - a;
+#pragma unroll
+        for (int c = 0; c < 3; c++) o[c] = ((s[c] * a) + ((d[c] * b) * a1)) / (a + (b * a1));
+        o[3] = a + (b * (1.0f - a));
+    } else {
+#pragma unroll
+        for (int c = 0; c < 4; c++) o[c] = s[c];
+    }
 }
 
 // Blend::blend(src, dst), color/blend.rs:9-31
 template <int BLEND>
 __device__ __forceinline__ uint32_t blend(uint32_t s, uint32_t d, Lut lut) {
-    if constexpr (BLEND == SR_BLEND_OVER) {
+    if constexpr (BLEND == SR_BLEND_OVER_NORMALIZED) {  // on unorm floats, quantised back
+        const float fs[4] = {lut(s), lut(s >> 8), lut(s >> 16), lut(s >> 24)};
+        const float fd[4] = {lut(d), lut(d >> 8), lut(d >> 16), lut(d >> 24)};
+        float o[4];
+        blend_f32<BLEND>(fs, fd, o);
+        return pack_q(o);
+    } else if constexpr (BLEND == SR_BLEND_OVER) {
         float a = lut(s >> 24), ia = 1.0f - a;
         uint32_t r = quant8((lut(s) * a) + (lut(d) * ia));
         uint32_t g = quant8((lut(s >> 8) * a) + (lut(d >> 8) * ia));

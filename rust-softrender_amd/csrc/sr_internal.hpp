@@ -12,40 +12,63 @@ namespace sr {
 constexpr int MAXK = 8;          // varyings per vertex
 constexpr int SCAN_TILE = 4096;  // elements per scan / radix block (256 threads x 16)
 
-// Screen- (or clip-) space vertices, structure of arrays:
-//   p[0..4)[i]  = x, y, z, w          p[4+k][i] = varying k
-// One allocation; stride = capacity in vertices.
+// Screen- (or clip-) space vertices in one allocation: the position as
+// four streams (x, y, z, w: the setup and binning passes read x, y, z
+// coalesced), then the varyings interleaved per vertex, `kstride` floats a
+// row (a shaded fragment reads each of its three vertices' varyings as one
+// short contiguous row instead of one word from each of K streams).
+//   comp(c)[i] = position component c of vertex i (c < 4)
+//   var(i)[k]  = varying k of vertex i (k < kstride)
+// stride = capacity in vertices.
 struct VertexBuf {
     float* base;
     uint32_t stride;
+    uint32_t kstride;
     __host__ __device__ float* comp(int c) const { return base + (size_t)c * stride; }
+    __host__ __device__ float* var(uint32_t i) const { return base + 4 * (size_t)stride + (size_t)i * kstride; }
 };
 
 // Per-triangle setup record, written once by k_tri_setup and read by every
-// raster workgroup whose tile the triangle touches.  80 bytes.
-struct alignas(16) TriRec {
+// raster workgroup whose tile the triangle touches: 64 bytes, one cache
+// line, what the raster's per-triangle loop reads (into SGPRs).  The
+// triangle's vertex ids, needed only to shade, are kept apart (TriVid).
+struct alignas(64) TriRec {
     double R;             // RN64(1 / (double)det): f32 quotients are formed as (float)(num * R)
     float x3, y3;         // screen position of the third vertex
     float e23y, e23x;     // (y2 - y3), (x3 - x2)   — triangle.rs:108
     float e31y, e31x;     // (y3 - y1), (x1 - x3)   — triangle.rs:109
     float z1, z2, z3;
     int32_t X0, X1, Y0, Y1;  // floor of the f32 bbox, clamped to [-1, W] x [-1, H]
-    uint32_t i0, i1, i2;     // vertex ids in the screen vertex buffer
-    uint32_t tmask;          // binning: bit i = i-th execution tile of the box (row-major)
-                             // may be covered; all ones when the box has over 32 tiles
-    uint32_t pad;
+    uint32_t strip;          // 1: a pixel centre outside the box may pass the coverage
+                             // test; the raster tests the box clamped into each screen
+                             // tile (clamp_as_int!), as the reference does
 };
-static_assert(sizeof(TriRec) == 80, "TriRec layout");
+static_assert(sizeof(TriRec) == 64, "TriRec layout");
 
-// Per-triangle binning record: what pair emission and the stencil strips
-// read (the box and the coverable-tile mask), apart from the full record
-// for boxes of more than 32 tiles.  Kept apart from TriRec so those passes
-// stream 20 bytes per triangle instead of 80.
-struct TriBin {
-    int32_t X0, X1, Y0, Y1;  // as in TriRec
-    uint32_t tmask;          // as in TriRec
+// The vertex ids of a triangle in the screen vertex buffer (the shading
+// gathers' addresses).
+struct TriVid {
+    uint32_t i0, i1, i2;
 };
-static_assert(sizeof(TriBin) == 20, "TriBin layout");
+static_assert(sizeof(TriVid) == 12, "TriVid layout");
+
+// Per-triangle binning record: what pair emission reads (the pixel box
+// whose execution tiles get the triangle and the coverable-tile mask),
+// apart from the full record for culled boxes of more than 32 tiles.
+// Kept apart from TriRec so emission streams 24 bytes per triangle
+// instead of 88.
+enum BinMode : uint32_t {
+    BIN_BOX = 0,    // the tiles of the box (or of its mask, with culling)
+    BIN_REACH = 1,  // every tile of X0..Y1: the box grown to the pixels that may pass
+    BIN_CLAMP = 2,  // every tile meeting the box clamped into some screen tile
+};
+struct TriBin {
+    int32_t X0, X1, Y0, Y1;  // the box (BIN_BOX, BIN_CLAMP) or the reach (BIN_REACH)
+    uint32_t tmask;          // bit i = i-th execution tile of the box (row-major) may be
+                             // covered; all ones when the box has over 32 tiles
+    uint32_t mode;           // BinMode
+};
+static_assert(sizeof(TriBin) == 24, "TriBin layout");
 
 // Per (line, screen tile) work item: the segment clipped to one screen
 // tile by liang_barsky_iterative (geometry/line.rs:6-54), as rasterize_line
@@ -96,11 +119,13 @@ struct RasterParams {
     uint32_t scols, srows;
     // execution tiles
     uint32_t ets, ntx, nty;
+    int nwv;                // waves sharing an execution tile (1, 2 or 4)
     const uint32_t* tile_start;
     const uint32_t* tile_end;
     const uint32_t* items;  // sorted item ids per execution tile
     // primitives
     const TriRec* tris;
+    const TriVid* vids;     // per triangle, beside tris
     const LineItem* line_items;
     const Plot* plots;
     VertexBuf verts;        // screen-space vertices
@@ -113,7 +138,10 @@ struct RasterParams {
     int s_test, s_op, s_wrap;
     uint32_t s_ref;
     int smode;              // 0: stencil untouched, 1: count visits, 2: track in order
-    const uint32_t* tex;
+    int exhaustive;         // every triangle in every tile: the box is clamped into each
+                            // screen tile exactly as clamp_as_int! does
+    const uint32_t* tex;    // RGBA8 texels, or RGBAF32 (4 floats each) when tex_fmt says so
+    int tex_fmt;            // SR_COLOR_*
     uint32_t texW, texH;
     int tex_filter, tex_edge;
     uint32_t tex_border;
@@ -124,6 +152,17 @@ struct RasterParams {
     uint32_t tile_base;
     int clear;
     uint32_t clear_rgba;
+};
+
+// Framebuffers of other formats (sr_fb_create_ex): the colour planes, the
+// depth plane of f64 depth, and how stencil behaves.
+struct FmtParams {
+    void* planes[SR_MAX_COLOR_PLANES];
+    int color_fmt[SR_MAX_COLOR_PLANES];
+    uint32_t n_color;
+    double* depth64;      // f64 depth (else RasterParams::depth)
+    int track_stencil;    // u8 stencil with a test or op: tracked per pixel in order
+    int has_stencil;      // 0: `()` stencil, no plane
 };
 
 // Raster launch: one scene given inline (one work-group per tile), or a
@@ -140,7 +179,8 @@ struct RasterBatch {
     uint32_t ngroups;
     uint32_t tile0, ntiles;          // global tile range of this launch
     const uint32_t* wg_start;        // run p = tiles [tile0 + wg_start[p], tile0 + wg_start[p+1])
-    const uint32_t* overflow;        // non-zero: binning overflowed, draw nothing
+    const uint32_t* overflow;        // [0] binning overflowed, [2] bit 0: a NaN screen coordinate:
+                                     // either draws nothing (no fused clear either)
 };
 
 }  // namespace sr

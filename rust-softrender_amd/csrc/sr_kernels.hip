@@ -48,10 +48,10 @@ __device__ __forceinline__ void normalize_pos(float x, float y, float z, float w
 // One vertex program invocation (the VertexShader::run closure stand-in,
 // vertex.rs:97-109), optionally fused with the viewport transform
 // (run_to_fragment, vertex.rs:139).  Writes slot `o` of the SoA output.
-__device__ __forceinline__ void vertex_one(const float* U, int program, int K, int normalize,
-                                           const sr_viewport& viewport, const float pin[4],
-                                           const float nin[3], float u, float v,
-                                           const VertexBuf& out, uint32_t o) {
+__device__ __forceinline__ void vertex_compute(const float* U, int program, int normalize,
+                                               const sr_viewport& viewport, const float pin[4],
+                                               const float nin[3], float u, float v, float outp[4],
+                                               float var[6]) {
     float clip[4];
 #pragma unroll
     for (int r = 0; r < 4; r++) {
@@ -69,7 +69,6 @@ __device__ __forceinline__ void vertex_one(const float* U, int program, int K, i
         n[r] = acc;
     }
     normalize3(n[0], n[1], n[2]);
-    float var[6];
     var[0] = n[0]; var[1] = n[1]; var[2] = n[2];
     if (program == SR_VP_MVP_NORMAL_WORLD) {
 #pragma unroll
@@ -84,17 +83,25 @@ __device__ __forceinline__ void vertex_one(const float* U, int program, int K, i
         var[4] = v;
         var[5] = 0.0f;
     }
-    float outp[4];
     if (normalize) {
         normalize_pos(clip[0], clip[1], clip[2], clip[3], viewport, outp);
     } else {
         outp[0] = clip[0]; outp[1] = clip[1]; outp[2] = clip[2]; outp[3] = clip[3];
     }
+}
+
+__device__ __forceinline__ void vertex_one(const float* U, int program, int K, int normalize,
+                                           const sr_viewport& viewport, const float pin[4],
+                                           const float nin[3], float u, float v,
+                                           const VertexBuf& out, uint32_t o) {
+    float outp[4], var[6];
+    vertex_compute(U, program, normalize, viewport, pin, nin, u, v, outp, var);
     out.comp(0)[o] = outp[0];
     out.comp(1)[o] = outp[1];
     out.comp(2)[o] = outp[2];
     out.comp(3)[o] = outp[3];
-    for (int k = 0; k < K; k++) out.comp(4 + k)[o] = var[k];
+    float* const row = out.var(o);
+    for (int k = 0; k < K; k++) row[k] = var[k];
 }
 
 // GeometryShader::finish on clip-space vertices a draw chain shares with
@@ -107,7 +114,9 @@ __global__ __launch_bounds__(256) void k_finish(VertexBuf in, VertexBuf out, uin
     normalize_pos(in.comp(0)[i], in.comp(1)[i], in.comp(2)[i], in.comp(3)[i], vp, sp);
 #pragma unroll
     for (int c = 0; c < 4; c++) out.comp(c)[i] = sp[c];
-    for (int c = 0; c < K; c++) out.comp(4 + c)[i] = in.comp(4 + c)[i];
+    const float* const src = in.var(i);
+    float* const dst = out.var(i);
+    for (int c = 0; c < K; c++) dst[c] = src[c];
 }
 
 // One vertex program invocation per thread.  Inputs are coalesced SoA
@@ -209,7 +218,7 @@ __device__ __forceinline__ void load_cv(const VertexBuf& v, uint32_t i, int K, C
 #pragma unroll
     for (int c = 0; c < 4; c++) o.p[c] = v.comp(c)[i];
 #pragma unroll
-    for (int c = 0; c < 6; c++) o.k[c] = c < K ? v.comp(4 + c)[i] : 0.0f;
+    for (int c = 0; c < 6; c++) o.k[c] = c < K ? v.var(i)[c] : 0.0f;
 }
 
 // ClippingPlane::has_inside, geometry/clip.rs:33-44 (Left..Far = 0..5)
@@ -321,7 +330,7 @@ __device__ __forceinline__ void store_screen(const GSParams& P, uint32_t slot, c
     // static indices into v.k (a runtime index would put v in scratch memory)
 #pragma unroll
     for (int c = 0; c < 6; c++)
-        if (c < P.K) P.out.comp(4 + c)[slot] = v.k[c];
+        if (c < P.K) P.out.var(slot)[c] = v.k[c];
 }
 
 template <int MODE, int PRIM>
@@ -529,16 +538,147 @@ __device__ __forceinline__ bool span_tile_covered(const CoverPlanes& c, const Ti
              plane_max(c.wa, c.wb, c.wc, x0, x1, y0, y1) < -c.dw);
 }
 
-__device__ __forceinline__ uint32_t tri_setup_one(const VertexBuf& v, uint32_t i0, uint32_t i1,
-                                                  uint32_t i2, int cull, uint32_t Wi, uint32_t Hi,
-                                                  uint32_t ets, TriRec& r, uint32_t* err, int empty_cull) {
-    const float* sx = v.comp(0);
-    const float* sy = v.comp(1);
-    const float* sz = v.comp(2);
-    const float x1 = sx[i0], y1 = sy[i0], x2 = sx[i1], y2 = sy[i1], x3 = sx[i2], y3 = sy[i2];
-    r.i0 = i0; r.i1 = i1; r.i2 = i2;
-    r.tmask = 0u;
-    r.pad = 0u;
+// Strip coverage.  The binned raster tests a triangle only over its box;
+// the reference also tests, in every screen tile the box misses, the box
+// clamped into that tile (clamp_as_int!, triangle.rs:66-99): pixels outside
+// the box, where f32 rounding can pass the coverage test (a sliver whose
+// sharp end points at such a pixel).  A pixel centre p can pass only if the
+// exact barycentrics of the record's f32 values satisfy u >= -eu, v >= -ev,
+// w >= -ew, where eu bounds |u_f32 - u|: the two products and the sum of
+// triangle.rs:108 and the difference x - x3 each round once (3 * 2^-24 of
+// |e23y dx| + |e23x dy|), the quotient rounds to f32 (2^-24 |u|), so
+// |u_f32 - u| <= 2^-22 (T_u + |u|) with T_u = (|e23y| |dx| + |e23x| |dy|) /
+// |det|; where all three tests pass, u_f32, v_f32 <= 1, so |u|, |v| <= 2;
+// w = (1 - u) - v adds the two subtractions' rounding.  (Bounds are taken
+// 4 times larger.)
+//
+// With |dx|, |dy| <= d (p at Chebyshev distance d from (x3, y3)) the
+// margins are affine in d: eu(d) = 2^-20 ((|e23y| + |e23x|) d / |det| + 2),
+// ev(d) likewise, ew = eu + ev + 2^-18, and p can pass only inside T(d),
+// the triangle with vertices (u, v) = (-eu, -ev), (1 + ev + ew, -ev),
+// (-eu, 1 + eu + ew) mapped back to the screen.  T(d) grows with d and its
+// vertices lie within c0 + c1 d of (x3, y3).  With c1 < 1/2, no pixel
+// beyond D = (c0 + 2) / (1 - c1) lies in its own T(d) (which stays inside
+// distance d), so every pixel that can pass lies in T(D).
+//
+// Returns BIN_BOX when T(D) holds no pixel centre of the framebuffer
+// outside the box; BIN_REACH with [rx0, rx1] x [ry0, ry1] = the pixel
+// centres of T(D) joined to the box (inside the framebuffer); BIN_CLAMP
+// when no such D is found (an extreme sliver), and the triangle goes to
+// every execution tile of its clamped boxes.
+__device__ __forceinline__ uint32_t strip_reach(const TriRec& r, float det, uint32_t Wi, uint32_t Hi, int& rx0,
+                                                int& rx1, int& ry0, int& ry1) {
+    if (!(det != 0.0f) || !(fabsf(det) < __builtin_huge_valf())) return BIN_BOX;  // never shades
+    const double e23y = r.e23y, e23x = r.e23x, e31y = r.e31y, e31x = r.e31x;
+    const double x3 = r.x3, y3 = r.y3, df = (double)det;
+    const double dm = e23y * e31x - e23x * e31y;  // the 2x2 determinant of the f32 edges
+    if (!(dm != 0.0)) return BIN_CLAMP;
+    const double k = df / dm;
+    const double q = 0x1p-20;
+    const double au = q * (fabs(e23y) + fabs(e23x)) / fabs(df), bu = 2.0 * q;  // eu(d) = au d + bu
+    const double av = q * (fabs(e31y) + fabs(e31x)) / fabs(df), bv = 2.0 * q;
+    const double aw = au + av, bw = bu + bv + 0x1p-18;
+    // the widened vertices' (u, v) = (U0 + U1 d, V0 + V1 d)
+    const double U0[3] = {-bu, 1.0 + bv + bw, -bu}, U1[3] = {-au, av + aw, -au};
+    const double V0[3] = {-bv, -bv, 1.0 + bu + bw}, V1[3] = {-av, -av, au + aw};
+    double c0 = 0.0, c1 = 0.0;
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+        const double ax = k * (e31x * U0[i] - e23x * V0[i]), ay = k * (-e31y * U0[i] + e23y * V0[i]);
+        const double bx = k * (e31x * U1[i] - e23x * V1[i]), by = k * (-e31y * U1[i] + e23y * V1[i]);
+        c0 = fmax(c0, fmax(fabs(ax), fabs(ay)));
+        c1 = fmax(c1, fmax(fabs(bx), fabs(by)));
+    }
+    if (!(c1 < 0.5)) return BIN_CLAMP;
+    const double D = (c0 + 2.0) / (1.0 - c1);
+    const double eu = au * D + bu, ev = av * D + bv, ew = eu + ev + 0x1p-18;
+    if (!(D < 1e9) || !(eu < 0.25 && ev < 0.25)) return BIN_CLAMP;
+    const double us[3] = {-eu, 1.0 + ev + ew, -eu}, vs[3] = {-ev, -ev, 1.0 + eu + ew};
+    double mnx = __builtin_huge_val(), mxx = -mnx, mny = mnx, mxy = -mnx;
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+        const double x = x3 + k * (e31x * us[i] - e23x * vs[i]);
+        const double y = y3 + k * (-e31y * us[i] + e23y * vs[i]);
+        mnx = fmin(mnx, x); mxx = fmax(mxx, x);
+        mny = fmin(mny, y); mxy = fmax(mxy, y);
+    }
+    if (!(fabs(mnx) + fabs(mxx) + fabs(mny) + fabs(mxy) < 1e300)) return BIN_CLAMP;
+    const double m = 1e-3;  // covers the f64 rounding of the vertices
+    const int W = (int)Wi, H = (int)Hi;
+    if (!((r.X0 >= 1 && !(mnx > (double)r.X0 - 0.5 + m)) || (r.X1 <= W - 2 && !(mxx < (double)r.X1 + 1.5 - m)) ||
+          (r.Y0 >= 1 && !(mny > (double)r.Y0 - 0.5 + m)) || (r.Y1 <= H - 2 && !(mxy < (double)r.Y1 + 1.5 - m))))
+        return BIN_BOX;
+    // pixels i whose centre i + 0.5 lies in [mn - m, mx + m], joined to the box
+    rx0 = (int)fmax(fmin(ceil(mnx - 0.5 - m), (double)r.X0), 0.0);
+    rx1 = (int)fmin(fmax(floor(mxx - 0.5 + m), (double)r.X1), (double)(W - 1));
+    ry0 = (int)fmax(fmin(ceil(mny - 0.5 - m), (double)r.Y0), 0.0);
+    ry1 = (int)fmin(fmax(floor(mxy - 0.5 + m), (double)r.Y1), (double)(H - 1));
+    return BIN_REACH;
+}
+
+// The execution-tile columns (or rows) that the box [a0, a1] clamped into
+// each screen tile meets along one axis (screen tile j spans [j t,
+// min(j t + t, amax)], j < n, as clamp_as_int! sees it): f(lo, hi) for each
+// maximal run of them, in increasing order.  The clamped ranges move right
+// with j, so runs are merged as they come.
+template <class F>
+__device__ __forceinline__ void clamp_runs(int a0, int a1, uint32_t t, uint32_t n, int amax, uint32_t ets, F&& f) {
+    int lo = 0, hi = -1;
+    for (uint32_t j = 0; j < n; j++) {
+        const int T0 = (int)(j * t), T1 = min((int)(j * t + t), amax);
+        const int c0 = min(max(a0, T0), T1) / (int)ets, c1 = min(max(a1, T0), T1) / (int)ets;
+        if (hi >= lo && c0 <= hi + 1) {
+            hi = max(hi, c1);
+            continue;
+        }
+        if (hi >= lo) f(lo, hi);
+        lo = c0;
+        hi = c1;
+    }
+    if (hi >= lo) f(lo, hi);
+}
+
+// A draw's pixel and tile geometry, as binning needs it.
+struct BinGeom {
+    uint32_t W, H, ets, tw, th;
+    __device__ uint32_t scols() const { return W > 1 ? (W - 1 + tw - 1) / tw : 0u; }  // fragment.rs:196
+    __device__ uint32_t srows() const { return H > 1 ? (H - 1 + th - 1) / th : 0u; }
+};
+
+// BIN_CLAMP: the execution tiles met by the box clamped into every screen
+// tile.  The clamped box of screen tile (i, j) is the product of a column
+// range that depends on j alone and a row range that depends on i alone,
+// so the tiles are the product of the column runs and the row runs.
+__device__ __forceinline__ uint32_t clamp_tile_count(const TriBin& b, const BinGeom& g) {
+    uint32_t nc = 0, nr = 0;
+    clamp_runs(b.X0, b.X1, g.tw, g.scols(), (int)g.W - 1, g.ets, [&](int lo, int hi) { nc += (uint32_t)(hi - lo + 1); });
+    clamp_runs(b.Y0, b.Y1, g.th, g.srows(), (int)g.H - 1, g.ets, [&](int lo, int hi) { nr += (uint32_t)(hi - lo + 1); });
+    return nc * nr;
+}
+
+__device__ __forceinline__ void emit_clamp_pairs(const TriBin& b, const BinGeom& g, uint32_t ntx, uint32_t key_base,
+                                                 uint32_t val, uint32_t* keys, uint32_t* vals) {
+    uint32_t o = 0;
+    clamp_runs(b.Y0, b.Y1, g.th, g.srows(), (int)g.H - 1, g.ets, [&](int ylo, int yhi) {
+        for (int ey = ylo; ey <= yhi; ey++)
+            clamp_runs(b.X0, b.X1, g.tw, g.scols(), (int)g.W - 1, g.ets, [&](int xlo, int xhi) {
+                for (int ex = xlo; ex <= xhi; ex++) {
+                    keys[o] = key_base + (uint32_t)ey * ntx + (uint32_t)ex;
+                    vals[o] = val;
+                    o++;
+                }
+            });
+    });
+}
+
+// Setup of the triangle with screen vertices (x1,y1,z1), (x2,y2,z2),
+// (x3,y3,z3); returns its pair count.  (The caller stores the vertex ids in
+// the triangle's TriVid.)
+__device__ __forceinline__ uint32_t tri_setup_xyz(float x1, float y1, float z1, float x2, float y2, float z2,
+                                                  float x3, float y3, float z3, int cull, const BinGeom& g,
+                                                  TriRec& r, TriBin& b, uint32_t* err, int empty_cull) {
+    r.strip = 0u;
+    b = TriBin{1, 0, 1, 0, 0u, BIN_BOX};
     if (cull != SR_CULL_NONE) {
         // shoelace, triangle.rs:56 (left-to-right evaluation)
         const float A = x1 * y2 + x2 * y3 + x3 * y1 - x2 * y1 - x3 * y2 - x1 * y3;
@@ -555,37 +695,86 @@ __device__ __forceinline__ uint32_t tri_setup_one(const VertexBuf& v, uint32_t i
     r.x3 = x3; r.y3 = y3;
     r.e23y = y2 - y3; r.e23x = x3 - x2;
     r.e31y = y3 - y1; r.e31x = x1 - x3;
-    r.z1 = sz[i0]; r.z2 = sz[i1]; r.z3 = sz[i2];
+    r.z1 = z1; r.z2 = z2; r.z3 = z3;
     if (mnx != mnx || mxx != mxx || mny != mny || mxy != mxy) {
         // cast(NaN).unwrap() panics in clamp_as_int!, triangle.rs:70
         atomicOr(&err[0], 1u);
         r.X0 = 1; r.X1 = 0; r.Y0 = 1; r.Y1 = 0;
         return 0;
     }
-    const float W = (float)Wi, H = (float)Hi;
+    const float W = (float)g.W, H = (float)g.H;
     r.X0 = (int32_t)floorf(clampf(mnx, -1.0f, W));
     r.X1 = (int32_t)floorf(clampf(mxx, -1.0f, W));
     r.Y0 = (int32_t)floorf(clampf(mny, -1.0f, H));
     r.Y1 = (int32_t)floorf(clampf(mxy, -1.0f, H));
-    r.tmask = ~0u;
+    b = TriBin{r.X0, r.X1, r.Y0, r.Y1, ~0u, BIN_BOX};
+    int rx0, rx1, ry0, ry1;
+    const uint32_t mode = strip_reach(r, det, g.W, g.H, rx0, rx1, ry0, ry1);
+    if (mode != BIN_BOX) {
+        // a pixel outside the box may pass: every tile it may lie in, no culling
+        atomicOr(&err[0], 2u);
+        r.strip = 1u;
+        b.mode = mode;
+        if (mode == BIN_CLAMP) return clamp_tile_count(b, g);
+        b.X0 = rx0; b.X1 = rx1; b.Y0 = ry0; b.Y1 = ry1;
+        TileSpan sp;
+        if (!tile_span(b, g.W, g.H, g.ets, sp)) return 0;
+        return (sp.ex1 - sp.ex0 + 1) * (sp.ey1 - sp.ey0 + 1);
+    }
     TileSpan sp;
-    if (!tile_span(r, Wi, Hi, ets, sp)) return 0;
+    if (!tile_span(r, g.W, g.H, g.ets, sp)) return 0;
     const uint32_t ntile = (sp.ex1 - sp.ex0 + 1) * (sp.ey1 - sp.ey0 + 1);
     if (!empty_cull) return ntile;
     const CoverPlanes cp = cover_planes(r, sp);
     if (cp.never) {
-        r.tmask = 0u;
+        b.tmask = 0u;
         return 0;
     }
     uint32_t n = 0, m = 0, i = 0;
     for (uint32_t ey = sp.ey0; ey <= sp.ey1; ey++)
         for (uint32_t ex = sp.ex0; ex <= sp.ex1; ex++, i++) {
-            const bool c = span_tile_covered(cp, sp, ets, ex, ey);
+            const bool c = span_tile_covered(cp, sp, g.ets, ex, ey);
             n += c ? 1u : 0u;
             if (c && i < 32) m |= 1u << i;
         }
-    if (ntile <= 32) r.tmask = m;
+    if (ntile <= 32) b.tmask = m;
     return n;
+}
+
+__device__ __forceinline__ uint32_t tri_setup_one(const VertexBuf& v, uint32_t i0, uint32_t i1,
+                                                  uint32_t i2, int cull, const BinGeom& g, TriRec& r,
+                                                  TriBin& b, uint32_t* err, int empty_cull) {
+    const float* sx = v.comp(0);
+    const float* sy = v.comp(1);
+    const float* sz = v.comp(2);
+    return tri_setup_xyz(sx[i0], sy[i0], sz[i0], sx[i1], sy[i1], sz[i1], sx[i2], sy[i2], sz[i2], cull, g, r, b,
+                         err, empty_cull);
+}
+
+// The vertex program of a non-indexed triangle's three vertices (3t, 3t+1,
+// 3t+2: each belongs to this triangle alone) fused with its setup: the
+// screen positions stay in registers (the raster reads only the record),
+// the varyings go to the vertex buffer's rows.
+__device__ __forceinline__ uint32_t vertex_setup_soup(const float* U, int program, int K,
+                                                      const sr_viewport& vp, const float* px, const float* py,
+                                                      const float* pz, const float* nx, const float* ny,
+                                                      const float* nz, const float* pu, const float* pv,
+                                                      uint32_t l, uint32_t o, const VertexBuf& out, int cull,
+                                                      const BinGeom& g, TriRec& r, TriBin& b, uint32_t* err,
+                                                      int empty_cull) {
+    float s[3][4];
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        const uint32_t i = 3 * l + k;
+        const float pin[4] = {px[i], py[i], pz[i], 1.0f};
+        const float nin[3] = {nx ? nx[i] : 0.0f, ny ? ny[i] : 0.0f, nz ? nz[i] : 0.0f};
+        float var[6];
+        vertex_compute(U, program, 1, vp, pin, nin, pu ? pu[i] : 0.0f, pv ? pv[i] : 0.0f, s[k], var);
+        float* const row = out.var(o + k);
+        for (int c = 0; c < K; c++) row[c] = var[c];
+    }
+    return tri_setup_xyz(s[0][0], s[0][1], s[0][2], s[1][0], s[1][1], s[1][2], s[2][0], s[2][1], s[2][2], cull,
+                         g, r, b, err, empty_cull);
 }
 
 // Writes the (tile, item) pairs of one triangle: every tile of its box, or
@@ -622,30 +811,37 @@ __device__ __forceinline__ void emit_pairs(const TriBin& b, const TriRec* rec, c
 
 // Setup records of one work-group, staged in LDS so that they leave with
 // consecutive threads on consecutive 16-byte (records) and 4-byte (box
-// records) addresses rather than one 80-byte record per thread.
+// records, vertex ids) addresses rather than one record per thread.
 struct SetupStage {
     TriRec rec[256];
     TriBin bin[256];
+    TriVid vid[256];
 };
 
-__device__ __forceinline__ void stage_setup(SetupStage& st, const TriRec& r) {
+__device__ __forceinline__ void stage_setup(SetupStage& st, const TriRec& r, const TriBin& b, uint32_t i0,
+                                            uint32_t i1, uint32_t i2) {
     st.rec[threadIdx.x] = r;
-    st.bin[threadIdx.x] = TriBin{r.X0, r.X1, r.Y0, r.Y1, r.tmask};
+    st.bin[threadIdx.x] = b;
+    st.vid[threadIdx.x] = TriVid{i0, i1, i2};
 }
 
 // Every thread of the work-group calls it (after staging its record, if any).
-__device__ __forceinline__ void store_setup(const SetupStage& st, TriRec* recs, TriBin* bins, uint32_t n) {
+__device__ __forceinline__ void store_setup(const SetupStage& st, TriRec* recs, TriBin* bins, TriVid* vids,
+                                            uint32_t n) {
     __syncthreads();
     const uint32_t b0 = blockIdx.x * 256u;
     if (b0 >= n) return;
     const uint32_t m = min(256u, n - b0);
-    constexpr uint32_t RW = sizeof(TriRec) / 16, BW = sizeof(TriBin) / 4;
+    constexpr uint32_t RW = sizeof(TriRec) / 16, BW = sizeof(TriBin) / 4, VW = sizeof(TriVid) / 4;
     const uint4* rs = (const uint4*)st.rec;
     uint4* rd = (uint4*)(recs + b0);
     for (uint32_t i = threadIdx.x; i < m * RW; i += 256u) rd[i] = rs[i];
     const uint32_t* bs = (const uint32_t*)st.bin;
     uint32_t* bd = (uint32_t*)(bins + b0);
     for (uint32_t i = threadIdx.x; i < m * BW; i += 256u) bd[i] = bs[i];
+    const uint32_t* vs = (const uint32_t*)st.vid;
+    uint32_t* vd = (uint32_t*)(vids + b0);
+    for (uint32_t i = threadIdx.x; i < m * VW; i += 256u) vd[i] = vs[i];
 }
 
 __global__ __launch_bounds__(256) void k_tri_setup(SetupParams P) {
@@ -653,27 +849,104 @@ __global__ __launch_bounds__(256) void k_tri_setup(SetupParams P) {
     const uint32_t t = blockIdx.x * 256u + threadIdx.x;
     if (t < P.n) {
         TriRec r;
-        P.counts[t] = tri_setup_one(P.v, prim_vertex(P.idx, 3 * t), prim_vertex(P.idx, 3 * t + 1),
-                                    prim_vertex(P.idx, 3 * t + 2), P.cull, P.W, P.H, P.ets, r, P.err,
-                                    P.empty_cull);
-        stage_setup(st, r);
+        TriBin b;
+        const uint32_t i0 = prim_vertex(P.idx, 3 * t), i1 = prim_vertex(P.idx, 3 * t + 1),
+                       i2 = prim_vertex(P.idx, 3 * t + 2);
+        const BinGeom g{P.W, P.H, P.ets, P.tw, P.th};
+        P.counts[t] = tri_setup_one(P.v, i0, i1, i2, P.cull, g, r, b, P.err, P.empty_cull);
+        stage_setup(st, r, b, i0, i1, i2);
     }
-    store_setup(st, P.recs, P.bins, P.n);
+    store_setup(st, P.recs, P.bins, P.vids, P.n);
 }
 
 __global__ __launch_bounds__(256) void k_tri_setup_batch(BatchTables T, VertexBuf v, TriRec* recs,
-                                                        TriBin* bins, uint32_t* counts, uint32_t* err) {
+                                                        TriBin* bins, TriVid* vids, uint32_t* counts,
+                                                        uint32_t* err) {
     __shared__ SetupStage st;
     const uint32_t t = blockIdx.x * 256u + threadIdx.x;
     with_scene(T, T.tend, T.nt, t, [&](const SceneGeom& S) {
         const uint32_t l = t - S.tbase;
         TriRec r;
-        counts[t] = tri_setup_one(v, S.vbase + prim_vertex(S.idx, 3 * l), S.vbase + prim_vertex(S.idx, 3 * l + 1),
-                                  S.vbase + prim_vertex(S.idx, 3 * l + 2), S.cull, S.W, S.H, S.ets, r, err,
-                                  S.empty_cull);
-        stage_setup(st, r);
+        TriBin b;
+        const uint32_t i0 = S.vbase + prim_vertex(S.idx, 3 * l), i1 = S.vbase + prim_vertex(S.idx, 3 * l + 1),
+                       i2 = S.vbase + prim_vertex(S.idx, 3 * l + 2);
+        const BinGeom g{S.W, S.H, S.ets, S.tw, S.th};
+        counts[t] = tri_setup_one(v, i0, i1, i2, S.cull, g, r, b, err, S.empty_cull);
+        stage_setup(st, r, b, i0, i1, i2);
     });
-    store_setup(st, recs, bins, T.nt);
+    store_setup(st, recs, bins, vids, T.nt);
+}
+
+// k_vertex + k_tri_setup for a non-indexed mesh without a geometry program.
+// One thread per triangle runs the vertex program on its three vertices and
+// the setup; the block's 768 vertices come in through LDS (each stream read
+// by consecutive threads) and their varyings leave through it (consecutive
+// threads write consecutive words of the rows), then the LDS holds the
+// setup records on their way out.
+__global__ __launch_bounds__(256) void k_vertex_setup_soup(VSParams V, SetupParams P) {
+    __shared__ union {
+        SetupStage st;
+        float io[8 * 768];  // the 8 vertex streams, then the varying rows
+    } sh;
+    const uint32_t b0 = blockIdx.x * 256u, t = b0 + threadIdx.x;
+    const uint32_t nt = min(256u, P.n - b0), v0 = 3u * b0, nv = 3u * nt;
+    const float* const src[8] = {V.px, V.py, V.pz, V.nx, V.ny, V.nz, V.u, V.v};
+#pragma unroll
+    for (int c = 0; c < 8; c++) {
+        if (!src[c]) continue;
+        for (uint32_t i = threadIdx.x; i < nv; i += 256u) sh.io[c * 768 + i] = src[c][v0 + i];
+    }
+    __syncthreads();
+    const bool live = t < P.n;
+    float s[3][4], var[3][6];
+    if (live) {
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            const uint32_t i = 3u * threadIdx.x + k;
+            const float pin[4] = {sh.io[i], sh.io[768 + i], sh.io[2 * 768 + i], 1.0f};
+            const float nin[3] = {V.nx ? sh.io[3 * 768 + i] : 0.0f, V.ny ? sh.io[4 * 768 + i] : 0.0f,
+                                  V.nz ? sh.io[5 * 768 + i] : 0.0f};
+            vertex_compute(V.U, V.program, 1, V.viewport, pin, nin, V.u ? sh.io[6 * 768 + i] : 0.0f,
+                           V.v ? sh.io[7 * 768 + i] : 0.0f, s[k], var[k]);
+        }
+    }
+    __syncthreads();
+    const uint32_t K = V.out.kstride;  // == V.K: rows of the vertex buffer
+    if (live)
+#pragma unroll
+        for (int k = 0; k < 3; k++)
+            for (uint32_t c = 0; c < K; c++) sh.io[(3u * threadIdx.x + k) * K + c] = var[k][c];
+    __syncthreads();
+    float* const dst = V.out.var(v0);
+    for (uint32_t i = threadIdx.x; i < nv * K; i += 256u) dst[i] = sh.io[i];
+    __syncthreads();
+    if (live) {
+        TriRec r;
+        TriBin b;
+        const BinGeom g{P.W, P.H, P.ets, P.tw, P.th};
+        P.counts[t] = tri_setup_xyz(s[0][0], s[0][1], s[0][2], s[1][0], s[1][1], s[1][2], s[2][0], s[2][1],
+                                    s[2][2], P.cull, g, r, b, P.err, P.empty_cull);
+        stage_setup(sh.st, r, b, 3 * t, 3 * t + 1, 3 * t + 2);
+    }
+    store_setup(sh.st, P.recs, P.bins, P.vids, P.n);
+}
+
+// The same over a batch whose meshes are all non-indexed.
+__global__ __launch_bounds__(256) void k_vertex_setup_soup_batch(BatchTables T, VertexBuf v, TriRec* recs,
+                                                                TriBin* bins, TriVid* vids, uint32_t* counts,
+                                                                uint32_t* err) {
+    __shared__ SetupStage st;
+    const uint32_t t = blockIdx.x * 256u + threadIdx.x;
+    with_scene(T, T.tend, T.nt, t, [&](const SceneGeom& S) {
+        const uint32_t l = t - S.tbase, o = S.vbase + 3 * l;
+        TriRec r;
+        TriBin b;
+        const BinGeom g{S.W, S.H, S.ets, S.tw, S.th};
+        counts[t] = vertex_setup_soup(S.U, S.program, S.K, S.viewport, S.px, S.py, S.pz, S.nx, S.ny, S.nz, S.u,
+                                      S.v, l, o, v, S.cull, g, r, b, err, S.empty_cull);
+        stage_setup(st, r, b, o, o + 1, o + 2);
+    });
+    store_setup(st, recs, bins, vids, T.nt);
 }
 
 // (execution tile, triangle) pairs of a batch; keys are global tile ids.
@@ -775,29 +1048,35 @@ __global__ __launch_bounds__(256) void k_tri_emit_batch(BatchTables T, const Tri
         atomicOr(overflow, 1u);
         c = 0;
     }
-    TriBin b{1, 0, 1, 0, 0u};
+    TriBin b{1, 0, 1, 0, 0u, BIN_BOX};
     TileSpan sp{0, 0, 0, 0, 0u, 0u, 0u, 0u};
     bool direct = false, all = false;
-    uint32_t ntx = 0, kbase = 0, W = 0, H = 0;
-    int ets = 8, empty_cull = 0;
+    uint32_t ntx = 0, kbase = 0;
+    BinGeom g{1, 1, 8, 128, 128};
+    int empty_cull = 0;
     // the scene's fields, through the scalar cache when the wave's
     // triangles share one scene (nearly always)
     with_scene(T, T.tend, T.nt, t, [&](const SceneGeom& S) {
-        ntx = S.ntx; kbase = S.tile_base; ets = (int)S.ets; empty_cull = S.empty_cull; W = S.W; H = S.H;
+        ntx = S.ntx; kbase = S.tile_base; empty_cull = S.empty_cull;
+        g = BinGeom{S.W, S.H, S.ets, S.tw, S.th};
     });
     if (c) {
         b = bins[t];
-        if (tile_span(b, W, H, (uint32_t)ets, sp)) {
+        if (b.mode == BIN_CLAMP) {
+            direct = false;
+        } else if (tile_span(b, g.W, g.H, g.ets, sp)) {
             const uint32_t nt = (sp.ex1 - sp.ex0 + 1) * (sp.ey1 - sp.ey0 + 1);
-            all = !empty_cull;
+            all = !empty_cull || b.mode == BIN_REACH;
             direct = all || nt <= 32;
         } else {
             c = 0;
         }
     }
     emit_wave(ew[threadIdx.x >> 6], direct, c, o, b.tmask, all, sp, ntx, kbase, t, keys, vals);
-    if (c && !direct)
-        emit_pairs(b, recs + t, sp, (uint32_t)ets, ntx, empty_cull, kbase, t, keys + o, vals + o);
+    if (c && !direct) {
+        if (b.mode == BIN_CLAMP) emit_clamp_pairs(b, g, ntx, kbase, t, keys + o, vals + o);
+        else emit_pairs(b, recs + t, sp, g.ets, ntx, empty_cull, kbase, t, keys + o, vals + o);
+    }
 }
 
 __global__ __launch_bounds__(256) void k_tri_emit(EmitParams P) {
@@ -807,23 +1086,31 @@ __global__ __launch_bounds__(256) void k_tri_emit(EmitParams P) {
     uint32_t c = t < P.n ? P.counts[t] : 0u;
     const uint32_t o = c ? P.offsets[t] : 0u;
     if (c && (uint64_t)o + c > P.cap) c = 0;  // the host emits again into a larger buffer
-    TriBin b{1, 0, 1, 0, 0u};
+    TriBin b{1, 0, 1, 0, 0u, BIN_BOX};
     TileSpan sp{0, 0, 0, 0, 0u, 0u, 0u, 0u};
-    bool direct = false;
+    bool direct = false, all = false;
     if (c) {
         b = P.bins[t];
-        if (tile_span(b, P.W, P.H, P.ets, sp)) {
+        if (b.mode == BIN_CLAMP) {
+            direct = false;
+        } else if (tile_span(b, P.W, P.H, P.ets, sp)) {
             const uint32_t nt = (sp.ex1 - sp.ex0 + 1) * (sp.ey1 - sp.ey0 + 1);
-            direct = !P.empty_cull || nt <= 32;
+            all = !P.empty_cull || b.mode == BIN_REACH;
+            direct = all || nt <= 32;
         } else {
             c = 0;
         }
     }
-    emit_wave(ew[threadIdx.x >> 6], direct, c, o, b.tmask, !P.empty_cull, sp, P.ntx, P.key_base, P.val_base + t,
-              P.keys, P.vals);
-    if (c && !direct)
-        emit_pairs(b, P.recs + t, sp, P.ets, P.ntx, P.empty_cull, P.key_base, P.val_base + t, P.keys + o,
-                   P.vals + o);
+    emit_wave(ew[threadIdx.x >> 6], direct, c, o, b.tmask, all, sp, P.ntx, P.key_base, P.val_base + t, P.keys,
+              P.vals);
+    if (c && !direct) {
+        if (b.mode == BIN_CLAMP)
+            emit_clamp_pairs(b, BinGeom{P.W, P.H, P.ets, P.tw, P.th}, P.ntx, P.key_base, P.val_base + t, P.keys + o,
+                             P.vals + o);
+        else
+            emit_pairs(b, P.recs + t, sp, P.ets, P.ntx, P.empty_cull, P.key_base, P.val_base + t, P.keys + o,
+                       P.vals + o);
+    }
 }
 
 // ===========================================================================
@@ -1187,6 +1474,15 @@ __global__ __launch_bounds__(256) void k_fill_u32(uint32_t* p, uint32_t v, uint3
     if (i < n) p[i] = v;
 }
 
+// p[i] = pattern[i % period] for i < n words (period 1, 2 or 4): a plane
+// cleared to one colour or depth value of any format.
+__global__ __launch_bounds__(256) void k_fill_pattern(uint32_t* p, uint4 pat, uint32_t period, size_t n) {
+    const size_t i = (size_t)blockIdx.x * 256u + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t k = (uint32_t)(i & (period - 1));
+    p[i] = k == 0 ? pat.x : k == 1 ? pat.y : k == 2 ? pat.z : pat.w;
+}
+
 __global__ __launch_bounds__(256) void k_iota_u32(uint32_t* p, uint32_t n) {
     const uint32_t i = blockIdx.x * 256u + threadIdx.x;
     if (i < n) p[i] = i;
@@ -1219,6 +1515,24 @@ __global__ __launch_bounds__(256) void k_partition(const uint32_t* prefix, uint3
 // may be far below the capacity the launch was sized for).
 // Each thread takes four consecutive keys (one 16-byte load when `keys` is
 // 16-byte aligned) and their two neighbours.
+// Every tile's [start, end) in the sorted keys by binary search: thread t
+// finds the first key >= t, which starts tile t and ends tile t - 1, so
+// empty tiles need no clearing beforehand.  For a frame's tiles (tens of
+// thousands); a batch's millions take k_ranges over the pairs instead.
+__global__ __launch_bounds__(256) void k_ranges_search(const uint32_t* keys, uint32_t n, uint32_t nkeys,
+                                                       uint32_t* start, uint32_t* end) {
+    const uint32_t t = blockIdx.x * 256u + threadIdx.x;
+    if (t > nkeys) return;
+    uint32_t lo = 0, hi = n;
+    while (lo < hi) {
+        const uint32_t m = (lo + hi) >> 1;
+        if (keys[m] < t) lo = m + 1;
+        else hi = m;
+    }
+    if (t < nkeys) start[t] = lo;
+    if (t > 0) end[t - 1] = lo;
+}
+
 __global__ __launch_bounds__(256) void k_ranges(const uint32_t* keys, uint32_t n, uint32_t key_base,
                                                uint32_t nkeys, uint32_t* start, uint32_t* end,
                                                const uint32_t* n_dev, int aligned) {
@@ -1348,7 +1662,7 @@ __global__ __launch_bounds__(256) void k_strips_accum(StripParams P) {
         const bool in = t < P.n;
         if (!hot && !in) break;
         if (hot && __syncthreads_or(in) == 0) break;
-        const TriBin* r = in ? &P.bins[t] : nullptr;
+        const TriRec* r = in ? &P.recs[t] : nullptr;  // the box itself (a bin may hold a reach)
         const int X0 = r ? r->X0 : 1, X1 = r ? r->X1 : 0, Y0 = r ? r->Y0 : 1, Y1 = r ? r->Y1 : 0;
         if (X0 <= X1) {  // not culled
             uint32_t nR, jL, nB, iA;  // columns right-of-box-status prefix etc.
@@ -1969,10 +2283,18 @@ void launch_tri_setup(const SetupParams& P, hipStream_t s) {
 void launch_vertex_batch(const BatchTables& T, const VertexBuf& out, hipStream_t s) {
     if (T.nv) hipLaunchKernelGGL(k_vertex_batch, dim3(blocks_for(T.nv, 256)), dim3(256), 0, s, T, out);
 }
-void launch_tri_setup_batch(const BatchTables& T, const VertexBuf& v, TriRec* recs, TriBin* bins,
+void launch_vertex_setup_soup(const VSParams& V, const SetupParams& P, hipStream_t s) {
+    if (P.n) hipLaunchKernelGGL(k_vertex_setup_soup, dim3(blocks_for(P.n, 256)), dim3(256), 0, s, V, P);
+}
+void launch_vertex_setup_soup_batch(const BatchTables& T, const VertexBuf& v, TriRec* recs, TriBin* bins,
+                                    TriVid* vids, uint32_t* counts, uint32_t* err, hipStream_t s) {
+    if (T.nt) hipLaunchKernelGGL(k_vertex_setup_soup_batch, dim3(blocks_for(T.nt, 256)), dim3(256), 0, s, T, v,
+                                 recs, bins, vids, counts, err);
+}
+void launch_tri_setup_batch(const BatchTables& T, const VertexBuf& v, TriRec* recs, TriBin* bins, TriVid* vids,
                             uint32_t* counts, uint32_t* err, hipStream_t s) {
     if (T.nt) hipLaunchKernelGGL(k_tri_setup_batch, dim3(blocks_for(T.nt, 256)), dim3(256), 0, s, T, v,
-                                 recs, bins, counts, err);
+                                 recs, bins, vids, counts, err);
 }
 void launch_tri_emit_batch(const BatchTables& T, const TriRec* recs, const TriBin* bins, const uint32_t* counts,
                            const uint32_t* offsets, uint32_t cap, uint32_t* keys, uint32_t* vals,
@@ -2060,8 +2382,20 @@ void launch_fill(uint32_t* p, uint32_t v, uint32_t n, hipStream_t s) {
     if (n) hipLaunchKernelGGL(k_fill_u32, dim3(blocks_for(n, 256)), dim3(256), 0, s, p, v, n);
 }
 
+void launch_fill_pattern(void* p, const uint32_t pat[4], uint32_t period, size_t n_words, hipStream_t s) {
+    if (n_words)
+        hipLaunchKernelGGL(k_fill_pattern, dim3((uint32_t)((n_words + 255) / 256)), dim3(256), 0, s, (uint32_t*)p,
+                           make_uint4(pat[0], pat[1], pat[2], pat[3]), period, n_words);
+}
+
 void launch_iota(uint32_t* p, uint32_t n, hipStream_t s) {
     if (n) hipLaunchKernelGGL(k_iota_u32, dim3(blocks_for(n, 256)), dim3(256), 0, s, p, n);
+}
+
+void launch_ranges_search(const uint32_t* keys, uint32_t n, uint32_t nkeys, uint32_t* start, uint32_t* end,
+                          hipStream_t s) {
+    hipLaunchKernelGGL(k_ranges_search, dim3(blocks_for((uint64_t)nkeys + 1, 256)), dim3(256), 0, s, keys, n, nkeys,
+                       start, end);
 }
 
 void launch_ranges(const uint32_t* keys, uint32_t n, uint32_t key_base, uint32_t nkeys, uint32_t* start,
@@ -2097,7 +2431,6 @@ void launch_raster_tris(const RasterBatch& P, uint32_t ntiles, hipStream_t s) {
     switch (P.one.ets) {
         case 8: launch_raster_tris_8(P, ntiles, s); break;
         case 32: launch_raster_tris_32(P, ntiles, s); break;
-        case 64: launch_raster_tris_64(P, ntiles, s); break;
         default: launch_raster_tris_16(P, ntiles, s); break;
     }
 }

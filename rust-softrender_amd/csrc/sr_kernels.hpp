@@ -34,8 +34,10 @@ struct SetupParams {
     uint32_t n;
     int cull;
     uint32_t W, H, ets;
+    uint32_t tw, th;  // screen tiles (clamp_as_int!'s boxes)
     TriRec* recs;
     TriBin* bins;
+    TriVid* vids;
     uint32_t* counts;
     uint32_t* err;
     int empty_cull;  // bin only execution tiles the triangle may cover (no stencil effects)
@@ -48,6 +50,7 @@ struct EmitParams {
     const uint32_t* offsets;
     uint32_t cap;  // pairs keys/vals hold: a triangle whose pairs would pass it writes none
     uint32_t n, W, H, ets, ntx;
+    uint32_t tw, th;  // screen tiles
     uint32_t key_base, val_base;
     uint32_t* keys;
     uint32_t* vals;
@@ -63,6 +66,7 @@ struct SceneGeom {
     uint32_t nt, tbase;       // triangles, first batch triangle
     int program, K, cull;
     uint32_t W, H, ets, ntx, tile_base;
+    uint32_t tw, th;  // screen tiles
     int empty_cull;
     sr_viewport viewport;
     float U[48];
@@ -126,7 +130,7 @@ struct PointSetupParams {
 };
 
 struct StripParams {
-    const TriBin* bins;
+    const TriRec* recs;
     uint32_t n;
     uint32_t W, H, tw, th, scols, srows;
     uint32_t* colL;  // [scols+1][H+1]  rows of column tx0_j, difference along y
@@ -154,9 +158,14 @@ void launch_gs_emit(int mode, int prim, const GSParams& P, hipStream_t s);
 // 3 per triangle (SR_GS_VERTEX_NORMALS) or 1 (SR_GS_FACE_NORMALS)
 void launch_gs_normals(int mode, const GSParams& P, hipStream_t s);
 void launch_tri_setup(const SetupParams& P, hipStream_t s);
+// The vertex program fused with setup, for non-indexed meshes without a
+// geometry program: varyings to V.out's rows, no screen positions written.
+void launch_vertex_setup_soup(const VSParams& V, const SetupParams& P, hipStream_t s);
+void launch_vertex_setup_soup_batch(const BatchTables& T, const VertexBuf& v, TriRec* recs, TriBin* bins,
+                                    TriVid* vids, uint32_t* counts, uint32_t* err, hipStream_t s);
 void launch_tri_emit(const EmitParams& P, hipStream_t s);
 void launch_vertex_batch(const BatchTables& T, const VertexBuf& out, hipStream_t s);
-void launch_tri_setup_batch(const BatchTables& T, const VertexBuf& v, TriRec* recs, TriBin* bins,
+void launch_tri_setup_batch(const BatchTables& T, const VertexBuf& v, TriRec* recs, TriBin* bins, TriVid* vids,
                             uint32_t* counts, uint32_t* err, hipStream_t s);
 void launch_tri_emit_batch(const BatchTables& T, const TriRec* recs, const TriBin* bins, const uint32_t* counts,
                            const uint32_t* offsets, uint32_t cap, uint32_t* keys, uint32_t* vals,
@@ -192,6 +201,12 @@ int radix_sort_groups(uint32_t* keys, uint32_t* vals, uint32_t* keys_tmp, uint32
                       const uint32_t* live, uint32_t* segs, uint32_t* hist, uint32_t* scan_tmp,
                       uint32_t* total, hipStream_t s);
 void launch_fill(uint32_t* p, uint32_t v, uint32_t n, hipStream_t s);
+// start/end of tiles [0, nkeys) in keys[0, n) (sorted), by binary search:
+// every tile written, no clearing needed
+void launch_ranges_search(const uint32_t* keys, uint32_t n, uint32_t nkeys, uint32_t* start, uint32_t* end,
+                          hipStream_t s);
+// p[i] = pat[i % period] over n_words 32-bit words (period 1, 2 or 4)
+void launch_fill_pattern(void* p, const uint32_t pat[4], uint32_t period, size_t n_words, hipStream_t s);
 void launch_iota(uint32_t* p, uint32_t n, hipStream_t s);  // p[i] = i
 // start/end of each key's run in sorted keys; keys outside [key_base,
 // key_base + nkeys) are skipped
@@ -211,10 +226,12 @@ void launch_fb_clear(uint32_t* color, float* depth, uint8_t* stencil, uint32_t n
 // grid = ntiles work-groups (inline scene) or a persistent grid (scene table);
 // the batch's `one` carries the launch-wide program choice (fp, blend, smode, ets)
 void launch_raster_tris(const RasterBatch& P, uint32_t grid, hipStream_t s);
+// The raster for other framebuffer formats, the multi-output program and the
+// normalised over-blend (sr_raster_fmt.hip): 8x8 execution tiles.
+void launch_raster_fmt(const RasterParams& P, const FmtParams& F, uint32_t ntiles, hipStream_t s);
 void launch_raster_tris_8(const RasterBatch& P, uint32_t grid, hipStream_t s);
 void launch_raster_tris_16(const RasterBatch& P, uint32_t grid, hipStream_t s);
 void launch_raster_tris_32(const RasterBatch& P, uint32_t grid, hipStream_t s);
-void launch_raster_tris_64(const RasterBatch& P, uint32_t grid, hipStream_t s);
 
 // lines / points / stencil strips
 void launch_line_tiles(const LineSetupParams& P, int pass, hipStream_t s);

@@ -1,10 +1,10 @@
 // sr_raster.hip — the raster work-groups.  Compiled once per execution
-// tile size (-DSR_ETS=8|16|32|64) so the template instances build in
+// tile size (-DSR_ETS=8|16|32) so the template instances build in
 // parallel; each object exports launch_raster_*_<ETS>.
 #include <type_traits>
 
-#include "sr_device.hpp"
 #include "sr_kernels.hpp"
+#include "sr_raster_common.hpp"
 
 #ifndef SR_ETS
 #define SR_ETS 16
@@ -26,121 +26,20 @@ namespace sr {
 // ===========================================================================
 // Raster: triangles
 // ===========================================================================
-// Screen tile grid of the reference (fragment.rs:188-216): column j spans
-// [j*tw, min((j+1)*tw, W-1)], rows likewise; both ends inclusive.
-__device__ __forceinline__ void sem_range(uint32_t lo_px, uint32_t hi_px, uint32_t t, uint32_t n,
-                                          uint32_t& jlo, uint32_t& jhi) {
-    // tiles whose inclusive span meets [lo_px, hi_px]
-    const uint32_t a = (lo_px + t - 1) / t;  // ceil(lo/t)
-    jlo = a > 0 ? a - 1 : 0;
-    jhi = hi_px / t;
-    if (jhi > n - 1) jhi = n - 1;
-}
-
-// Pixel ownership: an ETS x ETS execution tile, 256 threads (64 for ETS 8),
-// ETS*ETS/threads pixels per thread.  Wave w owns rows [w*ETS/4, (w+1)*ETS/4)
-// so a triangle whose box misses those rows is skipped by the whole wave.
-template <int ETS>
-constexpr int raster_threads() { return ETS == 8 ? 64 : 256; }
-
-// Loads through the constant address space: read-only for the kernel's
-// lifetime, so a wave-uniform address becomes an s_load into SGPRs.
-__device__ __forceinline__ uint32_t load_item(const uint32_t* base, uint32_t i) {
-#if defined(__HIP_DEVICE_COMPILE__)
-    return ((const __attribute__((address_space(4))) uint32_t*)base)[i];
-#else
-    return base[i];
-#endif
-}
-
-// 64 consecutive item ids, lane k holding the id at position c0 + k, the
-// position clamped into [beg, end) (beg < end) so every lane loads.
-__device__ __forceinline__ uint32_t load_ids(const uint32_t* items, uint32_t c0, uint32_t beg,
-                                             uint32_t end, int lane) {
-    const uint32_t c = min(max(c0 + (uint32_t)lane, beg), end - 1);
-#if defined(__HIP_DEVICE_COMPILE__)
-    return ((const __attribute__((address_space(1))) uint32_t*)items)[c];
-#else
-    return items[c];
-#endif
-}
-
-// A block of 64 triangle records, lane k holding the first 60 bytes of
-// record k (R, x3, y3, edges, z, bbox).
-typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
-typedef uint32_t u32x3 __attribute__((ext_vector_type(3)));
-struct RecBlock {
-    u32x4 a, b, c;
-    u32x3 d;
+// Pixel ownership: an ETS x ETS execution tile is split among NWV waves
+// (1, 2 or 4, chosen independently of the tile size): with 4 each wave owns
+// an ETS/2 square, with 2 a half of ETS/2 rows, with 1 the whole tile; a
+// lane owns PPT = pixels per wave / 64 pixels, rows 64/RW apart.  Every
+// wave walks the tile's triangle list for its own pixels.
+template <int ETS, int NWV>
+struct WaveRegion {
+    static_assert(NWV == 1 || NWV == 2 || NWV == 4, "waves per tile");
+    static_assert(ETS >= 16 || NWV == 1, "an 8x8 tile is one wave");
+    static constexpr int RW = NWV == 4 ? ETS / 2 : ETS;  // region width
+    static constexpr int RH = NWV == 1 ? ETS : ETS / 2;  // region height
+    static constexpr int PPT = RW * RH / 64;              // pixels per lane
+    static constexpr int RPP = 64 / RW;                   // region rows one pass of 64 lanes covers
 };
-
-__device__ __forceinline__ RecBlock load_block(const TriRec* tris, uint32_t id) {
-    RecBlock r;
-#if defined(__HIP_DEVICE_COMPILE__)
-    typedef const __attribute__((address_space(1))) u32x4 G4;
-    typedef const __attribute__((address_space(1))) u32x3 G3;
-#else
-    typedef const u32x4 G4;
-    typedef const u32x3 G3;
-#endif
-    const char* base = (const char*)(tris + id);
-    r.a = *(G4*)(base);
-    r.b = *(G4*)(base + 16);
-    r.c = *(G4*)(base + 32);
-    r.d = *(G3*)(base + 48);
-    return r;
-}
-
-// The fields of record k of a block, in SGPRs (k is wave-uniform).
-struct RecLane {
-    double R;
-    float x3, y3, e23y, e23x, e31y, e31x, z1, z2, z3;
-    int X0, X1, Y0, Y1;
-};
-
-__device__ __forceinline__ uint32_t lane_u32(uint32_t v, int k) {
-    return (uint32_t)__builtin_amdgcn_readlane((int)v, k);
-}
-__device__ __forceinline__ float lane_f32(uint32_t v, int k) {
-    return __builtin_bit_cast(float, lane_u32(v, k));
-}
-
-__device__ __forceinline__ RecLane rec_at(const RecBlock& b, int k) {
-    RecLane r;
-    const uint64_t lo = lane_u32(b.a.x, k), hi = lane_u32(b.a.y, k);
-    r.R = __builtin_bit_cast(double, lo | (hi << 32));
-    r.x3 = lane_f32(b.a.z, k);
-    r.y3 = lane_f32(b.a.w, k);
-    r.e23y = lane_f32(b.b.x, k);
-    r.e23x = lane_f32(b.b.y, k);
-    r.e31y = lane_f32(b.b.z, k);
-    r.e31x = lane_f32(b.b.w, k);
-    r.z1 = lane_f32(b.c.x, k);
-    r.z2 = lane_f32(b.c.y, k);
-    r.z3 = lane_f32(b.c.z, k);
-    r.X0 = (int)lane_u32(b.c.w, k);
-    r.X1 = (int)lane_u32(b.d.x, k);
-    r.Y0 = (int)lane_u32(b.d.y, k);
-    r.Y1 = (int)lane_u32(b.d.z, k);
-    return r;
-}
-
-__device__ __forceinline__ TriRec load_rec(const TriRec* base, uint32_t i) {
-#if defined(__HIP_DEVICE_COMPILE__)
-    const __attribute__((address_space(4))) TriRec* p =
-        (const __attribute__((address_space(4))) TriRec*)base + i;
-    TriRec r;
-    r.R = p->R;
-    r.x3 = p->x3; r.y3 = p->y3;
-    r.e23y = p->e23y; r.e23x = p->e23x; r.e31y = p->e31y; r.e31x = p->e31x;
-    r.z1 = p->z1; r.z2 = p->z2; r.z3 = p->z3;
-    r.X0 = p->X0; r.X1 = p->X1; r.Y0 = p->Y0; r.Y1 = p->Y1;
-    r.i0 = p->i0; r.i1 = p->i1; r.i2 = p->i2;
-    return r;
-#else
-    return base[i];
-#endif
-}
 
 // Fragments that pass stencil + depth wait in a per-wave LDS queue, in
 // order, until some pixel's queue is full; then the wave shades the whole
@@ -150,8 +49,9 @@ __device__ __forceinline__ TriRec load_rec(const TriRec* base, uint32_t i) {
 // and every fragment program here returns a colour (never Discard), so the
 // depth write (triangle.rs:142) does not depend on shading; blending, the
 // only colour-order dependence, runs in queue (= reference) order.
-template <int ETS>
-constexpr int queue_depth() { return ETS <= 16 ? 8 : (ETS == 32 ? 2 : 1); }
+// Queue slots per pixel: LDS holds 64 * PPT * Q queued fragments per wave.
+template <int PPT>
+constexpr int queue_depth() { return PPT == 1 ? 8 : (PPT == 2 ? 4 : (PPT == 4 ? 2 : 1)); }
 
 struct QRec {
     uint32_t tri;  // triangle id; after shading, the fragment colour
@@ -165,14 +65,19 @@ __device__ __forceinline__ void wave_sync_lds() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// Exclusive prefix sum over the 64 lanes, and the total, with DPP moves:
+// row_shr 1, 2, 4, 8 scan each row of 16 lanes (lanes shifted in from
+// outside the row read 0), then row_bcast 15 adds the end of row 0 to row 1
+// and of row 2 to row 3 (row mask 0xa), and row_bcast 31 adds the end of
+// row 1 to rows 2 and 3 (row mask 0xc).  No LDS, no lane-address registers.
 __device__ __forceinline__ uint32_t wave_excl_scan(uint32_t v, uint32_t& total) {
-    const int lane = threadIdx.x & 63;
     uint32_t x = v;
-#pragma unroll
-    for (int o = 1; o < 64; o <<= 1) {
-        const uint32_t y = __shfl_up(x, o, 64);
-        if (lane >= o) x += y;
-    }
+    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x111, 0xf, 0xf, true);  // row_shr:1
+    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x112, 0xf, 0xf, true);  // row_shr:2
+    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x114, 0xf, 0xf, true);  // row_shr:4
+    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x118, 0xf, 0xf, true);  // row_shr:8
+    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x142, 0xa, 0xf, false); // row_bcast:15
+    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x143, 0xc, 0xf, false); // row_bcast:31
     total = __builtin_amdgcn_readlane(x, 63);
     return x - v;
 }
@@ -184,26 +89,24 @@ __device__ __forceinline__ uint32_t shade_rec(const RasterParams& P, const QRec&
     constexpr int KV = fp_varyings<FP>();
     float k[KV > 0 ? KV : 1];
     if constexpr (KV > 0) {
-        const TriRec& r = P.tris[q.tri];
+        const TriVid& r = P.vids[q.tri];
         const uint32_t i0 = r.i0, i1 = r.i1, i2 = r.i2;
         const float u = q.u, v = q.v, w = (1.0f - u) - v;
+        const float *r0 = P.verts.var(i0), *r1 = P.verts.var(i1), *r2 = P.verts.var(i2);
 #pragma unroll
-        for (int c = 0; c < KV; c++) {
-            const float* vc = P.verts.comp(4 + c);
-            k[c] = ((vc[i0] * u) + (vc[i1] * v)) + (vc[i2] * w);
-        }
+        for (int c = 0; c < KV; c++) k[c] = ((r0[c] * u) + (r1[c] * v)) + (r2[c] * w);
     }
     return fragment<FP>(P, k, lut);
 }
 
-template <int ETS, int FP, int BLEND>
+template <int ETS, int NWV, int FP, int BLEND>
 struct RasterShared {
-    static constexpr int NT = raster_threads<ETS>();
-    static constexpr int NW = NT / 64;
-    static constexpr int S = ETS == 8 ? 8 : ETS / 2;  // each wave owns an S x S quadrant
-    static constexpr int PPT = (S * S) / 64;          // pixels per lane
+    using WR = WaveRegion<ETS, NWV>;
+    static constexpr int NT = 64 * NWV;
+    static constexpr int NW = NWV;
+    static constexpr int PPT = WR::PPT;               // pixels per lane
     static constexpr int WPIX = 64 * PPT;             // pixels per wave
-    static constexpr int Q = BLEND == SR_BLEND_OVER ? queue_depth<ETS>() : 1;
+    static constexpr int Q = BLEND == SR_BLEND_OVER ? queue_depth<PPT>() : 1;
     QRec qbuf[NW][WPIX * Q];  // first: queue stores take their offset in the instruction
     float lut[256];
     uint16_t flat[NW][WPIX * Q];
@@ -255,12 +158,23 @@ __device__ __forceinline__ uint32_t lower_bound_items(const uint32_t* items, uin
 }
 
 
-template <int ETS, int FP, int BLEND, int SMODE, class Draws>
+// CLAMP: the draw runs in the exhaustive mode (every triangle in every
+// tile), so a triangle's box is clamped into each screen tile exactly as
+// clamp_as_int! does (triangle.rs:66-78), strips of pixels outside the box
+// included.  Binned draws test the box itself (stencil counting) or nothing
+// (no stencil effects): a triangle reaches only the execution tiles its box
+// meets, and on every pixel that can pass, the clamped box, the box and no
+// box at all agree -- except for a triangle whose record carries the strip
+// flag (a pixel outside its box may pass the coverage test): setup bins it
+// to every execution tile such a pixel may lie in, and the raster tests it
+// in its clamped boxes.
+template <int ETS, int NWV, int FP, int BLEND, int SMODE, bool CLAMP, class Draws>
 __device__ __forceinline__ void raster_tile(const RasterParams& P, uint32_t tile, uint32_t gtile,
-                                            RasterShared<ETS, FP, BLEND>& sh, const Draws& draws) {
-    using SH = RasterShared<ETS, FP, BLEND>;
-    constexpr int S = SH::S, PPT = SH::PPT, Q = SH::Q;
-    constexpr int RPP = 64 / S;  // quadrant rows covered by one pixel pass
+                                            RasterShared<ETS, NWV, FP, BLEND>& sh, const Draws& draws) {
+    using SH = RasterShared<ETS, NWV, FP, BLEND>;
+    using WR = typename SH::WR;
+    constexpr int RW = WR::RW, RH = WR::RH, PPT = SH::PPT, Q = SH::Q;
+    constexpr int RPP = WR::RPP;
 
     // The empty asm makes the thread index opaque per tile: in the
     // persistent batch kernel the compiler would otherwise hoist every
@@ -281,9 +195,10 @@ __device__ __forceinline__ void raster_tile(const RasterParams& P, uint32_t tile
     // tile and wave pixel spans (clipped to the framebuffer)
     const uint32_t tx0 = etx * ETS, ty0 = ety * ETS;
     const int tx1 = (int)min(tx0 + ETS, P.W) - 1, ty1 = (int)min(ty0 + ETS, P.H) - 1;
-    const int qx = ETS == 8 ? 0 : (wave & 1) * S, qy = ETS == 8 ? 0 : (wave >> 1) * S;
+    const int qx = NWV == 4 ? (wave & 1) * RW : 0;
+    const int qy = NWV == 4 ? (wave >> 1) * RH : (NWV == 2 ? wave * RH : 0);
     const int wx0 = (int)tx0 + qx, wy0 = (int)ty0 + qy;
-    const int wx1 = min(wx0 + S - 1, tx1), wy1 = min(wy0 + S - 1, ty1);
+    const int wx1 = min(wx0 + RW - 1, tx1), wy1 = min(wy0 + RH - 1, ty1);
 
     int px[PPT], py[PPT];
     bool live[PPT];
@@ -291,8 +206,8 @@ __device__ __forceinline__ void raster_tile(const RasterParams& P, uint32_t tile
     float dep[PPT], fx[PPT], fy[PPT];
 #pragma unroll
     for (int j = 0; j < PPT; j++) {
-        px[j] = wx0 + lane % S;
-        py[j] = wy0 + lane / S + j * RPP;
+        px[j] = wx0 + lane % RW;
+        py[j] = wy0 + lane / RW + j * RPP;
         live[j] = (uint32_t)px[j] < P.W && (uint32_t)py[j] < P.H;
         const size_t idx = (size_t)py[j] * P.W + px[j];
         if (P.clear) {  // Framebuffer::clear: colour, depth far, stencil 0
@@ -376,47 +291,70 @@ __device__ __forceinline__ void raster_tile(const RasterParams& P, uint32_t tile
                 for (int j = 0; j < PPT; j++)
                     inT[j] = live[j] && px[j] >= Tx0 && px[j] <= Tx1 && py[j] >= Ty0 && py[j] <= Ty1;
 
-                // walk the tile's triangles in order, 64 at a time: lane k
-                // gathers the record of item c0 + k one block ahead of its use
-                // (item ids two blocks ahead) and the loop hands record fields
-                // to the whole wave with v_readlane, so the random record
-                // fetches overlap instead of stalling each triangle.  Loads
-                // are unconditional, with positions clamped into [beg, end),
-                // so no loaded register is copied (which would wait for it).
-                const uint32_t b0 = sb & ~63u;
-                uint32_t id = load_ids(P.items, b0, sb, se, lane);
-                RecBlock blk = load_block(P.tris, id);
-                uint32_t idn = load_ids(P.items, b0 + 64, sb, se, lane);
-                for (uint32_t c0 = b0; c0 < se; c0 += 64) {
-                    RecBlock nxt = load_block(P.tris, idn);
-                    const uint32_t idc = idn;
-                    idn = load_ids(P.items, c0 + 128, sb, se, lane);
-                    const uint32_t lo = max(c0, sb), hi = min(c0 + 64, se);
-                    for (uint32_t c = lo; c < hi; c++) {
-                    const int k = (int)(c - c0);
-                    const RecLane rc = rec_at(blk, k);
-                    const uint32_t cur = lane_u32(id, k);
-                    // tile-clamped bbox, clamp_as_int!, triangle.rs:66-78
+                // walk the tile's triangles in order.  Each record is read
+                // into SGPRs by a scalar load issued one triangle ahead (the
+                // asm ties that load's address to the current record, so it
+                // is issued only after the current record has arrived: a
+                // scalar load wait covers every scalar load in flight).  The
+                // item ids come 64 at a time, one block ahead, and lane k
+                // also touches the record of item c0 + 64 + k so that the
+                // scalar loads find their lines in L2.  Positions are clamped
+                // into [sb, se), so every id loaded is a valid triangle.
+                // One triangle against this wave's pixels (rc: its record,
+                // cur: its id).
+                auto tri = [&](const RecLane& rc, uint32_t cur) {
                     const int X0 = rc.X0, X1 = rc.X1, Y0 = rc.Y0, Y1 = rc.Y1;
-                    if (X0 > X1) continue;  // culled or invalid
-                    const int cx0 = min(max(X0, Tx0), Tx1), cx1 = min(max(X1, Tx0), Tx1);
-                    const int cy0 = min(max(Y0, Ty0), Ty1), cy1 = min(max(Y1, Ty0), Ty1);
-                    // (no wave-level early-out on the clamped box: binned
-                    // triangles nearly always reach the wave's pixels, and
-                    // the scalar test cost more than the rare skip saved)
+                    if constexpr (CLAMP) {
+                        if (X0 > X1) return;  // culled or invalid (binned draws never see these)
+                    }
+                    // The pixels tested: the box clamped into this screen
+                    // tile (clamp_as_int!, triangle.rs:66-78) in the
+                    // exhaustive mode and for a triangle whose record says a
+                    // pixel outside its box may pass (binned to every tile
+                    // such a pixel may lie in; rc.strip is wave-uniform, so
+                    // this is a scalar branch); the box for stencil
+                    // effects; else every pixel: in a binned draw no pixel
+                    // centre outside the box of any other triangle can pass
+                    // the coverage test, and inside the box the test is the
+                    // reference's.  Stencil counting counts the box itself
+                    // (the strips outside it are counted apart).
+                    const bool clampit = CLAMP || rc.strip != 0u;
+                    int cx0 = X0, cx1 = X1, cy0 = Y0, cy1 = Y1;
+                    if (clampit) {
+                        cx0 = min(max(X0, Tx0), Tx1); cx1 = min(max(X1, Tx0), Tx1);
+                        cy0 = min(max(Y0, Ty0), Ty1); cy1 = min(max(Y1, Ty0), Ty1);
+                    }
+                    // without stencil effects: the lane masks, set apart so
+                    // that other triangles skip the test
+                    bool inb[SMODE == 0 ? PPT : 1];
+                    if constexpr (SMODE == 0) {
+                        if (clampit) {
+#pragma unroll
+                            for (int j = 0; j < PPT; j++)
+                                inb[j] = (px[j] >= cx0) & (px[j] <= cx1) & (py[j] >= cy0) & (py[j] <= cy1);
+                        } else {
+#pragma unroll
+                            for (int j = 0; j < PPT; j++) inb[j] = true;
+                        }
+                    }
                     bool full = false;
 #pragma unroll
                     for (int j = 0; j < PPT; j++) {
                         // straight-line per pixel: every test is a mask, so
                         // the wave runs one exec-mask region per pixel
                         // instead of a nested branch per test
-                        bool ok = inT[j] & (px[j] >= cx0) & (px[j] <= cx1) & (py[j] >= cy0) & (py[j] <= cy1);
+                        bool ok = inT[j];
+                        if constexpr (SMODE == 0)
+                            ok = ok & inb[j];
+                        else
+                            ok = ok & (px[j] >= cx0) & (px[j] <= cx1) & (py[j] >= cy0) & (py[j] <= cy1);
                         if constexpr (SMODE == 2) {  // stencil before coverage, triangle.rs:90-99
                             const bool sp = ok && stencil_test(P.s_test, st[j], P.s_ref);
                             st[j] = sp ? stencil_op(P.s_op, P.s_wrap, st[j], P.s_ref) : st[j];
                             ok = sp;
                         } else if constexpr (SMODE == 1) {
-                            cnt[j] += (ok & (px[j] >= X0) & (px[j] <= X1) & (py[j] >= Y0) & (py[j] <= Y1)) ? 1u : 0u;
+                            const bool inr = !clampit || ((px[j] >= X0) & (px[j] <= X1) & (py[j] >= Y0) & (py[j] <= Y1));
+                            cnt[j] += (ok & inr) ? 1u : 0u;
                         }
                         const float dx = fx[j] - rc.x3, dy = fy[j] - rc.y3;
                         const float un = (rc.e23y * dx) + (rc.e23x * dy);  // triangle.rs:108
@@ -441,18 +379,56 @@ __device__ __forceinline__ void raster_tile(const RasterParams& P, uint32_t tile
                         if constexpr (BLEND == SR_BLEND_OVER) full |= qn[j] == (uint32_t)Q;
                     }
                     if constexpr (BLEND == SR_BLEND_OVER) {
-                        if (__ballot(full)) {
-                            // the record blocks are gathered again after the
-                            // flush rather than kept live across the shading
-                            // code: that keeps the kernel ~26 VGPRs smaller
-                            // (mid-walk flushes are rare)
-                            flush(PD);
-                            blk = load_block(P.tris, id);
-                            nxt = load_block(P.tris, idc);
+                        if (__ballot(full)) flush(PD);
+                    }
+                };
+
+                // Walk the tile's triangles in order.  Each record is read
+                // into SGPRs by a scalar load issued one triangle ahead
+                // (after_record ties that load's address to the current
+                // record, so it is issued only once the current record has
+                // arrived: a scalar-load wait covers every scalar load in
+                // flight).  Two record register sets take turns, so nothing
+                // is copied between triangles.  The item ids come 64 at a
+                // time, one block ahead, and lane k also touches the record
+                // of item c0 + 64 + k so that the scalar loads find their
+                // lines in L2.  Positions are clamped into [sb, se), so
+                // every id loaded is a valid triangle.
+                const uint32_t b0 = sb & ~63u;
+                uint32_t id = load_ids(P.items, b0, sb, se, lane);
+                uint32_t idn = load_ids(P.items, b0 + 64, sb, se, lane);
+                uint32_t warm = touch_rec(P.tris, id);
+                uint32_t cur_id = lane_u32(id, (int)(sb - b0));
+                RecLane ra = rec_s(P.tris, cur_id);
+                for (uint32_t c0 = b0; c0 < se; c0 += 64) {
+                    keep_live(warm);
+                    warm = touch_rec(P.tris, idn);
+                    const uint32_t idc = idn;
+                    idn = load_ids(P.items, c0 + 128, sb, se, lane);
+                    const uint32_t hi = min(c0 + 64, se);
+                    // lane k: the id at position c0 + k + 1 (lane 63 the
+                    // next block's first)
+                    uint32_t id1 = __shfl_down(id, 1, 64);
+                    id1 = lane == 63 ? lane_u32(idc, 0) : id1;
+                    auto next_id = [&](uint32_t c) { return lane_u32(id1, (int)(c - c0)); };
+                    uint32_t c = max(c0, sb);
+                    while (true) {
+                        uint32_t nid = next_id(c);
+                        after_record(nid, ra);
+                        const RecLane rb = rec_s(P.tris, nid);
+                        tri(ra, cur_id);
+                        cur_id = nid;
+                        if (++c >= hi) {
+                            ra = rb;
+                            break;
                         }
+                        nid = next_id(c);
+                        after_record(nid, rb);
+                        ra = rec_s(P.tris, nid);
+                        tri(rb, cur_id);
+                        cur_id = nid;
+                        if (++c >= hi) break;
                     }
-                    }
-                    blk = nxt;
                     id = idc;
                 }
             }
@@ -500,23 +476,23 @@ __device__ __forceinline__ void raster_tile(const RasterParams& P, uint32_t tile
 
 
 // One scene: one work-group per execution tile.
-template <int ETS, int FP, int BLEND, int SMODE>
+template <int ETS, int NWV, int FP, int BLEND, int SMODE, bool CLAMP>
 __global__ __launch_bounds__(256) SR_RASTER_OCCUPANCY void k_raster_tris(RasterParams P) {
-    using SH = RasterShared<ETS, FP, BLEND>;
+    using SH = RasterShared<ETS, NWV, FP, BLEND>;
     __shared__ SH sh;
     for (int i = threadIdx.x; i < 256; i += SH::NT) sh.lut[i] = (float)i / 255.0f;
     __syncthreads();
     const uint32_t tile = xcd_remap(blockIdx.x, gridDim.x);
-    raster_tile<ETS, FP, BLEND, SMODE>(P, tile, tile, sh, OneDraw{});
+    raster_tile<ETS, NWV, FP, BLEND, SMODE, CLAMP>(P, tile, tile, sh, OneDraw{});
 }
 
 // A batch: a persistent grid over the scene table (no stencil effects in
 // batches, so SMODE is 0).
-template <int ETS, int FP, int BLEND>
+template <int ETS, int NWV, int FP, int BLEND>
 __global__ __launch_bounds__(256) SR_RASTER_OCCUPANCY void k_raster_tris_batch(RasterBatch B) {
-    using SH = RasterShared<ETS, FP, BLEND>;
+    using SH = RasterShared<ETS, NWV, FP, BLEND>;
     __shared__ SH sh;
-    if (B.overflow && *B.overflow) return;
+    if (B.overflow && (B.overflow[0] | (B.overflow[2] & 1u))) return;  // overflow, NaN coordinate
     for (int i = threadIdx.x; i < 256; i += SH::NT) sh.lut[i] = (float)i / 255.0f;
     __syncthreads();
     // Work-group b takes run p of near-equal estimated cost (a contiguous
@@ -544,39 +520,60 @@ __global__ __launch_bounds__(256) SR_RASTER_OCCUPANCY void k_raster_tris_batch(R
         // (scalar-cache hits) rather than held in registers
         const uint32_t d0 = load_item(B.group_first, group), d1 = load_item(B.group_first, group + 1);
         const RasterParams P = load_params(B.scenes, d0);
-        raster_tile<ETS, FP, BLEND, 0>(P, g - P.tile_base, g, sh, GroupDraws{B.scenes, B.tri_end, d0, d1 - d0});
+        raster_tile<ETS, NWV, FP, BLEND, 0, false>(P, g - P.tile_base, g, sh,
+                                                   GroupDraws{B.scenes, B.tri_end, d0, d1 - d0});
     }
 }
 
-template <int ETS, int FP, int BLEND>
+template <int ETS, int NWV, int FP, int BLEND>
 static void raster_tris_smode(const RasterBatch& P, uint32_t ntiles, hipStream_t s) {
-    const dim3 g(ntiles), b(raster_threads<ETS>());
-    if (P.scenes) hipLaunchKernelGGL((k_raster_tris_batch<ETS, FP, BLEND>), g, b, 0, s, P);
-    else if (P.one.smode == 2) hipLaunchKernelGGL((k_raster_tris<ETS, FP, BLEND, 2>), g, b, 0, s, P.one);
-    else if (P.one.smode == 1) hipLaunchKernelGGL((k_raster_tris<ETS, FP, BLEND, 1>), g, b, 0, s, P.one);
-    else hipLaunchKernelGGL((k_raster_tris<ETS, FP, BLEND, 0>), g, b, 0, s, P.one);
+    const dim3 g(ntiles), b(64 * NWV);
+    const bool ex = P.one.exhaustive != 0;
+    if (P.scenes) hipLaunchKernelGGL((k_raster_tris_batch<ETS, NWV, FP, BLEND>), g, b, 0, s, P);
+    else if (P.one.smode == 2) hipLaunchKernelGGL((k_raster_tris<ETS, NWV, FP, BLEND, 2, true>), g, b, 0, s, P.one);
+    else if (P.one.smode == 1 && ex) hipLaunchKernelGGL((k_raster_tris<ETS, NWV, FP, BLEND, 1, true>), g, b, 0, s, P.one);
+    else if (P.one.smode == 1) hipLaunchKernelGGL((k_raster_tris<ETS, NWV, FP, BLEND, 1, false>), g, b, 0, s, P.one);
+    else if (ex) hipLaunchKernelGGL((k_raster_tris<ETS, NWV, FP, BLEND, 0, true>), g, b, 0, s, P.one);
+    else hipLaunchKernelGGL((k_raster_tris<ETS, NWV, FP, BLEND, 0, false>), g, b, 0, s, P.one);
 }
 
-template <int ETS, int FP>
+template <int ETS, int NWV, int FP>
 static void raster_tris_blend(const RasterBatch& P, uint32_t ntiles, hipStream_t s) {
-    if (P.one.blend == SR_BLEND_OVER) raster_tris_smode<ETS, FP, SR_BLEND_OVER>(P, ntiles, s);
-    else raster_tris_smode<ETS, FP, SR_BLEND_REPLACE>(P, ntiles, s);
+    if (P.one.blend == SR_BLEND_OVER) raster_tris_smode<ETS, NWV, FP, SR_BLEND_OVER>(P, ntiles, s);
+    else raster_tris_smode<ETS, NWV, FP, SR_BLEND_REPLACE>(P, ntiles, s);
 }
 
-template <int ETS>
+template <int ETS, int NWV>
 static void raster_tris_fp(const RasterBatch& P, uint32_t ntiles, hipStream_t s) {
     switch (P.one.fp) {
-        case SR_FP_BLINN_PHONG: raster_tris_blend<ETS, SR_FP_BLINN_PHONG>(P, ntiles, s); break;
-        case SR_FP_TEX_LAMBERT: raster_tris_blend<ETS, SR_FP_TEX_LAMBERT>(P, ntiles, s); break;
-        case SR_FP_UV_COLOR: raster_tris_blend<ETS, SR_FP_UV_COLOR>(P, ntiles, s); break;
-        default: raster_tris_blend<ETS, SR_FP_FLAT>(P, ntiles, s); break;
+        case SR_FP_BLINN_PHONG: raster_tris_blend<ETS, NWV, SR_FP_BLINN_PHONG>(P, ntiles, s); break;
+        case SR_FP_TEX_LAMBERT: raster_tris_blend<ETS, NWV, SR_FP_TEX_LAMBERT>(P, ntiles, s); break;
+        case SR_FP_UV_COLOR: raster_tris_blend<ETS, NWV, SR_FP_UV_COLOR>(P, ntiles, s); break;
+        default: raster_tris_blend<ETS, NWV, SR_FP_FLAT>(P, ntiles, s); break;
     }
 }
 
+// Waves per tile this object instantiates for its tile size: 8x8 one wave;
+// 16x16 one, two or four; 32x32 two or four.  Any other count runs the tile
+// size's default (the last listed).
 #define SR_CAT2(a, b) a##b
 #define SR_CAT(a, b) SR_CAT2(a, b)
 void SR_CAT(launch_raster_tris_, SR_ETS)(const RasterBatch& P, uint32_t ntiles, hipStream_t s) {
-    if (ntiles) raster_tris_fp<SR_ETS>(P, ntiles, s);
+    if (!ntiles) return;
+    const int w = P.one.nwv;
+#if SR_ETS == 8
+    (void)w;
+    raster_tris_fp<8, 1>(P, ntiles, s);
+#elif SR_ETS == 16
+    if (w == 1) raster_tris_fp<16, 1>(P, ntiles, s);
+    else if (w == 2) raster_tris_fp<16, 2>(P, ntiles, s);
+    else raster_tris_fp<16, 4>(P, ntiles, s);
+#elif SR_ETS == 32
+    if (w == 2) raster_tris_fp<32, 2>(P, ntiles, s);
+    else raster_tris_fp<32, 4>(P, ntiles, s);
+#else
+#error "SR_ETS must be 8, 16 or 32"
+#endif
 }
 
 }  // namespace sr

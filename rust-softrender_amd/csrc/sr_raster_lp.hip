@@ -273,11 +273,9 @@ __global__ __launch_bounds__(256) void k_raster_lines(RasterParams P) {
                     const float z = (omt * i_zs[si]) + (tt * i_ze[si]);                     // :77
                     if (!(z < 0.0f && z >= s_dep[lp])) return;
                     float kv[KV > 0 ? KV : 1];
+                    const float *r0 = P.verts.var(i_v0[si]), *r1 = P.verts.var(i_v1[si]);
 #pragma unroll
-                    for (int q = 0; q < KV; q++) {
-                        const float* vc = P.verts.comp(4 + q);
-                        kv[q] = (omt * vc[i_v0[si]]) + (tt * vc[i_v1[si]]);
-                    }
+                    for (int q = 0; q < KV; q++) kv[q] = (omt * r0[q]) + (tt * r1[q]);
                     uint32_t col = fragment<FP>(P, kv, lut);
                     col = mul_alpha(P.alpha_mode, col, pl.alpha);                         // :100
                     s_col[lp] = blend<BLEND>(col, s_col[lp], lut);
@@ -339,7 +337,7 @@ __global__ __launch_bounds__(256) void k_raster_points(RasterParams P) {
             if (!(z < 0.0f && z >= P.depth[idx])) return;        // point.rs:300-306
             float kv[KV > 0 ? KV : 1];
 #pragma unroll
-            for (int c = 0; c < KV; c++) kv[c] = P.verts.comp(4 + c)[v];
+            for (int c = 0; c < KV; c++) kv[c] = P.verts.var(v)[c];
             const uint32_t c = fragment<FP>(P, kv, lut);
             P.color[idx] = blend<BLEND>(c, P.color[idx], lut);
             P.depth[idx] = z;

@@ -96,8 +96,27 @@ int varyings_of(int vp) { return vp == SR_VP_MVP_NORMAL_WORLD ? 6 : 5; }
 int policy_tile(sr_context ctx, uint32_t w, uint32_t h) {
     const uint64_t px = (uint64_t)w * h;
     for (const auto& e : ctx->tile_policy)
-        if (px <= e.first) return e.second;
+        if (px <= e.first) return e.second & 0xFF;
     return 8;
+}
+
+static int valid_waves(uint32_t ets, int w) {
+    if (ets == 8) return 1;
+    if (ets == 16) return (w == 1 || w == 2) ? w : 4;
+    return w == 2 ? 2 : 4;  // 32
+}
+
+int draw_waves(sr_context ctx, const sr_draw_desc* d, uint32_t ets, uint32_t w, uint32_t h) {
+    int want = d->exec_waves;
+    if (!want && !d->exec_tile) {
+        const uint64_t px = (uint64_t)w * h;
+        for (const auto& e : ctx->tile_policy)
+            if (px <= e.first) {
+                want = e.second >> 8;
+                break;
+            }
+    }
+    return valid_waves(ets, want);
 }
 
 }  // namespace sr_rt
@@ -140,9 +159,7 @@ int bin_pairs(sr_context ctx, uint32_t npairs, uint32_t ntiles, const uint32_t**
                                    P<uint32_t>(ctx->scan_tmp), P<uint32_t>(ctx->misc) + 8, s);
     const uint32_t* skeys = swapped ? P<uint32_t>(ctx->keys2) : P<uint32_t>(ctx->keys);
     *sorted_vals = swapped ? P<uint32_t>(ctx->vals2) : P<uint32_t>(ctx->vals);
-    launch_fill(P<uint32_t>(ctx->tstart), 0u, ntiles, s);
-    launch_fill(P<uint32_t>(ctx->tend), 0u, ntiles, s);
-    launch_ranges(skeys, npairs, 0u, ntiles, P<uint32_t>(ctx->tstart), P<uint32_t>(ctx->tend), s);
+    launch_ranges_search(skeys, npairs, ntiles, P<uint32_t>(ctx->tstart), P<uint32_t>(ctx->tend), s);
     return SR_OK;
 }
 
@@ -157,8 +174,12 @@ int ensure_pairs(sr_context ctx, uint32_t npairs) {
 
 struct DrawPlan {
     uint32_t W, H, ets, ntx, nty, ntiles, tw, th, scols, srows;
+    int nwv = 1;  // waves per execution tile
     int K, smode;
     bool exhaustive, timing;
+    // the raster of sr_raster_fmt.hip (other framebuffer formats, the
+    // multi-output program, the normalised over-blend)
+    bool general = false;
 };
 
 void fill_raster_params(const DrawPlan& pl, sr_framebuffer fb, const sr_draw_desc* d,
@@ -167,13 +188,14 @@ void fill_raster_params(const DrawPlan& pl, sr_framebuffer fb, const sr_draw_des
     R.W = pl.W; R.H = pl.H;
     R.color = fb->color; R.depth = fb->depth; R.stencil = fb->stencil;
     R.tw = pl.tw; R.th = pl.th; R.scols = pl.scols; R.srows = pl.srows;
-    R.ets = pl.ets; R.ntx = pl.ntx; R.nty = pl.nty;
+    R.ets = pl.ets; R.ntx = pl.ntx; R.nty = pl.nty; R.nwv = pl.nwv;
     R.K = pl.K;
     R.fp = d->fragment_program; R.blend = d->blend; R.alpha_mode = d->alpha_mode;
     R.s_test = d->stencil_test; R.s_op = d->stencil_op; R.s_wrap = d->stencil_wrap;
     R.s_ref = d->stencil_ref & 0xFFu;
     R.smode = pl.smode;
-    if (tex) { R.tex = tex->d; R.texW = tex->w; R.texH = tex->h; }
+    R.exhaustive = pl.exhaustive ? 1 : 0;
+    if (tex) { R.tex = tex->d; R.tex_fmt = tex->fmt; R.texW = tex->w; R.texH = tex->h; }
     R.tex_filter = d->tex_filter; R.tex_edge = d->tex_edge; R.tex_border = d->tex_border;
     memcpy(R.fsu, d->fs_uniforms, sizeof(R.fsu));
 }
@@ -181,35 +203,42 @@ void fill_raster_params(const DrawPlan& pl, sr_framebuffer fb, const sr_draw_des
 // ---------------------------------------------------------------------------
 // The triangle draw
 // ---------------------------------------------------------------------------
+// `fuse`: the vertex stage of a non-indexed mesh, run inside setup (the
+// vertex kernel did not run; vb receives the varyings).
 int draw_tris(sr_context ctx, sr_framebuffer fb, const DrawPlan& pl, const sr_draw_desc* d,
-              sr_texture tex, VertexBuf vb, const uint32_t* idx, uint32_t ntris) {
+              sr_texture tex, VertexBuf vb, const uint32_t* idx, uint32_t ntris, const VSParams* fuse) {
     int rc;
     hipStream_t s = ctx->stream;
     ctx->last.triangles_setup = ntris;
-    // one buffer: the triangle records, then their binning records
+    // one buffer: the triangle records, their binning records, their vertex ids
     const uint32_t nrec = std::max<uint32_t>(ntris, 1);
-    if ((rc = ensure(ctx, ctx->recs, (sizeof(TriRec) + sizeof(TriBin)) * nrec))) return rc;
+    if ((rc = ensure(ctx, ctx->recs, (sizeof(TriRec) + sizeof(TriBin) + sizeof(TriVid)) * nrec))) return rc;
     TriBin* const bins = (TriBin*)(P<TriRec>(ctx->recs) + nrec);
+    TriVid* const vids = (TriVid*)(bins + nrec);
     if ((rc = ensure(ctx, ctx->counts, sizeof(uint32_t) * (ntris + 1)))) return rc;
     if ((rc = ensure(ctx, ctx->offsets, sizeof(uint32_t) * (ntris + 1)))) return rc;
     SetupParams sp;
     sp.v = vb; sp.idx = idx; sp.n = ntris; sp.cull = d->cull; sp.W = pl.W; sp.H = pl.H;
-    sp.ets = pl.ets; sp.recs = P<TriRec>(ctx->recs); sp.bins = bins; sp.counts = P<uint32_t>(ctx->counts);
+    sp.ets = pl.ets; sp.tw = pl.tw; sp.th = pl.th;
+    sp.recs = P<TriRec>(ctx->recs); sp.bins = bins; sp.vids = vids;
+    sp.counts = P<uint32_t>(ctx->counts);
     sp.err = P<uint32_t>(ctx->misc);
     // with no stencil effects, tiles whose pixels the triangle cannot cover
     // are left out of the bins
     sp.empty_cull = pl.smode == 0 && !pl.exhaustive;
-    launch_tri_setup(sp, s);
+    if (fuse) launch_vertex_setup_soup(*fuse, sp, s);
+    else launch_tri_setup(sp, s);
     mark(ctx, pl.timing, ST_SETUP);
 
     RasterParams R;
     fill_raster_params(pl, fb, d, tex, R);
     R.tris = P<TriRec>(ctx->recs);
+    R.vids = vids;
     R.verts = vb;
     R.error_flags = P<uint32_t>(ctx->misc);
     if (pl.exhaustive) {
         if ((rc = read_words(ctx, 1))) return rc;
-        if (ctx->h_misc[0]) return fail(ctx, SR_ERR_INVALID_COORDINATE, "NaN screen coordinate");
+        if (ctx->h_misc[0] & 1u) return fail(ctx, SR_ERR_INVALID_COORDINATE, "NaN screen coordinate");
         // every execution tile walks all triangles in order: identity item
         // list, every tile's range [0, ntris)
         if ((rc = ensure_pairs(ctx, ntris))) return rc;
@@ -236,7 +265,7 @@ int draw_tris(sr_context ctx, sr_framebuffer fb, const DrawPlan& pl, const sr_dr
         EmitParams ep;
         ep.recs = P<TriRec>(ctx->recs); ep.bins = bins; ep.counts = P<uint32_t>(ctx->counts);
         ep.offsets = P<uint32_t>(ctx->offsets); ep.n = ntris; ep.W = pl.W; ep.H = pl.H;
-        ep.ets = pl.ets; ep.ntx = pl.ntx; ep.key_base = 0; ep.val_base = 0;
+        ep.ets = pl.ets; ep.ntx = pl.ntx; ep.tw = pl.tw; ep.th = pl.th; ep.key_base = 0; ep.val_base = 0;
         ep.empty_cull = sp.empty_cull;
         auto emit = [&]() {
             ep.keys = P<uint32_t>(ctx->keys);
@@ -247,7 +276,11 @@ int draw_tris(sr_context ctx, sr_framebuffer fb, const DrawPlan& pl, const sr_dr
         };
         emit();
         HIP_OK(ctx, hipEventSynchronize(ctx->ev_read));
-        if (ctx->h_misc[0]) return fail(ctx, SR_ERR_INVALID_COORDINATE, "NaN screen coordinate");
+        if (ctx->h_misc[0] & 1u) return fail(ctx, SR_ERR_INVALID_COORDINATE, "NaN screen coordinate");
+        // a triangle that may pass the coverage test outside its box went
+        // to every tile such a pixel may lie in (the raster then tests it
+        // in its clamped boxes)
+        ctx->last.strip_slivers = (ctx->h_misc[0] & 2u) ? 1u : 0u;
         const uint32_t npairs = ctx->h_misc[1];
         ctx->last.bin_pairs = npairs;
         const size_t pb = sizeof(uint32_t) * std::max<uint32_t>(npairs, 1);
@@ -272,20 +305,40 @@ int draw_tris(sr_context ctx, sr_framebuffer fb, const DrawPlan& pl, const sr_dr
         const uint32_t* skeys = swapped ? P<uint32_t>(ctx->keys2) : P<uint32_t>(ctx->keys);
         const uint32_t* svals = swapped ? P<uint32_t>(ctx->vals2) : P<uint32_t>(ctx->vals);
         mark(ctx, pl.timing, ST_SORT);
-        launch_fill(P<uint32_t>(ctx->tstart), 0u, pl.ntiles, s);
-        launch_fill(P<uint32_t>(ctx->tend), 0u, pl.ntiles, s);
-        launch_ranges(skeys, npairs, 0u, pl.ntiles, P<uint32_t>(ctx->tstart), P<uint32_t>(ctx->tend), s);
+        launch_ranges_search(skeys, npairs, pl.ntiles, P<uint32_t>(ctx->tstart), P<uint32_t>(ctx->tend), s);
         mark(ctx, pl.timing, ST_RANGES);
         R.tile_start = P<uint32_t>(ctx->tstart);
         R.tile_end = P<uint32_t>(ctx->tend);
         R.items = svals;
+    }
+    if (pl.general) {
+        // every format through the per-pixel raster; a stencil with a test
+        // or op is tracked there in order (the draw is exhaustive then)
+        FmtParams F;
+        memset(&F, 0, sizeof(F));
+        for (uint32_t p = 0; p < fb->fmt.n_color; p++) {
+            F.planes[p] = fb->planes[p];
+            F.color_fmt[p] = fb->fmt.color[p];
+        }
+        if (fb->standard()) {
+            F.planes[0] = fb->color;
+            F.color_fmt[0] = SR_COLOR_RGBA8;
+        }
+        F.n_color = fb->fmt.n_color;
+        F.depth64 = fb->depth64;
+        F.has_stencil = fb->stencil != nullptr;
+        F.track_stencil = F.has_stencil && pl.smode != 0;
+        mark(ctx, pl.timing, ST_STRIPS);
+        launch_raster_fmt(R, F, pl.ntiles, s);
+        mark(ctx, pl.timing, ST_RASTER);
+        return SR_OK;
     }
     if (pl.smode == 1) {
         if ((rc = ensure(ctx, ctx->strips, sizeof(uint32_t) * (size_t)pl.W * pl.H))) return rc;
         const size_t aux = (size_t)(pl.scols + 1) * (pl.H + 1) * 2 + (size_t)(pl.srows + 1) * (pl.W + 1) * 2 +
                            (size_t)(pl.srows + 1) * (pl.scols + 1) * 4;
         StripParams stp;
-        stp.bins = bins; stp.n = ntris; stp.W = pl.W; stp.H = pl.H;
+        stp.recs = P<TriRec>(ctx->recs); stp.n = ntris; stp.W = pl.W; stp.H = pl.H;
         stp.tw = pl.tw; stp.th = pl.th; stp.scols = pl.scols; stp.srows = pl.srows;
         // after the difference arrays: the work-groups' partial hot rows
         // (written whole by each work-group, so not cleared)
@@ -335,21 +388,26 @@ int sr_rt::validate_draw(sr_context ctx, sr_framebuffer fb, sr_mesh mesh, const 
     if ((d->geometry_program == SR_GS_VERTEX_NORMALS || d->geometry_program == SR_GS_FACE_NORMALS) &&
         d->vertex_program != SR_VP_MVP_NORMAL_WORLD)
         return fail(ctx, SR_ERR_INVALID_ARGUMENT, "draw: normal visualisation needs world-space varyings");
-    if (d->fragment_program < SR_FP_FLAT || d->fragment_program > SR_FP_UV_COLOR)
+    if (d->fragment_program < SR_FP_FLAT || d->fragment_program > SR_FP_GBUFFER)
         return fail(ctx, SR_ERR_INVALID_ARGUMENT, "draw: unknown fragment program");
-    if (d->fragment_program == SR_FP_TEX_LAMBERT && !tex)
+    if (d->blend < SR_BLEND_REPLACE || d->blend > SR_BLEND_OVER_NORMALIZED)
+        return fail(ctx, SR_ERR_INVALID_ARGUMENT, "draw: unknown blend");
+    if ((d->fragment_program == SR_FP_TEX_LAMBERT || d->fragment_program == SR_FP_GBUFFER) && !tex)
         return fail(ctx, SR_ERR_INVALID_ARGUMENT, "draw: fragment program samples a texture");
     if (tex && tex->view && tex->fb == fb)
         return fail(ctx, SR_ERR_INVALID_ARGUMENT, "draw: samples the framebuffer it draws into");
     if (d->fragment_program == SR_FP_BLINN_PHONG && d->vertex_program != SR_VP_MVP_NORMAL_WORLD)
         return fail(ctx, SR_ERR_INVALID_ARGUMENT, "draw: Blinn-Phong needs world-space varyings");
-    if ((d->fragment_program == SR_FP_TEX_LAMBERT || d->fragment_program == SR_FP_UV_COLOR) &&
+    if ((d->fragment_program == SR_FP_TEX_LAMBERT || d->fragment_program == SR_FP_UV_COLOR ||
+         d->fragment_program == SR_FP_GBUFFER) &&
         d->vertex_program != SR_VP_MVP_NORMAL_UV)
         return fail(ctx, SR_ERR_INVALID_ARGUMENT, "draw: program needs uv varyings");
     if (d->stencil_test < 0 || d->stencil_test > 7 || d->stencil_op < 0 || d->stencil_op > 5)
         return fail(ctx, SR_ERR_INVALID_ARGUMENT, "draw: bad stencil config");
-    if (d->exec_tile != 0 && d->exec_tile != 8 && d->exec_tile != 16 && d->exec_tile != 32 && d->exec_tile != 64)
-        return fail(ctx, SR_ERR_INVALID_ARGUMENT, "draw: exec_tile must be 0, 8, 16, 32 or 64");
+    if (d->exec_tile != 0 && d->exec_tile != 8 && d->exec_tile != 16 && d->exec_tile != 32)
+        return fail(ctx, SR_ERR_INVALID_ARGUMENT, "draw: exec_tile must be 0, 8, 16 or 32");
+    if (d->exec_waves != 0 && d->exec_waves != 1 && d->exec_waves != 2 && d->exec_waves != 4)
+        return fail(ctx, SR_ERR_INVALID_ARGUMENT, "draw: exec_waves must be 0, 1, 2 or 4");
     return SR_OK;
 }
 
@@ -424,7 +482,7 @@ size_t batch_bytes_estimate(uint64_t tris, uint64_t verts, double pairs_per_tri)
     // offsets, four pair arrays of the sort, radix histograms; 25% growth
     // headroom
     const double pairs = (double)tris * pairs_per_tri + 65536.0;
-    const double b = 40.0 * (double)verts + (double)(sizeof(TriRec) + sizeof(TriBin)) * (double)tris +
+    const double b = 40.0 * (double)verts + (double)(sizeof(TriRec) + sizeof(TriBin) + sizeof(TriVid)) * (double)tris +
                      8.0 * (double)tris +
                      16.0 * pairs + 4.0 * (double)radix_hist_words((uint32_t)std::min(pairs, 2e9));
     return (size_t)(b * 1.25);
@@ -433,7 +491,8 @@ size_t batch_bytes_estimate(uint64_t tris, uint64_t verts, double pairs_per_tri)
 bool batchable(sr_mesh m, const sr_draw_desc* d) {
     return m->prim == SR_PRIM_TRIANGLES && d->geometry_program == SR_GS_NONE &&
            d->stencil_test == SR_STENCIL_ALWAYS && d->stencil_op == SR_STENCIL_KEEP &&
-           !(d->exec_flags & SR_EXEC_EXHAUSTIVE);
+           !(d->exec_flags & SR_EXEC_EXHAUSTIVE) && d->fragment_program != SR_FP_GBUFFER &&
+           (d->blend == SR_BLEND_REPLACE || d->blend == SR_BLEND_OVER);
 }
 
 int batch_plan(sr_context ctx, const sr_batch_item* items, uint32_t n, BatchPlan& pl) {
@@ -451,15 +510,18 @@ int batch_plan(sr_context ctx, const sr_batch_item* items, uint32_t n, BatchPlan
     }
     struct Order {
         uint32_t group;
-        uint32_t ets;
-        uint64_t cls;  // (ets, fp, blend)
+        uint32_t ets, nwv;
+        uint64_t cls;  // (ets, waves, fp, blend)
     };
     std::vector<Order> order(fbs.size());
     for (uint32_t g = 0; g < fbs.size(); g++) {
         const sr_batch_item& it = items[members[g][0]];
         const sr_draw_desc* d = it.desc;
         const uint32_t ets = d->exec_tile ? (uint32_t)d->exec_tile : (uint32_t)policy_tile(ctx, it.fb->w, it.fb->h);
-        order[g] = Order{g, ets, ((uint64_t)ets << 16) | ((uint64_t)d->fragment_program << 8) | (uint64_t)d->blend};
+        const uint32_t nwv = (uint32_t)draw_waves(ctx, d, ets, it.fb->w, it.fb->h);
+        order[g] = Order{g, ets, nwv,
+                         ((uint64_t)ets << 24) | ((uint64_t)nwv << 16) | ((uint64_t)d->fragment_program << 8) |
+                             (uint64_t)d->blend};
     }
     std::stable_sort(order.begin(), order.end(), [](const Order& a, const Order& b) { return a.cls < b.cls; });
     pl = BatchPlan();
@@ -489,8 +551,11 @@ int batch_plan(sr_context ctx, const sr_batch_item* items, uint32_t n, BatchPlan
             g.nz = m->soa[5]; g.u = m->soa[6]; g.v = m->soa[7]; g.idx = m->idx;
             g.nv = ntri ? m->nv : 0; g.vbase = (uint32_t)pl.NV;
             g.nt = ntri; g.tbase = (uint32_t)pl.NT;
+            if (m->indexed) pl.soup = false;
             g.program = d->vertex_program; g.K = varyings_of(d->vertex_program); g.cull = d->cull;
             g.W = W; g.H = H; g.ets = ets; g.ntx = ntx;
+            g.tw = d->tile_w > 0 ? (uint32_t)d->tile_w : 128u;
+            g.th = d->tile_h > 0 ? (uint32_t)d->tile_h : 128u;
             g.tile_base = tile_base;
             g.empty_cull = 1;  // batched draws have no stencil effects
             g.viewport = d->viewport;
@@ -503,8 +568,8 @@ int batch_plan(sr_context ctx, const sr_batch_item* items, uint32_t n, BatchPlan
         pl.group_tile_end.push_back((uint32_t)pl.NTILES);
         const sr_draw_desc* d0 = items[members[o.group][0]].desc;
         if (pl.classes.empty() || pl.classes.back().cls != o.cls)
-            pl.classes.push_back(BatchClass{o.cls, ets, (uint32_t)d0->fragment_program, (uint32_t)d0->blend,
-                                            tile_base, 0});
+            pl.classes.push_back(BatchClass{o.cls, ets, o.nwv, (uint32_t)d0->fragment_program,
+                                            (uint32_t)d0->blend, tile_base, 0});
         pl.classes.back().ntiles += ntiles;
     }
     pl.group_first.push_back(k);
@@ -524,7 +589,8 @@ int batch_plan(sr_context ctx, const sr_batch_item* items, uint32_t n, BatchPlan
 constexpr uint32_t TILE_FIXED_COST = 16;  // a tile's load/clear + resolve, in item iterations
 
 static uint32_t raster_runs(const BatchClass& C) {
-    return std::min<uint32_t>(C.ntiles, C.ets == 8 ? 256u * 16u * 32u : 256u * 4u * 32u);
+    // work-groups of 64 * nwv threads: 16 / nwv per CU
+    return std::min<uint32_t>(C.ntiles, 256u * (16u / C.nwv) * 32u);
 }
 
 static size_t raster_runs_total(const BatchPlan& pl) {
@@ -552,7 +618,7 @@ int batch_enqueue(sr_context ctx, BatchWork& W, const BatchPlan& pl, bool use_gr
     const uint64_t seg_words = 4ull * ng + 2;
     const size_t before = W.device_bytes();
     if ((rc = ensure(ctx, W.verts, sizeof(float) * 10ull * nvb, s)) ||
-        (rc = ensure(ctx, W.recs, (sizeof(TriRec) + sizeof(TriBin)) * std::max<uint64_t>(NT, 1), s)) ||
+        (rc = ensure(ctx, W.recs, (sizeof(TriRec) + sizeof(TriBin) + sizeof(TriVid)) * std::max<uint64_t>(NT, 1), s)) ||
         (rc = ensure(ctx, W.counts, sizeof(uint32_t) * (NT + 1), s)) ||
         (rc = ensure(ctx, W.offsets, sizeof(uint32_t) * (NT + 1), s)) ||
         (rc = ensure(ctx, W.keys, sizeof(uint32_t) * (size_t)cap, s)) ||
@@ -583,7 +649,7 @@ int batch_enqueue(sr_context ctx, BatchWork& W, const BatchPlan& pl, bool use_gr
     uint32_t* d_group_first = d_group_tile_end + ng;
     uint32_t* misc = P<uint32_t>(W.misc);  // [0] overflow [1] pairs [2] err [3..4] sort [5] tile cost total
                                             // [6] pairs to sort (0 on overflow)
-    const VertexBuf vb{P<float>(W.verts), nvb};
+    const VertexBuf vb{P<float>(W.verts), nvb, 6};  // rows of 6: the larger program's varyings
     const int swapped_passes = (local_bits + 7) / 8;
     const uint32_t* sorted_vals = (swapped_passes & 1) ? P<uint32_t>(W.vals2) : P<uint32_t>(W.vals);
     std::vector<RasterParams> rp(ns);
@@ -599,15 +665,18 @@ int batch_enqueue(sr_context ctx, BatchWork& W, const BatchPlan& pl, bool use_gr
         R.scols = R.W > 1 ? (R.W - 1 + R.tw - 1) / R.tw : 0;
         R.srows = R.H > 1 ? (R.H - 1 + R.th - 1) / R.th : 0;
         R.ets = pl.geom[k].ets; R.ntx = pl.geom[k].ntx; R.nty = (R.H + R.ets - 1) / R.ets;
+        R.nwv = (int)draw_waves(ctx, d, R.ets, R.W, R.H);
         R.tile_start = P<uint32_t>(W.tstart); R.tile_end = P<uint32_t>(W.tend);
         R.items = sorted_vals;
         R.tris = P<TriRec>(W.recs);
+        R.vids = (const TriVid*)((const TriBin*)(P<TriRec>(W.recs) + std::max<uint64_t>(NT, 1)) +
+                                 std::max<uint64_t>(NT, 1));
         R.verts = vb;
         R.K = pl.geom[k].K;
         R.fp = d->fragment_program; R.blend = d->blend; R.alpha_mode = d->alpha_mode;
         R.s_test = SR_STENCIL_ALWAYS; R.s_op = SR_STENCIL_KEEP;
         R.smode = 0;
-        if (it.tex) { R.tex = it.tex->d; R.texW = it.tex->w; R.texH = it.tex->h; }
+        if (it.tex) { R.tex = it.tex->d; R.tex_fmt = it.tex->fmt; R.texW = it.tex->w; R.texH = it.tex->h; }
         R.tex_filter = d->tex_filter; R.tex_edge = d->tex_edge; R.tex_border = d->tex_border;
         memcpy(R.fsu, d->fs_uniforms, sizeof(R.fsu));
         R.tile_base = pl.geom[k].tile_base;
@@ -652,10 +721,14 @@ int batch_enqueue(sr_context ctx, BatchWork& W, const BatchPlan& pl, bool use_gr
         }
         BatchTables T{d_geom, d_vend, d_tend, ns, (uint32_t)NV, (uint32_t)NT};
         launch_fill(misc, 0u, 8u, s);
-        launch_vertex_batch(T, vb, s);
+        if (!pl.soup) launch_vertex_batch(T, vb, s);
         // the triangle records, then their binning records, in one buffer
         TriBin* const bins = (TriBin*)(P<TriRec>(W.recs) + std::max<uint64_t>(NT, 1));
-        launch_tri_setup_batch(T, vb, P<TriRec>(W.recs), bins, P<uint32_t>(W.counts), misc + 2, s);
+        TriVid* const vids = (TriVid*)(bins + std::max<uint64_t>(NT, 1));
+        if (pl.soup)  // every mesh non-indexed: the vertex program runs inside setup
+            launch_vertex_setup_soup_batch(T, vb, P<TriRec>(W.recs), bins, vids, P<uint32_t>(W.counts), misc + 2, s);
+        else
+            launch_tri_setup_batch(T, vb, P<TriRec>(W.recs), bins, vids, P<uint32_t>(W.counts), misc + 2, s);
         launch_scan(P<uint32_t>(W.counts), P<uint32_t>(W.offsets), (uint32_t)NT, P<uint32_t>(W.scan), misc + 1, s);
         launch_tri_emit_batch(T, P<TriRec>(W.recs), bins, P<uint32_t>(W.counts), P<uint32_t>(W.offsets), cap,
                               P<uint32_t>(W.keys), P<uint32_t>(W.vals), misc + 0, s);
@@ -683,7 +756,8 @@ int batch_enqueue(sr_context ctx, BatchWork& W, const BatchPlan& pl, bool use_gr
             launch_partition(P<uint32_t>(W.cost), C.tile0, C.ntiles, G, runs, s);
             RasterBatch RB;
             memset(&RB, 0, sizeof(RB));
-            RB.one.fp = (int)C.fp; RB.one.blend = (int)C.blend; RB.one.ets = C.ets; RB.one.smode = 0;
+            RB.one.fp = (int)C.fp; RB.one.blend = (int)C.blend; RB.one.ets = C.ets; RB.one.nwv = (int)C.nwv;
+            RB.one.smode = 0;
             RB.scenes = d_rp;
             RB.group_tile_end = d_group_tile_end;
             RB.group_first = d_group_first;
@@ -716,7 +790,7 @@ int batch_enqueue(sr_context ctx, BatchWork& W, const BatchPlan& pl, bool use_gr
 
 int batch_result(sr_context ctx, BatchWork& W, const BatchPlan& pl, bool* again) {
     *again = false;
-    if (W.h_misc[2]) return fail(ctx, SR_ERR_INVALID_COORDINATE, "batch: NaN screen coordinate");
+    if (W.h_misc[2] & 1u) return fail(ctx, SR_ERR_INVALID_COORDINATE, "batch: NaN screen coordinate");
     if (!W.h_misc[0]) return SR_OK;
     // binning overflowed: nothing was drawn; grow and run again
     const uint64_t want = (uint64_t)W.h_misc[1] + W.h_misc[1] / 4 + 4096;
@@ -731,10 +805,34 @@ int batch_result(sr_context ctx, BatchWork& W, const BatchPlan& pl, bool* again)
 
 namespace {
 
-int batch_draw(sr_context ctx, const sr_batch_item* items, uint32_t n) {
+// Framebuffer::clear with an RGBA8 colour (a batch item's clear): on other
+// formats every RGBA8 plane takes it as is and every RGBAF32 plane its
+// unorm values (c / 255).
+int clear_rgba8(sr_framebuffer fb, uint32_t rgba) {
+    if (fb->standard()) return sr_fb_clear(fb, rgba, -3.40282347e+38f, 0);
+    uint8_t colors[16 * SR_MAX_COLOR_PLANES];
+    uint8_t* c = colors;
+    for (uint32_t p = 0; p < fb->fmt.n_color; p++) {
+        if (fb->fmt.color[p] == SR_COLOR_RGBAF32) {
+            const float f[4] = {(float)(rgba & 0xFFu) / 255.0f, (float)((rgba >> 8) & 0xFFu) / 255.0f,
+                                (float)((rgba >> 16) & 0xFFu) / 255.0f, (float)(rgba >> 24) / 255.0f};
+            memcpy(c, f, 16);
+            c += 16;
+        } else {
+            memcpy(c, &rgba, 4);
+            c += 4;
+        }
+    }
+    return sr_fb_clear_ex(fb, colors);
+}
+
+// Validates a batch and draws, one by one and in item order, the items of
+// every framebuffer whose items cannot form a one-pass group: they are all
+// batchable, share execution tile, program and blend, and only the first
+// clears (framebuffers are independent, so this reorders nothing visible).
+// The other items go to `fast`, in order.
+int split_batch(sr_context ctx, const sr_batch_item* items, uint32_t n, std::vector<sr_batch_item>& fast) {
     int rc;
-    hipSetDevice(ctx->device);
-    ctx->n_stage = 0;
     std::vector<sr_framebuffer> seen;  // distinct framebuffers, in order of first appearance
     for (uint32_t i = 0; i < n; i++) {
         const sr_batch_item& it = items[i];
@@ -749,25 +847,25 @@ int batch_draw(sr_context ctx, const sr_batch_item* items, uint32_t n) {
                     return fail(ctx, SR_ERR_INVALID_ARGUMENT,
                                 "batch: an item samples a framebuffer the batch draws into");
     }
-    // Per framebuffer: its items form one group of the batch when all are
-    // batchable, share execution tile, program and blend, and only the first
-    // clears; otherwise they are drawn one by one, in item order, before the
-    // batch (framebuffers are independent, so this reorders nothing visible).
-    std::vector<sr_batch_item> fast;
     for (sr_framebuffer fb : seen) {
         std::vector<uint32_t> mine;
         for (uint32_t i = 0; i < n; i++)
             if (items[i].fb == fb) mine.push_back(i);
         bool group = true;
-        int ets0 = -1;
+        int ets0 = -1, ew0 = -1;
         for (size_t j = 0; j < mine.size() && group; j++) {
             const sr_batch_item& it = items[mine[j]];
             const sr_draw_desc* d = it.desc;
             const int ets = d->exec_tile ? d->exec_tile : policy_tile(ctx, fb->w, fb->h);
+            const int ew = draw_waves(ctx, d, (uint32_t)ets, fb->w, fb->h);
             const sr_draw_desc* f = items[mine[0]].desc;
-            group = batchable(it.mesh, d) && (j == 0 || !it.clear) && (j == 0 || ets == ets0) &&
+            group = batchable(it.mesh, d) && fb->standard() && (j == 0 || !it.clear) &&
+                    (j == 0 || (ets == ets0 && ew == ew0)) &&
                     d->fragment_program == f->fragment_program && d->blend == f->blend;
-            if (j == 0) ets0 = ets;
+            if (j == 0) {
+                ets0 = ets;
+                ew0 = ew;
+            }
         }
         for (uint32_t i : mine) {
             const sr_batch_item& it = items[i];
@@ -775,10 +873,28 @@ int batch_draw(sr_context ctx, const sr_batch_item* items, uint32_t n) {
                 fast.push_back(it);
                 continue;
             }
-            if (it.clear && (rc = sr_fb_clear(it.fb, it.clear_rgba, -3.40282347e+38f, 0))) return rc;
+            if (it.clear && (rc = clear_rgba8(it.fb, it.clear_rgba))) return rc;
             if ((rc = sr_draw(ctx, it.fb, it.mesh, it.desc, it.tex))) return rc;
         }
     }
+    return SR_OK;
+}
+
+// After a one-pass batch ran on W: its error flags, or an overflow
+// (*again: it must run again with the grown capacity).
+int finish_batch(sr_context ctx, BatchWork& W, const BatchPlan& plan, bool* again) {
+    int rc;
+    if ((rc = batch_result(ctx, W, plan, again))) return rc;
+    ctx->last.strip_slivers = batch_strip_slivers(W) ? 1u : 0u;
+    return SR_OK;
+}
+
+int batch_draw(sr_context ctx, const sr_batch_item* items, uint32_t n) {
+    int rc;
+    hipSetDevice(ctx->device);
+    ctx->n_stage = 0;
+    std::vector<sr_batch_item> fast;
+    if ((rc = split_batch(ctx, items, n, fast))) return rc;
     if (fast.empty()) return SR_OK;
     if (!ctx->batch) {
         BatchWork* W = new (std::nothrow) BatchWork();
@@ -803,13 +919,51 @@ int batch_draw(sr_context ctx, const sr_batch_item* items, uint32_t n) {
         ctx->last.bin_pairs = W.h_misc[1];
         ctx->last.triangles_in = plan.NT;
         bool again = false;
-        if ((rc = batch_result(ctx, W, plan, &again))) return rc;
+        if ((rc = finish_batch(ctx, W, plan, &again))) return rc;
         if (!again) return SR_OK;
     }
     return fail(ctx, SR_ERR_DEVICE, "batch: capacity did not converge");
 }
 
+// ---------------------------------------------------------------------------
+// Asynchronous batches: the one-pass group queued on a slot's own stream,
+// finished by sr_batch_wait.
+// ---------------------------------------------------------------------------
+constexpr size_t kAsyncSlots = 4;
+
+// Runs the slot's batch to its end (overflow reruns, the one-by-one
+// fallback) and records its status; the slot is free afterwards.
+int async_complete(sr_context ctx, AsyncBatch& b) {
+    int rc = SR_OK;
+    for (int attempt = 0; attempt < 5; attempt++) {
+        if (hipEventSynchronize(b.W.ev1) != hipSuccess) {
+            rc = fail(ctx, SR_ERR_DEVICE, "batch: device failure");
+            break;
+        }
+        bool again = false;
+        if ((rc = finish_batch(ctx, b.W, b.plan, &again)) || !again) break;
+        if (attempt == 4) {
+            rc = fail(ctx, SR_ERR_DEVICE, "batch: capacity did not converge");
+            break;
+        }
+        if ((rc = batch_enqueue(ctx, b.W, b.plan, false))) break;
+    }
+    b.busy = false;
+    ctx->async_done[b.ticket] = rc;
+    return rc;
+}
+
+void destroy_async(sr_context ctx) {
+    for (AsyncBatch* b : ctx->async) {
+        b->W.release();
+        delete b;
+    }
+    ctx->async.clear();
+    ctx->async_done.clear();
+}
+
 void destroy_batch(sr_context ctx) {
+    destroy_async(ctx);
     if (!ctx->batch) return;
     ctx->batch->release();
     delete ctx->batch;
@@ -978,6 +1132,7 @@ int sr_texture_free(sr_texture t) {
 
 int sr_texture_from_fb(sr_framebuffer fb, sr_texture* out) {
     if (!fb || !out) return SR_ERR_INVALID_ARGUMENT;
+    if (fb->fmt.color[0] != SR_COLOR_RGBA8) return fail(fb->ctx, SR_ERR_UNSUPPORTED, "texture_from_fb: use sr_texture_from_fb_plane");
     sr_texture t = new (std::nothrow) sr_texture_t();
     if (!t) return SR_ERR_OUT_OF_MEMORY;
     t->ctx = fb->ctx; t->w = fb->w; t->h = fb->h;
@@ -1011,11 +1166,143 @@ int sr_fb_create(sr_context ctx, uint32_t w, uint32_t h, sr_framebuffer* out) {
     return SR_OK;
 }
 
+static size_t plane_bytes(int fmt) { return fmt == SR_COLOR_RGBAF32 ? 16 : 4; }
+
+// The functions that take one RGBA8 colour and f32 depth need plane 0 in
+// RGBA8 and f32 depth.
+static bool rgba8_depth32(sr_framebuffer fb) {
+    return fb->fmt.color[0] == SR_COLOR_RGBA8 && fb->fmt.depth == SR_DEPTH_F32;
+}
+
+int sr_fb_create_ex(sr_context ctx, uint32_t w, uint32_t h, const sr_fb_format* f, sr_framebuffer* out) {
+    if (!ctx || !out || !f) return SR_ERR_INVALID_ARGUMENT;
+    if (w == 0 || h == 0) return fail(ctx, SR_ERR_INVALID_ARGUMENT, "Framebuffer must have a non-zero width and height");
+    if (f->n_color < 1 || f->n_color > SR_MAX_COLOR_PLANES) return fail(ctx, SR_ERR_INVALID_ARGUMENT, "framebuffer: 1..4 colour planes");
+    for (uint32_t p = 0; p < f->n_color; p++)
+        if (f->color[p] != SR_COLOR_RGBA8 && f->color[p] != SR_COLOR_RGBAF32)
+            return fail(ctx, SR_ERR_INVALID_ARGUMENT, "framebuffer: unknown colour format");
+    if ((f->depth != SR_DEPTH_F32 && f->depth != SR_DEPTH_F64) || (f->stencil != SR_STENCIL_U8 && f->stencil != SR_STENCIL_NONE))
+        return fail(ctx, SR_ERR_INVALID_ARGUMENT, "framebuffer: unknown depth or stencil format");
+    sr_framebuffer fb = new (std::nothrow) sr_framebuffer_t();
+    if (!fb) return SR_ERR_OUT_OF_MEMORY;
+    fb->ctx = ctx; fb->w = w; fb->h = h;
+    fb->fmt = *f;
+    for (uint32_t p = f->n_color; p < SR_MAX_COLOR_PLANES; p++) fb->fmt.color[p] = 0;
+    const size_t n = (size_t)w * h;
+    bool ok = true;
+    for (uint32_t p = 0; p < f->n_color && ok; p++) ok = hipMalloc(&fb->planes[p], plane_bytes(f->color[p]) * n) == hipSuccess;
+    if (ok && f->depth == SR_DEPTH_F64) ok = hipMalloc(&fb->depth64, 8 * n) == hipSuccess;
+    if (ok && f->depth == SR_DEPTH_F32) ok = hipMalloc(&fb->depth, 4 * n) == hipSuccess;
+    if (ok && f->stencil == SR_STENCIL_U8) ok = hipMalloc(&fb->stencil, n) == hipSuccess;
+    if (!ok) {
+        sr_fb_free(fb);
+        return fail(ctx, SR_ERR_OUT_OF_MEMORY, "framebuffer: alloc failed");
+    }
+    if (f->color[0] == SR_COLOR_RGBA8) fb->color = (uint32_t*)fb->planes[0];
+    // with_dimensions: every plane Color::empty() (zero), depth far, stencil 0
+    // (texturebuffer.rs:102-111)
+    uint8_t zeros[16 * SR_MAX_COLOR_PLANES] = {};
+    int rc = sr_fb_clear_ex(fb, zeros);
+    if (rc == SR_OK && hipStreamSynchronize(ctx->stream) != hipSuccess) rc = SR_ERR_DEVICE;
+    if (rc != SR_OK) {
+        sr_fb_free(fb);
+        return fail(ctx, rc, "framebuffer: clear failed");
+    }
+    *out = fb;
+    return SR_OK;
+}
+
+int sr_fb_get_format(sr_framebuffer fb, sr_fb_format* out) {
+    if (!fb || !out) return SR_ERR_INVALID_ARGUMENT;
+    *out = fb->fmt;
+    return SR_OK;
+}
+
+int sr_fb_clear_ex(sr_framebuffer fb, const void* colors) {
+    if (!fb || !colors) return SR_ERR_INVALID_ARGUMENT;
+    const size_t n = (size_t)fb->w * fb->h;
+    hipStream_t s = fb->ctx->stream;
+    const uint8_t* c = (const uint8_t*)colors;
+    for (uint32_t p = 0; p < fb->fmt.n_color; p++) {
+        uint32_t pat[4] = {};
+        const size_t b = plane_bytes(fb->fmt.color[p]);
+        memcpy(pat, c, b);
+        c += b;
+        launch_fill_pattern(fb->planes[p], pat, (uint32_t)(b / 4), n * (b / 4), s);
+    }
+    if (fb->depth64) {
+        const double far = -1.7976931348623157e308;  // f64::MIN, depth.rs:31
+        uint32_t pat[4];
+        memcpy(pat, &far, 8);
+        launch_fill_pattern(fb->depth64, pat, 2, 2 * n, s);
+    }
+    if (fb->depth) {
+        const float far = -3.40282347e+38f;  // f32::MIN
+        uint32_t pat[4];
+        memcpy(pat, &far, 4);
+        launch_fill_pattern(fb->depth, pat, 1, n, s);
+    }
+    if (fb->stencil) HIP_OK(fb->ctx, hipMemsetAsync(fb->stencil, 0, n, s));
+    return SR_OK;
+}
+
+int sr_fb_read_ex(sr_framebuffer fb, void* const* planes, void* depth, uint8_t* stencil) {
+    if (!fb) return SR_ERR_INVALID_ARGUMENT;
+    sr_context ctx = fb->ctx;
+    const size_t n = (size_t)fb->w * fb->h;
+    for (uint32_t p = 0; planes && p < fb->fmt.n_color; p++)
+        if (planes[p])
+            HIP_OK(ctx, hipMemcpyAsync(planes[p], fb->planes[p], plane_bytes(fb->fmt.color[p]) * n,
+                                       hipMemcpyDeviceToHost, ctx->stream));
+    if (depth)
+        HIP_OK(ctx, hipMemcpyAsync(depth, fb->depth64 ? (void*)fb->depth64 : (void*)fb->depth,
+                                   (fb->depth64 ? 8 : 4) * n, hipMemcpyDeviceToHost, ctx->stream));
+    if (stencil && fb->stencil) HIP_OK(ctx, hipMemcpyAsync(stencil, fb->stencil, n, hipMemcpyDeviceToHost, ctx->stream));
+    HIP_OK(ctx, hipStreamSynchronize(ctx->stream));
+    return SR_OK;
+}
+
+int sr_fb_write_ex(sr_framebuffer fb, const void* const* planes, const void* depth, const uint8_t* stencil) {
+    if (!fb) return SR_ERR_INVALID_ARGUMENT;
+    sr_context ctx = fb->ctx;
+    const size_t n = (size_t)fb->w * fb->h;
+    for (uint32_t p = 0; planes && p < fb->fmt.n_color; p++)
+        if (planes[p])
+            HIP_OK(ctx, hipMemcpyAsync(fb->planes[p], planes[p], plane_bytes(fb->fmt.color[p]) * n,
+                                       hipMemcpyHostToDevice, ctx->stream));
+    if (depth)
+        HIP_OK(ctx, hipMemcpyAsync(fb->depth64 ? (void*)fb->depth64 : (void*)fb->depth, depth,
+                                   (fb->depth64 ? 8 : 4) * n, hipMemcpyHostToDevice, ctx->stream));
+    if (stencil && fb->stencil) HIP_OK(ctx, hipMemcpyAsync(fb->stencil, stencil, n, hipMemcpyHostToDevice, ctx->stream));
+    HIP_OK(ctx, hipStreamSynchronize(ctx->stream));
+    return SR_OK;
+}
+
+int sr_texture_from_fb_plane(sr_framebuffer fb, uint32_t plane, sr_texture* out) {
+    if (!fb || !out || plane >= fb->fmt.n_color) return SR_ERR_INVALID_ARGUMENT;
+    sr_texture t = new (std::nothrow) sr_texture_t();
+    if (!t) return SR_ERR_OUT_OF_MEMORY;
+    t->ctx = fb->ctx; t->w = fb->w; t->h = fb->h;
+    t->d = (uint32_t*)fb->planes[plane];  // row-major texels of the plane's format: the texture layout
+    t->fmt = fb->fmt.color[plane];
+    t->view = true;
+    t->fb = fb;
+    *out = t;
+    return SR_OK;
+}
+
 int sr_fb_free(sr_framebuffer fb) {
     if (!fb) return SR_ERR_INVALID_ARGUMENT;
     hipStreamSynchronize(fb->ctx->stream);
-    if (fb->color) hipFree(fb->color);
+    bool color_in_planes = false;
+    for (void* p : fb->planes)
+        if (p) {
+            color_in_planes |= p == (void*)fb->color;
+            hipFree(p);
+        }
+    if (fb->color && !color_in_planes) hipFree(fb->color);
     if (fb->depth) hipFree(fb->depth);
+    if (fb->depth64) hipFree(fb->depth64);
     if (fb->stencil) hipFree(fb->stencil);
     delete fb;
     return SR_OK;
@@ -1023,28 +1310,31 @@ int sr_fb_free(sr_framebuffer fb) {
 
 int sr_fb_clear(sr_framebuffer fb, uint32_t rgba, float depth, uint8_t stencil) {
     if (!fb) return SR_ERR_INVALID_ARGUMENT;
+    if (!fb->standard()) return fail(fb->ctx, SR_ERR_UNSUPPORTED, "fb_clear: use sr_fb_clear_ex for this format");
     launch_fb_clear(fb->color, fb->depth, fb->stencil, fb->w * fb->h, rgba, depth, stencil, fb->ctx->stream);
     return SR_OK;
 }
 
 int sr_fb_read(sr_framebuffer fb, uint32_t* rgba, float* depth, uint8_t* stencil) {
     if (!fb) return SR_ERR_INVALID_ARGUMENT;
+    if (!rgba8_depth32(fb)) return fail(fb->ctx, SR_ERR_UNSUPPORTED, "fb_read: use sr_fb_read_ex for this format");
     sr_context ctx = fb->ctx;
     const size_t n = (size_t)fb->w * fb->h;
     if (rgba) HIP_OK(ctx, hipMemcpyAsync(rgba, fb->color, 4 * n, hipMemcpyDeviceToHost, ctx->stream));
     if (depth) HIP_OK(ctx, hipMemcpyAsync(depth, fb->depth, 4 * n, hipMemcpyDeviceToHost, ctx->stream));
-    if (stencil) HIP_OK(ctx, hipMemcpyAsync(stencil, fb->stencil, n, hipMemcpyDeviceToHost, ctx->stream));
+    if (stencil && fb->stencil) HIP_OK(ctx, hipMemcpyAsync(stencil, fb->stencil, n, hipMemcpyDeviceToHost, ctx->stream));
     HIP_OK(ctx, hipStreamSynchronize(ctx->stream));
     return SR_OK;
 }
 
 int sr_fb_write(sr_framebuffer fb, const uint32_t* rgba, const float* depth, const uint8_t* stencil) {
     if (!fb) return SR_ERR_INVALID_ARGUMENT;
+    if (!rgba8_depth32(fb)) return fail(fb->ctx, SR_ERR_UNSUPPORTED, "fb_write: use sr_fb_write_ex for this format");
     sr_context ctx = fb->ctx;
     const size_t n = (size_t)fb->w * fb->h;
     if (rgba) HIP_OK(ctx, hipMemcpyAsync(fb->color, rgba, 4 * n, hipMemcpyHostToDevice, ctx->stream));
     if (depth) HIP_OK(ctx, hipMemcpyAsync(fb->depth, depth, 4 * n, hipMemcpyHostToDevice, ctx->stream));
-    if (stencil) HIP_OK(ctx, hipMemcpyAsync(fb->stencil, stencil, n, hipMemcpyHostToDevice, ctx->stream));
+    if (stencil && fb->stencil) HIP_OK(ctx, hipMemcpyAsync(fb->stencil, stencil, n, hipMemcpyHostToDevice, ctx->stream));
     HIP_OK(ctx, hipStreamSynchronize(ctx->stream));
     return SR_OK;
 }
@@ -1069,6 +1359,7 @@ static int draw_one(sr_context ctx, sr_framebuffer fb, sr_mesh mesh, const sr_dr
     pl.tw = d->tile_w > 0 ? (uint32_t)d->tile_w : 128u;
     pl.th = d->tile_h > 0 ? (uint32_t)d->tile_h : 128u;
     pl.ets = d->exec_tile ? (uint32_t)d->exec_tile : (uint32_t)policy_tile(ctx, fb->w, fb->h);
+    pl.nwv = draw_waves(ctx, d, pl.ets, fb->w, fb->h);
     pl.ntx = (pl.W + pl.ets - 1) / pl.ets;
     pl.nty = (pl.H + pl.ets - 1) / pl.ets;
     pl.ntiles = pl.ntx * pl.nty;
@@ -1081,6 +1372,21 @@ static int draw_one(sr_context ctx, sr_framebuffer fb, sr_mesh mesh, const sr_dr
     if (d->stencil_test == SR_STENCIL_ALWAYS) pl.smode = d->stencil_op == SR_STENCIL_KEEP ? 0 : 1;
     else pl.smode = 2;
     pl.exhaustive = (d->exec_flags & SR_EXEC_EXHAUSTIVE) != 0 || (value_test && d->stencil_op != SR_STENCIL_KEEP);
+    const bool no_stencil = fb->fmt.stencil == SR_STENCIL_NONE;  // `()`: no test, no op (stencil.rs:65-86)
+    const bool normals_gs = d->geometry_program == SR_GS_VERTEX_NORMALS || d->geometry_program == SR_GS_FACE_NORMALS;
+    pl.general = !fb->standard() || d->blend == SR_BLEND_OVER_NORMALIZED || d->fragment_program == SR_FP_GBUFFER;
+    if (pl.general) {
+        if (mesh->prim != SR_PRIM_TRIANGLES || normals_gs)
+            return fail(ctx, SR_ERR_UNSUPPORTED, "draw: lines and points need sr_fb_create's format, RGBA8 programs and blends");
+        // 8x8 execution tiles; a stencil with a test or op runs exhaustive
+        pl.ets = 8;
+        pl.nwv = 1;
+        pl.ntx = (pl.W + 7) / 8;
+        pl.nty = (pl.H + 7) / 8;
+        pl.ntiles = pl.ntx * pl.nty;
+        if (no_stencil) pl.smode = 0;
+        else if (pl.smode != 0) pl.exhaustive = true;
+    }
 
     const uint32_t per = (uint32_t)mesh->prim;
     const uint32_t nidx = mesh_count(mesh);
@@ -1089,7 +1395,8 @@ static int draw_one(sr_context ctx, sr_framebuffer fb, sr_mesh mesh, const sr_dr
     ctx->last.lines_in = mesh->prim == SR_PRIM_LINES ? nprim : 0;
     ctx->last.points_in = mesh->prim == SR_PRIM_POINTS ? nprim : 0;
     // StencilTest::Never: no pixel passes, nothing is written (stencil.rs:115)
-    if (pl.scols == 0 || pl.srows == 0 || d->stencil_test == SR_STENCIL_NEVER || nprim == 0) return SR_OK;
+    if (pl.scols == 0 || pl.srows == 0 || (d->stencil_test == SR_STENCIL_NEVER && !no_stencil) || nprim == 0)
+        return SR_OK;
 
     int rc;
     hipStream_t s = ctx->stream;
@@ -1102,7 +1409,7 @@ static int draw_one(sr_context ctx, sr_framebuffer fb, sr_mesh mesh, const sr_dr
     if (!clip &&
         (rc = ensure(ctx, ctx->verts, sizeof(float) * (size_t)(4 + pl.K) * std::max<uint32_t>(nv, 1))))
         return rc;
-    VertexBuf vb = clip ? *clip : VertexBuf{P<float>(ctx->verts), std::max<uint32_t>(nv, 1)};
+    VertexBuf vb = clip ? *clip : VertexBuf{P<float>(ctx->verts), std::max<uint32_t>(nv, 1), (uint32_t)pl.K};
     VSParams vs;
     memset(&vs, 0, sizeof(vs));
     vs.px = mesh->soa[0]; vs.py = mesh->soa[1]; vs.pz = mesh->soa[2];
@@ -1117,13 +1424,16 @@ static int draw_one(sr_context ctx, sr_framebuffer fb, sr_mesh mesh, const sr_dr
     vs.viewport = d->viewport;
     memcpy(vs.U, d->vs_uniforms, sizeof(vs.U));
     vs.out = vb;
+    // a non-indexed triangle mesh without a geometry program: each vertex
+    // belongs to one triangle, so setup runs the vertex program itself
+    const bool fuse = !clip && gs == SR_GS_NONE && mesh->prim == SR_PRIM_TRIANGLES && !mesh->indexed;
     if (!clip) {
-        launch_vertex(vs, s);
+        if (!fuse) launch_vertex(vs, s);
     } else if (gs == SR_GS_NONE) {
         // finish the shared clip-space vertices into this chain's own buffer
         if ((rc = ensure(ctx, ctx->gverts, sizeof(float) * (size_t)(4 + pl.K) * std::max<uint32_t>(nv, 1))))
             return rc;
-        const VertexBuf sv{P<float>(ctx->gverts), std::max<uint32_t>(nv, 1)};
+        const VertexBuf sv{P<float>(ctx->gverts), std::max<uint32_t>(nv, 1), (uint32_t)pl.K};
         launch_finish(*clip, sv, nv, pl.K, d->viewport, s);
         vb = sv;
     }
@@ -1143,7 +1453,7 @@ static int draw_one(sr_context ctx, sr_framebuffer fb, sr_mesh mesh, const sr_dr
         gp.in = vb; gp.idx = idx; gp.n_in = nprim; gp.K = pl.K;
         gp.viewport = d->viewport;
         memcpy(gp.G, d->gs_uniforms, sizeof(gp.G));
-        gp.out = VertexBuf{P<float>(ctx->gverts), (uint32_t)(2 * nl)};
+        gp.out = VertexBuf{P<float>(ctx->gverts), (uint32_t)(2 * nl), (uint32_t)pl.K};
         launch_gs_normals(d->geometry_program, gp, s);
         vb = gp.out;
         idx = nullptr;
@@ -1166,7 +1476,7 @@ static int draw_one(sr_context ctx, sr_framebuffer fb, sr_mesh mesh, const sr_dr
         const uint32_t ngen = ctx->h_misc[1];
         if ((rc = ensure(ctx, ctx->gverts, sizeof(float) * (size_t)(4 + pl.K) * std::max<uint32_t>(ngen * per, 1))))
             return rc;
-        gp.out = VertexBuf{P<float>(ctx->gverts), std::max<uint32_t>(ngen * per, 1)};
+        gp.out = VertexBuf{P<float>(ctx->gverts), std::max<uint32_t>(ngen * per, 1), (uint32_t)pl.K};
         launch_gs_emit(d->geometry_program, mesh->prim, gp, s);
         vb = gp.out;
         idx = nullptr;
@@ -1180,7 +1490,7 @@ static int draw_one(sr_context ctx, sr_framebuffer fb, sr_mesh mesh, const sr_dr
     if (nwork == 0) {
         rc = SR_OK;
     } else if (out_prim == SR_PRIM_TRIANGLES) {
-        rc = draw_tris(ctx, fb, pl, d, tex, vb, idx, nwork);
+        rc = draw_tris(ctx, fb, pl, d, tex, vb, idx, nwork, fuse ? &vs : nullptr);
     } else if (out_prim == SR_PRIM_LINES) {
         rc = draw_lines_impl(ctx, fb, pl, d, tex, vb, idx, nwork);
     } else {
@@ -1205,6 +1515,18 @@ int sr_draw(sr_context ctx, sr_framebuffer fb, sr_mesh mesh, const sr_draw_desc*
     hipSetDevice(ctx->device);
     return draw_one(ctx, fb, mesh, d, tex, nullptr);
 }
+
+}  // extern "C"
+
+// sr_draw on a framebuffer of `ctx` with a mesh and texture that another
+// context on the same device owns (the request server draws its clients'
+// meshes on its own context).  The draw was validated when it was queued.
+int sr_rt::draw_foreign(sr_context ctx, sr_framebuffer fb, sr_mesh mesh, const sr_draw_desc* d, sr_texture tex) {
+    hipSetDevice(ctx->device);
+    return draw_one(ctx, fb, mesh, d, tex, nullptr);
+}
+
+extern "C" {
 
 int sr_draw_static(sr_context ctx, sr_framebuffer fb, sr_mesh mesh, const sr_draw_desc* d, sr_texture tex) {
     if (int rc = validate_draw(ctx, fb, mesh, d, tex)) return rc;
@@ -1253,7 +1575,7 @@ int sr_draw_chains(sr_context ctx, sr_framebuffer fb, sr_mesh mesh, const sr_dra
     const uint32_t nv = mesh->nv;
     int rc;
     if ((rc = ensure(ctx, ctx->verts, sizeof(float) * (size_t)(4 + K) * std::max<uint32_t>(nv, 1)))) return rc;
-    const VertexBuf clip{P<float>(ctx->verts), std::max<uint32_t>(nv, 1)};
+    const VertexBuf clip{P<float>(ctx->verts), std::max<uint32_t>(nv, 1), (uint32_t)K};
     VSParams vs;
     memset(&vs, 0, sizeof(vs));
     vs.px = mesh->soa[0]; vs.py = mesh->soa[1]; vs.pz = mesh->soa[2];
@@ -1275,15 +1597,90 @@ int sr_batch_draw(sr_context ctx, const sr_batch_item* items, uint32_t n) {
     return batch_draw(ctx, items, n);
 }
 
+int sr_batch_submit(sr_context ctx, const sr_batch_item* items, uint32_t n, uint64_t* ticket) {
+    if (!ctx || (!items && n) || !ticket) return SR_ERR_INVALID_ARGUMENT;
+    int rc;
+    hipSetDevice(ctx->device);
+    std::vector<sr_batch_item> fast;
+    if ((rc = split_batch(ctx, items, n, fast))) return rc;
+    const uint64_t t = ctx->next_ticket++;
+    if (fast.empty()) {
+        ctx->async_done[t] = SR_OK;
+        *ticket = t;
+        return SR_OK;
+    }
+    // a free slot, else the oldest one completes first
+    AsyncBatch* slot = nullptr;
+    for (AsyncBatch* b : ctx->async)
+        if (!b->busy) {
+            slot = b;
+            break;
+        }
+    if (!slot && ctx->async.size() < kAsyncSlots) {
+        slot = new (std::nothrow) AsyncBatch();
+        if (!slot) return fail(ctx, SR_ERR_OUT_OF_MEMORY, "batch: slot");
+        if ((rc = slot->W.init(ctx, true))) {
+            slot->W.release();
+            delete slot;
+            return rc;
+        }
+        ctx->async.push_back(slot);
+    }
+    if (!slot) {
+        slot = ctx->async[0];
+        for (AsyncBatch* b : ctx->async)
+            if (b->ticket < slot->ticket) slot = b;
+        async_complete(ctx, *slot);  // its status waits in async_done for its own sr_batch_wait
+    }
+    slot->plan = BatchPlan();
+    if ((rc = batch_plan(ctx, fast.data(), (uint32_t)fast.size(), slot->plan))) return rc;
+    // work already queued on the context stream (the caller's clears and
+    // uploads, the items drawn one by one above) comes first
+    HIP_OK(ctx, hipEventRecord(ctx->ev_read, ctx->stream));
+    HIP_OK(ctx, hipStreamWaitEvent(slot->W.stream, ctx->ev_read, 0));
+    if ((rc = batch_enqueue(ctx, slot->W, slot->plan, false))) return rc;
+    slot->ticket = t;
+    slot->busy = true;
+    *ticket = t;
+    return SR_OK;
+}
+
+int sr_batch_query(sr_context ctx, uint64_t ticket) {
+    if (!ctx) return SR_ERR_INVALID_ARGUMENT;
+    for (AsyncBatch* b : ctx->async)
+        if (b->busy && b->ticket == ticket)
+            return hipEventQuery(b->W.ev1) == hipErrorNotReady ? SR_ERR_NOT_READY : SR_OK;
+    return ctx->async_done.count(ticket) ? SR_OK : fail(ctx, SR_ERR_INVALID_ARGUMENT, "batch: unknown ticket");
+}
+
+int sr_batch_wait(sr_context ctx, uint64_t ticket) {
+    if (!ctx) return SR_ERR_INVALID_ARGUMENT;
+    hipSetDevice(ctx->device);
+    for (AsyncBatch* b : ctx->async)
+        if (b->busy && b->ticket == ticket) async_complete(ctx, *b);
+    auto it = ctx->async_done.find(ticket);
+    if (it == ctx->async_done.end()) return fail(ctx, SR_ERR_INVALID_ARGUMENT, "batch: unknown ticket");
+    const int rc = it->second;
+    ctx->async_done.erase(it);
+    return rc;
+}
+
 int sr_ctx_set_exec_tile_policy(sr_context ctx, uint32_t n, const uint64_t* max_pixels,
                                 const int32_t* exec_tile) {
+    return sr_ctx_set_exec_policy(ctx, n, max_pixels, exec_tile, nullptr);
+}
+
+int sr_ctx_set_exec_policy(sr_context ctx, uint32_t n, const uint64_t* max_pixels, const int32_t* exec_tile,
+                           const int32_t* exec_waves) {
     if (!ctx || !n || !max_pixels || !exec_tile) return SR_ERR_INVALID_ARGUMENT;
     std::vector<std::pair<uint64_t, int>> p;
     for (uint32_t i = 0; i < n; i++) {
-        const int e = exec_tile[i];
-        if (e != 8 && e != 16 && e != 32 && e != 64)
-            return fail(ctx, SR_ERR_INVALID_ARGUMENT, "policy: exec tile must be 8, 16, 32 or 64");
-        p.emplace_back(max_pixels[i], e);
+        const int e = exec_tile[i], w = exec_waves ? exec_waves[i] : 0;
+        if (e != 8 && e != 16 && e != 32)
+            return fail(ctx, SR_ERR_INVALID_ARGUMENT, "policy: exec tile must be 8, 16 or 32");
+        if (w != 0 && w != 1 && w != 2 && w != 4)
+            return fail(ctx, SR_ERR_INVALID_ARGUMENT, "policy: waves must be 0, 1, 2 or 4");
+        p.emplace_back(max_pixels[i], e | (w << 8));
     }
     ctx->tile_policy = p;
     if (ctx->batch) ctx->batch->drop_graph();

@@ -8,6 +8,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <map>
 #include <new>
 #include <string>
 #include <vector>
@@ -26,6 +27,7 @@ enum Stage { ST_VERTEX, ST_GEOMETRY, ST_SETUP, ST_SCAN, ST_EMIT, ST_SORT, ST_RAN
              ST_RASTER, ST_COUNT };
 
 struct BatchWork;
+struct AsyncBatch;
 
 }  // namespace sr_rt
 
@@ -43,9 +45,13 @@ struct sr_context_t {
     sr_draw_stats last = {};
     // execution tile chosen when sr_draw_desc.exec_tile == 0: the first
     // entry whose pixel bound holds the framebuffer (set by the autotuner)
+    // (max pixels, exec tile | waves << 8); waves 0 = the tile size's default
     std::vector<std::pair<uint64_t, int>> tile_policy{{~0ull, 8}};
     uint32_t batch_cap_hint = 0;             // initial bin-pair capacity of new batch workspaces
     sr_rt::BatchWork* batch = nullptr;       // workspace of sr_batch_draw (on ctx->stream)
+    std::vector<sr_rt::AsyncBatch*> async;   // slots of sr_batch_submit, each with its own stream
+    std::map<uint64_t, int> async_done;      // finished tickets not yet waited for -> status
+    uint64_t next_ticket = 1;
     // While sr_draw_static runs: the draw's buffers are carved in order
     // from this fixed device region instead of being allocated.
     uint8_t* arena = nullptr;
@@ -68,7 +74,8 @@ inline uint32_t mesh_count(const sr_mesh_t* m) { return m->indexed ? m->ni : m->
 struct sr_texture_t {
     sr_context ctx;
     uint32_t w = 0, h = 0;
-    uint32_t* d = nullptr;
+    uint32_t* d = nullptr;            // texels (RGBAF32: 4 floats each)
+    int fmt = SR_COLOR_RGBA8;         // SR_COLOR_*
     bool view = false;              // aliases a framebuffer's colour plane (not owned)
     sr_framebuffer fb = nullptr;    // that framebuffer
 };
@@ -76,9 +83,18 @@ struct sr_texture_t {
 struct sr_framebuffer_t {
     sr_context ctx;
     uint32_t w = 0, h = 0;
-    uint32_t* color = nullptr;
-    float* depth = nullptr;
-    uint8_t* stencil = nullptr;
+    uint32_t* color = nullptr;    // plane 0 when it is RGBA8
+    float* depth = nullptr;       // f32 depth
+    uint8_t* stencil = nullptr;   // u8 stencil
+    sr_fb_format fmt{1, {SR_COLOR_RGBA8, 0, 0, 0}, SR_DEPTH_F32, SR_STENCIL_U8};
+    void* planes[SR_MAX_COLOR_PLANES] = {};
+    double* depth64 = nullptr;    // f64 depth
+    // sr_fb_create's format: one RGBA8 plane, f32 depth, u8 stencil (the
+    // format every draw path takes; the others take triangle draws only)
+    bool standard() const {
+        return fmt.n_color == 1 && fmt.color[0] == SR_COLOR_RGBA8 && fmt.depth == SR_DEPTH_F32 &&
+               fmt.stencil == SR_STENCIL_U8;
+    }
 };
 
 namespace sr_rt {
@@ -104,7 +120,11 @@ T* P(DevBuf& b) { return reinterpret_cast<T*>(b.p); }
 int bits_for(uint32_t n);  // bits to hold values 0..n-1
 int varyings_of(int vp);
 int policy_tile(sr_context ctx, uint32_t w, uint32_t h);
+// Waves per tile for execution tile `ets`: the desc's, else (auto tile) the
+// policy's, made valid for the tile size.
+int draw_waves(sr_context ctx, const sr_draw_desc* d, uint32_t ets, uint32_t w, uint32_t h);
 int validate_draw(sr_context ctx, sr_framebuffer fb, sr_mesh mesh, const sr_draw_desc* d, sr_texture tex);
+int draw_foreign(sr_context ctx, sr_framebuffer fb, sr_mesh mesh, const sr_draw_desc* d, sr_texture tex);
 
 // True when a draw can join a batch: triangles, no geometry program,
 // stencil Always/Keep, not the exhaustive (unbinned) debug mode.
@@ -125,7 +145,7 @@ struct BatchWork {
     hipGraphExec_t exec = nullptr;
     uint8_t* h_stage = nullptr;  // pinned staging for the scene tables
     size_t h_stage_bytes = 0;
-    uint32_t* h_misc = nullptr;  // pinned: [0] overflow [1] bin pairs [2] NaN flag
+    uint32_t* h_misc = nullptr;  // pinned: [0] overflow [1] bin pairs [2] flags (1 NaN, 2 strip slivers)
     hipStream_t stream = nullptr;
     bool own_stream = false;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -138,8 +158,15 @@ struct BatchWork {
 
 struct BatchClass {
     uint64_t cls;
-    uint32_t ets, fp, blend, tile0, ntiles;
+    uint32_t ets, nwv, fp, blend, tile0, ntiles;
 };
+
+// After batch_result: a triangle of the batch may pass the coverage test
+// outside its box (setup's strip-risk flag); the batched raster then drew
+// nothing and the batch's draws must go through sr_draw one by one.
+// A triangle of the batch could pass the coverage test outside its box (it
+// was binned to every tile such a pixel may lie in).
+inline bool batch_strip_slivers(const BatchWork& W) { return (W.h_misc[2] & 2u) != 0; }
 
 // Host tables of one batch.  `items` are copies (descriptors included), so a
 // plan outlives the caller's arrays.  Draws into the same framebuffer form a
@@ -153,7 +180,18 @@ struct BatchPlan {
     std::vector<uint32_t> group_tile_end, group_first; // per group (+1 entry for group_first)
     std::vector<BatchClass> classes;
     uint64_t NV = 0, NT = 0, NTILES = 0;
+    bool soup = true;  // every mesh non-indexed (vertex program fused into setup)
 };
+
+// One asynchronous batch (sr_batch_submit): its workspace and stream, its
+// plan (copies of the items and descriptors), its ticket.
+struct AsyncBatch {
+    BatchWork W;
+    BatchPlan plan;
+    uint64_t ticket = 0;
+    bool busy = false;
+};
+
 
 // Builds the tables for `items` (all batchable and validated; items on one
 // framebuffer share execution tile, program and blend, and only the first

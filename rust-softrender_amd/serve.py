@@ -141,6 +141,51 @@ class _HostBufs:
         self.bufs = {}
 
 
+def copy_ceiling(frame_bytes, streams=(1, 2, 4), seconds=0.5):
+    """Frames/s of device-to-host copies alone into pinned memory, the way
+    the server returns frames (hipMemcpyAsync per frame, round-robin over
+    `n` streams), for each n in `streams`.  `frame_bytes` lists the frame
+    sizes in the order they are copied (cycled)."""
+    hip = _HostBufs().hip
+    vp = C.c_void_p
+    hip.hipMalloc.argtypes = [C.POINTER(vp), C.c_size_t]
+    hip.hipFree.argtypes = [vp]
+    hip.hipStreamCreate.argtypes = [C.POINTER(vp)]
+    hip.hipStreamDestroy.argtypes = [vp]
+    hip.hipStreamSynchronize.argtypes = [vp]
+    hip.hipMemcpyAsync.argtypes = [vp, vp, C.c_size_t, C.c_int, vp]
+    big = max(frame_bytes)
+    dev, hosts = vp(), []
+    assert hip.hipMalloc(C.byref(dev), big) == 0
+    out = {}
+    for n in streams:
+        ss = []
+        for _ in range(n):
+            s = vp()
+            assert hip.hipStreamCreate(C.byref(s)) == 0
+            ss.append(s)
+            h = vp()
+            if len(hosts) < n:
+                assert hip.hipHostMalloc(C.byref(h), big, 0) == 0
+                hosts.append(h)
+        done, t0 = 0, time.perf_counter()
+        while time.perf_counter() - t0 < seconds:
+            for _ in range(32):
+                b = frame_bytes[done % len(frame_bytes)]
+                k = done % n
+                assert hip.hipMemcpyAsync(hosts[k], dev, b, 2, ss[k]) == 0  # hipMemcpyDeviceToHost
+                done += 1
+            for s in ss:
+                hip.hipStreamSynchronize(s)
+        out[str(n)] = done / (time.perf_counter() - t0)
+        for s in ss:
+            hip.hipStreamDestroy(s)
+    for h in hosts:
+        hip.hipHostFree(h)
+    hip.hipFree(dev)
+    return out
+
+
 def run_plain(ctx, world, reqs, arrivals, max_seconds=60.0):
     """One request at a time in arrival order; stops after max_seconds of
     wall time (the queue grows without bound above the path's capacity)."""
@@ -194,10 +239,15 @@ def run(rates=(200, 500, 1000, 2000), duration=10.0, n_meshes=64, capacity_n=512
     srv = sr.Server(ctx, **cfg)
     run_served(srv, world, reqs[:64], zero[:64])  # warm: pools and workspaces
     res["capacity_served"] = run_served(srv, world, reqs, zero)
+    print(json.dumps({"capacity_served_fps": res["capacity_served"]["fps"]}), file=sys.stderr, flush=True)
     if plain:
         run_plain(ctx, world, reqs[:16], zero[:16])
         res["capacity_plain"] = run_plain(ctx, world, reqs, zero)
         res["capacity_ratio"] = res["capacity_served"]["fps"] / res["capacity_plain"]["fps"]
+    # what copying the same frames back alone sustains: the served capacity's ceiling
+    fb_bytes = [4 * r["width"] * r["height"] for r in reqs]
+    res["copy_ceiling_fps"] = copy_ceiling(fb_bytes)
+    res["capacity_of_copy_ceiling"] = res["capacity_served"]["fps"] / max(res["copy_ceiling_fps"].values())
     for k, rate in enumerate(rates):
         arr = scenes.poisson_arrivals(rate, duration, seed + 0x30000 + k)
         reqs = scenes.serve_requests(len(arr), n_meshes=n_meshes, seed=seed + 0x40000 + k)

@@ -23,15 +23,24 @@ ERRORS = {
     -10: "SR_ERR_WORKSPACE",
 }
 SR_ERR_WORKSPACE = -10
+
+# framebuffer formats (sr_fb_format)
+COLOR_RGBA8, COLOR_RGBAF32 = 0, 1
+DEPTH_F32, DEPTH_F64 = 0, 1
+STENCIL_U8, STENCIL_NONE = 0, 1
+BLEND_REPLACE, BLEND_OVER, BLEND_OVER_NORMALIZED = 0, 1, 2
+FP_FLAT, FP_BLINN_PHONG, FP_TEX_LAMBERT, FP_UV_COLOR, FP_GBUFFER = 0, 1, 2, 3, 4
 STATIC_WORKSPACE_BYTES = 64 << 20  # SR_STATIC_WORKSPACE_MIB
 
 EXPORTED = [
     "sr_ctx_create", "sr_ctx_destroy", "sr_last_error", "sr_finish", "sr_ctx_stream",
     "sr_ctx_stage_times", "sr_ctx_last_stats", "sr_mesh_upload", "sr_mesh_free",
     "sr_texture_upload", "sr_texture_free", "sr_texture_from_fb", "sr_obj_parse", "sr_mesh_load_obj",
-    "sr_fb_create", "sr_fb_free", "sr_fb_clear",
+    "sr_fb_create", "sr_fb_free", "sr_fb_clear", "sr_fb_create_ex", "sr_fb_get_format", "sr_fb_clear_ex",
+    "sr_fb_read_ex", "sr_fb_write_ex", "sr_texture_from_fb_plane",
     "sr_fb_read", "sr_fb_write", "sr_fb_size", "sr_draw", "sr_draw_static", "sr_draw_chains", "sr_batch_draw",
-    "sr_ctx_set_exec_tile_policy", "sr_ctx_set_batch_capacity",
+    "sr_ctx_set_exec_tile_policy", "sr_ctx_set_exec_policy", "sr_ctx_set_batch_capacity",
+    "sr_batch_submit", "sr_batch_query", "sr_batch_wait",
     "sr_server_create", "sr_server_destroy", "sr_server_submit", "sr_server_poll",
     "sr_server_release", "sr_server_get_stats", "sr_server_last_error", "sr_server_clock",
 ]
@@ -56,19 +65,25 @@ class DrawDesc(C.Structure):
         ("tex_edge", C.c_int32), ("tex_border", C.c_uint32), ("viewport", Viewport),
         ("vs_uniforms", C.c_float * 48), ("fs_uniforms", C.c_float * 32),
         ("gs_uniforms", C.c_float * 20),
-        ("exec_tile", C.c_int32), ("exec_flags", C.c_uint32), ("reserved", C.c_int32 * 6),
+        ("exec_tile", C.c_int32), ("exec_flags", C.c_uint32), ("exec_waves", C.c_int32),
+        ("reserved", C.c_int32 * 5),
     ]
 
 
 class DrawStats(C.Structure):
     _fields_ = [(n, C.c_uint64) for n in (
         "triangles_in", "triangles_setup", "lines_in", "points_in", "bin_pairs",
-        "generated_tris", "generated_lines", "generated_points")]
+        "generated_tris", "generated_lines", "generated_points", "strip_slivers")]
 
 
 class BatchItem(C.Structure):
     _fields_ = [("fb", C.c_void_p), ("mesh", C.c_void_p), ("desc", C.POINTER(DrawDesc)),
                 ("tex", C.c_void_p), ("clear", C.c_int32), ("clear_rgba", C.c_uint32)]
+
+
+class FbFormat(C.Structure):
+    _fields_ = [("n_color", C.c_uint32), ("color", C.c_int32 * 4), ("depth", C.c_int32),
+                ("stencil", C.c_int32)]
 
 
 class ObjInfo(C.Structure):
@@ -201,6 +216,12 @@ def lib():
         L.sr_fb_write.argtypes = [vp, C.POINTER(C.c_uint32), C.POINTER(C.c_float),
                                   C.POINTER(C.c_uint8)]
         L.sr_fb_size.argtypes = [vp, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]
+        L.sr_fb_create_ex.argtypes = [vp, C.c_uint32, C.c_uint32, C.POINTER(FbFormat), C.POINTER(vp)]
+        L.sr_fb_get_format.argtypes = [vp, C.POINTER(FbFormat)]
+        L.sr_fb_clear_ex.argtypes = [vp, vp]
+        L.sr_fb_read_ex.argtypes = [vp, C.POINTER(vp), vp, C.POINTER(C.c_uint8)]
+        L.sr_fb_write_ex.argtypes = [vp, C.POINTER(vp), vp, C.POINTER(C.c_uint8)]
+        L.sr_texture_from_fb_plane.argtypes = [vp, C.c_uint32, C.POINTER(vp)]
         L.sr_draw.argtypes = [vp, vp, vp, C.POINTER(DrawDesc), vp]
         L.sr_draw_static.argtypes = [vp, vp, vp, C.POINTER(DrawDesc), vp]
         L.sr_draw_chains.argtypes = [vp, vp, vp, C.POINTER(DrawDesc), C.c_uint32, vp]
@@ -208,6 +229,11 @@ def lib():
         L.sr_ctx_set_exec_tile_policy.argtypes = [vp, C.c_uint32, C.POINTER(C.c_uint64),
                                                   C.POINTER(C.c_int32)]
         L.sr_ctx_set_batch_capacity.argtypes = [vp, C.c_uint32]
+        L.sr_ctx_set_exec_policy.argtypes = [vp, C.c_uint32, C.POINTER(C.c_uint64), C.POINTER(C.c_int32),
+                                             C.POINTER(C.c_int32)]
+        L.sr_batch_submit.argtypes = [vp, C.POINTER(BatchItem), C.c_uint32, C.POINTER(C.c_uint64)]
+        L.sr_batch_query.argtypes = [vp, C.c_uint64]
+        L.sr_batch_wait.argtypes = [vp, C.c_uint64]
         L.sr_server_create.argtypes = [vp, C.POINTER(ServerConfig), C.POINTER(vp)]
         L.sr_server_destroy.argtypes = [vp]
         L.sr_server_submit.argtypes = [vp, C.POINTER(Request), C.POINTER(C.c_uint64)]
@@ -288,8 +314,16 @@ class Context:
     def framebuffer(self, w, h) -> "Framebuffer":
         return Framebuffer(self, w, h)
 
+    def framebuffer_ex(self, w, h, planes, depth=DEPTH_F32, stencil=STENCIL_U8) -> "Framebuffer":
+        """A framebuffer / texture buffer of other formats (sr_fb_create_ex):
+        `planes` lists each colour plane's COLOR_* format."""
+        return Framebuffer(self, w, h, planes=planes, depth=depth, stencil=stencil)
+
     def texture_from_fb(self, fb: "Framebuffer") -> "Texture":
         return Texture(self, fb.w, fb.h_, fb=fb)
+
+    def texture_from_fb_plane(self, fb: "Framebuffer", plane: int) -> "Texture":
+        return Texture(self, fb.w, fb.h_, fb=fb, plane=plane)
 
     def draw(self, fb: "Framebuffer", mesh: "Mesh", desc: DrawDesc, tex: "Texture" = None):
         self._check(lib().sr_draw(self.h, fb.h, mesh.h, C.byref(desc), tex.h if tex else None))
@@ -304,8 +338,8 @@ class Context:
         arr = (DrawDesc * len(descs))(*descs)
         self._check(lib().sr_draw_chains(self.h, fb.h, mesh.h, arr, len(descs), tex.h if tex else None))
 
-    def batch_draw(self, items):
-        """items: list of (fb, mesh, desc, tex[, clear_rgba or None])."""
+    @staticmethod
+    def _batch_items(items):
         arr = (BatchItem * len(items))()
         keep = []
         for i, it in enumerate(items):
@@ -314,13 +348,42 @@ class Context:
             keep.append(desc)
             arr[i] = BatchItem(fb.h, mesh.h, C.pointer(desc), tex.h if tex else None,
                                0 if clear is None else 1, 0 if clear is None else clear)
+        return arr, keep
+
+    def batch_draw(self, items):
+        """items: list of (fb, mesh, desc, tex[, clear_rgba or None])."""
+        arr, keep = self._batch_items(items)
         self._check(lib().sr_batch_draw(self.h, arr, len(items)))
+
+    def batch_submit(self, items) -> int:
+        """sr_batch_submit: queues the batch and returns its ticket."""
+        arr, keep = self._batch_items(items)
+        t = C.c_uint64()
+        self._check(lib().sr_batch_submit(self.h, arr, len(items), C.byref(t)))
+        return t.value
+
+    def batch_ready(self, ticket: int) -> bool:
+        rc = lib().sr_batch_query(self.h, ticket)
+        if rc == -7:  # SR_ERR_NOT_READY
+            return False
+        self._check(rc)
+        return True
+
+    def batch_wait(self, ticket: int):
+        self._check(lib().sr_batch_wait(self.h, ticket))
 
     def set_exec_tile_policy(self, entries):
         """entries: [(max_pixels, exec_tile), ...] (first match wins)."""
         mp = (C.c_uint64 * len(entries))(*[int(e[0]) for e in entries])
         et = (C.c_int32 * len(entries))(*[int(e[1]) for e in entries])
         self._check(lib().sr_ctx_set_exec_tile_policy(self.h, len(entries), mp, et))
+
+    def set_exec_policy(self, entries):
+        """entries: [(max_pixels, exec_tile, exec_waves), ...] (first match wins)."""
+        mp = (C.c_uint64 * len(entries))(*[int(e[0]) for e in entries])
+        et = (C.c_int32 * len(entries))(*[int(e[1]) for e in entries])
+        ew = (C.c_int32 * len(entries))(*[int(e[2]) for e in entries])
+        self._check(lib().sr_ctx_set_exec_policy(self.h, len(entries), mp, et, ew))
 
     def set_batch_capacity(self, pairs: int):
         self._check(lib().sr_ctx_set_batch_capacity(self.h, pairs))
@@ -425,12 +488,15 @@ class Mesh:
 
 
 class Texture:
-    def __init__(self, ctx: Context, w: int, h: int, rgba=None, fb: "Framebuffer" = None):
+    def __init__(self, ctx: Context, w: int, h: int, rgba=None, fb: "Framebuffer" = None, plane=None):
         """An uploaded RGBA8 image, or (fb=...) a view of a framebuffer's
-        colour plane for render-to-texture."""
+        colour plane (plane 0, or `plane`) for render-to-texture."""
         self.h = C.c_void_p()
         if fb is not None:
-            ctx._check(lib().sr_texture_from_fb(fb.h, C.byref(self.h)))
+            if plane is None:
+                ctx._check(lib().sr_texture_from_fb(fb.h, C.byref(self.h)))
+            else:
+                ctx._check(lib().sr_texture_from_fb_plane(fb.h, plane, C.byref(self.h)))
             return
         a = np.ascontiguousarray(rgba, np.uint32)
         ctx._check(lib().sr_texture_upload(ctx.h, w, h, a.ctypes.data_as(C.POINTER(C.c_uint32)),
@@ -443,10 +509,56 @@ class Texture:
 
 
 class Framebuffer:
-    def __init__(self, ctx: Context, w: int, h: int):
+    def __init__(self, ctx: Context, w: int, h: int, planes=None, depth=DEPTH_F32, stencil=STENCIL_U8):
         self.ctx, self.w, self.h_ = ctx, w, h
         self.h = C.c_void_p()
-        ctx._check(lib().sr_fb_create(ctx.h, w, h, C.byref(self.h)))
+        if planes is None:
+            self.planes, self.depth_fmt, self.stencil_fmt = [COLOR_RGBA8], DEPTH_F32, STENCIL_U8
+            ctx._check(lib().sr_fb_create(ctx.h, w, h, C.byref(self.h)))
+            return
+        self.planes, self.depth_fmt, self.stencil_fmt = list(planes), depth, stencil
+        col = (C.c_int32 * 4)(*(list(planes) + [0] * (4 - len(planes))))
+        fmt = FbFormat(len(planes), col, depth, stencil)
+        ctx._check(lib().sr_fb_create_ex(ctx.h, w, h, C.byref(fmt), C.byref(self.h)))
+
+    def format(self) -> dict:
+        f = FbFormat()
+        self.ctx._check(lib().sr_fb_get_format(self.h, C.byref(f)))
+        return dict(planes=list(f.color)[:f.n_color], depth=f.depth, stencil=f.stencil)
+
+    def clear_ex(self, colors):
+        """Framebuffer::clear: colors[i] is plane i's colour (an RGBA8 uint32,
+        or 4 floats for an RGBAF32 plane); depth far, stencil 0."""
+        buf = b""
+        for fmt, c in zip(self.planes, colors):
+            buf += (np.broadcast_to(np.asarray(c, np.float32), (4,)).tobytes() if fmt == COLOR_RGBAF32
+                    else np.uint32(c).tobytes())
+        b = C.create_string_buffer(buf, len(buf))
+        self.ctx._check(lib().sr_fb_clear_ex(self.h, C.cast(b, C.c_void_p)))
+
+    def read_ex(self):
+        """(planes, depth, stencil): RGBA8 planes as (n,) uint32, RGBAF32
+        planes as (n, 4) float32, depth float32 or float64, stencil uint8 or
+        None."""
+        n = self.w * self.h_
+        planes = [np.empty((n, 4), np.float32) if f == COLOR_RGBAF32 else np.empty(n, np.uint32)
+                  for f in self.planes]
+        ptrs = (C.c_void_p * len(planes))(*[p.ctypes.data for p in planes])
+        dep = np.empty(n, np.float64 if self.depth_fmt == DEPTH_F64 else np.float32)
+        st = np.empty(n, np.uint8) if self.stencil_fmt == STENCIL_U8 else None
+        self.ctx._check(lib().sr_fb_read_ex(self.h, ptrs, C.c_void_p(dep.ctypes.data),
+                                            st.ctypes.data_as(C.POINTER(C.c_uint8)) if st is not None else None))
+        return planes, dep, st
+
+    def write_ex(self, planes, depth=None, stencil=None):
+        arrs = [None if p is None else np.ascontiguousarray(
+            p, np.float32 if f == COLOR_RGBAF32 else np.uint32) for p, f in zip(planes, self.planes)]
+        ptrs = (C.c_void_p * len(arrs))(*[a.ctypes.data if a is not None else None for a in arrs])
+        dt = np.float64 if self.depth_fmt == DEPTH_F64 else np.float32
+        d = None if depth is None else np.ascontiguousarray(depth, dt)
+        s = None if stencil is None else np.ascontiguousarray(stencil, np.uint8)
+        self.ctx._check(lib().sr_fb_write_ex(self.h, ptrs, C.c_void_p(d.ctypes.data) if d is not None else None,
+                                             s.ctypes.data_as(C.POINTER(C.c_uint8)) if s is not None else None))
 
     def clear(self, rgba: int, depth: float = float(np.finfo(np.float32).min), stencil: int = 0):
         self.ctx._check(lib().sr_fb_clear(self.h, rgba, depth, stencil))

@@ -40,7 +40,7 @@ def test_python_binding_covers_the_header():
 
 
 def test_desc_layout_matches_header():
-    # sr_draw_desc: 16 int32 + viewport(6 f32) + 48 + 32 f32 + exec_tile, exec_flags, 6 reserved
-    assert ctypes.sizeof(sr.DrawDesc) == 4 * (16 + 6 + 48 + 32 + 20 + 2 + 6)
+    # sr_draw_desc: 16 int32 + viewport(6 f32) + 48 + 32 + 20 f32 + exec_tile, exec_flags, exec_waves, 5 reserved
+    assert ctypes.sizeof(sr.DrawDesc) == 4 * (16 + 6 + 48 + 32 + 20 + 3 + 5)
     assert ctypes.sizeof(sr.VertexSoA) == 8 * 8
-    assert ctypes.sizeof(sr.DrawStats) == 8 * 8
+    assert ctypes.sizeof(sr.DrawStats) == 9 * 8

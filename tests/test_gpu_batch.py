@@ -169,3 +169,34 @@ def test_batch_many_draws_into_one_framebuffer(ctx):
     for f in (fb, fb2):
         f.free()
     tex.free()
+
+
+def test_batch_nan_coordinate_draws_nothing(ctx):
+    """A NaN screen coordinate in one item: sr_batch_draw reports it like
+    sr_draw does, and no framebuffer of the one-pass group changes (the
+    fused clears included), on a fresh batch and on a replayed graph."""
+    specs = mix(seed=90, n=5)
+    bad = dict(specs[2])
+    px = bad["px"].copy()
+    # every x of the second triangle: f32::min/max skip a NaN operand, so one
+    # NaN vertex leaves the box finite (and never shades); three make the
+    # box NaN, which the reference's cast panics on (triangle.rs:70)
+    px[3:6] = np.float32("nan")
+    bad["px"] = px
+    specs[2] = bad
+    tw, th, rgba = specs[0]["texture"]
+    tex = ctx.texture(tw, th, rgba)
+    items, keep = make_items(ctx, specs, tex)
+    before = [fb.read() for fb, *_ in items]
+    for _ in range(2):
+        with pytest.raises(sr.SoftrenderError) as e:
+            ctx.batch_draw(items)
+        assert e.value.code == -4  # SR_ERR_INVALID_COORDINATE
+        for (fb, *_), b in zip(items, before):
+            for x, y in zip(fb.read(), b):
+                assert np.array_equal(x, y)
+    # the same batch without the NaN draws normally afterwards
+    good = mix(seed=90, n=5)
+    items2, keep2 = make_items(ctx, good, tex)
+    ctx.batch_draw(items2)
+    check_all(items2, good)

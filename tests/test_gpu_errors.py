@@ -111,7 +111,8 @@ def test_zero_sized_framebuffer_is_refused(ctx):
         assert e.value.code == INVALID_ARGUMENT
 
 
-@pytest.mark.parametrize("field,value", [("exec_tile", 12), ("vertex_program", 9), ("geometry_program", 7),
+@pytest.mark.parametrize("field,value", [("exec_tile", 12), ("exec_tile", 64), ("exec_waves", 3),
+                                         ("vertex_program", 9), ("geometry_program", 7),
                                          ("fragment_program", 11), ("stencil_test", 8), ("stencil_op", 6)])
 def test_bad_descriptor_fields_are_refused(ctx, field, value):
     s = tri_scene()

@@ -149,3 +149,20 @@ def test_clip_frame_full_size(ctx):
     ref, st = render_ref(s)
     assert st[0]["gen_tris"] > 500_000 and st[1]["line_plots"] > 10_000_000
     assert_parity(render_gpu(ctx, s), ref)
+
+
+def test_points_scan_past_1024_blocks():
+    """4.5M points: the per-primitive count scan spans more than 1024 blocks
+    of 4096 elements, so the scan takes its three-level path (block sums
+    scanned by a top-level pass) rather than the fused one."""
+    import softrender_hip as sr
+
+    s = point_scene(4_500_000, 256, 192, seed=35, blend=1, tile_w=100, tile_h=64)
+    ctx = sr.Context(0)
+    try:
+        got = render_gpu(ctx, s)
+        assert ctx.last_stats()["points_in"] == 4_500_000
+    finally:
+        ctx.close()
+    ref, st = render_ref(s)
+    assert_parity(got, ref)

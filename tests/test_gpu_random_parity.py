@@ -35,7 +35,7 @@ def random_case(seed):
     fs = np.zeros(32, np.float32)
     fs[0:4] = rng.random(4)
     desc = dict(s["desc"], geometry_program=gs, cull=int(rng.integers(3)), blend=int(rng.integers(2)),
-                alpha_mode=int(rng.integers(2)), exec_tile=int(rng.choice([0, 8, 16, 32, 64])),
+                alpha_mode=int(rng.integers(2)), exec_tile=int(rng.choice([0, 8, 16, 32, 32])),
                 tile_w=int(rng.choice([16, 40, 64, 128])), tile_h=int(rng.choice([16, 33, 64, 128])),
                 stencil_test=test, stencil_op=op, stencil_wrap=int(rng.integers(2)),
                 stencil_ref=int(rng.integers(0, 4)))
