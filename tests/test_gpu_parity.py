@@ -21,7 +21,7 @@ def test_sphere_512(ctx):
     assert st[0]["shaded"] > 50_000, st
 
 
-@pytest.mark.parametrize("ets", [8, 16, 32, 64])
+@pytest.mark.parametrize("ets", [8, 16, 32])
 def test_soup_exec_tile_sizes(ctx, ets):
     s = scenes.soup_scene(6000, 320, 180, seed=11)
     ref, _ = render_ref(s)
