@@ -89,41 +89,55 @@ struct Lut {
     __device__ __forceinline__ float operator()(uint32_t c) const { return t[c & 0xFFu]; }
 };
 
-// One texel as RGBA floats: RGBA8 texels as unorm (c / 255), RGBAF32
-// texels (a texture buffer plane, texturebuffer.rs:12-49) as stored; the
-// border colour is RGBA8 either way.
-__device__ __forceinline__ void texel(const RasterParams& P, int ix, int iy, Lut lut, float o[4]) {
+// Texel (ix, iy) after the edge rule: its index, or -1 for the border
+// (textures hold fewer than 2^31 texels).
+__device__ __forceinline__ int texel_index(const RasterParams& P, int ix, int iy) {
     const int W = (int)P.texW, H = (int)P.texH;
-    bool border = false;
     if (P.tex_edge == SR_EDGE_WRAP) {
         ix = ((ix % W) + W) % W;
         iy = ((iy % H) + H) % H;
     } else if (P.tex_edge == SR_EDGE_BORDER) {
-        border = ix < 0 || iy < 0 || ix >= W || iy >= H;
+        if (ix < 0 || iy < 0 || ix >= W || iy >= H) return -1;
     } else {
         ix = ix < 0 ? 0 : (ix > W - 1 ? W - 1 : ix);
         iy = iy < 0 ? 0 : (iy > H - 1 ? H - 1 : iy);
     }
-    if (P.tex_fmt == SR_COLOR_RGBAF32 && !border) {
-        const float4 f = ((const float4*)P.tex)[(size_t)iy * W + ix];
-        o[0] = f.x; o[1] = f.y; o[2] = f.z; o[3] = f.w;
-        return;
-    }
-    const uint32_t c = border ? P.tex_border : P.tex[(size_t)iy * W + ix];
-    o[0] = lut(c);
-    o[1] = lut(c >> 8);
-    o[2] = lut(c >> 16);
-    o[3] = lut(c >> 24);
+    return iy * W + ix;
+}
+
+// Channel c of a texel as a float: RGBA8 texels as unorm (c / 255) from
+// the packed word, RGBAF32 texels (a texture buffer plane,
+// texturebuffer.rs:12-49) as stored; the border colour is RGBA8 either way.
+__device__ __forceinline__ float texel_f32(const RasterParams& P, int i, int c, Lut lut) {
+    return i >= 0 ? ((const float*)P.tex)[4 * (uint32_t)i + c] : lut(P.tex_border >> (8 * c));
+}
+__device__ __forceinline__ uint32_t texel_u8(const RasterParams& P, int i) {
+    return i >= 0 ? P.tex[(uint32_t)i] : P.tex_border;
 }
 
 // Texture sampling (texture.rs:14-62 declares Filter/Edge; the sampling
-// itself is build-defined): texel centres at (i + 0.5) / W.
+// itself is build-defined): texel centres at (i + 0.5) / W.  Bilinear
+// weights each channel's four texels in turn (an RGBA8 texel's packed word
+// is fetched once), so only a channel's four values are live at a time.
+// ANY_FMT = false: RGBA8 textures only (the triangle raster, whose draws
+// with an RGBAF32 texture take the per-pixel raster instead: the float
+// path costs it a wave per SIMD).
+template <bool ANY_FMT = true>
 __device__ __forceinline__ void sample(const RasterParams& P, float u, float v, Lut lut,
                                        float out[4]) {
+    const bool f32 = ANY_FMT && P.tex_fmt == SR_COLOR_RGBAF32;
     if (P.tex_filter == SR_FILTER_NEAREST) {
         float fx = clampf(u * (float)P.texW, -1.0e8f, 1.0e8f);
         float fy = clampf(v * (float)P.texH, -1.0e8f, 1.0e8f);
-        texel(P, (int)floorf(fx), (int)floorf(fy), lut, out);
+        const int i = texel_index(P, (int)floorf(fx), (int)floorf(fy));
+        if (f32) {
+#pragma unroll
+            for (int c = 0; c < 4; c++) out[c] = texel_f32(P, i, c, lut);
+        } else {
+            const uint32_t w = texel_u8(P, i);
+#pragma unroll
+            for (int c = 0; c < 4; c++) out[c] = lut(w >> (8 * c));
+        }
         return;
     }
     float fx = clampf(u * (float)P.texW - 0.5f, -1.0e8f, 1.0e8f);
@@ -131,17 +145,25 @@ __device__ __forceinline__ void sample(const RasterParams& P, float u, float v, 
     float x0 = floorf(fx), y0 = floorf(fy);
     float ax = fx - x0, ay = fy - y0;
     int ix = (int)x0, iy = (int)y0;
-    float t00[4], t10[4], t01[4], t11[4];
-    texel(P, ix, iy, lut, t00);
-    texel(P, ix + 1, iy, lut, t10);
-    texel(P, ix, iy + 1, lut, t01);
-    texel(P, ix + 1, iy + 1, lut, t11);
+    const int i00 = texel_index(P, ix, iy), i10 = texel_index(P, ix + 1, iy);
+    const int i01 = texel_index(P, ix, iy + 1), i11 = texel_index(P, ix + 1, iy + 1);
     float bx = 1.0f - ax, by = 1.0f - ay;
+    auto mix = [&](float t00, float t10, float t01, float t11) {
+        float top = (t00 * bx) + (t10 * ax);
+        float bot = (t01 * bx) + (t11 * ax);
+        return (top * by) + (bot * ay);
+    };
+    if (f32) {
 #pragma unroll
-    for (int c = 0; c < 4; c++) {
-        float top = (t00[c] * bx) + (t10[c] * ax);
-        float bot = (t01[c] * bx) + (t11[c] * ax);
-        out[c] = (top * by) + (bot * ay);
+        for (int c = 0; c < 4; c++)
+            out[c] = mix(texel_f32(P, i00, c, lut), texel_f32(P, i10, c, lut), texel_f32(P, i01, c, lut),
+                         texel_f32(P, i11, c, lut));
+    } else {
+        const uint32_t w00 = texel_u8(P, i00), w10 = texel_u8(P, i10);
+        const uint32_t w01 = texel_u8(P, i01), w11 = texel_u8(P, i11);
+#pragma unroll
+        for (int c = 0; c < 4; c++)
+            out[c] = mix(lut(w00 >> (8 * c)), lut(w10 >> (8 * c)), lut(w01 >> (8 * c)), lut(w11 >> (8 * c)));
     }
 }
 
@@ -149,7 +171,7 @@ __device__ __forceinline__ void sample(const RasterParams& P, float u, float v, 
 // fragment.rs:168-169).  k = interpolated varyings; o[i] = output i as
 // RGBA floats before quantisation.  Returns the program's output count
 // (SR_FP_GBUFFER: 4, the others 1).
-template <int FP>
+template <int FP, bool ANY_FMT = true>
 __device__ __forceinline__ int fragment_rgba(const RasterParams& P, const float* k, Lut lut, float o[][4]) {
     const float* U = P.fsu;
     if constexpr (FP == SR_FP_BLINN_PHONG) {
@@ -176,7 +198,7 @@ __device__ __forceinline__ int fragment_rgba(const RasterParams& P, const float*
         float diff = ndl > 0.0f ? ndl : 0.0f;
         float lit = U[3] + U[4] * diff;
         float t[4];
-        sample(P, k[3], k[4], lut, t);
+        sample<ANY_FMT>(P, k[3], k[4], lut, t);
         o[0][0] = t[0] * lit; o[0][1] = t[1] * lit; o[0][2] = t[2] * lit; o[0][3] = t[3] * U[5];
         if constexpr (FP == SR_FP_TEX_LAMBERT) {
             return 1;
@@ -200,10 +222,10 @@ __device__ __forceinline__ uint32_t pack_q(const float c[4]) {
 }
 
 // The program's (first) colour on an RGBA8 plane.
-template <int FP>
+template <int FP, bool ANY_FMT = true>
 __device__ __forceinline__ uint32_t fragment(const RasterParams& P, const float* k, Lut lut) {
     float o[FP == SR_FP_GBUFFER ? 4 : 1][4];
-    fragment_rgba<FP>(P, k, lut, o);
+    fragment_rgba<FP, ANY_FMT>(P, k, lut, o);
     return pack_q(o[0]);
 }
 

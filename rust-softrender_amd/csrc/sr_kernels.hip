@@ -90,20 +90,6 @@ __device__ __forceinline__ void vertex_compute(const float* U, int program, int 
     }
 }
 
-__device__ __forceinline__ void vertex_one(const float* U, int program, int K, int normalize,
-                                           const sr_viewport& viewport, const float pin[4],
-                                           const float nin[3], float u, float v,
-                                           const VertexBuf& out, uint32_t o) {
-    float outp[4], var[6];
-    vertex_compute(U, program, normalize, viewport, pin, nin, u, v, outp, var);
-    out.comp(0)[o] = outp[0];
-    out.comp(1)[o] = outp[1];
-    out.comp(2)[o] = outp[2];
-    out.comp(3)[o] = outp[3];
-    float* const row = out.var(o);
-    for (int k = 0; k < K; k++) row[k] = var[k];
-}
-
 // GeometryShader::finish on clip-space vertices a draw chain shares with
 // others (geometry.rs:60-129): the same normalisation the vertex kernel
 // fuses for run_to_fragment, varyings copied.
@@ -119,15 +105,37 @@ __global__ __launch_bounds__(256) void k_finish(VertexBuf in, VertexBuf out, uin
     for (int c = 0; c < K; c++) dst[c] = src[c];
 }
 
+// The varying rows of a block's vertices [i0, i0 + n) (n <= 256, row
+// stride out.kstride <= 6), staged in LDS so that consecutive threads store
+// consecutive words.  Every thread of the block calls it.
+__device__ __forceinline__ void store_rows(float* lds, const VertexBuf& out, uint32_t i0, uint32_t n,
+                                           const float var[6], bool live) {
+    const uint32_t K = out.kstride;
+    if (live)
+        for (uint32_t c = 0; c < K; c++) lds[threadIdx.x * K + c] = var[c];
+    __syncthreads();
+    float* const dst = out.var(i0);
+    for (uint32_t j = threadIdx.x; j < n * K; j += 256u) dst[j] = lds[j];
+}
+
 // One vertex program invocation per thread.  Inputs are coalesced SoA
-// streams; outputs are SoA (x, y, z, w, varyings).
+// streams; outputs are the SoA position streams (x, y, z, w) and the
+// per-vertex varying rows.
 __global__ __launch_bounds__(256) void k_vertex(VSParams P) {
-    const uint32_t i = blockIdx.x * 256u + threadIdx.x;
-    if (i >= P.n) return;
-    const float pin[4] = {P.px[i], P.py[i], P.pz[i], 1.0f};
-    const float nin[3] = {P.nx ? P.nx[i] : 0.0f, P.ny ? P.ny[i] : 0.0f, P.nz ? P.nz[i] : 0.0f};
-    vertex_one(P.U, P.program, P.K, P.normalize, P.viewport, pin, nin, P.u ? P.u[i] : 0.0f,
-               P.v ? P.v[i] : 0.0f, P.out, i);
+    __shared__ float rows[256 * 6];
+    const uint32_t b0 = blockIdx.x * 256u, i = b0 + threadIdx.x;
+    const bool live = i < P.n;
+    float var[6] = {};
+    if (live) {
+        const float pin[4] = {P.px[i], P.py[i], P.pz[i], 1.0f};
+        const float nin[3] = {P.nx ? P.nx[i] : 0.0f, P.ny ? P.ny[i] : 0.0f, P.nz ? P.nz[i] : 0.0f};
+        float outp[4];
+        vertex_compute(P.U, P.program, P.normalize, P.viewport, pin, nin, P.u ? P.u[i] : 0.0f,
+                       P.v ? P.v[i] : 0.0f, outp, var);
+#pragma unroll
+        for (int c = 0; c < 4; c++) P.out.comp(c)[i] = outp[c];
+    }
+    store_rows(rows, P.out, b0, min(256u, P.n - b0), var, live);
 }
 
 // scene owning element i of a batch: first s with i < end[s]
@@ -196,14 +204,22 @@ __device__ __forceinline__ void with_scene(const BatchTables& T, const uint32_t*
 }
 
 __global__ __launch_bounds__(256) void k_vertex_batch(BatchTables T, VertexBuf out) {
-    const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+    __shared__ float rows[256 * 6];
+    const uint32_t b0 = blockIdx.x * 256u, i = b0 + threadIdx.x;
+    float var[6] = {};
+    bool live = false;
     with_scene(T, T.vend, T.nv, i, [&](const SceneGeom& S) {
         const uint32_t l = i - S.vbase;
         const float pin[4] = {S.px[l], S.py[l], S.pz[l], 1.0f};
         const float nin[3] = {S.nx ? S.nx[l] : 0.0f, S.ny ? S.ny[l] : 0.0f, S.nz ? S.nz[l] : 0.0f};
-        vertex_one(S.U, S.program, S.K, 1, S.viewport, pin, nin, S.u ? S.u[l] : 0.0f,
-                   S.v ? S.v[l] : 0.0f, out, i);
+        float outp[4];
+        vertex_compute(S.U, S.program, 1, S.viewport, pin, nin, S.u ? S.u[l] : 0.0f, S.v ? S.v[l] : 0.0f, outp,
+                       var);
+#pragma unroll
+        for (int c = 0; c < 4; c++) out.comp(c)[i] = outp[c];
+        live = true;
     });
+    if (b0 < T.nv) store_rows(rows, out, b0, min(256u, T.nv - b0), var, live);
 }
 
 // ===========================================================================
@@ -751,32 +767,6 @@ __device__ __forceinline__ uint32_t tri_setup_one(const VertexBuf& v, uint32_t i
                          err, empty_cull);
 }
 
-// The vertex program of a non-indexed triangle's three vertices (3t, 3t+1,
-// 3t+2: each belongs to this triangle alone) fused with its setup: the
-// screen positions stay in registers (the raster reads only the record),
-// the varyings go to the vertex buffer's rows.
-__device__ __forceinline__ uint32_t vertex_setup_soup(const float* U, int program, int K,
-                                                      const sr_viewport& vp, const float* px, const float* py,
-                                                      const float* pz, const float* nx, const float* ny,
-                                                      const float* nz, const float* pu, const float* pv,
-                                                      uint32_t l, uint32_t o, const VertexBuf& out, int cull,
-                                                      const BinGeom& g, TriRec& r, TriBin& b, uint32_t* err,
-                                                      int empty_cull) {
-    float s[3][4];
-#pragma unroll
-    for (int k = 0; k < 3; k++) {
-        const uint32_t i = 3 * l + k;
-        const float pin[4] = {px[i], py[i], pz[i], 1.0f};
-        const float nin[3] = {nx ? nx[i] : 0.0f, ny ? ny[i] : 0.0f, nz ? nz[i] : 0.0f};
-        float var[6];
-        vertex_compute(U, program, 1, vp, pin, nin, pu ? pu[i] : 0.0f, pv ? pv[i] : 0.0f, s[k], var);
-        float* const row = out.var(o + k);
-        for (int c = 0; c < K; c++) row[c] = var[c];
-    }
-    return tri_setup_xyz(s[0][0], s[0][1], s[0][2], s[1][0], s[1][1], s[1][2], s[2][0], s[2][1], s[2][2], cull,
-                         g, r, b, err, empty_cull);
-}
-
 // Writes the (tile, item) pairs of one triangle: every tile of its box, or
 // with culling the tiles setup found coverable (its tile mask, or the test
 // again, on the full record `rec`, for boxes of more than 32 tiles), in
@@ -873,78 +863,6 @@ __global__ __launch_bounds__(256) void k_tri_setup_batch(BatchTables T, VertexBu
         const BinGeom g{S.W, S.H, S.ets, S.tw, S.th};
         counts[t] = tri_setup_one(v, i0, i1, i2, S.cull, g, r, b, err, S.empty_cull);
         stage_setup(st, r, b, i0, i1, i2);
-    });
-    store_setup(st, recs, bins, vids, T.nt);
-}
-
-// k_vertex + k_tri_setup for a non-indexed mesh without a geometry program.
-// One thread per triangle runs the vertex program on its three vertices and
-// the setup; the block's 768 vertices come in through LDS (each stream read
-// by consecutive threads) and their varyings leave through it (consecutive
-// threads write consecutive words of the rows), then the LDS holds the
-// setup records on their way out.
-__global__ __launch_bounds__(256) void k_vertex_setup_soup(VSParams V, SetupParams P) {
-    __shared__ union {
-        SetupStage st;
-        float io[8 * 768];  // the 8 vertex streams, then the varying rows
-    } sh;
-    const uint32_t b0 = blockIdx.x * 256u, t = b0 + threadIdx.x;
-    const uint32_t nt = min(256u, P.n - b0), v0 = 3u * b0, nv = 3u * nt;
-    const float* const src[8] = {V.px, V.py, V.pz, V.nx, V.ny, V.nz, V.u, V.v};
-#pragma unroll
-    for (int c = 0; c < 8; c++) {
-        if (!src[c]) continue;
-        for (uint32_t i = threadIdx.x; i < nv; i += 256u) sh.io[c * 768 + i] = src[c][v0 + i];
-    }
-    __syncthreads();
-    const bool live = t < P.n;
-    float s[3][4], var[3][6];
-    if (live) {
-#pragma unroll
-        for (int k = 0; k < 3; k++) {
-            const uint32_t i = 3u * threadIdx.x + k;
-            const float pin[4] = {sh.io[i], sh.io[768 + i], sh.io[2 * 768 + i], 1.0f};
-            const float nin[3] = {V.nx ? sh.io[3 * 768 + i] : 0.0f, V.ny ? sh.io[4 * 768 + i] : 0.0f,
-                                  V.nz ? sh.io[5 * 768 + i] : 0.0f};
-            vertex_compute(V.U, V.program, 1, V.viewport, pin, nin, V.u ? sh.io[6 * 768 + i] : 0.0f,
-                           V.v ? sh.io[7 * 768 + i] : 0.0f, s[k], var[k]);
-        }
-    }
-    __syncthreads();
-    const uint32_t K = V.out.kstride;  // == V.K: rows of the vertex buffer
-    if (live)
-#pragma unroll
-        for (int k = 0; k < 3; k++)
-            for (uint32_t c = 0; c < K; c++) sh.io[(3u * threadIdx.x + k) * K + c] = var[k][c];
-    __syncthreads();
-    float* const dst = V.out.var(v0);
-    for (uint32_t i = threadIdx.x; i < nv * K; i += 256u) dst[i] = sh.io[i];
-    __syncthreads();
-    if (live) {
-        TriRec r;
-        TriBin b;
-        const BinGeom g{P.W, P.H, P.ets, P.tw, P.th};
-        P.counts[t] = tri_setup_xyz(s[0][0], s[0][1], s[0][2], s[1][0], s[1][1], s[1][2], s[2][0], s[2][1],
-                                    s[2][2], P.cull, g, r, b, P.err, P.empty_cull);
-        stage_setup(sh.st, r, b, 3 * t, 3 * t + 1, 3 * t + 2);
-    }
-    store_setup(sh.st, P.recs, P.bins, P.vids, P.n);
-}
-
-// The same over a batch whose meshes are all non-indexed.
-__global__ __launch_bounds__(256) void k_vertex_setup_soup_batch(BatchTables T, VertexBuf v, TriRec* recs,
-                                                                TriBin* bins, TriVid* vids, uint32_t* counts,
-                                                                uint32_t* err) {
-    __shared__ SetupStage st;
-    const uint32_t t = blockIdx.x * 256u + threadIdx.x;
-    with_scene(T, T.tend, T.nt, t, [&](const SceneGeom& S) {
-        const uint32_t l = t - S.tbase, o = S.vbase + 3 * l;
-        TriRec r;
-        TriBin b;
-        const BinGeom g{S.W, S.H, S.ets, S.tw, S.th};
-        counts[t] = vertex_setup_soup(S.U, S.program, S.K, S.viewport, S.px, S.py, S.pz, S.nx, S.ny, S.nz, S.u,
-                                      S.v, l, o, v, S.cull, g, r, b, err, S.empty_cull);
-        stage_setup(st, r, b, o, o + 1, o + 2);
     });
     store_setup(st, recs, bins, vids, T.nt);
 }
@@ -2282,14 +2200,6 @@ void launch_tri_setup(const SetupParams& P, hipStream_t s) {
 }
 void launch_vertex_batch(const BatchTables& T, const VertexBuf& out, hipStream_t s) {
     if (T.nv) hipLaunchKernelGGL(k_vertex_batch, dim3(blocks_for(T.nv, 256)), dim3(256), 0, s, T, out);
-}
-void launch_vertex_setup_soup(const VSParams& V, const SetupParams& P, hipStream_t s) {
-    if (P.n) hipLaunchKernelGGL(k_vertex_setup_soup, dim3(blocks_for(P.n, 256)), dim3(256), 0, s, V, P);
-}
-void launch_vertex_setup_soup_batch(const BatchTables& T, const VertexBuf& v, TriRec* recs, TriBin* bins,
-                                    TriVid* vids, uint32_t* counts, uint32_t* err, hipStream_t s) {
-    if (T.nt) hipLaunchKernelGGL(k_vertex_setup_soup_batch, dim3(blocks_for(T.nt, 256)), dim3(256), 0, s, T, v,
-                                 recs, bins, vids, counts, err);
 }
 void launch_tri_setup_batch(const BatchTables& T, const VertexBuf& v, TriRec* recs, TriBin* bins, TriVid* vids,
                             uint32_t* counts, uint32_t* err, hipStream_t s) {

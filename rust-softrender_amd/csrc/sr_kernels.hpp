@@ -158,11 +158,6 @@ void launch_gs_emit(int mode, int prim, const GSParams& P, hipStream_t s);
 // 3 per triangle (SR_GS_VERTEX_NORMALS) or 1 (SR_GS_FACE_NORMALS)
 void launch_gs_normals(int mode, const GSParams& P, hipStream_t s);
 void launch_tri_setup(const SetupParams& P, hipStream_t s);
-// The vertex program fused with setup, for non-indexed meshes without a
-// geometry program: varyings to V.out's rows, no screen positions written.
-void launch_vertex_setup_soup(const VSParams& V, const SetupParams& P, hipStream_t s);
-void launch_vertex_setup_soup_batch(const BatchTables& T, const VertexBuf& v, TriRec* recs, TriBin* bins,
-                                    TriVid* vids, uint32_t* counts, uint32_t* err, hipStream_t s);
 void launch_tri_emit(const EmitParams& P, hipStream_t s);
 void launch_vertex_batch(const BatchTables& T, const VertexBuf& out, hipStream_t s);
 void launch_tri_setup_batch(const BatchTables& T, const VertexBuf& v, TriRec* recs, TriBin* bins, TriVid* vids,

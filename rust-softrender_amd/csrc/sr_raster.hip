@@ -49,9 +49,14 @@ struct WaveRegion {
 // and every fragment program here returns a colour (never Discard), so the
 // depth write (triangle.rs:142) does not depend on shading; blending, the
 // only colour-order dependence, runs in queue (= reference) order.
-// Queue slots per pixel: LDS holds 64 * PPT * Q queued fragments per wave.
+// Queue slots per pixel: LDS holds 64 * PPT * Q queued fragments per wave
+// (12 bytes each).  With one pixel per lane the LDS of a wave's queue
+// bounds the waves a CU holds (160 KB).
+#ifndef SR_QUEUE_DEPTH1
+#define SR_QUEUE_DEPTH1 6
+#endif
 template <int PPT>
-constexpr int queue_depth() { return PPT == 1 ? 8 : (PPT == 2 ? 4 : (PPT == 4 ? 2 : 1)); }
+constexpr int queue_depth() { return PPT == 1 ? SR_QUEUE_DEPTH1 : (PPT == 2 ? 4 : (PPT == 4 ? 2 : 1)); }
 
 struct QRec {
     uint32_t tri;  // triangle id; after shading, the fragment colour
@@ -96,7 +101,7 @@ __device__ __forceinline__ uint32_t shade_rec(const RasterParams& P, const QRec&
 #pragma unroll
         for (int c = 0; c < KV; c++) k[c] = ((r0[c] * u) + (r1[c] * v)) + (r2[c] * w);
     }
-    return fragment<FP>(P, k, lut);
+    return fragment<FP, false>(P, k, lut);  // RGBA8 textures (RGBAF32 ones go to the per-pixel raster)
 }
 
 template <int ETS, int NWV, int FP, int BLEND>

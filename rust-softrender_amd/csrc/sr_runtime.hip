@@ -203,10 +203,8 @@ void fill_raster_params(const DrawPlan& pl, sr_framebuffer fb, const sr_draw_des
 // ---------------------------------------------------------------------------
 // The triangle draw
 // ---------------------------------------------------------------------------
-// `fuse`: the vertex stage of a non-indexed mesh, run inside setup (the
-// vertex kernel did not run; vb receives the varyings).
 int draw_tris(sr_context ctx, sr_framebuffer fb, const DrawPlan& pl, const sr_draw_desc* d,
-              sr_texture tex, VertexBuf vb, const uint32_t* idx, uint32_t ntris, const VSParams* fuse) {
+              sr_texture tex, VertexBuf vb, const uint32_t* idx, uint32_t ntris) {
     int rc;
     hipStream_t s = ctx->stream;
     ctx->last.triangles_setup = ntris;
@@ -226,8 +224,7 @@ int draw_tris(sr_context ctx, sr_framebuffer fb, const DrawPlan& pl, const sr_dr
     // with no stencil effects, tiles whose pixels the triangle cannot cover
     // are left out of the bins
     sp.empty_cull = pl.smode == 0 && !pl.exhaustive;
-    if (fuse) launch_vertex_setup_soup(*fuse, sp, s);
-    else launch_tri_setup(sp, s);
+    launch_tri_setup(sp, s);
     mark(ctx, pl.timing, ST_SETUP);
 
     RasterParams R;
@@ -488,8 +485,14 @@ size_t batch_bytes_estimate(uint64_t tris, uint64_t verts, double pairs_per_tri)
     return (size_t)(b * 1.25);
 }
 
-bool batchable(sr_mesh m, const sr_draw_desc* d) {
-    return m->prim == SR_PRIM_TRIANGLES && d->geometry_program == SR_GS_NONE &&
+// An RGBAF32 texture read by the fragment program (a texture buffer plane)
+bool samples_f32(const sr_draw_desc* d, sr_texture tex) {
+    return tex && tex->fmt == SR_COLOR_RGBAF32 &&
+           (d->fragment_program == SR_FP_TEX_LAMBERT || d->fragment_program == SR_FP_GBUFFER);
+}
+
+bool batchable(sr_mesh m, const sr_draw_desc* d, sr_texture tex) {
+    return m->prim == SR_PRIM_TRIANGLES && d->geometry_program == SR_GS_NONE && !samples_f32(d, tex) &&
            d->stencil_test == SR_STENCIL_ALWAYS && d->stencil_op == SR_STENCIL_KEEP &&
            !(d->exec_flags & SR_EXEC_EXHAUSTIVE) && d->fragment_program != SR_FP_GBUFFER &&
            (d->blend == SR_BLEND_REPLACE || d->blend == SR_BLEND_OVER);
@@ -551,7 +554,6 @@ int batch_plan(sr_context ctx, const sr_batch_item* items, uint32_t n, BatchPlan
             g.nz = m->soa[5]; g.u = m->soa[6]; g.v = m->soa[7]; g.idx = m->idx;
             g.nv = ntri ? m->nv : 0; g.vbase = (uint32_t)pl.NV;
             g.nt = ntri; g.tbase = (uint32_t)pl.NT;
-            if (m->indexed) pl.soup = false;
             g.program = d->vertex_program; g.K = varyings_of(d->vertex_program); g.cull = d->cull;
             g.W = W; g.H = H; g.ets = ets; g.ntx = ntx;
             g.tw = d->tile_w > 0 ? (uint32_t)d->tile_w : 128u;
@@ -721,14 +723,11 @@ int batch_enqueue(sr_context ctx, BatchWork& W, const BatchPlan& pl, bool use_gr
         }
         BatchTables T{d_geom, d_vend, d_tend, ns, (uint32_t)NV, (uint32_t)NT};
         launch_fill(misc, 0u, 8u, s);
-        if (!pl.soup) launch_vertex_batch(T, vb, s);
+        launch_vertex_batch(T, vb, s);
         // the triangle records, then their binning records, in one buffer
         TriBin* const bins = (TriBin*)(P<TriRec>(W.recs) + std::max<uint64_t>(NT, 1));
         TriVid* const vids = (TriVid*)(bins + std::max<uint64_t>(NT, 1));
-        if (pl.soup)  // every mesh non-indexed: the vertex program runs inside setup
-            launch_vertex_setup_soup_batch(T, vb, P<TriRec>(W.recs), bins, vids, P<uint32_t>(W.counts), misc + 2, s);
-        else
-            launch_tri_setup_batch(T, vb, P<TriRec>(W.recs), bins, vids, P<uint32_t>(W.counts), misc + 2, s);
+        launch_tri_setup_batch(T, vb, P<TriRec>(W.recs), bins, vids, P<uint32_t>(W.counts), misc + 2, s);
         launch_scan(P<uint32_t>(W.counts), P<uint32_t>(W.offsets), (uint32_t)NT, P<uint32_t>(W.scan), misc + 1, s);
         launch_tri_emit_batch(T, P<TriRec>(W.recs), bins, P<uint32_t>(W.counts), P<uint32_t>(W.offsets), cap,
                               P<uint32_t>(W.keys), P<uint32_t>(W.vals), misc + 0, s);
@@ -859,7 +858,7 @@ int split_batch(sr_context ctx, const sr_batch_item* items, uint32_t n, std::vec
             const int ets = d->exec_tile ? d->exec_tile : policy_tile(ctx, fb->w, fb->h);
             const int ew = draw_waves(ctx, d, (uint32_t)ets, fb->w, fb->h);
             const sr_draw_desc* f = items[mine[0]].desc;
-            group = batchable(it.mesh, d) && fb->standard() && (j == 0 || !it.clear) &&
+            group = batchable(it.mesh, d, it.tex) && fb->standard() && (j == 0 || !it.clear) &&
                     (j == 0 || (ets == ets0 && ew == ew0)) &&
                     d->fragment_program == f->fragment_program && d->blend == f->blend;
             if (j == 0) {
@@ -1374,7 +1373,10 @@ static int draw_one(sr_context ctx, sr_framebuffer fb, sr_mesh mesh, const sr_dr
     pl.exhaustive = (d->exec_flags & SR_EXEC_EXHAUSTIVE) != 0 || (value_test && d->stencil_op != SR_STENCIL_KEEP);
     const bool no_stencil = fb->fmt.stencil == SR_STENCIL_NONE;  // `()`: no test, no op (stencil.rs:65-86)
     const bool normals_gs = d->geometry_program == SR_GS_VERTEX_NORMALS || d->geometry_program == SR_GS_FACE_NORMALS;
-    pl.general = !fb->standard() || d->blend == SR_BLEND_OVER_NORMALIZED || d->fragment_program == SR_FP_GBUFFER;
+    // every format, the normalised blend, the multi-output program and a
+    // triangle draw sampling an RGBAF32 texture take the per-pixel raster
+    pl.general = !fb->standard() || d->blend == SR_BLEND_OVER_NORMALIZED || d->fragment_program == SR_FP_GBUFFER ||
+                 (samples_f32(d, tex) && mesh->prim == SR_PRIM_TRIANGLES && !normals_gs);
     if (pl.general) {
         if (mesh->prim != SR_PRIM_TRIANGLES || normals_gs)
             return fail(ctx, SR_ERR_UNSUPPORTED, "draw: lines and points need sr_fb_create's format, RGBA8 programs and blends");
@@ -1424,11 +1426,8 @@ static int draw_one(sr_context ctx, sr_framebuffer fb, sr_mesh mesh, const sr_dr
     vs.viewport = d->viewport;
     memcpy(vs.U, d->vs_uniforms, sizeof(vs.U));
     vs.out = vb;
-    // a non-indexed triangle mesh without a geometry program: each vertex
-    // belongs to one triangle, so setup runs the vertex program itself
-    const bool fuse = !clip && gs == SR_GS_NONE && mesh->prim == SR_PRIM_TRIANGLES && !mesh->indexed;
     if (!clip) {
-        if (!fuse) launch_vertex(vs, s);
+        launch_vertex(vs, s);
     } else if (gs == SR_GS_NONE) {
         // finish the shared clip-space vertices into this chain's own buffer
         if ((rc = ensure(ctx, ctx->gverts, sizeof(float) * (size_t)(4 + pl.K) * std::max<uint32_t>(nv, 1))))
@@ -1490,7 +1489,7 @@ static int draw_one(sr_context ctx, sr_framebuffer fb, sr_mesh mesh, const sr_dr
     if (nwork == 0) {
         rc = SR_OK;
     } else if (out_prim == SR_PRIM_TRIANGLES) {
-        rc = draw_tris(ctx, fb, pl, d, tex, vb, idx, nwork, fuse ? &vs : nullptr);
+        rc = draw_tris(ctx, fb, pl, d, tex, vb, idx, nwork);
     } else if (out_prim == SR_PRIM_LINES) {
         rc = draw_lines_impl(ctx, fb, pl, d, tex, vb, idx, nwork);
     } else {

@@ -128,7 +128,8 @@ int draw_foreign(sr_context ctx, sr_framebuffer fb, sr_mesh mesh, const sr_draw_
 
 // True when a draw can join a batch: triangles, no geometry program,
 // stencil Always/Keep, not the exhaustive (unbinned) debug mode.
-bool batchable(sr_mesh m, const sr_draw_desc* d);
+bool samples_f32(const sr_draw_desc* d, sr_texture tex);
+bool batchable(sr_mesh m, const sr_draw_desc* d, sr_texture tex);
 
 // ---------------------------------------------------------------------------
 // Batched pipeline
@@ -180,7 +181,6 @@ struct BatchPlan {
     std::vector<uint32_t> group_tile_end, group_first; // per group (+1 entry for group_first)
     std::vector<BatchClass> classes;
     uint64_t NV = 0, NT = 0, NTILES = 0;
-    bool soup = true;  // every mesh non-indexed (vertex program fused into setup)
 };
 
 // One asynchronous batch (sr_batch_submit): its workspace and stream, its

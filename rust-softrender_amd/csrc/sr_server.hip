@@ -440,8 +440,8 @@ int sr_server_submit(sr_server S, const sr_request* r, uint64_t* ticket) {
     if (!r->mesh || r->mesh->ctx != S->user) why = "server: mesh missing or from another context";
     else if (r->tex && r->tex->ctx != S->user) why = "server: texture from another context";
     else if (r->width < 1 || r->height < 1 || r->width > 16384 || r->height > 16384) why = "server: bad framebuffer size";
-    else if (!batchable(r->mesh, &r->desc))
-        why = "server: requests are triangle draws without geometry program, stencil Always/Keep";
+    else if (!batchable(r->mesh, &r->desc, r->tex))
+        why = "server: requests are triangle draws without geometry program, stencil Always/Keep, RGBA8 texture";
     else if (r->desc.vertex_program != SR_VP_MVP_NORMAL_UV && r->desc.vertex_program != SR_VP_MVP_NORMAL_WORLD)
         why = "server: unknown vertex program";
     else if (r->desc.fragment_program < SR_FP_FLAT || r->desc.fragment_program > SR_FP_UV_COLOR)
