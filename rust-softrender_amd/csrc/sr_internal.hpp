@@ -146,7 +146,9 @@ struct RasterParams {
     int tex_filter, tex_edge;
     uint32_t tex_border;
     float fsu[32];
-    uint32_t* error_flags;
+    // non-null: nothing is drawn when skip_if[0] bit 0 (a NaN screen
+    // coordinate) or skip_if[4] (the bins overflowed) is set
+    const uint32_t* skip_if;
     // batch: this scene's first global execution tile, and a fused clear
     // (Framebuffer::clear, renderbuffer/mod.rs:125-133) applied on load
     uint32_t tile_base;

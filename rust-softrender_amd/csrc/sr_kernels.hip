@@ -1438,10 +1438,10 @@ __global__ __launch_bounds__(256) void k_partition(const uint32_t* prefix, uint3
 // empty tiles need no clearing beforehand.  For a frame's tiles (tens of
 // thousands); a batch's millions take k_ranges over the pairs instead.
 __global__ __launch_bounds__(256) void k_ranges_search(const uint32_t* keys, uint32_t n, uint32_t nkeys,
-                                                       uint32_t* start, uint32_t* end) {
+                                                       uint32_t* start, uint32_t* end, const uint32_t* n_dev) {
     const uint32_t t = blockIdx.x * 256u + threadIdx.x;
     if (t > nkeys) return;
-    uint32_t lo = 0, hi = n;
+    uint32_t lo = 0, hi = n_dev ? min(*n_dev, n) : n;
     while (lo < hi) {
         const uint32_t m = (lo + hi) >> 1;
         if (keys[m] < t) lo = m + 1;
@@ -1486,6 +1486,17 @@ __global__ __launch_bounds__(256) void k_ranges(const uint32_t* keys, uint32_t n
 // not fit were never written and hold stale data).
 __global__ void k_live_pairs(uint32_t* misc, int overflow_word, int total_word, int live_word) {
     if (blockIdx.x == 0 && threadIdx.x == 0) misc[live_word] = misc[overflow_word] ? 0u : misc[total_word];
+}
+
+// A plain draw's bins: misc[0] error flags (bit 0: NaN coordinate),
+// misc[1] the pair count; misc[4] = 1 when the count exceeds the pair
+// buffers' capacity, misc[5] = the pairs to sort (0 on overflow or NaN).
+__global__ void k_bins_live(uint32_t* misc, uint32_t cap) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        const bool over = misc[1] > cap;
+        misc[4] = over ? 1u : 0u;
+        misc[5] = (over || (misc[0] & 1u)) ? 0u : misc[1];
+    }
 }
 
 // ===========================================================================
@@ -2303,9 +2314,9 @@ void launch_iota(uint32_t* p, uint32_t n, hipStream_t s) {
 }
 
 void launch_ranges_search(const uint32_t* keys, uint32_t n, uint32_t nkeys, uint32_t* start, uint32_t* end,
-                          hipStream_t s) {
+                          hipStream_t s, const uint32_t* n_dev) {
     hipLaunchKernelGGL(k_ranges_search, dim3(blocks_for((uint64_t)nkeys + 1, 256)), dim3(256), 0, s, keys, n, nkeys,
-                       start, end);
+                       start, end, n_dev);
 }
 
 void launch_ranges(const uint32_t* keys, uint32_t n, uint32_t key_base, uint32_t nkeys, uint32_t* start,
@@ -2314,6 +2325,10 @@ void launch_ranges(const uint32_t* keys, uint32_t n, uint32_t key_base, uint32_t
     if (n) hipLaunchKernelGGL(k_ranges, dim3(std::min<uint32_t>(blocks_for(n, 1024), 8192u)), dim3(256), 0, s,
                               keys, n, key_base, nkeys, start, end, n_dev, aligned);
 }
+void launch_bins_live(uint32_t* misc, uint32_t cap, hipStream_t s) {
+    hipLaunchKernelGGL(k_bins_live, dim3(1), dim3(64), 0, s, misc, cap);
+}
+
 void launch_live_pairs(uint32_t* misc, int overflow_word, int total_word, int live_word, hipStream_t s) {
     hipLaunchKernelGGL(k_live_pairs, dim3(1), dim3(64), 0, s, misc, overflow_word, total_word, live_word);
 }

@@ -197,9 +197,10 @@ int radix_sort_groups(uint32_t* keys, uint32_t* vals, uint32_t* keys_tmp, uint32
                       uint32_t* total, hipStream_t s);
 void launch_fill(uint32_t* p, uint32_t v, uint32_t n, hipStream_t s);
 // start/end of tiles [0, nkeys) in keys[0, n) (sorted), by binary search:
-// every tile written, no clearing needed
+// every tile written, no clearing needed.  n_dev: the key count on the
+// device (at most n), when only the device knows it.
 void launch_ranges_search(const uint32_t* keys, uint32_t n, uint32_t nkeys, uint32_t* start, uint32_t* end,
-                          hipStream_t s);
+                          hipStream_t s, const uint32_t* n_dev = nullptr);
 // p[i] = pat[i % period] over n_words 32-bit words (period 1, 2 or 4)
 void launch_fill_pattern(void* p, const uint32_t pat[4], uint32_t period, size_t n_words, hipStream_t s);
 void launch_iota(uint32_t* p, uint32_t n, hipStream_t s);  // p[i] = i
@@ -209,6 +210,8 @@ void launch_ranges(const uint32_t* keys, uint32_t n, uint32_t key_base, uint32_t
                    uint32_t* end, hipStream_t s, const uint32_t* n_dev = nullptr);
 // misc[live_word] = misc[overflow_word] ? 0 : misc[total_word]
 void launch_live_pairs(uint32_t* misc, int overflow_word, int total_word, int live_word, hipStream_t s);
+// misc[4] = misc[1] > cap; misc[5] = pairs to sort (0 on overflow or a NaN flag in misc[0])
+void launch_bins_live(uint32_t* misc, uint32_t cap, hipStream_t s);
 // persistent-raster partition: cost[t] = fixed + items of tile t for
 // t < n, cost[n] = 0 (scan it into a prefix of n + 1 entries), then G runs
 // of near-equal cost over tiles [tile0, tile0 + n): wg_start[0..G]

@@ -485,6 +485,7 @@ template <int ETS, int NWV, int FP, int BLEND, int SMODE, bool CLAMP>
 __global__ __launch_bounds__(256) SR_RASTER_OCCUPANCY void k_raster_tris(RasterParams P) {
     using SH = RasterShared<ETS, NWV, FP, BLEND>;
     __shared__ SH sh;
+    if (P.skip_if && ((load_item(P.skip_if, 0) & 1u) | load_item(P.skip_if, 4))) return;  // NaN, overflow
     for (int i = threadIdx.x; i < 256; i += SH::NT) sh.lut[i] = (float)i / 255.0f;
     __syncthreads();
     const uint32_t tile = xcd_remap(blockIdx.x, gridDim.x);

@@ -30,6 +30,7 @@ __device__ __forceinline__ void shade_varyings(const RasterParams& P, uint32_t t
 template <int FP, int BLEND>
 __global__ __launch_bounds__(64) void k_raster_fmt(RasterParams P, FmtParams F) {
     __shared__ float lut_s[256];
+    if (P.skip_if && ((load_item(P.skip_if, 0) & 1u) | load_item(P.skip_if, 4))) return;  // NaN, overflow
     for (int i = threadIdx.x; i < 256; i += 64) lut_s[i] = (float)i / 255.0f;
     __syncthreads();
     const Lut lut{lut_s};

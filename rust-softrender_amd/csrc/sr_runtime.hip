@@ -172,6 +172,36 @@ int ensure_pairs(sr_context ctx, uint32_t npairs) {
     return SR_OK;
 }
 
+// Pairs the pair buffers, the radix histogram and the scan space hold: the
+// largest n with n entries in each pair buffer, radix_hist_words(n) + 256
+// histogram words and scan_tmp_words(radix_hist_words(n) + 256) + 16 scan
+// words (ensure_pair_capacity(n) makes all three hold).
+uint32_t pair_capacity(sr_context ctx) {
+    uint64_t n = std::min(std::min(ctx->keys.bytes, ctx->vals.bytes), std::min(ctx->keys2.bytes, ctx->vals2.bytes)) /
+                 sizeof(uint32_t);
+    n = std::min<uint64_t>(n, 0x7FFFFFFFu);
+    // radix_hist_words(n) = 256 * blocks_for(n, SCAN_TILE): the blocks the histogram holds
+    const uint64_t hw = ctx->hist.bytes / sizeof(uint32_t);
+    const uint64_t hist_blocks = hw > 256 ? (hw - 256) / 256 : 0;
+    n = std::min<uint64_t>(n, hist_blocks * SCAN_TILE);
+    // the histogram's scan: scan_tmp_words(256 * blocks + 256) + 16 words
+    while (n > 0 && sizeof(uint32_t) * ((uint64_t)scan_tmp_words(radix_hist_words((uint32_t)n) + 256) + 16) >
+                        ctx->scan_tmp.bytes)
+        n = n > SCAN_TILE ? ((n + SCAN_TILE - 1) / SCAN_TILE - 1) * (uint64_t)SCAN_TILE : 0;
+    return (uint32_t)n;
+}
+
+// Grows the pair buffers, histogram and scan space to hold n pairs.
+int ensure_pair_capacity(sr_context ctx, uint64_t n) {
+    if (n >= 0x7FFFFFFFull) return fail(ctx, SR_ERR_OUT_OF_MEMORY, "draw: bin pairs exceed 2^31");
+    if (pair_capacity(ctx) >= n) return SR_OK;
+    int rc;
+    const uint32_t m = (uint32_t)n;
+    if ((rc = ensure_pairs(ctx, m))) return rc;
+    if ((rc = ensure(ctx, ctx->hist, sizeof(uint32_t) * (radix_hist_words(m) + 256)))) return rc;
+    return ensure(ctx, ctx->scan_tmp, sizeof(uint32_t) * (scan_tmp_words(radix_hist_words(m) + 256) + 16));
+}
+
 struct DrawPlan {
     uint32_t W, H, ets, ntx, nty, ntiles, tw, th, scols, srows;
     int nwv = 1;  // waves per execution tile
@@ -203,6 +233,8 @@ void fill_raster_params(const DrawPlan& pl, sr_framebuffer fb, const sr_draw_des
 // ---------------------------------------------------------------------------
 // The triangle draw
 // ---------------------------------------------------------------------------
+int launch_tri_raster(sr_context ctx, sr_framebuffer fb, const DrawPlan& pl, RasterParams R, uint32_t ntris);
+
 int draw_tris(sr_context ctx, sr_framebuffer fb, const DrawPlan& pl, const sr_draw_desc* d,
               sr_texture tex, VertexBuf vb, const uint32_t* idx, uint32_t ntris) {
     int rc;
@@ -232,7 +264,7 @@ int draw_tris(sr_context ctx, sr_framebuffer fb, const DrawPlan& pl, const sr_dr
     R.tris = P<TriRec>(ctx->recs);
     R.vids = vids;
     R.verts = vb;
-    R.error_flags = P<uint32_t>(ctx->misc);
+    R.skip_if = nullptr;
     if (pl.exhaustive) {
         if ((rc = read_words(ctx, 1))) return rc;
         if (ctx->h_misc[0] & 1u) return fail(ctx, SR_ERR_INVALID_COORDINATE, "NaN screen coordinate");
@@ -253,25 +285,52 @@ int draw_tris(sr_context ctx, sr_framebuffer fb, const DrawPlan& pl, const sr_dr
         if ((rc = scan_counts(ctx, P<uint32_t>(ctx->counts), P<uint32_t>(ctx->offsets), ntris)))
             return rc;
         mark(ctx, pl.timing, ST_SCAN);
-        // The error flag and the pair count travel to the host while the
-        // pairs are emitted into the pair buffers as they are; a draw with
-        // more pairs than they hold (its first, or a larger one) emits
-        // again after they grow.
+        // The error flags and the pair count travel to the host while the
+        // bins are built and the raster runs, all sized on the device by
+        // the pair count: the pair buffers hold `cap` pairs; a draw with
+        // more (misc[4] set) or with a NaN coordinate draws nothing, and
+        // the host, once the count arrives, grows the buffers and bins and
+        // draws again.  So the host waits for setup only, never between
+        // kernels.
         HIP_OK(ctx, hipMemcpyAsync(ctx->h_misc, ctx->misc.p, 2 * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
         HIP_OK(ctx, hipEventRecord(ctx->ev_read, s));
-        EmitParams ep;
-        ep.recs = P<TriRec>(ctx->recs); ep.bins = bins; ep.counts = P<uint32_t>(ctx->counts);
-        ep.offsets = P<uint32_t>(ctx->offsets); ep.n = ntris; ep.W = pl.W; ep.H = pl.H;
-        ep.ets = pl.ets; ep.ntx = pl.ntx; ep.tw = pl.tw; ep.th = pl.th; ep.key_base = 0; ep.val_base = 0;
-        ep.empty_cull = sp.empty_cull;
-        auto emit = [&]() {
+        if ((rc = ensure(ctx, ctx->tstart, sizeof(uint32_t) * pl.ntiles)) ||
+            (rc = ensure(ctx, ctx->tend, sizeof(uint32_t) * pl.ntiles)))
+            return rc;
+        // a first guess for a context's first draws: 4 pairs per triangle
+        // (none in the fixed workspace, which sizes them exactly)
+        if (!ctx->arena && (rc = ensure_pair_capacity(ctx, (uint64_t)ntris * 4 + 4096))) return rc;
+        auto bin_and_draw = [&]() -> int {
+            const uint32_t cap = pair_capacity(ctx);
+            EmitParams ep;
+            ep.recs = P<TriRec>(ctx->recs); ep.bins = bins; ep.counts = P<uint32_t>(ctx->counts);
+            ep.offsets = P<uint32_t>(ctx->offsets); ep.n = ntris; ep.W = pl.W; ep.H = pl.H;
+            ep.ets = pl.ets; ep.ntx = pl.ntx; ep.tw = pl.tw; ep.th = pl.th; ep.key_base = 0; ep.val_base = 0;
+            ep.empty_cull = sp.empty_cull;
             ep.keys = P<uint32_t>(ctx->keys);
             ep.vals = P<uint32_t>(ctx->vals);
-            ep.cap = (uint32_t)std::min<size_t>(std::min(ctx->keys.bytes, ctx->vals.bytes) / sizeof(uint32_t),
-                                                0xFFFFFFFFu);
-            if (ep.keys && ep.vals) launch_tri_emit(ep, s);
+            ep.cap = cap;
+            launch_tri_emit(ep, s);
+            uint32_t* misc = P<uint32_t>(ctx->misc);
+            launch_bins_live(misc, cap, s);  // misc[4]: overflow, misc[5]: pairs to sort (0 on overflow or NaN)
+            mark(ctx, pl.timing, ST_EMIT);
+            const int swapped = radix_sort(P<uint32_t>(ctx->keys), P<uint32_t>(ctx->vals),
+                                           P<uint32_t>(ctx->keys2), P<uint32_t>(ctx->vals2), cap,
+                                           bits_for(pl.ntiles), P<uint32_t>(ctx->hist),
+                                           P<uint32_t>(ctx->scan_tmp), misc + 2, s, misc + 5);
+            const uint32_t* skeys = swapped ? P<uint32_t>(ctx->keys2) : P<uint32_t>(ctx->keys);
+            const uint32_t* svals = swapped ? P<uint32_t>(ctx->vals2) : P<uint32_t>(ctx->vals);
+            mark(ctx, pl.timing, ST_SORT);
+            launch_ranges_search(skeys, cap, pl.ntiles, P<uint32_t>(ctx->tstart), P<uint32_t>(ctx->tend), s,
+                                 misc + 5);
+            mark(ctx, pl.timing, ST_RANGES);
+            R.tile_start = P<uint32_t>(ctx->tstart);
+            R.tile_end = P<uint32_t>(ctx->tend);
+            R.items = svals;
+            R.skip_if = misc;
+            return launch_tri_raster(ctx, fb, pl, R, ntris);
         };
-        emit();
+        if ((rc = bin_and_draw())) return rc;
         HIP_OK(ctx, hipEventSynchronize(ctx->ev_read));
         if (ctx->h_misc[0] & 1u) return fail(ctx, SR_ERR_INVALID_COORDINATE, "NaN screen coordinate");
         // a triangle that may pass the coverage test outside its box went
@@ -280,34 +339,23 @@ int draw_tris(sr_context ctx, sr_framebuffer fb, const DrawPlan& pl, const sr_dr
         ctx->last.strip_slivers = (ctx->h_misc[0] & 2u) ? 1u : 0u;
         const uint32_t npairs = ctx->h_misc[1];
         ctx->last.bin_pairs = npairs;
-        const size_t pb = sizeof(uint32_t) * std::max<uint32_t>(npairs, 1);
-        if (npairs > ep.cap || !ep.keys || !ep.vals) {
-            if ((rc = ensure(ctx, ctx->keys, pb)) || (rc = ensure(ctx, ctx->vals, pb))) return rc;
-            emit();
+        if (npairs > pair_capacity(ctx)) {
+            const uint64_t want = ctx->arena ? (uint64_t)npairs : (uint64_t)npairs + npairs / 4 + 4096;
+            if ((rc = ensure_pair_capacity(ctx, want))) return rc;
+            if (npairs > pair_capacity(ctx)) return fail(ctx, SR_ERR_OUT_OF_MEMORY, "draw: bin capacity did not grow");
+            if ((rc = bin_and_draw())) return rc;
         }
-        if ((rc = ensure(ctx, ctx->keys2, pb)) || (rc = ensure(ctx, ctx->vals2, pb))) return rc;
-        if ((rc = ensure(ctx, ctx->hist, sizeof(uint32_t) * (radix_hist_words(npairs) + 256))))
-            return rc;
-        if ((rc = ensure(ctx, ctx->scan_tmp,
-                         sizeof(uint32_t) * (scan_tmp_words(radix_hist_words(npairs) + 256) + 16))))
-            return rc;
-        if ((rc = ensure(ctx, ctx->tstart, sizeof(uint32_t) * pl.ntiles)) ||
-            (rc = ensure(ctx, ctx->tend, sizeof(uint32_t) * pl.ntiles)))
-            return rc;
-        mark(ctx, pl.timing, ST_EMIT);
-        const int swapped = radix_sort(P<uint32_t>(ctx->keys), P<uint32_t>(ctx->vals),
-                                       P<uint32_t>(ctx->keys2), P<uint32_t>(ctx->vals2), npairs,
-                                       bits_for(pl.ntiles), P<uint32_t>(ctx->hist),
-                                       P<uint32_t>(ctx->scan_tmp), P<uint32_t>(ctx->misc) + 2, s);
-        const uint32_t* skeys = swapped ? P<uint32_t>(ctx->keys2) : P<uint32_t>(ctx->keys);
-        const uint32_t* svals = swapped ? P<uint32_t>(ctx->vals2) : P<uint32_t>(ctx->vals);
-        mark(ctx, pl.timing, ST_SORT);
-        launch_ranges_search(skeys, npairs, pl.ntiles, P<uint32_t>(ctx->tstart), P<uint32_t>(ctx->tend), s);
-        mark(ctx, pl.timing, ST_RANGES);
-        R.tile_start = P<uint32_t>(ctx->tstart);
-        R.tile_end = P<uint32_t>(ctx->tend);
-        R.items = svals;
+        return SR_OK;
     }
+    return launch_tri_raster(ctx, fb, pl, R, ntris);
+}
+
+// The raster of a triangle draw whose bins R names: the per-pixel raster for
+// the general formats, else the stencil strips (counting draws) and the
+// tile raster.
+int launch_tri_raster(sr_context ctx, sr_framebuffer fb, const DrawPlan& pl, RasterParams R, uint32_t ntris) {
+    int rc;
+    hipStream_t s = ctx->stream;
     if (pl.general) {
         // every format through the per-pixel raster; a stencil with a test
         // or op is tracked there in order (the draw is exhaustive then)
@@ -1403,8 +1451,8 @@ static int draw_one(sr_context ctx, sr_framebuffer fb, sr_mesh mesh, const sr_dr
     int rc;
     hipStream_t s = ctx->stream;
     if (pl.timing) hipEventRecord(ctx->ev[0], s);
-    // clear the error word
-    launch_fill(P<uint32_t>(ctx->misc), 0u, 4u, s);
+    // clear the error word, the pair count and the overflow flag
+    launch_fill(P<uint32_t>(ctx->misc), 0u, 6u, s);
 
     // ---- vertex stage
     const uint32_t nv = mesh->nv;
