@@ -632,6 +632,24 @@ __device__ __forceinline__ uint32_t strip_reach(const TriRec& r, float det, uint
     return BIN_REACH;
 }
 
+// The same claim, sufficient and cheap (no division): taking d as the
+// distance to the farthest pixel centre of the framebuffer, every widened
+// vertex lies within |k| (|e31x| + |e23x|) E horizontally (|e31y| + |e23y|
+// vertically) of its vertex, E = eu + ev + ew bounding each (u, v)
+// displacement; when both stay below 0.45 px the widened triangle cannot
+// reach a pixel centre outside the box (those lie at least 0.5 px beyond
+// the vertices' extremes).  Multiplied through by |det|: nothing to divide.
+__device__ __forceinline__ bool strip_box_fast(const TriRec& r, float det, uint32_t Wi, uint32_t Hi) {
+    const double e23y = r.e23y, e23x = r.e23x, e31y = r.e31y, e31x = r.e31x;
+    const double x3 = r.x3, y3 = r.y3, ad = fabs((double)det);
+    const double D = fmax(fmax(fabs(x3), fabs((double)Wi - x3)), fmax(fabs(y3), fabs((double)Hi - y3))) + 1.0;
+    const double adm = fabs(e23y * e31x - e23x * e31y);
+    const double Lu = fabs(e23y) + fabs(e23x), Lv = fabs(e31y) + fabs(e31x);
+    const double dE = 0x1p-19 * ((Lu + Lv) * D + 4.0 * ad) + 0x1p-18 * ad;  // |det| E
+    const double Sx = fabs(e31x) + fabs(e23x), Sy = fabs(e31y) + fabs(e23y);
+    return Sx * dE < 0.45 * adm && Sy * dE < 0.45 * adm;
+}
+
 // The execution-tile columns (or rows) that the box [a0, a1] clamped into
 // each screen tile meets along one axis (screen tile j spans [j t,
 // min(j t + t, amax)], j < n, as clamp_as_int! sees it): f(lo, hi) for each
@@ -725,7 +743,8 @@ __device__ __forceinline__ uint32_t tri_setup_xyz(float x1, float y1, float z1, 
     r.Y1 = (int32_t)floorf(clampf(mxy, -1.0f, H));
     b = TriBin{r.X0, r.X1, r.Y0, r.Y1, ~0u, BIN_BOX};
     int rx0, rx1, ry0, ry1;
-    const uint32_t mode = strip_reach(r, det, g.W, g.H, rx0, rx1, ry0, ry1);
+    const uint32_t mode = strip_box_fast(r, det, g.W, g.H) ? (uint32_t)BIN_BOX
+                                                           : strip_reach(r, det, g.W, g.H, rx0, rx1, ry0, ry1);
     if (mode != BIN_BOX) {
         // a pixel outside the box may pass: every tile it may lie in, no culling
         atomicOr(&err[0], 2u);
@@ -960,8 +979,11 @@ __global__ __launch_bounds__(256) void k_tri_emit_batch(BatchTables T, const Tri
     __shared__ EmitWave ew[4];
     const uint32_t t = blockIdx.x * 256u + threadIdx.x;
     if (__builtin_amdgcn_readfirstlane(blockIdx.x * 256u + (threadIdx.x & ~63u)) >= T.nt) return;  // whole wave
-    uint32_t c = t < T.nt ? counts[t] : 0u;
-    const uint32_t o = c ? offsets[t] : 0u;
+    // the three loads issue together (none waits on another)
+    const bool in = t < T.nt;
+    uint32_t c = in ? counts[t] : 0u;
+    const uint32_t o = in ? offsets[t] : 0u;
+    const TriBin bt = in ? bins[t] : TriBin{1, 0, 1, 0, 0u, BIN_BOX};
     if (c && (uint64_t)o + c > cap) {
         atomicOr(overflow, 1u);
         c = 0;
@@ -979,7 +1001,7 @@ __global__ __launch_bounds__(256) void k_tri_emit_batch(BatchTables T, const Tri
         g = BinGeom{S.W, S.H, S.ets, S.tw, S.th};
     });
     if (c) {
-        b = bins[t];
+        b = bt;
         if (b.mode == BIN_CLAMP) {
             direct = false;
         } else if (tile_span(b, g.W, g.H, g.ets, sp)) {
@@ -1001,14 +1023,17 @@ __global__ __launch_bounds__(256) void k_tri_emit(EmitParams P) {
     __shared__ EmitWave ew[4];
     const uint32_t t = blockIdx.x * 256u + threadIdx.x;
     if (__builtin_amdgcn_readfirstlane(blockIdx.x * 256u + (threadIdx.x & ~63u)) >= P.n) return;  // whole wave
-    uint32_t c = t < P.n ? P.counts[t] : 0u;
-    const uint32_t o = c ? P.offsets[t] : 0u;
+    // the three loads issue together (none waits on another)
+    const bool in = t < P.n;
+    uint32_t c = in ? P.counts[t] : 0u;
+    const uint32_t o = in ? P.offsets[t] : 0u;
+    const TriBin bt = in ? P.bins[t] : TriBin{1, 0, 1, 0, 0u, BIN_BOX};
     if (c && (uint64_t)o + c > P.cap) c = 0;  // the host emits again into a larger buffer
     TriBin b{1, 0, 1, 0, 0u, BIN_BOX};
     TileSpan sp{0, 0, 0, 0, 0u, 0u, 0u, 0u};
     bool direct = false, all = false;
     if (c) {
-        b = P.bins[t];
+        b = bt;
         if (b.mode == BIN_CLAMP) {
             direct = false;
         } else if (tile_span(b, P.W, P.H, P.ets, sp)) {

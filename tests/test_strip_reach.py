@@ -75,6 +75,14 @@ def strip_reach_np(x, y, W, H, widen=1.0):
         ry0 = np.fmax(np.fmin(np.ceil(mny - 0.5 - m), by0), 0)
         ry1 = np.fmin(np.fmax(np.floor(mxy - 0.5 + m), by1), H - 1)
     mode = np.where(never, BIN_BOX, np.where(clamp, BIN_CLAMP, np.where(out, BIN_REACH, BIN_BOX)))
+    # the cheap sufficient test setup runs first (strip_box_fast)
+    with np.errstate(all="ignore"):
+        D = np.maximum(np.maximum(np.abs(X3), np.abs(W - X3)), np.maximum(np.abs(Y3), np.abs(H - Y3))) + 1.0
+        ad, adm = np.abs(df), np.abs(dm)
+        Lu = np.abs(e23y) + np.abs(e23x); Lv = np.abs(e31y) + np.abs(e31x)
+        dE = widen * (2.0 ** -19 * ((Lu + Lv) * D + 4.0 * ad) + 2.0 ** -18 * ad)
+        fast = ((np.abs(e31x) + np.abs(e23x)) * dE < 0.45 * adm) & ((np.abs(e31y) + np.abs(e23y)) * dE < 0.45 * adm)
+    mode = np.where(fast, BIN_BOX, mode)
     return mode, (bx0, bx1, by0, by1), (rx0, rx1, ry0, ry1)
 
 
