@@ -329,30 +329,9 @@ __device__ __forceinline__ void raster_tile(const RasterParams& P, uint32_t tile
                         cx0 = min(max(X0, Tx0), Tx1); cx1 = min(max(X1, Tx0), Tx1);
                         cy0 = min(max(Y0, Ty0), Ty1); cy1 = min(max(Y1, Ty0), Ty1);
                     }
-                    // without stencil effects: the lane masks, set apart so
-                    // that other triangles skip the test
-                    bool inb[SMODE == 0 ? PPT : 1];
-                    if constexpr (SMODE == 0) {
-                        if (clampit) {
-#pragma unroll
-                            for (int j = 0; j < PPT; j++)
-                                inb[j] = (px[j] >= cx0) & (px[j] <= cx1) & (py[j] >= cy0) & (py[j] <= cy1);
-                        } else {
-#pragma unroll
-                            for (int j = 0; j < PPT; j++) inb[j] = true;
-                        }
-                    }
-                    bool full = false;
-#pragma unroll
-                    for (int j = 0; j < PPT; j++) {
-                        // straight-line per pixel: every test is a mask, so
-                        // the wave runs one exec-mask region per pixel
-                        // instead of a nested branch per test
-                        bool ok = inT[j];
-                        if constexpr (SMODE == 0)
-                            ok = ok & inb[j];
-                        else
-                            ok = ok & (px[j] >= cx0) & (px[j] <= cx1) & (py[j] >= cy0) & (py[j] <= cy1);
+                    // One pixel: stencil, coverage, depth, queue (ok: the
+                    // pixel is in the tested region).
+                    auto pixel = [&](int j, bool ok) {
                         if constexpr (SMODE == 2) {  // stencil before coverage, triangle.rs:90-99
                             const bool sp = ok && stencil_test(P.s_test, st[j], P.s_ref);
                             st[j] = sp ? stencil_op(P.s_op, P.s_wrap, st[j], P.s_ref) : st[j];
@@ -379,11 +358,29 @@ __device__ __forceinline__ void raster_tile(const RasterParams& P, uint32_t tile
                                 qn[j] = 1;
                             }
                         }
-                        // tested on every lane, outside the branch: the
-                        // ballot below then takes the compare's mask as is
-                        if constexpr (BLEND == SR_BLEND_OVER) full |= qn[j] == (uint32_t)Q;
+                    };
+                    if constexpr (SMODE == 0 && !CLAMP) {
+                        // no stencil effects: no test at all for a triangle
+                        // that cannot pass outside its box; for one that can,
+                        // the lanes outside its clamped box sit out
+                        if (clampit) {
+#pragma unroll
+                            for (int j = 0; j < PPT; j++)
+                                if ((px[j] >= cx0) & (px[j] <= cx1) & (py[j] >= cy0) & (py[j] <= cy1)) pixel(j, inT[j]);
+                        } else {
+#pragma unroll
+                            for (int j = 0; j < PPT; j++) pixel(j, inT[j]);
+                        }
+                    } else {
+#pragma unroll
+                        for (int j = 0; j < PPT; j++)
+                            pixel(j, inT[j] & (px[j] >= cx0) & (px[j] <= cx1) & (py[j] >= cy0) & (py[j] <= cy1));
                     }
                     if constexpr (BLEND == SR_BLEND_OVER) {
+                        // a full queue on any lane: shade and blend the queues
+                        bool full = false;
+#pragma unroll
+                        for (int j = 0; j < PPT; j++) full |= qn[j] == (uint32_t)Q;
                         if (__ballot(full)) flush(PD);
                     }
                 };
