@@ -125,7 +125,7 @@ struct RasterParams {
     const uint32_t* items;  // sorted item ids per execution tile
     // primitives
     const TriRec* tris;
-    const TriVid* vids;     // per triangle, beside tris
+    const TriVid* vids;     // per triangle, beside tris; null: triangle t's vertices are 3t, 3t+1, 3t+2
     const LineItem* line_items;
     const Plot* plots;
     VertexBuf verts;        // screen-space vertices

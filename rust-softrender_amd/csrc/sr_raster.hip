@@ -94,8 +94,13 @@ __device__ __forceinline__ uint32_t shade_rec(const RasterParams& P, const QRec&
     constexpr int KV = fp_varyings<FP>();
     float k[KV > 0 ? KV : 1];
     if constexpr (KV > 0) {
-        const TriVid& r = P.vids[q.tri];
-        const uint32_t i0 = r.i0, i1 = r.i1, i2 = r.i2;
+        // a triangle list without indices needs no vertex-id load: its
+        // vertices are 3t, 3t+1, 3t+2
+        uint32_t i0 = 3u * q.tri, i1 = i0 + 1u, i2 = i0 + 2u;
+        if (P.vids) {
+            const TriVid& r = P.vids[q.tri];
+            i0 = r.i0; i1 = r.i1; i2 = r.i2;
+        }
         const float u = q.u, v = q.v, w = (1.0f - u) - v;
         const float *r0 = P.verts.var(i0), *r1 = P.verts.var(i1), *r2 = P.verts.var(i2);
 #pragma unroll

@@ -19,9 +19,13 @@ __device__ __forceinline__ void shade_varyings(const RasterParams& P, uint32_t t
                                                float* k) {
     constexpr int KV = fp_varyings<FP>();
     if constexpr (KV > 0) {
-        const TriVid& r = P.vids[tri];
+        uint32_t i0 = 3u * tri, i1 = i0 + 1u, i2 = i0 + 2u;  // no indices: 3t, 3t+1, 3t+2
+        if (P.vids) {
+            const TriVid& r = P.vids[tri];
+            i0 = r.i0; i1 = r.i1; i2 = r.i2;
+        }
         const float w = (1.0f - u) - v;
-        const float *r0 = P.verts.var(r.i0), *r1 = P.verts.var(r.i1), *r2 = P.verts.var(r.i2);
+        const float *r0 = P.verts.var(i0), *r1 = P.verts.var(i1), *r2 = P.verts.var(i2);
 #pragma unroll
         for (int c = 0; c < KV; c++) k[c] = ((r0[c] * u) + (r1[c] * v)) + (r2[c] * w);  // interpolate.rs:43-64
     }

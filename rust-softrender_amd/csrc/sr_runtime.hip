@@ -262,7 +262,7 @@ int draw_tris(sr_context ctx, sr_framebuffer fb, const DrawPlan& pl, const sr_dr
     RasterParams R;
     fill_raster_params(pl, fb, d, tex, R);
     R.tris = P<TriRec>(ctx->recs);
-    R.vids = vids;
+    R.vids = idx ? vids : nullptr;  // no indices: vertices 3t, 3t+1, 3t+2
     R.verts = vb;
     R.skip_if = nullptr;
     if (pl.exhaustive) {
@@ -719,8 +719,12 @@ int batch_enqueue(sr_context ctx, BatchWork& W, const BatchPlan& pl, bool use_gr
         R.tile_start = P<uint32_t>(W.tstart); R.tile_end = P<uint32_t>(W.tend);
         R.items = sorted_vals;
         R.tris = P<TriRec>(W.recs);
-        R.vids = (const TriVid*)((const TriBin*)(P<TriRec>(W.recs) + std::max<uint64_t>(NT, 1)) +
-                                 std::max<uint64_t>(NT, 1));
+        // a scene without indices whose vertices start at 3x its first
+        // triangle needs no vertex ids: vertex 3t+k
+        const bool ident = !it.mesh->indexed && pl.geom[k].vbase == 3ull * pl.geom[k].tbase;
+        R.vids = ident ? nullptr
+                       : (const TriVid*)((const TriBin*)(P<TriRec>(W.recs) + std::max<uint64_t>(NT, 1)) +
+                                         std::max<uint64_t>(NT, 1));
         R.verts = vb;
         R.K = pl.geom[k].K;
         R.fp = d->fragment_program; R.blend = d->blend; R.alpha_mode = d->alpha_mode;
