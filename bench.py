@@ -426,6 +426,7 @@ def extras(a):
     keep = ("fps", "p50_ms", "p99_ms", "max_ms", "mean_batch", "completed", "truncated")
     srv = {"meshes": sv["meshes"], "mean_mesh_tris": round(sv["mean_mesh_tris"]),
            "capacity_served_fps": round(sv["capacity_served"]["fps"], 1),
+           "capacity_served_runs": [round(v, 1) for v in sv["capacity_served"]["fps_runs"]],
            "capacity_plain_fps": round(sv["capacity_plain"]["fps"], 1),
            "capacity_ratio": round(sv["capacity_ratio"], 2),
            "copy_ceiling_fps": {k: round(v, 1) for k, v in sv["copy_ceiling_fps"].items()},

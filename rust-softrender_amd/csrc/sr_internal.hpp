@@ -39,11 +39,21 @@ struct alignas(64) TriRec {
     float e31y, e31x;     // (y3 - y1), (x1 - x3)   — triangle.rs:109
     float z1, z2, z3;
     int32_t X0, X1, Y0, Y1;  // floor of the f32 bbox, clamped to [-1, W] x [-1, H]
-    uint32_t strip;          // 1: a pixel centre outside the box may pass the coverage
-                             // test; the raster tests the box clamped into each screen
-                             // tile (clamp_as_int!), as the reference does
+    uint32_t zs;             // bit 0 (strip): a pixel centre outside the box may pass the
+                             // coverage test; the raster tests the box clamped into each
+                             // screen tile (clamp_as_int!), as the reference does.
+                             // Bits 1..31: depth_key(zhi) rounded up to even, zhi an
+                             // upper bound of every depth the triangle can write
+                             // (depth_bound_key)
 };
 static_assert(sizeof(TriRec) == 64, "TriRec layout");
+
+// Order-preserving key of an f32 (a < b  <=>  key(a) < key(b) for all
+// non-NaN a, b, with -0 just below +0); a NaN of either sign lies beyond
+// the infinity of its sign.
+__host__ __device__ __forceinline__ uint32_t depth_key(uint32_t bits) {
+    return bits ^ ((uint32_t)((int32_t)bits >> 31) | 0x80000000u);
+}
 
 // The vertex ids of a triangle in the screen vertex buffer (the shading
 // gathers' addresses).

@@ -705,13 +705,35 @@ __device__ __forceinline__ void emit_clamp_pairs(const TriBin& b, const BinGeom&
     });
 }
 
+// An upper bound of every depth a fragment of the triangle can write, as a
+// depth_key rounded up to even (the raster's hierarchical depth test
+// compares it with the least key of a wave's pixels).  A fragment passes
+// only with u, v, w >= 0, and then u, v, w <= 1 and |u + v + w - 1| <=
+// 2^-23 (w = (1 - u) - v rounds twice), so the exact z1 u + z2 v + z3 w is
+// at most M + K 2^-23 (M = max z_i, K = max |z_i|); the three products and
+// two sums of triangle.rs:115-117 add at most 8.001 * 2^-24 K + 5 * 2^-150.
+// zhi = M + K 2^-19 + 2^-140, rounded up to f32, covers both.  A NaN or
+// infinite depth gives the key of +inf.
+__device__ __forceinline__ uint32_t depth_bound_key(float z1, float z2, float z3) {
+    const double M = fmax(fmax((double)z1, (double)z2), (double)z3);
+    const double K = fmax(fmax(fabs((double)z1), fabs((double)z2)), fabs((double)z3));
+    const double b = M + K * 0x1p-19 + 0x1p-140;
+    uint32_t key = 0xFF800000u;  // depth_key(+inf)
+    if (z1 == z1 && z2 == z2 && z3 == z3 && b < (double)INFINITY) {
+        float f = (float)b;
+        if ((double)f < b) f = nextafterf(f, INFINITY);
+        key = depth_key(__float_as_uint(f));
+    }
+    return key >= 0xFFFFFFFEu ? 0xFFFFFFFEu : (key + 1u) & ~1u;
+}
+
 // Setup of the triangle with screen vertices (x1,y1,z1), (x2,y2,z2),
 // (x3,y3,z3); returns its pair count.  (The caller stores the vertex ids in
 // the triangle's TriVid.)
 __device__ __forceinline__ uint32_t tri_setup_xyz(float x1, float y1, float z1, float x2, float y2, float z2,
                                                   float x3, float y3, float z3, int cull, const BinGeom& g,
                                                   TriRec& r, TriBin& b, uint32_t* err, int empty_cull) {
-    r.strip = 0u;
+    r.zs = depth_bound_key(z1, z2, z3);
     b = TriBin{1, 0, 1, 0, 0u, BIN_BOX};
     if (cull != SR_CULL_NONE) {
         // shoelace, triangle.rs:56 (left-to-right evaluation)
@@ -748,7 +770,7 @@ __device__ __forceinline__ uint32_t tri_setup_xyz(float x1, float y1, float z1, 
     if (mode != BIN_BOX) {
         // a pixel outside the box may pass: every tile it may lie in, no culling
         atomicOr(&err[0], 2u);
-        r.strip = 1u;
+        r.zs |= 1u;
         b.mode = mode;
         if (mode == BIN_CLAMP) return clamp_tile_count(b, g);
         b.X0 = rx0; b.X1 = rx1; b.Y0 = ry0; b.Y1 = ry1;

@@ -58,6 +58,29 @@ struct WaveRegion {
 template <int PPT>
 constexpr int queue_depth() { return PPT == 1 ? SR_QUEUE_DEPTH1 : (PPT == 2 ? 4 : (PPT == 4 ? 2 : 1)); }
 
+// Triangles between refreshes of the hierarchical depth key.
+#ifndef SR_HIZ_PERIOD
+#define SR_HIZ_PERIOD 8u
+#endif
+
+// The least depth_key of the wave's PPT x 64 pixel depths (DPP min: row_shr
+// 1, 2, 4, 8 within rows of 16 lanes, shifted-in lanes keeping the
+// identity, then row_bcast 15 and 31 as in wave_excl_scan; lane 63 ends
+// with the minimum).
+template <int PPT>
+__device__ __forceinline__ uint32_t wave_min_key(const float (&dep)[PPT]) {
+    uint32_t m = 0xFFFFFFFFu;
+#pragma unroll
+    for (int j = 0; j < PPT; j++) m = min(m, depth_key(__float_as_uint(dep[j])));
+    m = min(m, (uint32_t)__builtin_amdgcn_update_dpp(-1, (int)m, 0x111, 0xf, 0xf, false));  // row_shr:1
+    m = min(m, (uint32_t)__builtin_amdgcn_update_dpp(-1, (int)m, 0x112, 0xf, 0xf, false));  // row_shr:2
+    m = min(m, (uint32_t)__builtin_amdgcn_update_dpp(-1, (int)m, 0x114, 0xf, 0xf, false));  // row_shr:4
+    m = min(m, (uint32_t)__builtin_amdgcn_update_dpp(-1, (int)m, 0x118, 0xf, 0xf, false));  // row_shr:8
+    m = min(m, (uint32_t)__builtin_amdgcn_update_dpp(-1, (int)m, 0x142, 0xa, 0xf, false));  // row_bcast:15
+    m = min(m, (uint32_t)__builtin_amdgcn_update_dpp(-1, (int)m, 0x143, 0xc, 0xf, false));  // row_bcast:31
+    return (uint32_t)__builtin_amdgcn_readlane((int)m, 63);
+}
+
 struct QRec {
     uint32_t tri;  // triangle id; after shading, the fragment colour
     float u, v;    // barycentrics (w = (1 - u) - v is recomputed exactly)
@@ -220,13 +243,15 @@ __device__ __forceinline__ void raster_tile(const RasterParams& P, uint32_t tile
         py[j] = wy0 + lane / RW + j * RPP;
         live[j] = (uint32_t)px[j] < P.W && (uint32_t)py[j] < P.H;
         const size_t idx = (size_t)py[j] * P.W + px[j];
+        // a pixel outside the framebuffer holds a positive NaN depth: it
+        // passes no depth test and lies above every key of wave_min_key
         if (P.clear) {  // Framebuffer::clear: colour, depth far, stencil 0
             col[j] = P.clear_rgba;
-            dep[j] = -3.40282347e+38f;
+            dep[j] = live[j] ? -3.40282347e+38f : __uint_as_float(0x7FFFFFFFu);
             st[j] = 0u;
         } else {
             col[j] = live[j] ? P.color[idx] : 0u;
-            dep[j] = live[j] ? P.depth[idx] : 0.0f;
+            dep[j] = live[j] ? P.depth[idx] : __uint_as_float(0x7FFFFFFFu);
             st[j] = (SMODE == 2 && live[j]) ? P.stencil[idx] : 0u;
         }
         cnt[j] = 0;
@@ -281,6 +306,14 @@ __device__ __forceinline__ void raster_tile(const RasterParams& P, uint32_t tile
     };
 
 
+    // Hierarchical depth (draws without stencil effects): hz_key is a lower
+    // bound of the depth keys of this wave's pixels, refreshed every
+    // SR_HIZ_PERIOD triangles (depths only grow: a fragment passes with
+    // z >= its pixel's depth).  A triangle whose depth bound key lies below
+    // it cannot pass the depth test at any of these pixels, and then
+    // changes nothing here: it is skipped.
+    uint32_t hz_key = 0u, hz_n = 1u;
+
     // One draw's items [sb, se): the screen tiles meeting this wave's pixels
     // in the reference's row-major order, each walking the items in order.
     auto walk = [&](uint32_t sb, uint32_t se, const RasterParams& PD) {
@@ -317,18 +350,25 @@ __device__ __forceinline__ void raster_tile(const RasterParams& P, uint32_t tile
                     if constexpr (CLAMP) {
                         if (X0 > X1) return;  // culled or invalid (binned draws never see these)
                     }
+                    if constexpr (SMODE == 0) {
+                        if (--hz_n == 0u) {
+                            hz_n = SR_HIZ_PERIOD;
+                            hz_key = wave_min_key<PPT>(dep);
+                        }
+                        if ((rc.zs | 1u) < hz_key) return;  // behind every pixel here
+                    }
                     // The pixels tested: the box clamped into this screen
                     // tile (clamp_as_int!, triangle.rs:66-78) in the
                     // exhaustive mode and for a triangle whose record says a
                     // pixel outside its box may pass (binned to every tile
-                    // such a pixel may lie in; rc.strip is wave-uniform, so
+                    // such a pixel may lie in; rc.zs is wave-uniform, so
                     // this is a scalar branch); the box for stencil
                     // effects; else every pixel: in a binned draw no pixel
                     // centre outside the box of any other triangle can pass
                     // the coverage test, and inside the box the test is the
                     // reference's.  Stencil counting counts the box itself
                     // (the strips outside it are counted apart).
-                    const bool clampit = CLAMP || rc.strip != 0u;
+                    const bool clampit = CLAMP || (rc.zs & 1u) != 0u;
                     int cx0 = X0, cx1 = X1, cy0 = Y0, cy1 = Y1;
                     if (clampit) {
                         cx0 = min(max(X0, Tx0), Tx1); cx1 = min(max(X1, Tx0), Tx1);

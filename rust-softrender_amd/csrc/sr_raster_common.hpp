@@ -46,7 +46,7 @@ struct RecLane {
     double R;
     float x3, y3, e23y, e23x, e31y, e31x, z1, z2, z3;
     int X0, X1, Y0, Y1;
-    uint32_t strip;
+    uint32_t zs;  // bit 0: strip flag; the rest: depth bound key (TriRec::zs)
 };
 
 __device__ __forceinline__ uint32_t lane_u32(uint32_t v, int k) {
@@ -68,12 +68,12 @@ __device__ __forceinline__ RecLane rec_s(const TriRec* tris, uint32_t id) {
     r.e23y = p->e23y; r.e23x = p->e23x; r.e31y = p->e31y; r.e31x = p->e31x;
     r.z1 = p->z1; r.z2 = p->z2; r.z3 = p->z3;
     r.X0 = p->X0; r.X1 = p->X1; r.Y0 = p->Y0; r.Y1 = p->Y1;
-    r.strip = p->strip;
+    r.zs = p->zs;
     return r;
 #else
     const TriRec& t = tris[id];
     return RecLane{t.R, t.x3, t.y3, t.e23y, t.e23x, t.e31y, t.e31x, t.z1, t.z2, t.z3, t.X0, t.X1, t.Y0, t.Y1,
-                   t.strip};
+                   t.zs};
 #endif
 }
 

@@ -79,7 +79,7 @@ __global__ __launch_bounds__(64) void k_raster_fmt(RasterParams P, FmtParams F) 
                 const uint32_t id = load_item(P.items, it);
                 const RecLane rc = rec_s(P.tris, id);
                 int cx0 = rc.X0, cx1 = rc.X1, cy0 = rc.Y0, cy1 = rc.Y1;
-                if (P.exhaustive || rc.strip) {  // every box, or one a pixel outside may pass
+                if (P.exhaustive || (rc.zs & 1u)) {  // every box, or one a pixel outside may pass
                     if (rc.X0 > rc.X1) continue;  // culled or invalid
                     // clamp_as_int! into the screen tile, triangle.rs:66-78
                     cx0 = min(max(rc.X0, Tx0), Tx1); cx1 = min(max(rc.X1, Tx0), Tx1);

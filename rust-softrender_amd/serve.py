@@ -237,8 +237,12 @@ def run(rates=(200, 500, 1000, 2000), duration=10.0, n_meshes=64, capacity_n=512
     reqs = scenes.serve_requests(capacity_n, n_meshes=n_meshes, seed=seed + 0x20000)
     zero = np.zeros(capacity_n)
     srv = sr.Server(ctx, **cfg)
-    run_served(srv, world, reqs[:64], zero[:64])  # warm: pools and workspaces
-    res["capacity_served"] = run_served(srv, world, reqs, zero)
+    # warm with the whole set: the first pass at this depth grows the frame,
+    # result and workspace pools (allocations that a steady server has made)
+    run_served(srv, world, reqs, zero)
+    runs = sorted((run_served(srv, world, reqs, zero) for _ in range(3)), key=lambda r: r["fps"])
+    res["capacity_served"] = runs[1]  # the median of three passes
+    res["capacity_served"]["fps_runs"] = [r["fps"] for r in runs]
     print(json.dumps({"capacity_served_fps": res["capacity_served"]["fps"]}), file=sys.stderr, flush=True)
     if plain:
         run_plain(ctx, world, reqs[:16], zero[:16])
