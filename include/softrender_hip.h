@@ -63,6 +63,14 @@ extern "C" {
  * world-space normal n' = normalize(N * n). */
 #define SR_VP_MVP_NORMAL_UV 0    /* varyings: nx ny nz u v        (K=5) */
 #define SR_VP_MVP_NORMAL_WORLD 1 /* varyings: nx ny nz wx wy wz   (K=6) */
+/* The full example's vertex shader (full_example/src/shaders.rs:8-31):
+ * clip = MVP * (p,1); varyings the world position model * (p,1) and the
+ * world normal normalize(mit * (n,0)), both as 4-vectors (K=8: px py pz pw
+ * nx ny nz nw).  model is vs_uniforms[25..37) (rows 0-2) and [40..44) (row
+ * 3); mit (the model matrix's inverse transpose) is [16..25) (rows 0-2,
+ * columns 0-2) and [37..40) (row 3, columns 0-2).  The 4-vector norm sums
+ * ((x*x + y*y) + z*z) + w*w.  No geometry program takes it. */
+#define SR_VP_WORLD4_NORMAL4 2
 
 /* Geometry programs (GeometryShader::run, geometry.rs:132-258). */
 #define SR_GS_NONE 0           /* run_to_fragment / finish without a program */
@@ -95,6 +103,19 @@ extern "C" {
  * A framebuffer with fewer planes keeps the first ones; every other
  * program writes its one colour to every plane. */
 #define SR_FP_GBUFFER 4
+/* The full example's fragment shader (full_example/src/shaders.rs:92-163,
+ * color.rs:19-56): four point lights with Fresnel-Schlick (ior 1.45)
+ * blended Lambert diffuse (albedo 0.7) and Blinn-Phong specular (NdotH^64),
+ * inverse-square falloff, on a grey material decode_gamma(g, 2.2); then
+ * ACES filmic tone mapping (A 2.51, B 0.03, C 2.43, D 0.59, E 0.14) and
+ * encode_gamma(1/2.2).  Alpha 1.  Needs SR_VP_WORLD4_NORMAL4.
+ * fs_uniforms: [0..3) camera position (w = 1); light i (0..3) at
+ * [3+7i, 10+7i): position xyz (w = 1), colour rgb, intensity; [31] g.
+ * Vectors are 4-vectors as in the example (dot and norm over 4
+ * components); powi(x, n) multiplies by repeated squaring (x^5 = x * x^4).
+ * Transcendentals (powf, sqrtf) are the platform's: colours agree with the
+ * oracle within a few ulps, not bit for bit. */
+#define SR_FP_FRESNEL4 5
 
 /* Blend (color/blend.rs:9-74): `()` replaces, OVER is src-alpha over,
  * OVER_NORMALIZED the full example's blend (full_example/src/color.rs:5-17):
@@ -150,6 +171,15 @@ extern "C" {
 /* texture.rs:20-40 */
 #define SR_FILTER_NEAREST 0
 #define SR_FILTER_BILINEAR 1
+/* The full example's sampler (full_example/src/texture.rs:29-84): texel
+ * coordinates u * (W - 1) (nearest: rounded half away from zero; bilinear:
+ * + 0.5, floor, the next texel weighted by the fraction), edges Clamp
+ * (u, v clamped to [0, 1]) or Wrap (u - trunc(u)); SR_EDGE_BORDER acts as
+ * Clamp.  The texel is then decode_gamma(2.2)'d (r, g, b = c^2.2).  Texel
+ * indices past the last texel, where the example would panic, use the
+ * last.  Per-pixel raster (as RGBAF32 textures are). */
+#define SR_FILTER_SPAN_NEAREST 2
+#define SR_FILTER_SPAN_BILINEAR 3
 #define SR_EDGE_CLAMP 0
 #define SR_EDGE_WRAP 1
 #define SR_EDGE_BORDER 2

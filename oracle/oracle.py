@@ -102,6 +102,8 @@ def load(path: str):
     L.or_gs_normal_lines.argtypes = [C.c_int] + [fp] * 6 + [C.c_int, fp, fp, fp, C.POINTER(C.c_int)]
     L.or_vertex.argtypes = [C.POINTER(DrawDesc), C.POINTER(OrMesh), C.c_uint32, fp, fp]
     L.or_vertex.restype = C.c_int
+    L.or_fragment.argtypes = [C.POINTER(DrawDesc), fp, C.POINTER(OrTexture), fp]
+    L.or_fragment.restype = None
     return L
 
 
@@ -201,6 +203,25 @@ def vertex(mesh: dict, desc: DrawDesc, i: int, L=None):
     fp = C.POINTER(C.c_float)
     K = (L or lib()).or_vertex(C.byref(desc), C.byref(om), i, clip.ctypes.data_as(fp), k.ctypes.data_as(fp))
     return clip, k[:K].copy()
+
+
+def fragment(desc: DrawDesc, k, tex=None, L=None):
+    """Output 0 of the fragment program on varyings k: RGBA floats (4,)."""
+    k = np.ascontiguousarray(np.r_[np.asarray(k, np.float32), np.zeros(8, np.float32)][:8])
+    ot, keep = _texture(tex)
+    out = np.zeros(4, np.float32)
+    fp = C.POINTER(C.c_float)
+    (L or lib()).or_fragment(C.byref(desc), k.ctypes.data_as(fp), C.byref(ot) if ot is not None else None,
+                             out.ctypes.data_as(fp))
+    return out
+
+
+def sample(tex, u: float, v: float, filter: int, edge: int, border: int = 0, L=None):
+    """or_sample: RGBA floats (4,)."""
+    ot, keep = _texture(tex)
+    out = np.zeros(4, np.float32)
+    (L or lib()).or_sample(C.byref(ot), u, v, filter, edge, border, out.ctypes.data_as(C.POINTER(C.c_float)))
+    return out
 
 
 def draw(fb: Framebuffer, mesh: dict, desc: DrawDesc, tex=None, nthreads: int = 1, L=None):

@@ -66,7 +66,21 @@ static inline void normalize3(float* v) {
     v[2] = v[2] / len;
 }
 
-static int vp_varyings(int vp) { return vp == SR_VP_MVP_NORMAL_WORLD ? 6 : 5; }
+static int vp_varyings(int vp) {
+    return vp == SR_VP_WORLD4_NORMAL4 ? 8 : (vp == SR_VP_MVP_NORMAL_WORLD ? 6 : 5);
+}
+
+/* 4-vectors of the full example (nalgebra Vector4<f32>): dot sums from +0
+ * in index order; norm = sqrt(dot(v, v)); normalize = v / norm. */
+static inline float dot4(const float* a, const float* b) {
+    float acc = 0.0f;
+    for (int c = 0; c < 4; c++) acc = acc + a[c] * b[c];
+    return acc;
+}
+static inline void normalize4(float* v) {
+    float n = sqrtf(dot4(v, v));
+    for (int c = 0; c < 4; c++) v[c] = v[c] / n;
+}
 
 static void vs_run(const sr_draw_desc* d, const or_mesh* m, uint32_t i, ovtx* o) {
     const float* U = d->vs_uniforms;
@@ -77,6 +91,28 @@ static void vs_run(const sr_draw_desc* d, const or_mesh* m, uint32_t i, ovtx* o)
         float acc = 0.0f;
         for (int k = 0; k < 4; k++) acc = acc + U[r * 4 + k] * pin[k];
         o->p[r] = acc;
+    }
+    if (d->vertex_program == SR_VP_WORLD4_NORMAL4) {
+        /* full_example/src/shaders.rs:8-31: world_position = model * p and
+         * normal = (mit * n.to_homogeneous()).normalize(), 4-vectors; model
+         * rows 0-2 at U[25..37), row 3 at U[40..44); mit rows 0-2 (columns
+         * 0-2) at U[16..25), row 3 at U[37..40) (column 3 meets w = 0). */
+        for (int r = 0; r < 4; r++) {
+            const float* row = r < 3 ? U + 25 + r * 4 : U + 40;
+            float acc = 0.0f;
+            for (int k = 0; k < 4; k++) acc = acc + row[k] * pin[k];
+            o->k[r] = acc;
+        }
+        float n4[4];
+        for (int r = 0; r < 4; r++) {
+            const float* row = r < 3 ? U + 16 + r * 3 : U + 37;
+            float acc = 0.0f;
+            for (int k = 0; k < 3; k++) acc = acc + row[k] * nin[k];
+            n4[r] = acc;
+        }
+        normalize4(n4);
+        for (int c = 0; c < 4; c++) o->k[4 + c] = n4[c];
+        return;
     }
     float n[3];
     for (int r = 0; r < 3; r++) {
@@ -383,9 +419,66 @@ static void texel(const or_texture* t, int ix, int iy, int edge, uint32_t border
 
 static inline float clampf(float x, float lo, float hi) { return fminf(fmaxf(x, lo), hi); }
 
+/* float -> u32 as Rust's `as` casts it: NaN and negatives to 0, saturating. */
+static inline uint32_t f2u_sat(float f) {
+    if (!(f > 0.0f)) return 0u;
+    if (f >= 4294967296.0f) return 0xFFFFFFFFu;
+    return (uint32_t)f;
+}
+
+/* RgbaImage::get_pixel(x, y) as a Color (full_example/src/texture.rs:38-45:
+ * c / 255; RGBAF32 texels as stored).  get_pixel panics past the image;
+ * here such an index takes the last texel. */
+static void span_texel(const or_texture* t, uint32_t x, uint32_t y, float o[4]) {
+    if (x > t->w - 1) x = t->w - 1;
+    if (y > t->h - 1) y = t->h - 1;
+    size_t i = (size_t)y * t->w + x;
+    if (!t->rgba) {
+        for (int c = 0; c < 4; c++) o[c] = t->rgbaf[4 * i + c];
+        return;
+    }
+    uint32_t w = t->rgba[i];
+    for (int c = 0; c < 4; c++) o[c] = unorm8(w >> (8 * c));
+}
+
+/* The full example's Texture::sample (full_example/src/texture.rs:47-84),
+ * EdgeBehavior::edge (texture.rs:27-34: Clamp u.min(1).max(0), Wrap
+ * u.fract() = u - trunc(u)), then decode_gamma(texel, SRGB_GAMMA)
+ * (color.rs:47-56: r, g, b = c.powf(2.2)). */
+static void sample_span(const or_texture* t, float u, float v, int filter, int edge, float out[4]) {
+    if (edge == SR_EDGE_WRAP) {
+        u = u - truncf(u);
+        v = v - truncf(v);
+    } else {
+        u = fmaxf(fminf(u, 1.0f), 0.0f);
+        v = fmaxf(fminf(v, 1.0f), 0.0f);
+    }
+    const float w1 = (float)(t->w - 1), h1 = (float)(t->h - 1);
+    if (filter == SR_FILTER_SPAN_NEAREST) {  /* texture.rs:51-56 */
+        span_texel(t, f2u_sat(roundf(u * w1)), f2u_sat(roundf(v * h1)), out);
+    } else {                                  /* texture.rs:57-80 */
+        const float uu = (u * w1) + 0.5f, vv = (v * h1) + 0.5f;
+        const uint32_t x = f2u_sat(floorf(uu)), y = f2u_sat(floorf(vv));
+        const float ur = uu - (float)x, vr = vv - (float)y;
+        const float uo = 1.0f - ur, vo = 1.0f - vr;
+        float t00[4], t10[4], t01[4], t11[4];
+        span_texel(t, x, y, t00);
+        span_texel(t, x + 1, y, t10);
+        span_texel(t, x, y + 1, t01);
+        span_texel(t, x + 1, y + 1, t11);
+        for (int c = 0; c < 4; c++)
+            out[c] = (((t00[c] * uo) + (t10[c] * ur)) * vo) + (((t01[c] * uo) + (t11[c] * ur)) * vr);
+    }
+    for (int c = 0; c < 3; c++) out[c] = powf(out[c], 2.2f);
+}
+
 /* Texel centres at (i+0.5)/W (GL convention); Filter/Edge per texture.rs:20-40. */
 void or_sample(const or_texture* t, float u, float v, int filter, int edge, uint32_t border,
                float out[4]) {
+    if (filter >= SR_FILTER_SPAN_NEAREST) {
+        sample_span(t, u, v, filter, edge, out);
+        return;
+    }
     if (filter == SR_FILTER_NEAREST) {
         float fx = clampf(u * (float)t->w, -1.0e8f, 1.0e8f);
         float fy = clampf(v * (float)t->h, -1.0e8f, 1.0e8f);
@@ -408,6 +501,60 @@ void or_sample(const or_texture* t, float u, float v, int filter, int edge, uint
         float bot = (t01[c] * bx) + (t11[c] * ax);
         out[c] = (top * by) + (bot * ay);
     }
+}
+
+/* powi by repeated squaring (r *= x for each set bit, x squared between:
+ * x^5 = x * x^4), the order compiler-rt's __powisf2 multiplies in. */
+static float powi_sq(float x, int n) {
+    float r = 1.0f;
+    for (;;) {
+        if (n & 1) r = r * x;
+        n >>= 1;
+        if (n == 0) break;
+        x = x * x;
+    }
+    return r;
+}
+static inline float saturate1(float x) { return x < 0.0f ? 0.0f : (x > 1.0f ? 1.0f : x); } /* shaders.rs:101-103 */
+
+/* The full example's fragment_shader (full_example/src/shaders.rs:105-161)
+ * with fresnel_schlick (shaders.rs:94-99), aces_filmic_tonemap
+ * (color.rs:19-38) and encode_gamma (color.rs:40-47).  k = world position
+ * (4) then world normal (4); U: camera xyz, four lights (position xyz,
+ * colour rgb, intensity) at U[3 + 7i], U[31] the grey of the material. */
+static void fresnel4(const float* U, const float* k, float out[4]) {
+    const float* pos = k;
+    const float* nrm = k + 4;
+    float view[4] = {U[0] - pos[0], U[1] - pos[1], U[2] - pos[2], 1.0f - pos[3]};
+    normalize4(view);
+    const float mat = powf(U[31], 2.2f);  /* decode_gamma(Color {0.25, ..}, SRGB_GAMMA) */
+    const float albedo = 0.7f, ior = 1.45f;
+    float col[3] = {0.0f, 0.0f, 0.0f};
+    for (int l = 0; l < 4; l++) {
+        const float* L = U + 3 + 7 * l;
+        float d[4] = {L[0] - pos[0], L[1] - pos[1], L[2] - pos[2], 1.0f - pos[3]};
+        const float dist = sqrtf(dot4(d, d));
+        normalize4(d);
+        float h[4] = {d[0] + view[0], d[1] + view[1], d[2] + view[2], d[3] + view[3]};
+        normalize4(h);
+        const float intensity = L[6] / powi_sq(dist, 2);
+        const float ndl = saturate1(dot4(d, nrm));
+        const float ndh = saturate1(dot4(nrm, h));
+        const float vdh = saturate1(dot4(view, h));
+        const float f0 = powi_sq((1.0f - ior) / (1.0f + ior), 2);
+        const float f = f0 + (1.0f - f0) * powi_sq(1.0f - vdh, 5);
+        const float diffuse = (1.0f - f) * ndl;
+        const float specular = f * powi_sq(ndh, 64);  /* shininess * 2 */
+        for (int c = 0; c < 3; c++)
+            col[c] = col[c] + (intensity * L[3 + c]) * (specular + ((diffuse * albedo) * mat));
+    }
+    const float inv_gamma = 1.0f / 2.2f;
+    for (int c = 0; c < 3; c++) {
+        const float x = col[c];
+        const float t = (x * ((2.51f * x) + 0.03f)) / ((x * ((2.43f * x) + 0.59f)) + 0.14f);
+        out[c] = powf(t, inv_gamma);
+    }
+    out[3] = 1.0f;
 }
 
 /* Fragment programs; k = interpolated varyings.  Each program's colours
@@ -457,10 +604,21 @@ static int fs_run_f(const sr_draw_desc* d, const float* k, const or_texture* tex
         case SR_FP_UV_COLOR:
             out[0][0] = k[3]; out[0][1] = k[4]; out[0][2] = U[0]; out[0][3] = U[1];
             return 1;
+        case SR_FP_FRESNEL4:
+            fresnel4(U, k, out[0]);
+            return 1;
         default:
             out[0][0] = U[0]; out[0][1] = U[1]; out[0][2] = U[2]; out[0][3] = U[3];
             return 1;
     }
+}
+
+/* Output 0 of the fragment program on varyings k (RGBA floats before any
+ * quantisation or blend): the programs on their own, for the tests. */
+void or_fragment(const sr_draw_desc* d, const float* k, const or_texture* tex, float out[4]) {
+    float o[4][4];
+    fs_run_f(d, k, tex, o);
+    for (int c = 0; c < 4; c++) out[c] = o[0][c];
 }
 
 static inline uint32_t pack_q(const float c[4]) {

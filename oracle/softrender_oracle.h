@@ -83,6 +83,8 @@ void or_fb_clear(or_fb* fb, uint32_t rgba, float depth, uint8_t stencil);
  * (clip space, before the viewport transform); returns K, the number of
  * varyings written to k. */
 int or_vertex(const sr_draw_desc* d, const or_mesh* m, uint32_t i, float clip[4], float* k);
+/* Output 0 of d's fragment program on varyings k (RGBA floats). */
+void or_fragment(const sr_draw_desc* d, const float* k, const or_texture* tex, float out[4]);
 /* ClipVertex::normalize on a bare position. out = screen x,y,z,w */
 void or_normalize(const float clip[4], const sr_viewport* vp, float out[4]);
 /* clip_primitives on one triangle with K varyings per vertex (vertex layout:

@@ -122,9 +122,70 @@ __device__ __forceinline__ uint32_t texel_u8(const RasterParams& P, int i) {
 // ANY_FMT = false: RGBA8 textures only (the triangle raster, whose draws
 // with an RGBAF32 texture take the per-pixel raster instead: the float
 // path costs it a wave per SIMD).
+// float -> u32 as Rust's `as` casts it: NaN and negatives to 0, saturating.
+__device__ __forceinline__ uint32_t f2u_sat(float f) {
+    if (!(f > 0.0f)) return 0u;
+    if (f >= 4294967296.0f) return 0xFFFFFFFFu;
+    return (uint32_t)f;
+}
+
+// Texel (x, y) of the full example's sampler: RGBA8 as c / 255, RGBAF32 as
+// stored; indices past the last texel (a panic there) use the last.
+__device__ __forceinline__ void span_texel(const RasterParams& P, uint32_t x, uint32_t y, Lut lut, float o[4]) {
+    x = min(x, P.texW - 1u);
+    y = min(y, P.texH - 1u);
+    const uint32_t i = y * P.texW + x;
+    if (P.tex_fmt == SR_COLOR_RGBAF32) {
+#pragma unroll
+        for (int c = 0; c < 4; c++) o[c] = ((const float*)P.tex)[4 * i + c];
+    } else {
+        const uint32_t w = P.tex[i];
+#pragma unroll
+        for (int c = 0; c < 4; c++) o[c] = lut(w >> (8 * c));
+    }
+}
+
+// The full example's Texture::sample (full_example/src/texture.rs:29-84)
+// with its decode_gamma (color.rs:49-56, gamma 2.2), in Rust's evaluation
+// order.
+__device__ __forceinline__ void sample_span(const RasterParams& P, float u, float v, Lut lut, float out[4]) {
+    if (P.tex_edge == SR_EDGE_WRAP) {  // f32::fract
+        u = u - truncf(u);
+        v = v - truncf(v);
+    } else {  // u.min(1.0).max(0.0)
+        u = fmaxf(fminf(u, 1.0f), 0.0f);
+        v = fmaxf(fminf(v, 1.0f), 0.0f);
+    }
+    const float w1 = (float)(P.texW - 1u), h1 = (float)(P.texH - 1u);
+    if (P.tex_filter == SR_FILTER_SPAN_NEAREST) {
+        span_texel(P, f2u_sat(roundf(u * w1)), f2u_sat(roundf(v * h1)), lut, out);
+    } else {
+        const float uu = (u * w1) + 0.5f, vv = (v * h1) + 0.5f;
+        const uint32_t x = f2u_sat(floorf(uu)), y = f2u_sat(floorf(vv));
+        const float ur = uu - (float)x, vr = vv - (float)y;
+        const float uo = 1.0f - ur, vo = 1.0f - vr;
+        float t00[4], t10[4], t01[4], t11[4];
+        span_texel(P, x, y, lut, t00);
+        span_texel(P, x + 1u, y, lut, t10);
+        span_texel(P, x, y + 1u, lut, t01);
+        span_texel(P, x + 1u, y + 1u, lut, t11);
+#pragma unroll
+        for (int c = 0; c < 4; c++)
+            out[c] = (((t00[c] * uo) + (t10[c] * ur)) * vo) + (((t01[c] * uo) + (t11[c] * ur)) * vr);
+    }
+#pragma unroll
+    for (int c = 0; c < 3; c++) out[c] = powf(out[c], 2.2f);
+}
+
 template <bool ANY_FMT = true>
 __device__ __forceinline__ void sample(const RasterParams& P, float u, float v, Lut lut,
                                        float out[4]) {
+    if constexpr (ANY_FMT) {
+        if (P.tex_filter >= SR_FILTER_SPAN_NEAREST) {
+            sample_span(P, u, v, lut, out);
+            return;
+        }
+    }
     const bool f32 = ANY_FMT && P.tex_fmt == SR_COLOR_RGBAF32;
     if (P.tex_filter == SR_FILTER_NEAREST) {
         float fx = clampf(u * (float)P.texW, -1.0e8f, 1.0e8f);
@@ -165,6 +226,74 @@ __device__ __forceinline__ void sample(const RasterParams& P, float u, float v, 
         for (int c = 0; c < 4; c++)
             out[c] = mix(lut(w00 >> (8 * c)), lut(w10 >> (8 * c)), lut(w01 >> (8 * c)), lut(w11 >> (8 * c)));
     }
+}
+
+// 4-vectors of the full example (nalgebra Vector4<f32>): dot sums from +0
+// in index order, norm = sqrt(dot(v, v)), normalize divides each component.
+__device__ __forceinline__ float dot4(const float a[4], const float b[4]) {
+    float acc = 0.0f;
+#pragma unroll
+    for (int c = 0; c < 4; c++) acc = acc + a[c] * b[c];
+    return acc;
+}
+__device__ __forceinline__ void normalize4(float v[4]) {
+    const float n = sqrtf(dot4(v, v));
+#pragma unroll
+    for (int c = 0; c < 4; c++) v[c] = v[c] / n;
+}
+// powi by repeated squaring: r *= x for each set bit, x squared between.
+template <int N>
+__device__ __forceinline__ float powi_sq(float x) {
+    float r = 1.0f;
+    int n = N;
+#pragma unroll
+    for (int i = 0; i < 8; i++) {
+        if (n & 1) r = r * x;
+        n >>= 1;
+        if (n == 0) break;
+        x = x * x;
+    }
+    return r;
+}
+__device__ __forceinline__ float saturate1(float x) { return x < 0.0f ? 0.0f : (x > 1.0f ? 1.0f : x); }
+
+// The full example's fragment_shader (full_example/src/shaders.rs:92-163)
+// with aces_filmic_tonemap and encode_gamma (color.rs:19-47): k = world
+// position (4) then world normal (4).
+__device__ __forceinline__ void fresnel4(const float* U, const float* k, float out[4]) {
+    const float pos[4] = {k[0], k[1], k[2], k[3]}, nrm[4] = {k[4], k[5], k[6], k[7]};
+    float view[4] = {U[0] - pos[0], U[1] - pos[1], U[2] - pos[2], 1.0f - pos[3]};
+    normalize4(view);
+    const float mat = powf(U[31], 2.2f);  // decode_gamma of the grey material
+    const float albedo = 0.7f, ior = 1.45f;
+    float col[3] = {0.0f, 0.0f, 0.0f};
+#pragma unroll
+    for (int l = 0; l < 4; l++) {
+        const float* L = U + 3 + 7 * l;
+        float d[4] = {L[0] - pos[0], L[1] - pos[1], L[2] - pos[2], 1.0f - pos[3]};
+        const float dist = sqrtf(dot4(d, d));
+        normalize4(d);  // light_dir
+        float h[4] = {d[0] + view[0], d[1] + view[1], d[2] + view[2], d[3] + view[3]};
+        normalize4(h);
+        const float intensity = L[6] / powi_sq<2>(dist);
+        const float ndl = saturate1(dot4(d, nrm));
+        const float ndh = saturate1(dot4(nrm, h));
+        const float vdh = saturate1(dot4(view, h));
+        const float f0 = powi_sq<2>((1.0f - ior) / (1.0f + ior));
+        const float f = f0 + (1.0f - f0) * powi_sq<5>(1.0f - vdh);
+        const float diffuse = (1.0f - f) * ndl;
+        const float specular = f * powi_sq<64>(ndh);
+#pragma unroll
+        for (int c = 0; c < 3; c++) col[c] = col[c] + (intensity * L[3 + c]) * (specular + ((diffuse * albedo) * mat));
+    }
+    const float inv_gamma = 1.0f / 2.2f;
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+        const float x = col[c];
+        const float t = (x * ((2.51f * x) + 0.03f)) / ((x * ((2.43f * x) + 0.59f)) + 0.14f);
+        out[c] = powf(t, inv_gamma);
+    }
+    out[3] = 1.0f;
 }
 
 // Fragment programs (the closure passed to FragmentShader::run,
@@ -211,6 +340,9 @@ __device__ __forceinline__ int fragment_rgba(const RasterParams& P, const float*
     } else if constexpr (FP == SR_FP_UV_COLOR) {
         o[0][0] = k[3]; o[0][1] = k[4]; o[0][2] = U[0]; o[0][3] = U[1];
         return 1;
+    } else if constexpr (FP == SR_FP_FRESNEL4) {
+        fresnel4(U, k, o[0]);
+        return 1;
     } else {
         o[0][0] = U[0]; o[0][1] = U[1]; o[0][2] = U[2]; o[0][3] = U[3];
         return 1;
@@ -231,8 +363,9 @@ __device__ __forceinline__ uint32_t fragment(const RasterParams& P, const float*
 
 template <int FP>
 constexpr int fp_varyings() {
-    return FP == SR_FP_BLINN_PHONG ? 6
-                                   : (FP == SR_FP_TEX_LAMBERT || FP == SR_FP_UV_COLOR || FP == SR_FP_GBUFFER) ? 5 : 0;
+    return FP == SR_FP_FRESNEL4     ? 8
+           : FP == SR_FP_BLINN_PHONG ? 6
+           : (FP == SR_FP_TEX_LAMBERT || FP == SR_FP_UV_COLOR || FP == SR_FP_GBUFFER) ? 5 : 0;
 }
 
 // Blend::blend(src, dst) on RGBA floats (RGBAF32 planes): `()` returns

@@ -51,7 +51,7 @@ __device__ __forceinline__ void normalize_pos(float x, float y, float z, float w
 __device__ __forceinline__ void vertex_compute(const float* U, int program, int normalize,
                                                const sr_viewport& viewport, const float pin[4],
                                                const float nin[3], float u, float v, float outp[4],
-                                               float var[6]) {
+                                               float var[MAXK]) {
     float clip[4];
 #pragma unroll
     for (int r = 0; r < 4; r++) {
@@ -59,6 +59,39 @@ __device__ __forceinline__ void vertex_compute(const float* U, int program, int 
 #pragma unroll
         for (int k = 0; k < 4; k++) acc = acc + U[r * 4 + k] * pin[k];
         clip[r] = acc;
+    }
+    if (program == SR_VP_WORLD4_NORMAL4) {
+        // the full example's vertex_shader (shaders.rs:8-31): model * (p,1)
+        // and normalize(mit * (n,0)) as 4-vectors
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+            const float* row = r < 3 ? U + 25 + r * 4 : U + 40;
+            float acc = 0.0f;
+#pragma unroll
+            for (int k = 0; k < 4; k++) acc = acc + row[k] * pin[k];
+            var[r] = acc;
+        }
+        float n4[4];
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+            const float* row = r < 3 ? U + 16 + r * 3 : U + 37;
+            float acc = 0.0f;
+#pragma unroll
+            for (int k = 0; k < 3; k++) acc = acc + row[k] * nin[k];
+            n4[r] = acc;
+        }
+        float len = 0.0f;
+#pragma unroll
+        for (int c = 0; c < 4; c++) len = len + n4[c] * n4[c];
+        len = sqrtf(len);
+#pragma unroll
+        for (int c = 0; c < 4; c++) var[4 + c] = n4[c] / len;
+        if (normalize) {
+            normalize_pos(clip[0], clip[1], clip[2], clip[3], viewport, outp);
+        } else {
+            outp[0] = clip[0]; outp[1] = clip[1]; outp[2] = clip[2]; outp[3] = clip[3];
+        }
+        return;
     }
     float n[3];
 #pragma unroll
@@ -106,10 +139,10 @@ __global__ __launch_bounds__(256) void k_finish(VertexBuf in, VertexBuf out, uin
 }
 
 // The varying rows of a block's vertices [i0, i0 + n) (n <= 256, row
-// stride out.kstride <= 6), staged in LDS so that consecutive threads store
+// stride out.kstride <= MAXK), staged in LDS so that consecutive threads store
 // consecutive words.  Every thread of the block calls it.
 __device__ __forceinline__ void store_rows(float* lds, const VertexBuf& out, uint32_t i0, uint32_t n,
-                                           const float var[6], bool live) {
+                                           const float var[MAXK], bool live) {
     const uint32_t K = out.kstride;
     if (live)
         for (uint32_t c = 0; c < K; c++) lds[threadIdx.x * K + c] = var[c];
@@ -122,10 +155,10 @@ __device__ __forceinline__ void store_rows(float* lds, const VertexBuf& out, uin
 // streams; outputs are the SoA position streams (x, y, z, w) and the
 // per-vertex varying rows.
 __global__ __launch_bounds__(256) void k_vertex(VSParams P) {
-    __shared__ float rows[256 * 6];
+    __shared__ float rows[256 * MAXK];
     const uint32_t b0 = blockIdx.x * 256u, i = b0 + threadIdx.x;
     const bool live = i < P.n;
-    float var[6] = {};
+    float var[MAXK] = {};
     if (live) {
         const float pin[4] = {P.px[i], P.py[i], P.pz[i], 1.0f};
         const float nin[3] = {P.nx ? P.nx[i] : 0.0f, P.ny ? P.ny[i] : 0.0f, P.nz ? P.nz[i] : 0.0f};
@@ -204,9 +237,9 @@ __device__ __forceinline__ void with_scene(const BatchTables& T, const uint32_t*
 }
 
 __global__ __launch_bounds__(256) void k_vertex_batch(BatchTables T, VertexBuf out) {
-    __shared__ float rows[256 * 6];
+    __shared__ float rows[256 * MAXK];
     const uint32_t b0 = blockIdx.x * 256u, i = b0 + threadIdx.x;
-    float var[6] = {};
+    float var[MAXK] = {};
     bool live = false;
     with_scene(T, T.vend, T.nv, i, [&](const SceneGeom& S) {
         const uint32_t l = i - S.vbase;

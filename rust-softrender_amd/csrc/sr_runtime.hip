@@ -91,7 +91,7 @@ int bits_for(uint32_t n) {
     return b;
 }
 
-int varyings_of(int vp) { return vp == SR_VP_MVP_NORMAL_WORLD ? 6 : 5; }
+int varyings_of(int vp) { return vp == SR_VP_WORLD4_NORMAL4 ? 8 : (vp == SR_VP_MVP_NORMAL_WORLD ? 6 : 5); }
 
 int policy_tile(sr_context ctx, uint32_t w, uint32_t h) {
     const uint64_t px = (uint64_t)w * h;
@@ -426,14 +426,23 @@ int draw_points_impl(sr_context ctx, sr_framebuffer fb, const DrawPlan& pl, cons
 int sr_rt::validate_draw(sr_context ctx, sr_framebuffer fb, sr_mesh mesh, const sr_draw_desc* d, sr_texture tex) {
     if (!ctx || !fb || !mesh || !d) return fail(ctx, SR_ERR_INVALID_ARGUMENT, "draw: null argument");
     if (fb->ctx != ctx || mesh->ctx != ctx) return fail(ctx, SR_ERR_INVALID_ARGUMENT, "draw: handle from another context");
-    if (d->vertex_program != SR_VP_MVP_NORMAL_UV && d->vertex_program != SR_VP_MVP_NORMAL_WORLD)
+    if (d->vertex_program < SR_VP_MVP_NORMAL_UV || d->vertex_program > SR_VP_WORLD4_NORMAL4)
         return fail(ctx, SR_ERR_INVALID_ARGUMENT, "draw: unknown vertex program");
+    if (d->vertex_program == SR_VP_WORLD4_NORMAL4 && d->geometry_program != SR_GS_NONE)
+        return fail(ctx, SR_ERR_INVALID_ARGUMENT, "draw: no geometry program takes the 4-vector varyings");
+    if (d->fragment_program == SR_FP_FRESNEL4 && d->vertex_program != SR_VP_WORLD4_NORMAL4)
+        return fail(ctx, SR_ERR_INVALID_ARGUMENT, "draw: the Fresnel program needs the 4-vector varyings");
+    if (d->vertex_program == SR_VP_WORLD4_NORMAL4 && d->fragment_program != SR_FP_FRESNEL4 &&
+        d->fragment_program != SR_FP_FLAT)
+        return fail(ctx, SR_ERR_INVALID_ARGUMENT, "draw: the 4-vector varyings feed the Fresnel and flat programs only");
+    if (d->tex_filter < SR_FILTER_NEAREST || d->tex_filter > SR_FILTER_SPAN_BILINEAR)
+        return fail(ctx, SR_ERR_INVALID_ARGUMENT, "draw: unknown texture filter");
     if (d->geometry_program < SR_GS_NONE || d->geometry_program > SR_GS_FACE_NORMALS)
         return fail(ctx, SR_ERR_INVALID_ARGUMENT, "draw: unknown geometry program");
     if ((d->geometry_program == SR_GS_VERTEX_NORMALS || d->geometry_program == SR_GS_FACE_NORMALS) &&
         d->vertex_program != SR_VP_MVP_NORMAL_WORLD)
         return fail(ctx, SR_ERR_INVALID_ARGUMENT, "draw: normal visualisation needs world-space varyings");
-    if (d->fragment_program < SR_FP_FLAT || d->fragment_program > SR_FP_GBUFFER)
+    if (d->fragment_program < SR_FP_FLAT || d->fragment_program > SR_FP_FRESNEL4)
         return fail(ctx, SR_ERR_INVALID_ARGUMENT, "draw: unknown fragment program");
     if (d->blend < SR_BLEND_REPLACE || d->blend > SR_BLEND_OVER_NORMALIZED)
         return fail(ctx, SR_ERR_INVALID_ARGUMENT, "draw: unknown blend");
@@ -533,14 +542,16 @@ size_t batch_bytes_estimate(uint64_t tris, uint64_t verts, double pairs_per_tri)
     return (size_t)(b * 1.25);
 }
 
-// An RGBAF32 texture read by the fragment program (a texture buffer plane)
+// A texture read only the per-pixel raster's sampler does: an RGBAF32
+// texture (a texture buffer plane) or the full example's sampler
 bool samples_f32(const sr_draw_desc* d, sr_texture tex) {
-    return tex && tex->fmt == SR_COLOR_RGBAF32 &&
+    return tex && (tex->fmt == SR_COLOR_RGBAF32 || d->tex_filter >= SR_FILTER_SPAN_NEAREST) &&
            (d->fragment_program == SR_FP_TEX_LAMBERT || d->fragment_program == SR_FP_GBUFFER);
 }
 
 bool batchable(sr_mesh m, const sr_draw_desc* d, sr_texture tex) {
     return m->prim == SR_PRIM_TRIANGLES && d->geometry_program == SR_GS_NONE && !samples_f32(d, tex) &&
+           d->vertex_program != SR_VP_WORLD4_NORMAL4 &&
            d->stencil_test == SR_STENCIL_ALWAYS && d->stencil_op == SR_STENCIL_KEEP &&
            !(d->exec_flags & SR_EXEC_EXHAUSTIVE) && d->fragment_program != SR_FP_GBUFFER &&
            (d->blend == SR_BLEND_REPLACE || d->blend == SR_BLEND_OVER);
@@ -1428,6 +1439,7 @@ static int draw_one(sr_context ctx, sr_framebuffer fb, sr_mesh mesh, const sr_dr
     // every format, the normalised blend, the multi-output program and a
     // triangle draw sampling an RGBAF32 texture take the per-pixel raster
     pl.general = !fb->standard() || d->blend == SR_BLEND_OVER_NORMALIZED || d->fragment_program == SR_FP_GBUFFER ||
+                 d->fragment_program == SR_FP_FRESNEL4 ||
                  (samples_f32(d, tex) && mesh->prim == SR_PRIM_TRIANGLES && !normals_gs);
     if (pl.general) {
         if (mesh->prim != SR_PRIM_TRIANGLES || normals_gs)

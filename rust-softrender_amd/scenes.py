@@ -280,6 +280,52 @@ def normals_scene(pos, nrm, idx, width: int = 320, height: int = 320,
                 clear=0xFF000000, texture=None, draws=[lit, lines])
 
 
+def example_lights() -> np.ndarray:
+    """The full example's four lights (full_example/src/lib.rs:55-64):
+    rows of position xyz, colour rgb, intensity."""
+    return np.array([[-1.0, 1.0, -1.0, 1.0, 1.0, 1.0, 9.0],
+                     [1.0, 1.0, 1.0, 0.6, 0.6, 1.0, 9.0],
+                     [0.0, 3.0, -1.0, 1.0, 0.3, 0.3, 25.0],
+                     [-2.0, -1.0, 1.0, 0.7, 1.0, 0.7, 25.0]], np.float64)
+
+
+def fresnel_scene(pos, nrm, idx, width: int = 320, height: int = 180, translate=(0.0, 0.0, 0.0),
+                  object_deg: float = 45.0):
+    """The full example's first frame (full_example/src/main.rs:11-42,
+    lib.rs:22-66): the model rotated about y (plus `translate`, which makes
+    the normal's w non-zero), the camera at (cos 75deg, 1, sin 75deg) * 2
+    looking at (0, 0.25, 0), 65deg field of view, four lights, the Fresnel
+    program (SR_VP_WORLD4_NORMAL4 + SR_FP_FRESNEL4) and the normalised
+    over-blend, on an RGBAF32 framebuffer cleared to (0.01, 0.01, 0.01, 1)."""
+    model = rotation_y(object_deg)
+    model[:3, 3] = translate
+    mit = np.linalg.inv(model).T
+    a = np.deg2rad(75.0)
+    cam = np.array([np.cos(a) * 2.0, 2.0, np.sin(a) * 2.0])
+    view = look_at(cam, (0.0, 0.25, 0.0), (0.0, 1.0, 0.0))
+    proj = perspective_zero_to_one(65.0, width / height, 0.1, 1000.0)
+    mvp = proj @ view @ model
+    vsu = np.zeros(48, np.float32)
+    vsu[0:16] = mvp.reshape(16)
+    vsu[16:25] = mit[:3, :3].reshape(9)
+    vsu[25:37] = model[:3].reshape(12)
+    vsu[37:40] = mit[3, :3]
+    vsu[40:44] = model[3]
+    fs = np.zeros(32, np.float32)
+    fs[0:3] = cam
+    fs[3:31] = example_lights().reshape(28)
+    fs[31] = 0.25
+    pos = np.asarray(pos, np.float32)
+    nrm = np.asarray(nrm, np.float32)
+    mesh = dict(primitive=3, px=pos[:, 0].copy(), py=pos[:, 1].copy(), pz=pos[:, 2].copy(),
+                nx=nrm[:, 0].copy(), ny=nrm[:, 1].copy(), nz=nrm[:, 2].copy(), u=None, v=None,
+                indices=np.asarray(idx, np.uint32))
+    desc = dict(vertex_program=2, fragment_program=5, blend=2, cull=0, vs_uniforms=vsu, fs_uniforms=fs,
+                viewport=(0.0, 0.0, float(width), float(height), 0.0, 1.0))
+    return dict(mesh, name=f"fresnel_{width}x{height}", width=width, height=height, texture=None,
+                clear_f32=(0.01, 0.01, 0.01, 1.0), desc=desc)
+
+
 # ---------------------------------------------------------------------------
 # Clip-space triangle soup (the 1080p / 1M-triangle frame)
 # ---------------------------------------------------------------------------

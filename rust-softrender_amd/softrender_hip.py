@@ -30,6 +30,9 @@ DEPTH_F32, DEPTH_F64 = 0, 1
 STENCIL_U8, STENCIL_NONE = 0, 1
 BLEND_REPLACE, BLEND_OVER, BLEND_OVER_NORMALIZED = 0, 1, 2
 FP_FLAT, FP_BLINN_PHONG, FP_TEX_LAMBERT, FP_UV_COLOR, FP_GBUFFER = 0, 1, 2, 3, 4
+FP_FRESNEL4 = 5
+VP_MVP_NORMAL_UV, VP_MVP_NORMAL_WORLD, VP_WORLD4_NORMAL4 = 0, 1, 2
+FILTER_NEAREST, FILTER_BILINEAR, FILTER_SPAN_NEAREST, FILTER_SPAN_BILINEAR = 0, 1, 2, 3
 STATIC_WORKSPACE_BYTES = 64 << 20  # SR_STATIC_WORKSPACE_MIB
 
 EXPORTED = [
