@@ -71,6 +71,10 @@ extern "C" {
  * columns 0-2) and [37..40) (row 3, columns 0-2).  The 4-vector norm sums
  * ((x*x + y*y) + z*z) + w*w.  No geometry program takes it. */
 #define SR_VP_WORLD4_NORMAL4 2
+/* clip = MVP * (p,1) and one varying: the mesh's integer attribute as f32
+ * (`R::from(i32)`, rounded to nearest), K=1.  Needs sr_mesh_set_int_attrib;
+ * no geometry program takes it (clipping would cast at every clip vertex). */
+#define SR_VP_MVP_INT 3
 
 /* Geometry programs (GeometryShader::run, geometry.rs:132-258). */
 #define SR_GS_NONE 0           /* run_to_fragment / finish without a program */
@@ -116,6 +120,12 @@ extern "C" {
  * Transcendentals (powf, sqrtf) are the platform's: colours agree with the
  * oracle within a few ulps, not bit for bit. */
 #define SR_FP_FRESNEL4 5
+/* The integer varying back as i32 (`<i32 as NumCast>::from`, truncation
+ * toward zero; NaN and values out of i32's range, where the reference's
+ * unwrap panics, saturate as Rust's `as` does), its index m = id mod 8 in
+ * 0..7, and colour fs_uniforms[4m .. 4m+4) (RGBA floats).  Needs
+ * SR_VP_MVP_INT.  Per-pixel raster. */
+#define SR_FP_INT_PALETTE 6
 
 /* Blend (color/blend.rs:9-74): `()` replaces, OVER is src-alpha over,
  * OVER_NORMALIZED the full example's blend (full_example/src/color.rs:5-17):
@@ -276,6 +286,11 @@ int sr_mesh_upload(sr_context ctx, const sr_vertex_soa* vertices, uint32_t n_ver
                    const uint32_t* indices, uint32_t n_indices, int32_t primitive,
                    sr_mesh* out);
 int sr_mesh_free(sr_mesh mesh);
+/* An integer vertex attribute (an i32 member of the vertex's uniforms,
+ * interpolated through f32 and cast back: interpolate.rs:43-64), one value
+ * per vertex (count == the mesh's n_vertices); replaces an earlier one.
+ * SR_VP_MVP_INT reads it. */
+int sr_mesh_set_int_attrib(sr_mesh mesh, const int32_t* values, uint32_t count);
 
 /* ---- texture (texture.rs:14-62; sampling is build-defined) -------------- */
 int sr_texture_upload(sr_context ctx, uint32_t width, uint32_t height,

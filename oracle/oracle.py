@@ -29,7 +29,7 @@ class OrFb(C.Structure):
 
 class OrMesh(C.Structure):
     _fields_ = [("v", VertexSoA), ("n_vertices", C.c_uint32), ("indices", C.POINTER(C.c_uint32)),
-                ("n_indices", C.c_uint32), ("primitive", C.c_int32)]
+                ("n_indices", C.c_uint32), ("primitive", C.c_int32), ("iattr", C.POINTER(C.c_int32))]
 
 
 class OrTexture(C.Structure):
@@ -187,12 +187,15 @@ def _or_mesh(mesh: dict):
     soa, keep = vertex_soa(mesh)
     idx = mesh.get("indices")
     n = len(keep["px"])
+    ia = mesh.get("iattr")
+    ia = None if ia is None else np.ascontiguousarray(ia, np.int32)
+    iap = None if ia is None else ia.ctypes.data_as(C.POINTER(C.c_int32))
     if idx is not None:
         idx = np.ascontiguousarray(idx, np.uint32)
-        om = OrMesh(soa, n, idx.ctypes.data_as(C.POINTER(C.c_uint32)), len(idx), mesh["primitive"])
+        om = OrMesh(soa, n, idx.ctypes.data_as(C.POINTER(C.c_uint32)), len(idx), mesh["primitive"], iap)
     else:
-        om = OrMesh(soa, n, None, 0, mesh["primitive"])
-    return om, (keep, idx)
+        om = OrMesh(soa, n, None, 0, mesh["primitive"], iap)
+    return om, (keep, idx, ia)
 
 
 def vertex(mesh: dict, desc: DrawDesc, i: int, L=None):

@@ -67,7 +67,7 @@ static inline void normalize3(float* v) {
 }
 
 static int vp_varyings(int vp) {
-    return vp == SR_VP_WORLD4_NORMAL4 ? 8 : (vp == SR_VP_MVP_NORMAL_WORLD ? 6 : 5);
+    return vp == SR_VP_WORLD4_NORMAL4 ? 8 : vp == SR_VP_MVP_INT ? 1 : (vp == SR_VP_MVP_NORMAL_WORLD ? 6 : 5);
 }
 
 /* 4-vectors of the full example (nalgebra Vector4<f32>): dot sums from +0
@@ -91,6 +91,12 @@ static void vs_run(const sr_draw_desc* d, const or_mesh* m, uint32_t i, ovtx* o)
         float acc = 0.0f;
         for (int k = 0; k < 4; k++) acc = acc + U[r * 4 + k] * pin[k];
         o->p[r] = acc;
+    }
+    if (d->vertex_program == SR_VP_MVP_INT) {
+        /* an i32 member of the uniforms; interpolate.rs:47-49 converts each
+         * vertex's value with R::from (i32 -> f32, rounded to nearest) */
+        o->k[0] = m->iattr ? (float)m->iattr[i] : 0.0f;
+        return;
     }
     if (d->vertex_program == SR_VP_WORLD4_NORMAL4) {
         /* full_example/src/shaders.rs:8-31: world_position = model * p and
@@ -607,6 +613,17 @@ static int fs_run_f(const sr_draw_desc* d, const float* k, const or_texture* tex
         case SR_FP_FRESNEL4:
             fresnel4(U, k, out[0]);
             return 1;
+        case SR_FP_INT_PALETTE: {
+            /* <i32 as NumCast>::from(f32) (interpolate.rs:47-49, num-traits
+             * 0.1): truncation toward zero; NaN and out-of-range values, where
+             * the reference's unwrap panics, saturate as Rust's `as` does
+             * (parity unpinned there: the crate would abort). */
+            const float x = k[0];
+            const int32_t id = x != x ? 0 : (x >= 2147483648.0f ? INT32_MAX : (x < -2147483648.0f ? INT32_MIN : (int32_t)x));
+            const int32_t m = ((id % 8) + 8) % 8;
+            for (int c = 0; c < 4; c++) out[0][c] = U[4 * m + c];
+            return 1;
+        }
         default:
             out[0][0] = U[0]; out[0][1] = U[1]; out[0][2] = U[2]; out[0][3] = U[3];
             return 1;

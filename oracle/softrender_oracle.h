@@ -44,6 +44,7 @@ typedef struct or_mesh {
     const uint32_t* indices; /* NULL: index i = i */
     uint32_t n_indices;
     int32_t primitive; /* SR_PRIM_* */
+    const int32_t* iattr; /* integer attribute per vertex (SR_VP_MVP_INT), or NULL */
 } or_mesh;
 
 typedef struct or_texture {

@@ -343,6 +343,15 @@ __device__ __forceinline__ int fragment_rgba(const RasterParams& P, const float*
     } else if constexpr (FP == SR_FP_FRESNEL4) {
         fresnel4(U, k, o[0]);
         return 1;
+    } else if constexpr (FP == SR_FP_INT_PALETTE) {
+        // <i32 as NumCast>::from(f32): truncation toward zero (saturating
+        // where the reference's unwrap would panic)
+        const float x = k[0];
+        const int32_t id = !(x == x) ? 0 : (x >= 2147483648.0f ? 2147483647 : (x < -2147483648.0f ? (-2147483647 - 1) : (int32_t)x));
+        const int32_t m = ((id % 8) + 8) % 8;
+#pragma unroll
+        for (int c = 0; c < 4; c++) o[0][c] = U[4 * m + c];
+        return 1;
     } else {
         o[0][0] = U[0]; o[0][1] = U[1]; o[0][2] = U[2]; o[0][3] = U[3];
         return 1;
@@ -364,6 +373,7 @@ __device__ __forceinline__ uint32_t fragment(const RasterParams& P, const float*
 template <int FP>
 constexpr int fp_varyings() {
     return FP == SR_FP_FRESNEL4     ? 8
+           : FP == SR_FP_INT_PALETTE ? 1
            : FP == SR_FP_BLINN_PHONG ? 6
            : (FP == SR_FP_TEX_LAMBERT || FP == SR_FP_UV_COLOR || FP == SR_FP_GBUFFER) ? 5 : 0;
 }

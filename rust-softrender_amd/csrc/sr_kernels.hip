@@ -50,8 +50,8 @@ __device__ __forceinline__ void normalize_pos(float x, float y, float z, float w
 // (run_to_fragment, vertex.rs:139).  Writes slot `o` of the SoA output.
 __device__ __forceinline__ void vertex_compute(const float* U, int program, int normalize,
                                                const sr_viewport& viewport, const float pin[4],
-                                               const float nin[3], float u, float v, float outp[4],
-                                               float var[MAXK]) {
+                                               const float nin[3], float u, float v, int32_t ia,
+                                               float outp[4], float var[MAXK]) {
     float clip[4];
 #pragma unroll
     for (int r = 0; r < 4; r++) {
@@ -59,6 +59,15 @@ __device__ __forceinline__ void vertex_compute(const float* U, int program, int 
 #pragma unroll
         for (int k = 0; k < 4; k++) acc = acc + U[r * 4 + k] * pin[k];
         clip[r] = acc;
+    }
+    if (program == SR_VP_MVP_INT) {  // R::from(i32): rounded to nearest
+        var[0] = (float)ia;
+        if (normalize) {
+            normalize_pos(clip[0], clip[1], clip[2], clip[3], viewport, outp);
+        } else {
+            outp[0] = clip[0]; outp[1] = clip[1]; outp[2] = clip[2]; outp[3] = clip[3];
+        }
+        return;
     }
     if (program == SR_VP_WORLD4_NORMAL4) {
         // the full example's vertex_shader (shaders.rs:8-31): model * (p,1)
@@ -164,7 +173,7 @@ __global__ __launch_bounds__(256) void k_vertex(VSParams P) {
         const float nin[3] = {P.nx ? P.nx[i] : 0.0f, P.ny ? P.ny[i] : 0.0f, P.nz ? P.nz[i] : 0.0f};
         float outp[4];
         vertex_compute(P.U, P.program, P.normalize, P.viewport, pin, nin, P.u ? P.u[i] : 0.0f,
-                       P.v ? P.v[i] : 0.0f, outp, var);
+                       P.v ? P.v[i] : 0.0f, P.ia ? P.ia[i] : 0, outp, var);
 #pragma unroll
         for (int c = 0; c < 4; c++) P.out.comp(c)[i] = outp[c];
     }
@@ -246,8 +255,8 @@ __global__ __launch_bounds__(256) void k_vertex_batch(BatchTables T, VertexBuf o
         const float pin[4] = {S.px[l], S.py[l], S.pz[l], 1.0f};
         const float nin[3] = {S.nx ? S.nx[l] : 0.0f, S.ny ? S.ny[l] : 0.0f, S.nz ? S.nz[l] : 0.0f};
         float outp[4];
-        vertex_compute(S.U, S.program, 1, S.viewport, pin, nin, S.u ? S.u[l] : 0.0f, S.v ? S.v[l] : 0.0f, outp,
-                       var);
+        vertex_compute(S.U, S.program, 1, S.viewport, pin, nin, S.u ? S.u[l] : 0.0f, S.v ? S.v[l] : 0.0f, 0,
+                       outp, var);
 #pragma unroll
         for (int c = 0; c < 4; c++) out.comp(c)[i] = outp[c];
         live = true;

@@ -7,6 +7,7 @@ namespace sr {
 
 struct VSParams {
     const float *px, *py, *pz, *nx, *ny, *nz, *u, *v;
+    const int32_t* ia;  // integer attribute (SR_VP_MVP_INT), or null
     uint32_t n;
     int program;
     int K;

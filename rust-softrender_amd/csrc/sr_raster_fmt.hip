@@ -154,6 +154,7 @@ void launch_raster_fmt(const RasterParams& P, const FmtParams& F, uint32_t ntile
         case SR_FP_UV_COLOR: raster_fmt_blend<SR_FP_UV_COLOR>(P, F, ntiles, s); break;
         case SR_FP_GBUFFER: raster_fmt_blend<SR_FP_GBUFFER>(P, F, ntiles, s); break;
         case SR_FP_FRESNEL4: raster_fmt_blend<SR_FP_FRESNEL4>(P, F, ntiles, s); break;
+        case SR_FP_INT_PALETTE: raster_fmt_blend<SR_FP_INT_PALETTE>(P, F, ntiles, s); break;
         default: raster_fmt_blend<SR_FP_FLAT>(P, F, ntiles, s); break;
     }
 }

@@ -91,7 +91,9 @@ int bits_for(uint32_t n) {
     return b;
 }
 
-int varyings_of(int vp) { return vp == SR_VP_WORLD4_NORMAL4 ? 8 : (vp == SR_VP_MVP_NORMAL_WORLD ? 6 : 5); }
+int varyings_of(int vp) {
+    return vp == SR_VP_WORLD4_NORMAL4 ? 8 : vp == SR_VP_MVP_INT ? 1 : (vp == SR_VP_MVP_NORMAL_WORLD ? 6 : 5);
+}
 
 int policy_tile(sr_context ctx, uint32_t w, uint32_t h) {
     const uint64_t px = (uint64_t)w * h;
@@ -426,10 +428,19 @@ int draw_points_impl(sr_context ctx, sr_framebuffer fb, const DrawPlan& pl, cons
 int sr_rt::validate_draw(sr_context ctx, sr_framebuffer fb, sr_mesh mesh, const sr_draw_desc* d, sr_texture tex) {
     if (!ctx || !fb || !mesh || !d) return fail(ctx, SR_ERR_INVALID_ARGUMENT, "draw: null argument");
     if (fb->ctx != ctx || mesh->ctx != ctx) return fail(ctx, SR_ERR_INVALID_ARGUMENT, "draw: handle from another context");
-    if (d->vertex_program < SR_VP_MVP_NORMAL_UV || d->vertex_program > SR_VP_WORLD4_NORMAL4)
+    if (d->vertex_program < SR_VP_MVP_NORMAL_UV || d->vertex_program > SR_VP_MVP_INT)
         return fail(ctx, SR_ERR_INVALID_ARGUMENT, "draw: unknown vertex program");
     if (d->vertex_program == SR_VP_WORLD4_NORMAL4 && d->geometry_program != SR_GS_NONE)
         return fail(ctx, SR_ERR_INVALID_ARGUMENT, "draw: no geometry program takes the 4-vector varyings");
+    if (d->vertex_program == SR_VP_MVP_INT && d->geometry_program != SR_GS_NONE)
+        return fail(ctx, SR_ERR_INVALID_ARGUMENT, "draw: no geometry program takes the integer varying");
+    if (d->vertex_program == SR_VP_MVP_INT && !mesh->iattr)
+        return fail(ctx, SR_ERR_INVALID_ARGUMENT, "draw: the integer vertex program needs sr_mesh_set_int_attrib");
+    if (d->fragment_program == SR_FP_INT_PALETTE && d->vertex_program != SR_VP_MVP_INT)
+        return fail(ctx, SR_ERR_INVALID_ARGUMENT, "draw: the palette program needs the integer varying");
+    if (d->vertex_program == SR_VP_MVP_INT && d->fragment_program != SR_FP_INT_PALETTE &&
+        d->fragment_program != SR_FP_FLAT)
+        return fail(ctx, SR_ERR_INVALID_ARGUMENT, "draw: the integer varying feeds the palette and flat programs only");
     if (d->fragment_program == SR_FP_FRESNEL4 && d->vertex_program != SR_VP_WORLD4_NORMAL4)
         return fail(ctx, SR_ERR_INVALID_ARGUMENT, "draw: the Fresnel program needs the 4-vector varyings");
     if (d->vertex_program == SR_VP_WORLD4_NORMAL4 && d->fragment_program != SR_FP_FRESNEL4 &&
@@ -442,7 +453,7 @@ int sr_rt::validate_draw(sr_context ctx, sr_framebuffer fb, sr_mesh mesh, const 
     if ((d->geometry_program == SR_GS_VERTEX_NORMALS || d->geometry_program == SR_GS_FACE_NORMALS) &&
         d->vertex_program != SR_VP_MVP_NORMAL_WORLD)
         return fail(ctx, SR_ERR_INVALID_ARGUMENT, "draw: normal visualisation needs world-space varyings");
-    if (d->fragment_program < SR_FP_FLAT || d->fragment_program > SR_FP_FRESNEL4)
+    if (d->fragment_program < SR_FP_FLAT || d->fragment_program > SR_FP_INT_PALETTE)
         return fail(ctx, SR_ERR_INVALID_ARGUMENT, "draw: unknown fragment program");
     if (d->blend < SR_BLEND_REPLACE || d->blend > SR_BLEND_OVER_NORMALIZED)
         return fail(ctx, SR_ERR_INVALID_ARGUMENT, "draw: unknown blend");
@@ -551,7 +562,7 @@ bool samples_f32(const sr_draw_desc* d, sr_texture tex) {
 
 bool batchable(sr_mesh m, const sr_draw_desc* d, sr_texture tex) {
     return m->prim == SR_PRIM_TRIANGLES && d->geometry_program == SR_GS_NONE && !samples_f32(d, tex) &&
-           d->vertex_program != SR_VP_WORLD4_NORMAL4 &&
+           d->vertex_program != SR_VP_WORLD4_NORMAL4 && d->vertex_program != SR_VP_MVP_INT &&
            d->stencil_test == SR_STENCIL_ALWAYS && d->stencil_op == SR_STENCIL_KEEP &&
            !(d->exec_flags & SR_EXEC_EXHAUSTIVE) && d->fragment_program != SR_FP_GBUFFER &&
            (d->blend == SR_BLEND_REPLACE || d->blend == SR_BLEND_OVER);
@@ -1162,7 +1173,24 @@ int sr_mesh_free(sr_mesh m) {
     for (float* p : m->soa)
         if (p) hipFree(p);
     if (m->idx) hipFree(m->idx);
+    if (m->iattr) hipFree(m->iattr);
     delete m;
+    return SR_OK;
+}
+
+int sr_mesh_set_int_attrib(sr_mesh m, const int32_t* values, uint32_t count) {
+    if (!m || (!values && count)) return fail(m ? m->ctx : nullptr, SR_ERR_INVALID_ARGUMENT, "mesh: null argument");
+    sr_context ctx = m->ctx;
+    if (count != m->nv) return fail(ctx, SR_ERR_INVALID_ARGUMENT, "mesh: one integer attribute per vertex");
+    hipSetDevice(ctx->device);
+    if (!m->iattr && count && hipMalloc(&m->iattr, sizeof(int32_t) * count) != hipSuccess) {
+        m->iattr = nullptr;
+        return fail(ctx, SR_ERR_OUT_OF_MEMORY, "mesh: attribute upload failed");
+    }
+    if (count && (hipMemcpyAsync(m->iattr, values, sizeof(int32_t) * count, hipMemcpyHostToDevice, ctx->stream) !=
+                      hipSuccess ||
+                  hipStreamSynchronize(ctx->stream) != hipSuccess))
+        return fail(ctx, SR_ERR_DEVICE, "mesh: attribute upload failed");
     return SR_OK;
 }
 
@@ -1439,7 +1467,7 @@ static int draw_one(sr_context ctx, sr_framebuffer fb, sr_mesh mesh, const sr_dr
     // every format, the normalised blend, the multi-output program and a
     // triangle draw sampling an RGBAF32 texture take the per-pixel raster
     pl.general = !fb->standard() || d->blend == SR_BLEND_OVER_NORMALIZED || d->fragment_program == SR_FP_GBUFFER ||
-                 d->fragment_program == SR_FP_FRESNEL4 ||
+                 d->fragment_program == SR_FP_FRESNEL4 || d->fragment_program == SR_FP_INT_PALETTE ||
                  (samples_f32(d, tex) && mesh->prim == SR_PRIM_TRIANGLES && !normals_gs);
     if (pl.general) {
         if (mesh->prim != SR_PRIM_TRIANGLES || normals_gs)
@@ -1480,7 +1508,7 @@ static int draw_one(sr_context ctx, sr_framebuffer fb, sr_mesh mesh, const sr_dr
     memset(&vs, 0, sizeof(vs));
     vs.px = mesh->soa[0]; vs.py = mesh->soa[1]; vs.pz = mesh->soa[2];
     vs.nx = mesh->soa[3]; vs.ny = mesh->soa[4]; vs.nz = mesh->soa[5];
-    vs.u = mesh->soa[6]; vs.v = mesh->soa[7];
+    vs.u = mesh->soa[6]; vs.v = mesh->soa[7]; vs.ia = mesh->iattr;
     vs.n = nv; vs.program = d->vertex_program; vs.K = pl.K;
     // normal visualisation re-emits lines and points unchanged (shaders.rs:59,88):
     // for them it is run_to_fragment
@@ -1643,7 +1671,7 @@ int sr_draw_chains(sr_context ctx, sr_framebuffer fb, sr_mesh mesh, const sr_dra
     memset(&vs, 0, sizeof(vs));
     vs.px = mesh->soa[0]; vs.py = mesh->soa[1]; vs.pz = mesh->soa[2];
     vs.nx = mesh->soa[3]; vs.ny = mesh->soa[4]; vs.nz = mesh->soa[5];
-    vs.u = mesh->soa[6]; vs.v = mesh->soa[7];
+    vs.u = mesh->soa[6]; vs.v = mesh->soa[7]; vs.ia = mesh->iattr;
     vs.n = nv; vs.program = descs[0].vertex_program; vs.K = K;
     vs.normalize = 0;
     vs.viewport = descs[0].viewport;

@@ -61,6 +61,7 @@ struct sr_context_t {
 struct sr_mesh_t {
     sr_context ctx;
     float* soa[8] = {};  // px py pz nx ny nz u v (null if absent)
+    int32_t* iattr = nullptr;  // integer attribute (sr_mesh_set_int_attrib), per vertex
     uint32_t* idx = nullptr;
     uint32_t nv = 0, ni = 0;
     bool indexed = false;  // an index list was given (it may be empty: nothing is drawn)

@@ -30,14 +30,14 @@ DEPTH_F32, DEPTH_F64 = 0, 1
 STENCIL_U8, STENCIL_NONE = 0, 1
 BLEND_REPLACE, BLEND_OVER, BLEND_OVER_NORMALIZED = 0, 1, 2
 FP_FLAT, FP_BLINN_PHONG, FP_TEX_LAMBERT, FP_UV_COLOR, FP_GBUFFER = 0, 1, 2, 3, 4
-FP_FRESNEL4 = 5
-VP_MVP_NORMAL_UV, VP_MVP_NORMAL_WORLD, VP_WORLD4_NORMAL4 = 0, 1, 2
+FP_FRESNEL4, FP_INT_PALETTE = 5, 6
+VP_MVP_NORMAL_UV, VP_MVP_NORMAL_WORLD, VP_WORLD4_NORMAL4, VP_MVP_INT = 0, 1, 2, 3
 FILTER_NEAREST, FILTER_BILINEAR, FILTER_SPAN_NEAREST, FILTER_SPAN_BILINEAR = 0, 1, 2, 3
 STATIC_WORKSPACE_BYTES = 64 << 20  # SR_STATIC_WORKSPACE_MIB
 
 EXPORTED = [
     "sr_ctx_create", "sr_ctx_destroy", "sr_last_error", "sr_finish", "sr_ctx_stream",
-    "sr_ctx_stage_times", "sr_ctx_last_stats", "sr_mesh_upload", "sr_mesh_free",
+    "sr_ctx_stage_times", "sr_ctx_last_stats", "sr_mesh_upload", "sr_mesh_free", "sr_mesh_set_int_attrib",
     "sr_texture_upload", "sr_texture_free", "sr_texture_from_fb", "sr_obj_parse", "sr_mesh_load_obj",
     "sr_fb_create", "sr_fb_free", "sr_fb_clear", "sr_fb_create_ex", "sr_fb_get_format", "sr_fb_clear_ex",
     "sr_fb_read_ex", "sr_fb_write_ex", "sr_texture_from_fb_plane",
@@ -204,6 +204,7 @@ def lib():
         L.sr_mesh_upload.argtypes = [vp, C.POINTER(VertexSoA), C.c_uint32, C.POINTER(C.c_uint32),
                                      C.c_uint32, C.c_int32, C.POINTER(vp)]
         L.sr_mesh_free.argtypes = [vp]
+        L.sr_mesh_set_int_attrib.argtypes = [vp, C.POINTER(C.c_int32), C.c_uint32]
         L.sr_texture_upload.argtypes = [vp, C.c_uint32, C.c_uint32, C.POINTER(C.c_uint32),
                                         C.POINTER(vp)]
         L.sr_texture_free.argtypes = [vp]
@@ -466,8 +467,9 @@ class Server:
 
 class Mesh:
     def __init__(self, ctx: Context, s: dict = None, obj: str = None, model: int = 0):
-        """Uploads a mesh dict (px..v streams, indices, primitive) or, with
-        obj=path, model `model` of a Wavefront OBJ file."""
+        """Uploads a mesh dict (px..v streams, indices, primitive, and an
+        optional per-vertex int32 "iattr") or, with obj=path, model `model`
+        of a Wavefront OBJ file."""
         self.ctx = ctx
         self.h = C.c_void_p()
         if obj is not None:
@@ -483,6 +485,13 @@ class Mesh:
             ip, ni = None, 0
         ctx._check(lib().sr_mesh_upload(ctx.h, C.byref(soa), n, ip, ni, s["primitive"],
                                         C.byref(self.h)))
+        if s.get("iattr") is not None:
+            ia = np.ascontiguousarray(s["iattr"], np.int32)
+            try:
+                ctx._check(lib().sr_mesh_set_int_attrib(self.h, ia.ctypes.data_as(C.POINTER(C.c_int32)), len(ia)))
+            except Exception:
+                self.free()
+                raise
 
     def free(self):
         if self.h:
