@@ -33,18 +33,20 @@ struct VertexBuf {
 // line, what the raster's per-triangle loop reads (into SGPRs).  The
 // triangle's vertex ids, needed only to shade, are kept apart (TriVid).
 struct alignas(64) TriRec {
+    // bytes 0-47: what the raster tests every pixel with (staged into LDS
+    // 16 bytes at a time); 48-63: the box, read where a mode needs it
     double R;             // RN64(1 / (double)det): f32 quotients are formed as (float)(num * R)
+    uint32_t zs;          // bit 0 (strip): a pixel centre outside the box may pass the
+                          // coverage test; the raster tests the box clamped into each
+                          // screen tile (clamp_as_int!), as the reference does.
+                          // Bits 1..31: depth_key(zhi) rounded up to even, zhi an
+                          // upper bound of every depth the triangle can write
+                          // (depth_bound_key)
     float x3, y3;         // screen position of the third vertex
     float e23y, e23x;     // (y2 - y3), (x3 - x2)   — triangle.rs:108
     float e31y, e31x;     // (y3 - y1), (x1 - x3)   — triangle.rs:109
     float z1, z2, z3;
     int32_t X0, X1, Y0, Y1;  // floor of the f32 bbox, clamped to [-1, W] x [-1, H]
-    uint32_t zs;             // bit 0 (strip): a pixel centre outside the box may pass the
-                             // coverage test; the raster tests the box clamped into each
-                             // screen tile (clamp_as_int!), as the reference does.
-                             // Bits 1..31: depth_key(zhi) rounded up to even, zhi an
-                             // upper bound of every depth the triangle can write
-                             // (depth_bound_key)
 };
 static_assert(sizeof(TriRec) == 64, "TriRec layout");
 

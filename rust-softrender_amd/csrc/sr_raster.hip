@@ -53,7 +53,7 @@ struct WaveRegion {
 // (12 bytes each).  With one pixel per lane the LDS of a wave's queue
 // bounds the waves a CU holds (160 KB).
 #ifndef SR_QUEUE_DEPTH1
-#define SR_QUEUE_DEPTH1 6
+#define SR_QUEUE_DEPTH1 4
 #endif
 template <int PPT>
 constexpr int queue_depth() { return PPT == 1 ? SR_QUEUE_DEPTH1 : (PPT == 2 ? 4 : (PPT == 4 ? 2 : 1)); }
@@ -141,6 +141,7 @@ struct RasterShared {
     static constexpr int WPIX = 64 * PPT;             // pixels per wave
     static constexpr int Q = BLEND == SR_BLEND_OVER ? queue_depth<PPT>() : 1;
     QRec qbuf[NW][WPIX * Q];  // first: queue stores take their offset in the instruction
+    uint4 stage[NW][3 * 64];  // per wave: the block's records, bytes 0-47 (chunk c of item k at c*64 + k)
     float lut[256];
     uint16_t flat[NW][WPIX * Q];
 };
@@ -208,6 +209,11 @@ __device__ __forceinline__ void raster_tile(const RasterParams& P, uint32_t tile
     using WR = typename SH::WR;
     constexpr int RW = WR::RW, RH = WR::RH, PPT = SH::PPT, Q = SH::Q;
     constexpr int RPP = WR::RPP;
+    // the record's box is read for every triangle when the mode tests it
+    // (stencil effects, the exhaustive mode); else only for strip triangles
+    constexpr bool BOX = SMODE != 0 || CLAMP;
+    // this wave's record stage: 3 chunks of 64 x 16 bytes
+    uint4* const stage = sh.stage[__builtin_amdgcn_readfirstlane(threadIdx.x >> 6)];
 
     // The empty asm makes the thread index opaque per tile: in the
     // persistent batch kernel the compiler would otherwise hoist every
@@ -346,15 +352,11 @@ __device__ __forceinline__ void raster_tile(const RasterParams& P, uint32_t tile
                 // One triangle against this wave's pixels (rc: its record,
                 // cur: its id).
                 auto tri = [&](const RecLane& rc, uint32_t cur) {
-                    const int X0 = rc.X0, X1 = rc.X1, Y0 = rc.Y0, Y1 = rc.Y1;
+                    int X0 = rc.X0, X1 = rc.X1, Y0 = rc.Y0, Y1 = rc.Y1;  // (BOX: else read below)
                     if constexpr (CLAMP) {
                         if (X0 > X1) return;  // culled or invalid (binned draws never see these)
                     }
                     if constexpr (SMODE == 0) {
-                        if (--hz_n == 0u) {
-                            hz_n = SR_HIZ_PERIOD;
-                            hz_key = wave_min_key<PPT>(dep);
-                        }
                         if ((rc.zs | 1u) < hz_key) return;  // behind every pixel here
                     }
                     // The pixels tested: the box clamped into this screen
@@ -369,6 +371,13 @@ __device__ __forceinline__ void raster_tile(const RasterParams& P, uint32_t tile
                     // reference's.  Stencil counting counts the box itself
                     // (the strips outside it are counted apart).
                     const bool clampit = CLAMP || (rc.zs & 1u) != 0u;
+                    if constexpr (!BOX) {
+                        if (clampit) {  // a strip triangle: its box now
+                            RecLane b;
+                            rec_box(P.tris, cur, b);
+                            X0 = b.X0; X1 = b.X1; Y0 = b.Y0; Y1 = b.Y1;
+                        }
+                    }
                     int cx0 = X0, cx1 = X1, cy0 = Y0, cy1 = Y1;
                     if (clampit) {
                         cx0 = min(max(X0, Tx0), Tx1); cx1 = min(max(X1, Tx0), Tx1);
@@ -430,53 +439,44 @@ __device__ __forceinline__ void raster_tile(const RasterParams& P, uint32_t tile
                     }
                 };
 
-                // Walk the tile's triangles in order.  Each record is read
-                // into SGPRs by a scalar load issued one triangle ahead
-                // (after_record ties that load's address to the current
-                // record, so it is issued only once the current record has
-                // arrived: a scalar-load wait covers every scalar load in
-                // flight).  Two record register sets take turns, so nothing
-                // is copied between triangles.  The item ids come 64 at a
-                // time, one block ahead, and lane k also touches the record
-                // of item c0 + 64 + k so that the scalar loads find their
-                // lines in L2.  Positions are clamped into [sb, se), so
-                // every id loaded is a valid triangle.
-                const uint32_t b0 = sb & ~63u;
-                uint32_t id = load_ids(P.items, b0, sb, se, lane);
-                uint32_t idn = load_ids(P.items, b0 + 64, sb, se, lane);
-                uint32_t warm = touch_rec(P.tris, id);
-                uint32_t cur_id = lane_u32(id, (int)(sb - b0));
-                RecLane ra = rec_s(P.tris, cur_id);
-                for (uint32_t c0 = b0; c0 < se; c0 += 64) {
-                    keep_live(warm);
-                    warm = touch_rec(P.tris, idn);
-                    const uint32_t idc = idn;
-                    idn = load_ids(P.items, c0 + 128, sb, se, lane);
-                    const uint32_t hi = min(c0 + 64, se);
-                    // lane k: the id at position c0 + k + 1 (lane 63 the
-                    // next block's first)
-                    uint32_t id1 = __shfl_down(id, 1, 64);
-                    id1 = lane == 63 ? lane_u32(idc, 0) : id1;
-                    auto next_id = [&](uint32_t c) { return lane_u32(id1, (int)(c - c0)); };
-                    uint32_t c = max(c0, sb);
-                    while (true) {
-                        uint32_t nid = next_id(c);
-                        after_record(nid, ra);
-                        const RecLane rb = rec_s(P.tris, nid);
-                        tri(ra, cur_id);
-                        cur_id = nid;
-                        if (++c >= hi) {
-                            ra = rb;
-                            break;
+                // Walk the tile's triangles in order, 64 items at a time.
+                // The item ids come a block ahead.  At a block's start lane
+                // k copies the first 48 bytes of item c0 + k's record into
+                // this wave's LDS stage (three 16-byte LDS DMA loads, waited
+                // for once per block, before the stage's first read); the
+                // block's survivors are its items in [sb, se) whose depth
+                // bound is not below hz_key, and each survivor's record is
+                // read from the stage by three broadcast LDS reads.  Every
+                // SR_HIZ_PERIOD triangles tested hz_key is refreshed and the
+                // block's remaining survivors filtered by it.  Positions are
+                // clamped into [sb, se), so every id loaded is a valid
+                // triangle.
+                uint32_t id = load_ids(P.items, sb & ~63u, sb, se, lane);
+                for (uint32_t c0 = sb & ~63u; c0 < se; c0 += 64) {
+                    stage_records(P.tris, id, stage);
+                    wait_vm_all();  // the stage has landed (the next ids load after this)
+                    const uint32_t idn = load_ids(P.items, c0 + 64, sb, se, lane);
+                    const uint32_t zsk = stage[lane].z;  // item c0 + k's depth key word
+                    const uint32_t pos = c0 + (uint32_t)lane;
+                    bool keep = pos >= sb && pos < se;
+                    if constexpr (SMODE == 0) keep = keep && (zsk | 1u) >= hz_key;
+                    uint64_t m = __ballot(keep);
+                    while (m != 0) {
+                        const int k = __builtin_amdgcn_readfirstlane(__builtin_ctzll(m));
+                        m &= m - 1;
+                        if constexpr (SMODE == 0) {
+                            if (--hz_n == 0u) {
+                                hz_n = SR_HIZ_PERIOD;
+                                hz_key = wave_min_key<PPT>(dep);
+                                m &= __ballot((zsk | 1u) >= hz_key);
+                            }
                         }
-                        nid = next_id(c);
-                        after_record(nid, rb);
-                        ra = rec_s(P.tris, nid);
-                        tri(rb, cur_id);
-                        cur_id = nid;
-                        if (++c >= hi) break;
+                        const uint32_t cur = lane_u32(id, k);
+                        RecLane rc = rec_stage(stage, k);
+                        if constexpr (BOX) rec_box(P.tris, cur, rc);
+                        tri(rc, cur);
                     }
-                    id = idc;
+                    id = idn;
                 }
             }
         }
