@@ -63,15 +63,15 @@ constexpr int queue_depth() { return PPT == 1 ? SR_QUEUE_DEPTH1 : (PPT == 2 ? 4 
 #define SR_HIZ_PERIOD 8u
 #endif
 
-// The least depth_key of the wave's PPT x 64 pixel depths (DPP min: row_shr
-// 1, 2, 4, 8 within rows of 16 lanes, shifted-in lanes keeping the
-// identity, then row_bcast 15 and 31 as in wave_excl_scan; lane 63 ends
-// with the minimum).
+// The least depth_key of the depths of the wave's pixels in `in` (DPP
+// min: row_shr 1, 2, 4, 8 within rows of 16 lanes, shifted-in lanes
+// keeping the identity, then row_bcast 15 and 31 as in wave_excl_scan;
+// lane 63 ends with the minimum).
 template <int PPT>
-__device__ __forceinline__ uint32_t wave_min_key(const float (&dep)[PPT]) {
+__device__ __forceinline__ uint32_t wave_min_key(const float (&dep)[PPT], const bool (&in)[PPT]) {
     uint32_t m = 0xFFFFFFFFu;
 #pragma unroll
-    for (int j = 0; j < PPT; j++) m = min(m, depth_key(__float_as_uint(dep[j])));
+    for (int j = 0; j < PPT; j++) m = min(m, in[j] ? depth_key(__float_as_uint(dep[j])) : 0xFFFFFFFFu);
     m = min(m, (uint32_t)__builtin_amdgcn_update_dpp(-1, (int)m, 0x111, 0xf, 0xf, false));  // row_shr:1
     m = min(m, (uint32_t)__builtin_amdgcn_update_dpp(-1, (int)m, 0x112, 0xf, 0xf, false));  // row_shr:2
     m = min(m, (uint32_t)__builtin_amdgcn_update_dpp(-1, (int)m, 0x114, 0xf, 0xf, false));  // row_shr:4
@@ -313,11 +313,12 @@ __device__ __forceinline__ void raster_tile(const RasterParams& P, uint32_t tile
 
 
     // Hierarchical depth (draws without stencil effects): hz_key is a lower
-    // bound of the depth keys of this wave's pixels, refreshed every
-    // SR_HIZ_PERIOD triangles (depths only grow: a fragment passes with
-    // z >= its pixel's depth).  A triangle whose depth bound key lies below
-    // it cannot pass the depth test at any of these pixels, and then
-    // changes nothing here: it is skipped.
+    // bound of the depth keys of this wave's pixels in the screen tile being
+    // walked, refreshed every SR_HIZ_PERIOD triangles (depths only grow: a
+    // fragment passes with z >= its pixel's depth).  A triangle whose depth
+    // bound key lies below it cannot pass the depth test at any of those
+    // pixels (the only ones it may touch in this walk), and then changes
+    // nothing: it is skipped.
     uint32_t hz_key = 0u, hz_n = 1u;
 
     // One draw's items [sb, se): the screen tiles meeting this wave's pixels
@@ -339,6 +340,11 @@ __device__ __forceinline__ void raster_tile(const RasterParams& P, uint32_t tile
 #pragma unroll
                 for (int j = 0; j < PPT; j++)
                     inT[j] = live[j] && px[j] >= Tx0 && px[j] <= Tx1 && py[j] >= Ty0 && py[j] <= Ty1;
+                // hz_key bounds the depths of this screen tile's pixels only
+                // (a pixel on a screen tile seam is walked once per tile it
+                // lies in; the others wait): start again per screen tile
+                hz_key = 0u;
+                hz_n = 1u;
 
                 // walk the tile's triangles in order.  Each record is read
                 // into SGPRs by a scalar load issued one triangle ahead (the
@@ -467,7 +473,7 @@ __device__ __forceinline__ void raster_tile(const RasterParams& P, uint32_t tile
                         if constexpr (SMODE == 0) {
                             if (--hz_n == 0u) {
                                 hz_n = SR_HIZ_PERIOD;
-                                hz_key = wave_min_key<PPT>(dep);
+                                hz_key = wave_min_key<PPT>(dep, inT);
                                 m &= __ballot((zsk | 1u) >= hz_key);
                             }
                         }
