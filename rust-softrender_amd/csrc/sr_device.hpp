@@ -177,6 +177,68 @@ __device__ __forceinline__ void sample_span(const RasterParams& P, float u, floa
     for (int c = 0; c < 3; c++) out[c] = powf(out[c], 2.2f);
 }
 
+// Bilinear weights of texels (x0, y0) .. (x0 + 1, y0 + 1) in each channel:
+// the top row, the bottom row, then the two rows.
+struct BilinearAt {
+    float ax, ay, bx, by;
+    __device__ __forceinline__ float operator()(float t00, float t10, float t01, float t11) const {
+        float top = (t00 * bx) + (t10 * ax);
+        float bot = (t01 * bx) + (t11 * ax);
+        return (top * by) + (bot * ay);
+    }
+};
+
+// The texel grid position of (u, v) for a bilinear sample: texel indices
+// of the four texels and their weights.
+__device__ __forceinline__ void bilinear_at(const RasterParams& P, float u, float v, int i[4], BilinearAt& w) {
+    float fx = clampf(u * (float)P.texW - 0.5f, -1.0e8f, 1.0e8f);
+    float fy = clampf(v * (float)P.texH - 0.5f, -1.0e8f, 1.0e8f);
+    float x0 = floorf(fx), y0 = floorf(fy);
+    w.ax = fx - x0;
+    w.ay = fy - y0;
+    int ix = (int)x0, iy = (int)y0;
+    i[0] = texel_index(P, ix, iy);
+    i[1] = texel_index(P, ix + 1, iy);
+    i[2] = texel_index(P, ix, iy + 1);
+    i[3] = texel_index(P, ix + 1, iy + 1);
+    w.bx = 1.0f - w.ax;
+    w.by = 1.0f - w.ay;
+}
+
+// An RGBA8 sample in two steps: tex_fetch8 loads the texel words and
+// tex_mix8 weights them, so a caller can compute something else while the
+// loads are in flight.  Together they are the RGBA8 case of sample().  A
+// Nearest sample is the bilinear mix of four copies of its texel with
+// weights ax = ay = 0, bx = by = 1, which returns that texel's values
+// exactly (t * 1 + t * 0 = t for the finite unorm values), so the mix
+// takes no branch on the filter.
+struct TexFetch8 {
+    uint32_t w[4];
+    BilinearAt at;
+};
+__device__ __forceinline__ TexFetch8 tex_fetch8(const RasterParams& P, float u, float v) {
+    TexFetch8 f;
+    if (P.tex_filter == SR_FILTER_NEAREST) {
+        float fx = clampf(u * (float)P.texW, -1.0e8f, 1.0e8f);
+        float fy = clampf(v * (float)P.texH, -1.0e8f, 1.0e8f);
+        f.w[0] = texel_u8(P, texel_index(P, (int)floorf(fx), (int)floorf(fy)));
+        f.w[1] = f.w[2] = f.w[3] = f.w[0];
+        f.at = BilinearAt{0.0f, 0.0f, 1.0f, 1.0f};
+        return f;
+    }
+    int i[4];
+    bilinear_at(P, u, v, i, f.at);
+#pragma unroll
+    for (int t = 0; t < 4; t++) f.w[t] = texel_u8(P, i[t]);
+    return f;
+}
+__device__ __forceinline__ void tex_mix8(const TexFetch8& f, Lut lut, float out[4]) {
+#pragma unroll
+    for (int c = 0; c < 4; c++)
+        out[c] = f.at(lut(f.w[0] >> (8 * c)), lut(f.w[1] >> (8 * c)), lut(f.w[2] >> (8 * c)),
+                      lut(f.w[3] >> (8 * c)));
+}
+
 template <bool ANY_FMT = true>
 __device__ __forceinline__ void sample(const RasterParams& P, float u, float v, Lut lut,
                                        float out[4]) {
@@ -185,47 +247,26 @@ __device__ __forceinline__ void sample(const RasterParams& P, float u, float v, 
             sample_span(P, u, v, lut, out);
             return;
         }
-    }
-    const bool f32 = ANY_FMT && P.tex_fmt == SR_COLOR_RGBAF32;
-    if (P.tex_filter == SR_FILTER_NEAREST) {
-        float fx = clampf(u * (float)P.texW, -1.0e8f, 1.0e8f);
-        float fy = clampf(v * (float)P.texH, -1.0e8f, 1.0e8f);
-        const int i = texel_index(P, (int)floorf(fx), (int)floorf(fy));
-        if (f32) {
+        if (P.tex_fmt == SR_COLOR_RGBAF32) {
+            if (P.tex_filter == SR_FILTER_NEAREST) {
+                float fx = clampf(u * (float)P.texW, -1.0e8f, 1.0e8f);
+                float fy = clampf(v * (float)P.texH, -1.0e8f, 1.0e8f);
+                const int i = texel_index(P, (int)floorf(fx), (int)floorf(fy));
 #pragma unroll
-            for (int c = 0; c < 4; c++) out[c] = texel_f32(P, i, c, lut);
-        } else {
-            const uint32_t w = texel_u8(P, i);
+                for (int c = 0; c < 4; c++) out[c] = texel_f32(P, i, c, lut);
+                return;
+            }
+            int i[4];
+            BilinearAt at;
+            bilinear_at(P, u, v, i, at);
 #pragma unroll
-            for (int c = 0; c < 4; c++) out[c] = lut(w >> (8 * c));
+            for (int c = 0; c < 4; c++)
+                out[c] = at(texel_f32(P, i[0], c, lut), texel_f32(P, i[1], c, lut), texel_f32(P, i[2], c, lut),
+                            texel_f32(P, i[3], c, lut));
+            return;
         }
-        return;
     }
-    float fx = clampf(u * (float)P.texW - 0.5f, -1.0e8f, 1.0e8f);
-    float fy = clampf(v * (float)P.texH - 0.5f, -1.0e8f, 1.0e8f);
-    float x0 = floorf(fx), y0 = floorf(fy);
-    float ax = fx - x0, ay = fy - y0;
-    int ix = (int)x0, iy = (int)y0;
-    const int i00 = texel_index(P, ix, iy), i10 = texel_index(P, ix + 1, iy);
-    const int i01 = texel_index(P, ix, iy + 1), i11 = texel_index(P, ix + 1, iy + 1);
-    float bx = 1.0f - ax, by = 1.0f - ay;
-    auto mix = [&](float t00, float t10, float t01, float t11) {
-        float top = (t00 * bx) + (t10 * ax);
-        float bot = (t01 * bx) + (t11 * ax);
-        return (top * by) + (bot * ay);
-    };
-    if (f32) {
-#pragma unroll
-        for (int c = 0; c < 4; c++)
-            out[c] = mix(texel_f32(P, i00, c, lut), texel_f32(P, i10, c, lut), texel_f32(P, i01, c, lut),
-                         texel_f32(P, i11, c, lut));
-    } else {
-        const uint32_t w00 = texel_u8(P, i00), w10 = texel_u8(P, i10);
-        const uint32_t w01 = texel_u8(P, i01), w11 = texel_u8(P, i11);
-#pragma unroll
-        for (int c = 0; c < 4; c++)
-            out[c] = mix(lut(w00 >> (8 * c)), lut(w10 >> (8 * c)), lut(w01 >> (8 * c)), lut(w11 >> (8 * c)));
-    }
+    tex_mix8(tex_fetch8(P, u, v), lut, out);
 }
 
 // 4-vectors of the full example (nalgebra Vector4<f32>): dot sums from +0
@@ -321,13 +362,25 @@ __device__ __forceinline__ int fragment_rgba(const RasterParams& P, const float*
         o[0][2] = U[8] * lit + spec; o[0][3] = U[12];
         return 1;
     } else if constexpr (FP == SR_FP_TEX_LAMBERT || FP == SR_FP_GBUFFER) {
+        // RGBA8-only callers issue the texel loads first and light the
+        // fragment while they are in flight
+        TexFetch8 tf;
+        if constexpr (!ANY_FMT) tf = tex_fetch8(P, k[3], k[4]);
         float nx = k[0], ny = k[1], nz = k[2];
         normalize3(nx, ny, nz);
         float ndl = dot3(nx, ny, nz, U[0], U[1], U[2]);
         float diff = ndl > 0.0f ? ndl : 0.0f;
         float lit = U[3] + U[4] * diff;
         float t[4];
-        sample<ANY_FMT>(P, k[3], k[4], lut, t);
+        if constexpr (!ANY_FMT) {
+            // keep the lighting above the texel reads: the scheduler would
+            // otherwise hoist the mix's table reads, and the wait for the
+            // texels with them, above it
+            __builtin_amdgcn_sched_barrier(0);
+            tex_mix8(tf, lut, t);
+        } else {
+            sample<ANY_FMT>(P, k[3], k[4], lut, t);
+        }
         o[0][0] = t[0] * lit; o[0][1] = t[1] * lit; o[0][2] = t[2] * lit; o[0][3] = t[3] * U[5];
         if constexpr (FP == SR_FP_TEX_LAMBERT) {
             return 1;
