@@ -32,11 +32,17 @@ __device__ __forceinline__ float dot3(float ax, float ay, float az, float bx, fl
     return ((ax * bx) + (ay * by)) + (az * bz);
 }
 
+// v / |v| per component.  The three f32 quotients x / len are formed as
+// quot(x, RN64(1 / len)): the same values as correctly rounded division
+// (the argument at quot), for one f64 reciprocal instead of three f32
+// division sequences.  len = 0, inf or NaN gives what the division gives
+// (x * inf, x * 0, NaN).
 __device__ __forceinline__ void normalize3(float& x, float& y, float& z) {
-    float len = sqrtf(dot3(x, y, z, x, y, z));
-    x = x / len;
-    y = y / len;
-    z = z / len;
+    const float len = sqrtf(dot3(x, y, z, x, y, z));
+    const double R = 1.0 / (double)len;
+    x = quot(x, R);
+    y = quot(y, R);
+    z = quot(z, R);
 }
 
 __device__ __forceinline__ float clampf(float x, float lo, float hi) {
