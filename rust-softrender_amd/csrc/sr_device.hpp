@@ -285,8 +285,9 @@ __device__ __forceinline__ float dot4(const float a[4], const float b[4]) {
 }
 __device__ __forceinline__ void normalize4(float v[4]) {
     const float n = sqrtf(dot4(v, v));
+    const double R = 1.0 / (double)n;  // v[c] / n as quot(v[c], R), as in normalize3
 #pragma unroll
-    for (int c = 0; c < 4; c++) v[c] = v[c] / n;
+    for (int c = 0; c < 4; c++) v[c] = quot(v[c], R);
 }
 // powi by repeated squaring: r *= x for each set bit, x squared between.
 template <int N>

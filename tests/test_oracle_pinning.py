@@ -436,3 +436,39 @@ def test_f64_reciprocal_quotient_equals_f32_division():
         got = (a.astype(np.float64) * (1.0 / b.astype(np.float64))).astype(np.float32)
     same = (ref.view(np.uint32) == got.view(np.uint32)) | (np.isnan(ref) & np.isnan(got))
     assert same.all()
+
+
+def _recip_quot(x, d):
+    with np.errstate(all="ignore"):
+        return (x.astype(np.float64) * (1.0 / d.astype(np.float64))).astype(np.float32)
+
+
+def _same_f32(a, b):
+    return (a.view(np.uint32) == b.view(np.uint32)) | (np.isnan(a) & np.isnan(b))
+
+
+def test_f64_reciprocal_normalize_equals_f32_division():
+    # the device normalises vectors as quot(x, 1.0 / (double)len) per
+    # component: equal to x / len in f32 over components and lengths of any
+    # magnitude, denormals, zeros, infinities and NaN included
+    rng = np.random.default_rng(29)
+    n = 2_000_000
+    v = rng.standard_normal((3, n)) * np.exp2(rng.integers(-140, 120, n))
+    v = v.astype(np.float32)
+    with np.errstate(all="ignore"):
+        ln = np.sqrt(((v[0] * v[0]) + (v[1] * v[1])) + (v[2] * v[2]))
+        for c in range(3):
+            assert _same_f32(v[c] / ln, _recip_quot(v[c], ln)).all()
+        # arbitrary numerators against every significand of one binade of divisors
+        d = (np.arange(1 << 23, dtype=np.uint32) | np.uint32(0x3F800000)).view(np.float32)
+        for e in (-126, -60, 0, 23, 100):
+            x = (rng.random(d.size) * 2.0 - 1.0).astype(np.float32) * np.float32(2.0 ** e)
+            assert _same_f32(x / d, _recip_quot(x, d)).all()
+        # denormal numerators of either sign
+        bits = rng.integers(0, 1 << 23, d.size).astype(np.uint32) | (rng.integers(0, 2, d.size).astype(np.uint32) << 31)
+        x = bits.view(np.float32)
+        assert _same_f32(x / d, _recip_quot(x, d)).all()
+        assert _same_f32(x / (d * np.float32(2.0 ** -100)), _recip_quot(x, d * np.float32(2.0 ** -100))).all()
+        sp = np.array([0.0, -0.0, np.inf, -np.inf, np.nan, 1e-45, -1e-45, 3.4028235e38], np.float32)
+        xs, ds = np.meshgrid(sp, sp)
+        assert _same_f32(xs / ds, _recip_quot(xs, ds)).all()
