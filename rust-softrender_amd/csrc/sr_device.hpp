@@ -194,55 +194,91 @@ struct BilinearAt {
     }
 };
 
-// The texel grid position of (u, v) for a bilinear sample: texel indices
-// of the four texels and their weights.
-__device__ __forceinline__ void bilinear_at(const RasterParams& P, float u, float v, int i[4], BilinearAt& w) {
+// The texel grid position of (u, v) for a bilinear sample: the first
+// texel (ix, iy) before the edge rule, and the weights.
+__device__ __forceinline__ void bilinear_pos(const RasterParams& P, float u, float v, int& ix, int& iy,
+                                             BilinearAt& w) {
     float fx = clampf(u * (float)P.texW - 0.5f, -1.0e8f, 1.0e8f);
     float fy = clampf(v * (float)P.texH - 0.5f, -1.0e8f, 1.0e8f);
     float x0 = floorf(fx), y0 = floorf(fy);
     w.ax = fx - x0;
     w.ay = fy - y0;
-    int ix = (int)x0, iy = (int)y0;
-    i[0] = texel_index(P, ix, iy);
-    i[1] = texel_index(P, ix + 1, iy);
-    i[2] = texel_index(P, ix, iy + 1);
-    i[3] = texel_index(P, ix + 1, iy + 1);
+    ix = (int)x0;
+    iy = (int)y0;
     w.bx = 1.0f - w.ax;
     w.by = 1.0f - w.ay;
 }
 
+// ... and the indices of the four texels after the edge rule.
+__device__ __forceinline__ void bilinear_at(const RasterParams& P, float u, float v, int i[4], BilinearAt& w) {
+    int ix, iy;
+    bilinear_pos(P, u, v, ix, iy, w);
+    i[0] = texel_index(P, ix, iy);
+    i[1] = texel_index(P, ix + 1, iy);
+    i[2] = texel_index(P, ix, iy + 1);
+    i[3] = texel_index(P, ix + 1, iy + 1);
+}
+
+// Two adjacent RGBA8 texels in one 8-byte load (4-byte aligned).
+__device__ __forceinline__ uint2 texel_pair(const uint32_t* p) {
+    uint2 r;
+    __builtin_memcpy(&r, p, sizeof(r));
+    return r;
+}
+
 // An RGBA8 sample in two steps: tex_fetch8 loads the texel words and
 // tex_mix8 weights them, so a caller can compute something else while the
-// loads are in flight.  Together they are the RGBA8 case of sample().  A
-// Nearest sample is the bilinear mix of four copies of its texel with
-// weights ax = ay = 0, bx = by = 1, which returns that texel's values
-// exactly (t * 1 + t * 0 = t for the finite unorm values), so the mix
-// takes no branch on the filter.
+// loads are in flight.  Together they are the RGBA8 case of sample().  The
+// four texels are the two words of row pair r0 and of r1, picked by two
+// flags in tex_mix8 (after the wait): texel (x0, y) is r.y when left_hi,
+// else r.x; texel (x0 + 1, y) is r.x when right_lo, else r.y.  A Nearest
+// sample is the bilinear mix of four copies of its texel with weights
+// ax = ay = 0, bx = by = 1, which returns that texel's values exactly
+// (t * 1 + t * 0 = t for the finite unorm values), so the mix takes no
+// branch on the filter.
 struct TexFetch8 {
-    uint32_t w[4];
+    uint2 r0, r1;
+    bool left_hi, right_lo;
     BilinearAt at;
 };
 __device__ __forceinline__ TexFetch8 tex_fetch8(const RasterParams& P, float u, float v) {
     TexFetch8 f;
+    f.left_hi = f.right_lo = false;
     if (P.tex_filter == SR_FILTER_NEAREST) {
         float fx = clampf(u * (float)P.texW, -1.0e8f, 1.0e8f);
         float fy = clampf(v * (float)P.texH, -1.0e8f, 1.0e8f);
-        f.w[0] = texel_u8(P, texel_index(P, (int)floorf(fx), (int)floorf(fy)));
-        f.w[1] = f.w[2] = f.w[3] = f.w[0];
+        const uint32_t w = texel_u8(P, texel_index(P, (int)floorf(fx), (int)floorf(fy)));
+        f.r0 = make_uint2(w, w);
+        f.r1 = f.r0;
         f.at = BilinearAt{0.0f, 0.0f, 1.0f, 1.0f};
         return f;
     }
-    int i[4];
-    bilinear_at(P, u, v, i, f.at);
-#pragma unroll
-    for (int t = 0; t < 4; t++) f.w[t] = texel_u8(P, i[t]);
+    int ix, iy;
+    bilinear_pos(P, u, v, ix, iy, f.at);
+    if (P.tex_edge == SR_EDGE_CLAMP && P.texW >= 2u) {
+        // Clamp: texels (ix, y) and (ix + 1, y) after the edge rule both lie
+        // in the pair at columns b, b + 1 with b = clamp(ix, 0, W - 2) (the
+        // left one is b unless ix >= W - 1, the right one b + 1 unless
+        // ix < 0): one 8-byte load per row instead of two 4-byte loads
+        const int W = (int)P.texW, H = (int)P.texH;
+        const int b = min(max(ix, 0), W - 2);
+        const int y0 = min(max(iy, 0), H - 1), y1 = min(max(iy + 1, 0), H - 1);
+        f.r0 = texel_pair(P.tex + (uint32_t)(y0 * W + b));
+        f.r1 = texel_pair(P.tex + (uint32_t)(y1 * W + b));
+        f.left_hi = ix >= W - 1;
+        f.right_lo = ix < 0;
+        return f;
+    }
+    f.r0 = make_uint2(texel_u8(P, texel_index(P, ix, iy)), texel_u8(P, texel_index(P, ix + 1, iy)));
+    f.r1 = make_uint2(texel_u8(P, texel_index(P, ix, iy + 1)), texel_u8(P, texel_index(P, ix + 1, iy + 1)));
     return f;
 }
 __device__ __forceinline__ void tex_mix8(const TexFetch8& f, Lut lut, float out[4]) {
+    const uint32_t w00 = f.left_hi ? f.r0.y : f.r0.x, w10 = f.right_lo ? f.r0.x : f.r0.y;
+    const uint32_t w01 = f.left_hi ? f.r1.y : f.r1.x, w11 = f.right_lo ? f.r1.x : f.r1.y;
 #pragma unroll
     for (int c = 0; c < 4; c++)
-        out[c] = f.at(lut(f.w[0] >> (8 * c)), lut(f.w[1] >> (8 * c)), lut(f.w[2] >> (8 * c)),
-                      lut(f.w[3] >> (8 * c)));
+        out[c] = f.at(lut(w00 >> (8 * c)), lut(w10 >> (8 * c)), lut(w01 >> (8 * c)), lut(w11 >> (8 * c)));
 }
 
 template <bool ANY_FMT = true>

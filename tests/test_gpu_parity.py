@@ -89,6 +89,23 @@ def test_sampler_modes(ctx, flt, edge):
     assert_parity(render_gpu(ctx, s, ov), ref)
 
 
+@pytest.mark.parametrize("tw,th", [(2, 1), (3, 2), (5, 7), (1, 4), (64, 3)])
+@pytest.mark.parametrize("flt,edge", [(1, 0), (0, 0), (1, 1), (1, 2)])
+def test_sampler_small_random_textures(ctx, tw, th, flt, edge):
+    # every texel differs from its neighbours, and uv runs well past [0, 1]:
+    # a wrong texel of a bilinear pair at a clamped edge (or the 1-wide case)
+    # changes the image
+    s = scenes.soup_scene(3000, 256, 144, seed=18)
+    s["u"] = (s["u"] * 3.0 - 1.0).astype(np.float32)
+    s["v"] = (s["v"] * 3.0 - 1.0).astype(np.float32)
+    rng = np.random.default_rng(tw * 31 + th)
+    s["texture"] = (tw, th, rng.integers(0, 1 << 32, tw * th, dtype=np.uint64).astype(np.uint32))
+    ov = dict(tex_filter=flt, tex_edge=edge, tex_border=0xFF3366CC)
+    ref, st = render_ref(s, ov)
+    assert st[0]["shaded"] > 1000
+    assert_parity(render_gpu(ctx, s, ov), ref)
+
+
 def test_exhaustive_matches_binned(ctx):
     s = scenes.soup_scene(1500, 200, 120, seed=16)
     a = render_gpu(ctx, s, dict(exec_flags=1))
