@@ -219,10 +219,12 @@ __device__ __forceinline__ void bilinear_at(const RasterParams& P, float u, floa
     i[3] = texel_index(P, ix + 1, iy + 1);
 }
 
-// Two adjacent RGBA8 texels in one 8-byte load (4-byte aligned).
+// Two adjacent RGBA8 texels in one 8-byte load (4-byte aligned; a global
+// load, which leaves the LDS counter alone).
 __device__ __forceinline__ uint2 texel_pair(const uint32_t* p) {
+    typedef __attribute__((address_space(1))) const uint32_t GU32;
     uint2 r;
-    __builtin_memcpy(&r, p, sizeof(r));
+    __builtin_memcpy(&r, (GU32*)p, sizeof(r));
     return r;
 }
 
