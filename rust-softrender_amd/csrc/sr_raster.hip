@@ -346,17 +346,8 @@ __device__ __forceinline__ void raster_tile(const RasterParams& P, uint32_t tile
                 hz_key = 0u;
                 hz_n = 1u;
 
-                // walk the tile's triangles in order.  Each record is read
-                // into SGPRs by a scalar load issued one triangle ahead (the
-                // asm ties that load's address to the current record, so it
-                // is issued only after the current record has arrived: a
-                // scalar load wait covers every scalar load in flight).  The
-                // item ids come 64 at a time, one block ahead, and lane k
-                // also touches the record of item c0 + 64 + k so that the
-                // scalar loads find their lines in L2.  Positions are clamped
-                // into [sb, se), so every id loaded is a valid triangle.
                 // One triangle against this wave's pixels (rc: its record,
-                // cur: its id).
+                // read from the stage; cur: its id).
                 auto tri = [&](const RecLane& rc, uint32_t cur) {
                     int X0 = rc.X0, X1 = rc.X1, Y0 = rc.Y0, Y1 = rc.Y1;  // (BOX: else read below)
                     if constexpr (CLAMP) {

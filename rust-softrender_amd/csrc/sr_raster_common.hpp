@@ -126,32 +126,4 @@ __device__ __forceinline__ RecLane rec_stage(const uint4* stage, int k) {
     return r;
 }
 
-// Orders the scalar load of the next record after the arrival of the
-// current one: `next` passes through an empty asm that also reads a field
-// of `cur`, so its address is not known before that field is.
-__device__ __forceinline__ void after_record(uint32_t& next, const RecLane& cur) {
-#if defined(__HIP_DEVICE_COMPILE__)
-    asm volatile("" : "+s"(next) : "s"(cur.X0));
-#endif
-}
-
-// One vector load per lane from record `id` (a record is one 64-byte
-// line): brings it into L2 ahead of the scalar load that reads it.
-__device__ __forceinline__ uint32_t touch_rec(const TriRec* tris, uint32_t id) {
-#if defined(__HIP_DEVICE_COMPILE__)
-    return *(const __attribute__((address_space(1))) uint32_t*)(tris + id);
-#else
-    return *(const uint32_t*)(tris + id);
-#endif
-}
-
-// Keeps a loaded value alive (its load is then neither removed nor left
-// unwaited): an empty asm that reads it.
-__device__ __forceinline__ void keep_live(uint32_t v) {
-#if defined(__HIP_DEVICE_COMPILE__)
-    asm volatile("" ::"v"(v));
-#endif
-}
-
-
 }  // namespace sr
