@@ -174,8 +174,10 @@ __global__ __launch_bounds__(256) void k_vertex(VSParams P) {
         float outp[4];
         vertex_compute(P.U, P.program, P.normalize, P.viewport, pin, nin, P.u ? P.u[i] : 0.0f,
                        P.v ? P.v[i] : 0.0f, P.ia ? P.ia[i] : 0, outp, var);
+        // a screen-space (normalised) vertex is read for x, y, z only
 #pragma unroll
-        for (int c = 0; c < 4; c++) P.out.comp(c)[i] = outp[c];
+        for (int c = 0; c < 3; c++) P.out.comp(c)[i] = outp[c];
+        if (!P.normalize) P.out.comp(3)[i] = outp[3];
     }
     store_rows(rows, P.out, b0, min(256u, P.n - b0), var, live);
 }
@@ -257,8 +259,9 @@ __global__ __launch_bounds__(256) void k_vertex_batch(BatchTables T, VertexBuf o
         float outp[4];
         vertex_compute(S.U, S.program, 1, S.viewport, pin, nin, S.u ? S.u[l] : 0.0f, S.v ? S.v[l] : 0.0f, 0,
                        outp, var);
+        // screen space: x, y, z are all that setup and shading read
 #pragma unroll
-        for (int c = 0; c < 4; c++) out.comp(c)[i] = outp[c];
+        for (int c = 0; c < 3; c++) out.comp(c)[i] = outp[c];
         live = true;
     });
     if (b0 < T.nv) store_rows(rows, out, b0, min(256u, T.nv - b0), var, live);
@@ -912,6 +915,7 @@ __device__ __forceinline__ void store_setup(const SetupStage& st, TriRec* recs, 
     const uint32_t* bs = (const uint32_t*)st.bin;
     uint32_t* bd = (uint32_t*)(bins + b0);
     for (uint32_t i = threadIdx.x; i < m * BW; i += 256u) bd[i] = bs[i];
+    if (!vids) return;  // no draw reads them (vertices 3t, 3t+1, 3t+2)
     const uint32_t* vs = (const uint32_t*)st.vid;
     uint32_t* vd = (uint32_t*)(vids + b0);
     for (uint32_t i = threadIdx.x; i < m * VW; i += 256u) vd[i] = vs[i];

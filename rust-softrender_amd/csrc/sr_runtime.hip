@@ -252,7 +252,7 @@ int draw_tris(sr_context ctx, sr_framebuffer fb, const DrawPlan& pl, const sr_dr
     SetupParams sp;
     sp.v = vb; sp.idx = idx; sp.n = ntris; sp.cull = d->cull; sp.W = pl.W; sp.H = pl.H;
     sp.ets = pl.ets; sp.tw = pl.tw; sp.th = pl.th;
-    sp.recs = P<TriRec>(ctx->recs); sp.bins = bins; sp.vids = vids;
+    sp.recs = P<TriRec>(ctx->recs); sp.bins = bins; sp.vids = idx ? vids : nullptr;
     sp.counts = P<uint32_t>(ctx->counts);
     sp.err = P<uint32_t>(ctx->misc);
     // with no stencil effects, tiles whose pixels the triangle cannot cover
@@ -725,6 +725,7 @@ int batch_enqueue(sr_context ctx, BatchWork& W, const BatchPlan& pl, bool use_gr
     const int swapped_passes = (local_bits + 7) / 8;
     const uint32_t* sorted_vals = (swapped_passes & 1) ? P<uint32_t>(W.vals2) : P<uint32_t>(W.vals);
     std::vector<RasterParams> rp(ns);
+    bool need_vids = false;  // some draw shades through the vertex-id table
     for (uint32_t k = 0; k < ns; k++) {
         const sr_batch_item& it = pl.items[k];
         const sr_draw_desc* d = &pl.descs[k];
@@ -744,6 +745,7 @@ int batch_enqueue(sr_context ctx, BatchWork& W, const BatchPlan& pl, bool use_gr
         // a scene without indices whose vertices start at 3x its first
         // triangle needs no vertex ids: vertex 3t+k
         const bool ident = !it.mesh->indexed && pl.geom[k].vbase == 3ull * pl.geom[k].tbase;
+        need_vids = need_vids || !ident;
         R.vids = ident ? nullptr
                        : (const TriVid*)((const TriBin*)(P<TriRec>(W.recs) + std::max<uint64_t>(NT, 1)) +
                                          std::max<uint64_t>(NT, 1));
@@ -801,7 +803,8 @@ int batch_enqueue(sr_context ctx, BatchWork& W, const BatchPlan& pl, bool use_gr
         // the triangle records, then their binning records, in one buffer
         TriBin* const bins = (TriBin*)(P<TriRec>(W.recs) + std::max<uint64_t>(NT, 1));
         TriVid* const vids = (TriVid*)(bins + std::max<uint64_t>(NT, 1));
-        launch_tri_setup_batch(T, vb, P<TriRec>(W.recs), bins, vids, P<uint32_t>(W.counts), misc + 2, s);
+        launch_tri_setup_batch(T, vb, P<TriRec>(W.recs), bins, need_vids ? vids : nullptr, P<uint32_t>(W.counts),
+                               misc + 2, s);
         launch_scan(P<uint32_t>(W.counts), P<uint32_t>(W.offsets), (uint32_t)NT, P<uint32_t>(W.scan), misc + 1, s);
         launch_tri_emit_batch(T, P<TriRec>(W.recs), bins, P<uint32_t>(W.counts), P<uint32_t>(W.offsets), cap,
                               P<uint32_t>(W.keys), P<uint32_t>(W.vals), misc + 0, s);
