@@ -330,7 +330,7 @@ def main():
     ntris = a.tris
     pairs = stats.get("bin_pairs") or 0
     W, H = 1920, 1080
-    setup_bytes = ntris * (24 + 12 + 80 + 4)        # read x,y,z per vertex; write record + count
+    setup_bytes = ntris * (36 + 64 + 24 + 4)        # read x,y,z of 3 vertices; write record, binning record, count
     sort_bytes = pairs * 8 * 2 * 2 + pairs * 4 * 2  # 2 radix passes (r+w keys/vals) + hist reads
     if stages.get("setup"):
         out["roofline_setup"] = {"bound": "hbm", "achieved": round(setup_bytes / stages["setup"] / 1e6, 1),
