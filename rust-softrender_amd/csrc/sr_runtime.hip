@@ -721,7 +721,11 @@ int batch_enqueue(sr_context ctx, BatchWork& W, const BatchPlan& pl, bool use_gr
     uint32_t* d_group_first = d_group_tile_end + ng;
     uint32_t* misc = P<uint32_t>(W.misc);  // [0] overflow [1] pairs [2] err [3..4] sort [5] tile cost total
                                             // [6] pairs to sort (0 on overflow)
-    const VertexBuf vb{P<float>(W.verts), nvb, 6};  // rows of 6: the larger program's varyings
+    // varying rows as wide as the widest vertex program of the batch needs
+    // (at most 6: the workspace holds 10 floats a vertex)
+    uint32_t kmax = 1;
+    for (const SceneGeom& g : pl.geom) kmax = std::max(kmax, (uint32_t)g.K);
+    const VertexBuf vb{P<float>(W.verts), nvb, kmax};
     const int swapped_passes = (local_bits + 7) / 8;
     const uint32_t* sorted_vals = (swapped_passes & 1) ? P<uint32_t>(W.vals2) : P<uint32_t>(W.vals);
     std::vector<RasterParams> rp(ns);
